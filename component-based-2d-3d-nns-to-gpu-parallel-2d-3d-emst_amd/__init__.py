@@ -1,0 +1,230 @@
+"""ctypes binding of libemst_amd.so (the C ABI in include/emst.h).
+
+The class `Calculateur` mirrors the reference's entry surface
+(emst_coalition/include/CalculateurEMST.h:43-58: initialize, run, activate,
+getLinks, getAdaptiveMap) over an in-memory point array.  The library must be
+built (see build()); nothing here falls back to a CPU path: a missing or
+unloadable library raises.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libemst_amd.so")
+
+EMST_OK = 0
+STATUS = {0: "EMST_OK", 1: "EMST_E_ARG", 2: "EMST_E_HIP", 3: "EMST_E_OOM", 4: "EMST_E_NONFINITE",
+          5: "EMST_E_STATE", 6: "EMST_E_NODEVICE", 7: "EMST_E_LIMIT", 8: "EMST_E_IO"}
+
+# every symbol include/emst.h declares
+EXPORTS = [
+    "emst_default_options", "emst_create", "emst_destroy", "emst_last_error", "emst_set_points",
+    "emst_run", "emst_step", "emst_get_edges", "emst_get_weight", "emst_get_labels",
+    "emst_get_rescaled", "emst_get_trace", "emst_get_candidates", "emst_get_cell_grid",
+    "emst_get_times", "emst_synth", "emst_version",
+]
+
+
+class EmstError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("level_radius", ctypes.c_int32),
+                ("max_rounds", ctypes.c_int32), ("bucket_size", ctypes.c_int32),
+                ("trace", ctypes.c_int32), ("device", ctypes.c_int32),
+                ("radius_mode", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+
+
+class RoundTrace(ctypes.Structure):
+    _fields_ = [("iteration", ctypes.c_int32), ("size", ctypes.c_int32), ("length", ctypes.c_double),
+                ("time_flatten", ctypes.c_float), ("time_termination", ctypes.c_float),
+                ("time_find_next_closest", ctypes.c_float), ("time_find_min_pair", ctypes.c_float),
+                ("time_connect_union", ctypes.c_float), ("searched", ctypes.c_int64)]
+
+
+def build(jobs=4):
+    """Compile libemst_amd.so for gfx950 with hipcc (package Makefile)."""
+    subprocess.run(["make", "-s", f"-j{jobs}", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run build() (hipcc, gfx950) first")
+    L = ctypes.CDLL(LIB_PATH)
+    P, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+    L.emst_default_options.argtypes = [P]
+    L.emst_create.argtypes = [P]
+    L.emst_create.restype = P
+    L.emst_destroy.argtypes = [P]
+    L.emst_last_error.argtypes = [P]
+    L.emst_last_error.restype = ctypes.c_char_p
+    L.emst_set_points.argtypes = [P, P, i64, i32, i32]
+    L.emst_run.argtypes = [P]
+    L.emst_step.argtypes = [P, P]
+    L.emst_get_edges.argtypes = [P, P, P, P, P]
+    L.emst_get_weight.argtypes = [P, P, P]
+    L.emst_get_labels.argtypes = [P, P]
+    L.emst_get_rescaled.argtypes = [P, P, P]
+    L.emst_get_trace.argtypes = [P, P, P]
+    L.emst_get_candidates.argtypes = [P, P, P, P]
+    L.emst_get_cell_grid.argtypes = [P, i32, P, P, P, P]
+    L.emst_get_times.argtypes = [P, P, P]
+    L.emst_synth.argtypes = [i32, i64, i32, ctypes.c_uint64, P]
+    L.emst_version.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def synth(kind, n, dim, seed):
+    """Synthetic cloud (0 uniform, 1 Gaussian mixture, 2 power-law), n x dim float32."""
+    out = np.empty((n, dim), np.float32)
+    rc = lib().emst_synth(kind, n, dim, seed, _p(out))
+    if rc:
+        raise EmstError(rc, "emst_synth")
+    return out
+
+
+class Calculateur:
+    """One EMST context on one device (CalculateurEMST / SolutionEMST)."""
+
+    def __init__(self, dim=3, level_radius=1, max_rounds=100, bucket_size=0, trace=False, device=0):
+        L = lib()
+        o = Options()
+        L.emst_default_options(ctypes.byref(o))
+        o.dim, o.level_radius, o.max_rounds = dim, level_radius, max_rounds
+        o.bucket_size, o.trace, o.device = bucket_size, int(trace), device
+        self._L = L
+        self._c = L.emst_create(ctypes.byref(o))
+        if not self._c:
+            raise EmstError(6, L.emst_last_error(None).decode())
+        self.dim = dim
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_c", None):
+            self._L.emst_destroy(self._c)
+            self._c = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc != EMST_OK:
+            raise EmstError(rc, f"{what}: {self._L.emst_last_error(self._c).decode()}")
+
+    # -- reference entry surface -----------------------------------------
+    def initialize(self, points):
+        """SolutionEMST::initialize(NetLink&, pMin, pMax) over a host array."""
+        pts = np.ascontiguousarray(points, dtype=np.float32)
+        self._keep = pts
+        n, dim = pts.shape
+        self._chk(self._L.emst_set_points(self._c, _p(pts), n, dim, 0), "set_points")
+        self.n, self.dim = n, dim
+
+    def initialize_device(self, dev_ptr, n, dim):
+        """Same, for points already resident on the device (AoS float32)."""
+        self._chk(self._L.emst_set_points(self._c, ctypes.c_void_p(dev_ptr), n, dim, 1), "set_points")
+        self.n, self.dim = n, dim
+
+    def run(self):
+        self._chk(self._L.emst_run(self._c), "run")
+
+    def activate(self):
+        """One Boruvka round; returns False once the termination test holds."""
+        fin = ctypes.c_int32(0)
+        self._chk(self._L.emst_step(self._c, ctypes.byref(fin)), "step")
+        return fin.value == 0
+
+    def getLinks(self):
+        """Forest edges (u < v, sorted) and their float32 d2 (rescaled)."""
+        m = ctypes.c_int64(0)
+        self._chk(self._L.emst_get_edges(self._c, None, None, None, ctypes.byref(m)), "edges")
+        u = np.empty(m.value, np.int32)
+        v = np.empty(m.value, np.int32)
+        d = np.empty(m.value, np.float32)
+        self._chk(self._L.emst_get_edges(self._c, _p(u), _p(v), _p(d), ctypes.byref(m)), "edges")
+        return u, v, d
+
+    def getAdaptiveMap(self):
+        out = np.empty((self.n, self.dim), np.float32)
+        ux = ctypes.c_double(0)
+        self._chk(self._L.emst_get_rescaled(self._c, _p(out), ctypes.byref(ux)), "rescaled")
+        return out, ux.value
+
+    def evaluate(self):
+        """(FP64 weight on input coordinates, edge count)."""
+        w = ctypes.c_double(0)
+        m = ctypes.c_int64(0)
+        self._chk(self._L.emst_get_weight(self._c, ctypes.byref(w), ctypes.byref(m)), "weight")
+        return w.value, m.value
+
+    def labels(self):
+        out = np.empty(self.n, np.int32)
+        if self.n:
+            self._chk(self._L.emst_get_labels(self._c, _p(out)), "labels")
+        return out
+
+    def trace(self):
+        k = ctypes.c_int32(0)
+        self._chk(self._L.emst_get_trace(self._c, None, ctypes.byref(k)), "trace")
+        rows = (RoundTrace * max(k.value, 1))()
+        self._chk(self._L.emst_get_trace(self._c, rows, ctypes.byref(k)), "trace")
+        return [dict((f, getattr(rows[i], f)) for f, _ in RoundTrace._fields_) for i in range(k.value)]
+
+    def candidates(self):
+        k = ctypes.c_int32(0)
+        self._chk(self._L.emst_get_candidates(self._c, None, None, ctypes.byref(k)), "candidates")
+        cand = np.empty((self.n, k.value), np.int32)
+        cd2 = np.empty((self.n, k.value), np.float32)
+        self._chk(self._L.emst_get_candidates(self._c, _p(cand), _p(cd2), ctypes.byref(k)), "candidates")
+        return cand, cd2
+
+    def cell_grid(self, level_radius=1):
+        ext = np.zeros(3, np.int32)
+        self._chk(self._L.emst_get_cell_grid(self._c, level_radius, _p(ext), None, None, None), "cell grid")
+        ncell = int(ext[0]) * int(ext[1]) * int(ext[2])
+        cell_of = np.empty((self.n, 3), np.int32)
+        start = np.empty(ncell + 1, np.int64)
+        cpts = np.empty(max(self.n, 1), np.int32)
+        self._chk(self._L.emst_get_cell_grid(self._c, level_radius, _p(ext), _p(cell_of), _p(start),
+                                             _p(cpts)), "cell grid")
+        return ext, cell_of, start, cpts[: self.n]
+
+    def times(self):
+        t = (ctypes.c_float * 4)()
+        k = ctypes.c_int32(4)
+        self._chk(self._L.emst_get_times(self._c, t, ctypes.byref(k)), "times")
+        return list(t)[: k.value]
+
+
+def emst(points, **kw):
+    """Whole path on a host array: returns (u, v, d2, weight)."""
+    pts = np.ascontiguousarray(points, dtype=np.float32)
+    with Calculateur(dim=pts.shape[1], **kw) as c:
+        c.initialize(pts)
+        c.run()
+        u, v, d = c.getLinks()
+        w, _ = c.evaluate()
+    return u, v, d, w

@@ -1,0 +1,644 @@
+// emst_api.hip -- host side of the C ABI (include/emst.h).
+//
+// A context holds one point set on one device and one HIP stream.  The build
+// (bbox, rescale, Morton sort, hierarchy, round-1 cone search) runs inside
+// the first emst_run/emst_step, so that a timed run starts from resident
+// input points, like the reference's cuda_duree span plus its cell build.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/emst.h"
+#include "emst_internal.h"
+
+namespace emst {
+void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const uint64_t* cand_key,
+                        const int32_t* cand_pos, int32_t* label, int32_t* label2, int32_t* corr,
+                        uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, int32_t* fixed,
+                        Counters* C, int32_t* eu, int32_t* ev, float* ed2, int32_t* eround,
+                        int round, hipStream_t s);
+}
+
+using namespace emst;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+constexpr int kWeightBlocks = 512;
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace
+
+struct emst_ctx {
+    emst_options opt{};
+    hipStream_t stream = nullptr;
+    std::string err;
+    // point set
+    int64_t n = 0;
+    int dim = 0;
+    int K = 0;
+    int bucket = 32;
+    int64_t L = 0;
+    const float* xyz = nullptr;       // input points on the device (AoS)
+    bool have_points = false;
+    bool built = false;
+    bool finished = false;
+    int rounds = 0;
+    uint32_t edges = 0;
+    uint32_t zero_or_empty = 0;
+    std::vector<emst_round_trace> trace;
+    float times[4] = {0, 0, 0, 0};
+    hipEvent_t ev[8] = {};
+    // device buffers
+    DevBuf<float> xyz_own, resc;
+    DevBuf<uint64_t> keys0, keys1, cand_key, corr_key, comp1;
+    DevBuf<uint32_t> vals0, vals1, hist, scan_tmp, comp2;
+    DevBuf<float4> pts;
+    DevBuf<int32_t> inv, cand_pos, label, label2, corr, fixed, list, eu, ev_, eround, geo;
+    DevBuf<Node> nodes;
+    DevBuf<float> ed2;
+    DevBuf<double> partial;
+    Params* P = nullptr;
+    Counters* C = nullptr;
+};
+
+namespace {
+
+int fail(emst_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                     \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(ctx, EMST_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCHCHK(ctx)                                                                        \
+    do {                                                                                      \
+        hipError_t e_ = hipGetLastError();                                                    \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(ctx, EMST_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int ensure(emst_ctx* c, DevBuf<T>& b, size_t count)
+{
+    if (count == 0) count = 1;
+    if (b.cap >= count) return EMST_OK;
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    hipError_t e = hipMalloc((void**)&b.p, sizeof(T) * count);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        b.p = nullptr;
+        return fail(c, EMST_E_OOM, "hipMalloc of " + std::to_string(sizeof(T) * count) + " bytes failed");
+    }
+    b.cap = count;
+    return EMST_OK;
+}
+
+template <typename T>
+void release(DevBuf<T>& b)
+{
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+int choose_bucket(const emst_options& o, int dim)
+{
+    int b = o.bucket_size;
+    if (b <= 0) b = dim == 3 ? 32 : 32;
+    return std::max(4, std::min(b, 256));
+}
+
+int alloc_all(emst_ctx* c)
+{
+    const int64_t n = c->n;
+    const int K = c->K;
+    int rc;
+#define ENS(buf, cnt) \
+    if ((rc = ensure(c, c->buf, (size_t)(cnt))) != EMST_OK) return rc;
+    ENS(resc, n * c->dim);
+    ENS(keys0, n);
+    ENS(keys1, n);
+    ENS(vals0, n);
+    ENS(vals1, n);
+    ENS(hist, radix_hist_words(n));
+    ENS(scan_tmp, scan_tmp_words((int64_t)radix_hist_words(n)) + scan_tmp_words(n) + 16);
+    ENS(pts, n);
+    ENS(inv, n);
+    ENS(nodes, 2 * c->L);
+    ENS(cand_key, n * K);
+    ENS(cand_pos, n * K);
+    ENS(label, n);
+    ENS(label2, n);
+    ENS(corr, n);
+    ENS(fixed, n);
+    ENS(list, n);
+    ENS(corr_key, n);
+    ENS(comp1, n);
+    ENS(comp2, n);
+    ENS(eu, n);
+    ENS(ev_, n);
+    ENS(eround, n);
+    ENS(ed2, n);
+    ENS(partial, kWeightBlocks);
+    ENS(geo, 16);
+#undef ENS
+    return EMST_OK;
+}
+
+int read_counters(emst_ctx* c, Counters* h)
+{
+    HIPCHK(c, hipMemcpyAsync(h, c->C, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return EMST_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return ms;
+}
+
+// bbox, rescale, Morton sort, hierarchy, round-1 cone search
+int build(emst_ctx* c)
+{
+    const int64_t n = c->n;
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipEventRecord(c->ev[0], s));
+    launch_bbox(c->xyz, n, c->dim, c->P, s);
+    launch_params(c->P, n, c->dim, s);
+    LAUNCHCHK(c);
+    uint32_t nonfinite = 0;
+    HIPCHK(c, hipMemcpyAsync(&nonfinite, &c->P->nonfinite, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (nonfinite) return fail(c, EMST_E_NONFINITE, "input holds a NaN or Inf coordinate");
+    launch_rescale_morton(c->xyz, n, c->dim, c->P, c->resc.p, c->keys0.p, s);
+    radix_sort(c->keys0.p, c->vals0.p, c->keys1.p, c->vals1.p, n, c->dim == 3 ? 63 : 64, true,
+               c->hist.p, c->scan_tmp.p, s);
+    launch_gather_sorted(c->resc.p, c->vals0.p, n, c->dim, c->pts.p, c->inv.p, s);
+    launch_build_tree(c->pts.p, n, c->bucket, c->L, c->nodes.p, s);
+    launch_round_init(n, c->label.p, c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->C, s);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipEventRecord(c->ev[1], s));
+    launch_cone_search(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, nullptr, n, c->label.p, false,
+                       c->inv.p, c->cand_key.p, c->cand_pos.p, s);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipEventRecord(c->ev[2], s));
+    launch_flag_research(c->K, n, c->cand_key.p, c->list.p, c->C, s);
+    LAUNCHCHK(c);
+    Counters h{};
+    int rc = read_counters(c, &h);
+    if (rc) return rc;
+    c->zero_or_empty = h.zero_or_empty;
+    c->times[0] = elapsed(c->ev[0], c->ev[1]);
+    c->times[1] = elapsed(c->ev[1], c->ev[2]);
+    c->built = true;
+    c->rounds = 0;
+    c->edges = 0;
+    c->finished = n <= 1;
+    c->trace.clear();
+    return EMST_OK;
+}
+
+int weight_of(emst_ctx* c, int64_t m, double* w)
+{
+    *w = 0;
+    if (m <= 0) return EMST_OK;
+    launch_weight(c->xyz, c->dim, c->eu.p, c->ev_.p, m, c->partial.p, kWeightBlocks, c->stream);
+    LAUNCHCHK(c);
+    std::vector<double> h(kWeightBlocks);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->partial.p, sizeof(double) * kWeightBlocks, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double s = 0;
+    for (double x : h) s += x;
+    *w = s;
+    return EMST_OK;
+}
+
+// One Boruvka round (SolutionEMST::activate).  Termination is tested first.
+int round_once(emst_ctx* c, int32_t* finished)
+{
+    const int64_t n = c->n;
+    hipStream_t s = c->stream;
+    if (c->finished || (int64_t)c->edges >= n - 1 || c->rounds >= c->opt.max_rounds) {
+        c->finished = true;
+        *finished = 1;
+        return EMST_OK;
+    }
+    *finished = 0;
+    const bool tr = c->opt.trace != 0;
+    int64_t searched = c->rounds == 0 ? n : 0;
+    if (tr) HIPCHK(c, hipEventRecord(c->ev[3], s));
+    if (c->rounds > 0 && c->zero_or_empty > 0) {
+        HIPCHK(c, hipMemsetAsync(&c->C->research, 0, sizeof(uint32_t), s));
+        launch_flag_research(c->K, n, c->cand_key.p, c->list.p, c->C, s);
+        LAUNCHCHK(c);
+        Counters h{};
+        int rc = read_counters(c, &h);
+        if (rc) return rc;
+        if (h.research > 0)
+            launch_cone_search(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, c->list.p, h.research,
+                               c->label.p, true, c->inv.p, c->cand_key.p, c->cand_pos.p, s);
+        LAUNCHCHK(c);
+        searched = h.research;
+    }
+    const int round = c->rounds + 1;
+    for (int ph = 0; ph < 4; ++ph) {
+        launch_round_phase(ph, c->K, n, c->pts.p, c->cand_key.p, c->cand_pos.p, c->label.p, c->label2.p,
+                           c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->fixed.p, c->C, c->eu.p,
+                           c->ev_.p, c->ed2.p, c->eround.p, round, s);
+        if (tr) HIPCHK(c, hipEventRecord(c->ev[4 + ph], s));
+    }
+    LAUNCHCHK(c);
+    Counters h{};
+    int rc = read_counters(c, &h);
+    if (rc) return rc;
+    for (int again = 0; h.unflat; ++again) {
+        if (again == 64) return fail(c, EMST_E_HIP, "flatten did not converge (cycle in the hook forest)");
+        HIPCHK(c, hipMemsetAsync(&c->C->unflat, 0, sizeof(uint32_t), s));
+        launch_round_phase(3, c->K, n, c->pts.p, c->cand_key.p, c->cand_pos.p, c->label.p, c->label2.p,
+                           c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->fixed.p, c->C, c->eu.p,
+                           c->ev_.p, c->ed2.p, c->eround.p, round, s);
+        LAUNCHCHK(c);
+        if ((rc = read_counters(c, &h))) return rc;
+    }
+    const uint32_t before = c->edges;
+    c->edges = h.edges;
+    c->rounds = round;
+    if (tr) {
+        emst_round_trace row{};
+        row.iteration = round;
+        row.size = (int32_t)c->edges;
+        double w = 0;
+        rc = weight_of(c, c->edges, &w);
+        if (rc) return rc;
+        row.length = w;
+        row.time_find_next_closest = elapsed(c->ev[3], c->ev[4]);
+        row.time_find_min_pair = elapsed(c->ev[4], c->ev[5]);
+        row.time_connect_union = elapsed(c->ev[5], c->ev[6]);
+        row.time_flatten = elapsed(c->ev[6], c->ev[7]);
+        row.time_termination = 0;
+        row.searched = searched;
+        c->trace.push_back(row);
+    }
+    if (c->edges == before) c->finished = true;     // no component has an outgoing candidate
+    return EMST_OK;
+}
+
+int need_points(emst_ctx* c)
+{
+    if (!c) return EMST_E_ARG;
+    if (!c->have_points) return fail(c, EMST_E_STATE, "no points set");
+    return EMST_OK;
+}
+
+int ensure_built(emst_ctx* c)
+{
+    int rc = need_points(c);
+    if (rc) return rc;
+    if (!c->built) return build(c);
+    return EMST_OK;
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+void emst_default_options(emst_options* o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->dim = 3;
+    o->level_radius = 1;
+    o->max_rounds = 100;
+    o->bucket_size = 0;
+    o->trace = 0;
+    o->device = 0;
+    o->radius_mode = 0;
+}
+
+emst_ctx* emst_create(const emst_options* opt)
+{
+    emst_options o;
+    if (opt) o = *opt;
+    else emst_default_options(&o);
+    if (o.dim != 2 && o.dim != 3) { g_create_error = "dim must be 2 or 3"; return nullptr; }
+    if (o.max_rounds <= 0) o.max_rounds = 100;
+    if (o.level_radius <= 0) o.level_radius = 1;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        g_create_error = "no HIP device";
+        return nullptr;
+    }
+    if (o.device < 0 || o.device >= ndev) { g_create_error = "device ordinal out of range"; return nullptr; }
+    if (hipSetDevice(o.device) != hipSuccess) { g_create_error = "hipSetDevice failed"; return nullptr; }
+    emst_ctx* c = new emst_ctx();
+    c->opt = o;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_create_error = "hipStreamCreate failed";
+        delete c;
+        return nullptr;
+    }
+    for (auto& e : c->ev) hipEventCreate(&e);
+    if (hipMalloc((void**)&c->P, sizeof(Params)) != hipSuccess ||
+        hipMalloc((void**)&c->C, sizeof(Counters)) != hipSuccess) {
+        g_create_error = "hipMalloc of context state failed";
+        emst_destroy(c);
+        return nullptr;
+    }
+    return c;
+}
+
+int emst_destroy(emst_ctx* c)
+{
+    if (!c) return EMST_E_ARG;
+    hipSetDevice(c->opt.device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    release(c->xyz_own); release(c->resc); release(c->keys0); release(c->keys1); release(c->cand_key);
+    release(c->corr_key); release(c->comp1); release(c->vals0); release(c->vals1); release(c->hist);
+    release(c->scan_tmp); release(c->comp2); release(c->pts); release(c->inv); release(c->cand_pos);
+    release(c->label); release(c->label2); release(c->corr); release(c->fixed); release(c->list);
+    release(c->eu); release(c->ev_); release(c->eround); release(c->geo); release(c->nodes);
+    release(c->ed2); release(c->partial);
+    if (c->P) hipFree(c->P);
+    if (c->C) hipFree(c->C);
+    for (auto& e : c->ev) if (e) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return EMST_OK;
+}
+
+const char* emst_last_error(const emst_ctx* c)
+{
+    if (!c) return g_create_error.c_str();
+    return c->err.c_str();
+}
+
+int emst_set_points(emst_ctx* c, const float* xyz, int64_t n, int32_t dim, int32_t on_device)
+{
+    if (!c) return EMST_E_ARG;
+    if (n < 0 || (n > 0 && !xyz)) return fail(c, EMST_E_ARG, "n < 0 or NULL points");
+    if (dim != 2 && dim != 3) return fail(c, EMST_E_ARG, "dim must be 2 or 3");
+    if (n >= ((int64_t)1 << 31) - 1) return fail(c, EMST_E_LIMIT, "n must be below 2^31 - 1");
+    HIPCHK(c, hipSetDevice(c->opt.device));
+    c->n = n;
+    c->dim = dim;
+    c->K = 2 * dim * (1 << (dim - 1));
+    c->bucket = choose_bucket(c->opt, dim);
+    const int64_t nb = std::max<int64_t>(1, (n + c->bucket - 1) / c->bucket);
+    int64_t L = 1;
+    while (L < nb) L <<= 1;
+    c->L = L;
+    int rc = alloc_all(c);
+    if (rc) return rc;
+    if (on_device) {
+        c->xyz = xyz;
+    } else {
+        rc = ensure(c, c->xyz_own, (size_t)(n * dim));
+        if (rc) return rc;
+        if (n > 0)
+            HIPCHK(c, hipMemcpyAsync(c->xyz_own.p, xyz, sizeof(float) * n * dim, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->xyz = c->xyz_own.p;
+    }
+    c->have_points = true;
+    c->built = false;
+    c->finished = false;
+    c->rounds = 0;
+    c->edges = 0;
+    c->trace.clear();
+    return EMST_OK;
+}
+
+int emst_run(emst_ctx* c)
+{
+    int rc = need_points(c);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->opt.device));
+    rc = ensure_built(c);
+    if (rc) return rc;
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    int32_t fin = 0;
+    while (!fin) {
+        rc = round_once(c, &fin);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev[3]));
+    c->times[2] = elapsed(c->ev[2], c->ev[3]);
+    c->times[3] = c->times[0] + c->times[1] + c->times[2];
+    return EMST_OK;
+}
+
+int emst_step(emst_ctx* c, int32_t* finished)
+{
+    if (!finished) return fail(c, EMST_E_ARG, "finished must not be NULL");
+    int rc = need_points(c);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->opt.device));
+    rc = ensure_built(c);
+    if (rc) return rc;
+    return round_once(c, finished);
+}
+
+int emst_get_edges(emst_ctx* c, int32_t* u, int32_t* v, float* d2, int64_t* m)
+{
+    if (!m) return fail(c, EMST_E_ARG, "m must not be NULL");
+    int rc = need_points(c);
+    if (rc) return rc;
+    const int64_t cnt = c->built ? (int64_t)c->edges : 0;
+    if (!u && !v && !d2) { *m = cnt; return EMST_OK; }
+    if (*m < cnt) { *m = cnt; return fail(c, EMST_E_ARG, "edge arrays too small"); }
+    *m = cnt;
+    if (cnt == 0) return EMST_OK;
+    HIPCHK(c, hipSetDevice(c->opt.device));
+    hipStream_t s = c->stream;
+    launch_edge_keys(c->eu.p, c->ev_.p, cnt, c->keys0.p, s);
+    radix_sort(c->keys0.p, c->vals0.p, c->keys1.p, c->vals1.p, cnt, 64, true, c->hist.p, c->scan_tmp.p, s);
+    // reuse label2/list as staging (free between rounds)
+    int32_t* du = c->label2.p;
+    int32_t* dv = c->list.p;
+    float* dd = reinterpret_cast<float*>(c->vals1.p);
+    launch_edges_out(c->keys0.p, c->vals0.p, c->ed2.p, cnt, du, dv, dd, s);
+    LAUNCHCHK(c);
+    if (u) HIPCHK(c, hipMemcpyAsync(u, du, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost, s));
+    if (v) HIPCHK(c, hipMemcpyAsync(v, dv, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost, s));
+    if (d2) HIPCHK(c, hipMemcpyAsync(d2, dd, sizeof(float) * cnt, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    return EMST_OK;
+}
+
+int emst_get_weight(emst_ctx* c, double* w, int64_t* size)
+{
+    if (!w) return fail(c, EMST_E_ARG, "weight must not be NULL");
+    int rc = need_points(c);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->opt.device));
+    const int64_t cnt = c->built ? (int64_t)c->edges : 0;
+    if (size) *size = cnt;
+    return weight_of(c, cnt, w);
+}
+
+int emst_get_labels(emst_ctx* c, int32_t* labels)
+{
+    if (!labels) return fail(c, EMST_E_ARG, "labels must not be NULL");
+    int rc = ensure_built(c);
+    if (rc) return rc;
+    if (c->n == 0) return EMST_OK;
+    launch_labels_out(c->n, c->label.p, c->pts.p, c->inv.p, c->list.p, c->stream);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipMemcpyAsync(labels, c->list.p, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return EMST_OK;
+}
+
+int emst_get_rescaled(emst_ctx* c, float* xyz, double* unit_x)
+{
+    int rc = ensure_built(c);
+    if (rc) return rc;
+    if (xyz && c->n > 0)
+        HIPCHK(c, hipMemcpyAsync(xyz, c->resc.p, sizeof(float) * c->n * c->dim, hipMemcpyDeviceToHost, c->stream));
+    if (unit_x) HIPCHK(c, hipMemcpyAsync(unit_x, &c->P->unit_x, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return EMST_OK;
+}
+
+int emst_get_trace(emst_ctx* c, emst_round_trace* rows, int32_t* nrows)
+{
+    if (!c || !nrows) return EMST_E_ARG;
+    const int32_t cnt = (int32_t)c->trace.size();
+    if (!rows) { *nrows = cnt; return EMST_OK; }
+    const int32_t k = std::min(*nrows, cnt);
+    for (int32_t i = 0; i < k; ++i) rows[i] = c->trace[i];
+    *nrows = k;
+    return EMST_OK;
+}
+
+int emst_get_candidates(emst_ctx* c, int32_t* cand, float* cd2, int32_t* k)
+{
+    int rc = ensure_built(c);
+    if (rc) return rc;
+    if (k) *k = c->K;
+    if (!cand && !cd2) return EMST_OK;
+    if (!cand || !cd2) return fail(c, EMST_E_ARG, "cand and cand_d2 are both required");
+    if (c->n == 0) return EMST_OK;
+    // staging in keys1/vals1 would be too small for n*K: use cand_pos-sized scratch
+    DevBuf<int32_t> tc;
+    DevBuf<float> td;
+    if ((rc = ensure(c, tc, (size_t)(c->n * c->K))) != EMST_OK) return rc;
+    if ((rc = ensure(c, td, (size_t)(c->n * c->K))) != EMST_OK) { release(tc); return rc; }
+    launch_cands_out(c->K, c->n, c->cand_key.p, c->inv.p, tc.p, td.p, c->stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(cand, tc.p, sizeof(int32_t) * c->n * c->K, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(cd2, td.p, sizeof(float) * c->n * c->K, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    release(tc);
+    release(td);
+    if (e != hipSuccess) return fail(c, EMST_E_HIP, std::string("candidates: ") + hipGetErrorString(e));
+    return EMST_OK;
+}
+
+int emst_get_cell_grid(emst_ctx* c, int32_t R, int32_t* ext_dual, int32_t* cell_of, int64_t* cell_start,
+                       int32_t* cell_pts)
+{
+    if (!ext_dual) return fail(c, EMST_E_ARG, "ext_dual must not be NULL");
+    if (R <= 0) return fail(c, EMST_E_ARG, "level radius must be positive");
+    int rc = ensure_built(c);
+    if (rc) return rc;
+    Params hp{};
+    HIPCHK(c, hipMemcpyAsync(&hp, c->P, sizeof(Params), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // extents (SolutionEMST.cu:420-430) and ViewGridQuadMD recurrences (ViewGrid.h:251-265);
+    // the rescaled minimum is 0 (the box corner maps to 0 exactly).
+    int32_t geo[16] = {0};
+    for (int k = 0; k < 3; ++k) {
+        int32_t ext = 1, pc = 0, PC = 0, ed = 1;
+        if (k < c->dim) {
+            ext = (int32_t)std::ceil((double)hp.newmax[k] - 0.0) + 1;
+            pc = ext / 2;
+            PC = (pc + (R - 1)) / R;
+            const int32_t eb = (PC + 1) + ((ext - pc) + (R - 1)) / R;
+            const int32_t PCD = PC / 2;
+            ed = (PCD + 1) + ((eb - PC) + 1) / 2;
+        }
+        geo[k] = ed;
+        geo[3 + k] = PC;
+        geo[6 + k] = pc;
+    }
+    geo[9] = R;
+    geo[10] = c->dim;
+    for (int k = 0; k < 3; ++k) ext_dual[k] = geo[k];
+    if (!cell_of && !cell_start && !cell_pts) return EMST_OK;
+    const int64_t ncell = (int64_t)geo[0] * geo[1] * geo[2];
+    const int64_t n = c->n;
+    DevBuf<int32_t> dcell;
+    DevBuf<uint32_t> dcount, dstart, tmp;
+    auto cleanup = [&]() { release(dcell); release(dcount); release(dstart); release(tmp); };
+    if ((rc = ensure(c, dcell, (size_t)(3 * n))) || (rc = ensure(c, dcount, (size_t)ncell + 1)) ||
+        (rc = ensure(c, dstart, (size_t)ncell + 1)) || (rc = ensure(c, tmp, scan_tmp_words(ncell + 1) + 16))) {
+        cleanup();
+        return rc;
+    }
+    hipStream_t s = c->stream;
+    hipError_t e = hipMemcpyAsync(c->geo.p, geo, sizeof(geo), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemsetAsync(dcount.p, 0, sizeof(uint32_t) * (ncell + 1), s);
+    if (e == hipSuccess) {
+        launch_cell_grid(c->resc.p, n, c->dim, c->geo.p, dcell.p, c->keys0.p, dcount.p, s);
+        exclusive_scan_u32(dcount.p, dstart.p, ncell + 1, tmp.p, s);
+        int bits = 8;
+        while (bits < 64 && ((uint64_t)ncell >> bits) != 0) bits += 8;
+        radix_sort(c->keys0.p, c->vals0.p, c->keys1.p, c->vals1.p, n, bits, true, c->hist.p, c->scan_tmp.p, s);
+        e = hipGetLastError();
+    }
+    std::vector<uint32_t> hstart;
+    if (e == hipSuccess && cell_of) e = hipMemcpyAsync(cell_of, dcell.p, sizeof(int32_t) * 3 * n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && cell_start) {
+        hstart.resize(ncell + 1);
+        e = hipMemcpyAsync(hstart.data(), dstart.p, sizeof(uint32_t) * (ncell + 1), hipMemcpyDeviceToHost, s);
+    }
+    if (e == hipSuccess && cell_pts) e = hipMemcpyAsync(cell_pts, c->vals0.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    cleanup();
+    if (e != hipSuccess) return fail(c, EMST_E_HIP, std::string("cell grid: ") + hipGetErrorString(e));
+    if (cell_start)
+        for (int64_t i = 0; i <= ncell; ++i) cell_start[i] = hstart[i];
+    // the count of points inside the grid closes the CSR (outside points sort last)
+    return EMST_OK;
+}
+
+int emst_get_times(emst_ctx* c, float* times, int32_t* n_times)
+{
+    if (!c || !times || !n_times) return EMST_E_ARG;
+    const int32_t k = std::min<int32_t>(*n_times, 4);
+    for (int i = 0; i < k; ++i) times[i] = c->times[i];
+    *n_times = k;
+    return EMST_OK;
+}
+
+const char* emst_version(void) { return "emst_amd 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+// emst_internal.h -- shared types of the EMST engine (host + device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace emst {
+
+constexpr uint64_t kEmptyKey = ~0ull;   // empty cone / no candidate
+constexpr int kWave = 64;
+
+// Device-side scalar parameters of one point set.
+struct Params {
+    float pmin[3];          // bounding box of the input (A1)
+    float pmax[3];
+    double unit_x;          // rescale factor (A2), double as computed
+    float unit_xf;          // (float)unit_x, the factor applied per coordinate
+    float newmax[3];        // rescaled extremes (newmin is 0 by monotonicity)
+    float qscale;           // Morton quantisation scale
+    uint32_t nonfinite;     // set when a coordinate is NaN or Inf
+    uint32_t pad[3];
+};
+
+// Round counters, one device struct per context.
+struct Counters {
+    uint32_t edges;         // total forest edges
+    uint32_t research;      // points queued for a repeated cone search
+    uint32_t zero_or_empty; // points holding a zero-distance or no candidate
+    uint32_t unflat;        // flatten stopped at its step cap (run it again)
+};
+
+// Search-hierarchy node: tight box of a contiguous range of Morton-sorted points.
+struct alignas(32) Node {
+    float lo[3];
+    float hi[3];
+    float pad[2];
+};
+
+// ---- launch wrappers (emst_kernels.hip); all asynchronous on `s` ----
+void launch_bbox(const float* xyz, int64_t n, int dim, Params* P, hipStream_t s);
+void launch_params(Params* P, int64_t n, int dim, hipStream_t s);
+void launch_rescale_morton(const float* xyz, int64_t n, int dim, const Params* P, float* resc,
+                           uint64_t* keys, hipStream_t s);
+// LSD radix sort of (keys, vals) by bits [0, bits); result lands in (k0, v0).
+// vals_identity: values start as 0..n-1 (v0 is not read).
+void radix_sort(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, int64_t n, int bits,
+                bool vals_identity, uint32_t* hist, uint32_t* scan_tmp, hipStream_t s);
+size_t radix_hist_words(int64_t n);
+size_t scan_tmp_words(int64_t n);
+void launch_gather_sorted(const float* resc, const uint32_t* perm, int64_t n, int dim, float4* pts,
+                          int32_t* inv, hipStream_t s);
+void launch_build_tree(const float4* pts, int64_t n, int bucket, int64_t L, Node* nodes,
+                       hipStream_t s);
+// Cone search over queries [0,n) (qlist == nullptr) or qlist[0..nq).
+// check_labels: skip zero-distance points of the query's own component.
+void launch_cone_search(int dim, const float4* pts, const Node* nodes, int64_t L, int bucket,
+                        int64_t n, const int32_t* qlist, int64_t nq, const int32_t* label,
+                        bool check_labels, const int32_t* inv, uint64_t* cand_key,
+                        int32_t* cand_pos, hipStream_t s);
+void launch_round_init(int64_t n, int32_t* label, int32_t* corr, uint64_t* corr_key,
+                       uint64_t* comp1, uint32_t* comp2, Counters* C, hipStream_t s);
+void launch_flag_research(int K, int64_t n, const uint64_t* cand_key, int32_t* list,
+                          Counters* C, hipStream_t s);
+void launch_round(int K, int64_t n, const float4* pts, const uint64_t* cand_key,
+                  const int32_t* cand_pos, int32_t* label, int32_t* label2, int32_t* corr,
+                  uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, int32_t* fixed,
+                  Counters* C, int32_t* eu, int32_t* ev, float* ed2, int32_t* eround,
+                  int round, float* t_ms, hipEvent_t* ev_marks, hipStream_t s);
+void launch_weight(const float* xyz, int dim, const int32_t* eu, const int32_t* ev, int64_t m,
+                   double* partial, int nblocks, hipStream_t s);
+void launch_labels_out(int64_t n, const int32_t* label, const float4* pts, const int32_t* inv,
+                       int32_t* out, hipStream_t s);
+void launch_cands_out(int K, int64_t n, const uint64_t* cand_key, const int32_t* inv,
+                      int32_t* cand, float* cd2, hipStream_t s);
+void launch_edge_keys(const int32_t* eu, const int32_t* ev, int64_t m, uint64_t* keys,
+                      hipStream_t s);
+void launch_edges_out(const uint64_t* keys, const uint32_t* idx, const float* ed2, int64_t m,
+                      int32_t* u, int32_t* v, float* d2, hipStream_t s);
+// Reference cellular matrix (findCell + CSR).
+// geo: int32[16] = ext_dual[3], PC[3], pc[3], R, dim (host-computed, see emst_api.hip).
+void launch_cell_grid(const float* resc, int64_t n, int dim, const int32_t* geo, int32_t* cell_of,
+                      uint64_t* keys, uint32_t* cell_count, hipStream_t s);
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
+                        hipStream_t s);
+
+}  // namespace emst

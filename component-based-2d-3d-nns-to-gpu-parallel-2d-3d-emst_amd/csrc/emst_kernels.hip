@@ -1,0 +1,1117 @@
+// emst_kernels.hip -- gfx950 kernels of the EMST engine.
+//
+// Stages (reference rows they implement, paths relative to the reference root):
+//   bbox / params / rescale ..... InputRW.h:541-559, SolutionEMST.cu:305-379
+//   Morton keys + radix sort .... the cell-grid build (count, wave-level scan,
+//                                 scatter); cells are Morton prefixes
+//   gather / tree ............... leaf cells of <= bucket points with tight boxes
+//   cone search ................. NodeComputeOctantMD::computeOctants
+//                                 (NodeEMST.h:1018-1296): per point, the nearest
+//                                 point of each of the 2D x 2^(D-1) cones
+//   round kernels ............... search (NodeEMST.h:1303-1384), component list +
+//                                 min pair (NodeEMST.h:817-955), hooking
+//                                 (EMSTOperators.h:1267-1303), flatten
+//                                 (EMSTOperators.h:412-433)
+//   weight ...................... K_EMST_evaluate_ST (EMSTOperators.h:262-292)
+//   cell grid ................... ViewGridQuadMD::findCell (ViewGrid.h:339-359)
+//
+// Squared distances are fl(fl(dx*dx) + dy*dy) [+ dz*dz] with each "+ a*a"
+// fused (__builtin_fmaf); the file is built with -ffp-contract=off so no other
+// operation is contracted.
+#include "emst_internal.h"
+
+namespace emst {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ float d2_of(float dx, float dy) { return __builtin_fmaf(dy, dy, dx * dx); }
+__device__ __forceinline__ float d2_of(float dx, float dy, float dz)
+{
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+// Order-preserving float <-> uint maps for atomic min/max of floats.
+__device__ __forceinline__ uint32_t f2ord(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u)
+{
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt()
+{
+    const int lane = __lane_id();
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// ------------------------------------------------------------ bbox (A1)
+// Params.pmin/pmax hold order-mapped uints during the reduction.
+__global__ void k_bbox_init(Params* P)
+{
+    uint32_t* lo = reinterpret_cast<uint32_t*>(P->pmin);
+    uint32_t* hi = reinterpret_cast<uint32_t*>(P->pmax);
+    for (int k = 0; k < 3; ++k) { lo[k] = 0xffffffffu; hi[k] = 0u; }
+    P->nonfinite = 0;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_bbox(const float* __restrict__ xyz, int64_t n, Params* P)
+{
+    uint32_t lo[D], hi[D];
+    for (int k = 0; k < D; ++k) { lo[k] = 0xffffffffu; hi[k] = 0u; }
+    uint32_t bad = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        for (int k = 0; k < D; ++k) {
+            float c = xyz[i * D + k];
+            bad |= !__builtin_isfinite(c);
+            uint32_t o = f2ord(c);
+            lo[k] = min(lo[k], o);
+            hi[k] = max(hi[k], o);
+        }
+    }
+    for (int k = 0; k < D; ++k)
+        for (int off = 32; off > 0; off >>= 1) {
+            lo[k] = min(lo[k], (uint32_t)__shfl_xor((int)lo[k], off));
+            hi[k] = max(hi[k], (uint32_t)__shfl_xor((int)hi[k], off));
+        }
+    bad = __any(bad);
+    if (__lane_id() == 0) {
+        uint32_t* plo = reinterpret_cast<uint32_t*>(P->pmin);
+        uint32_t* phi = reinterpret_cast<uint32_t*>(P->pmax);
+        for (int k = 0; k < D; ++k) {
+            atomicMin(&plo[k], lo[k]);
+            atomicMax(&phi[k], hi[k]);
+        }
+        if (bad) atomicOr(&P->nonfinite, 1u);
+    }
+}
+
+// ---------------------------------------------------------- params (A2)
+// wg = (double)(pMax.x - pMin.x) etc. are float subtractions widened to double,
+// _wnd / unitX in double (SolutionEMST.cu:305-342).  The rescaled extremes are
+// the rescaled box corners: the map p -> (p - pMin) * unitXf is monotone.
+__global__ void k_params(Params* P, int64_t n, int dim)
+{
+    uint32_t* plo = reinterpret_cast<uint32_t*>(P->pmin);
+    uint32_t* phi = reinterpret_cast<uint32_t*>(P->pmax);
+    for (int k = 0; k < 3; ++k) {
+        if (k < dim) {
+            P->pmin[k] = ord2f(plo[k]);
+            P->pmax[k] = ord2f(phi[k]);
+        } else {
+            P->pmin[k] = 0.f;
+            P->pmax[k] = 0.f;
+        }
+    }
+    double wg = (double)(P->pmax[0] - P->pmin[0]);
+    double hg = (double)(P->pmax[1] - P->pmin[1]);
+    double dg = dim >= 3 ? (double)(P->pmax[2] - P->pmin[2]) : 0.0;
+    if (wg == 0) wg = 1;
+    if (hg == 0) hg = 1;
+    const double nn = (double)n;
+    double wnd = sqrt(nn * wg / hg) * 2;
+    if (dim >= 3 && dg != 0) wnd = pow((nn * wg * wg) / (hg * dg), 0.333);
+    const double unit = (ceil(wnd) + 1) / wg;
+    P->unit_x = unit;
+    const float fu = (float)unit;
+    P->unit_xf = fu;
+    float mx = 0.f;
+    for (int k = 0; k < 3; ++k) {
+        P->newmax[k] = k < dim ? (P->pmax[k] - P->pmin[k]) * fu : 0.f;
+        mx = fmaxf(mx, P->newmax[k]);
+    }
+    const int bits = dim == 3 ? 21 : 32;
+    const double maxq = (double)((1ull << bits) - 1);
+    P->qscale = mx > 0 ? (float)(maxq / (double)mx) : 0.f;
+}
+
+// ------------------------------------------ rescale + Morton keys (A2, A5)
+__device__ __forceinline__ uint64_t spread3(uint64_t x)
+{
+    x &= 0x1fffff;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+__device__ __forceinline__ uint64_t spread2(uint64_t x)
+{
+    x &= 0xffffffffull;
+    x = (x | (x << 16)) & 0x0000ffff0000ffffull;
+    x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
+    x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_rescale_morton(const float* __restrict__ xyz, int64_t n,
+                                                           const Params* __restrict__ P,
+                                                           float* __restrict__ resc,
+                                                           uint64_t* __restrict__ keys)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float fu = P->unit_xf;
+    const float qs = P->qscale;
+    const uint32_t maxq = D == 3 ? 0x1fffffu : 0xffffffffu;
+    uint64_t code = 0;
+    for (int k = 0; k < D; ++k) {
+        float d = xyz[i * D + k] - P->pmin[k];     // PointE::operator- (Node.h:574-580)
+        float v = d * fu;                           // PointE::operator*(float) (Node.h:588-593)
+        resc[i * D + k] = v;
+        float qf = v * qs;
+        uint32_t q = qf >= (float)maxq ? maxq : (uint32_t)qf;
+        code |= (D == 3 ? spread3(q) : spread2(q)) << k;
+    }
+    keys[i] = code;
+}
+
+// ------------------------------------------------------------------ scan
+// Exclusive scan of uint32 in three passes: per-block scan of 1024 elements,
+// scan of block totals (one block, chunked), add-back.
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const int lane = __lane_id();
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t t = (uint32_t)__shfl_up((int)v, off);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh, uint32_t& total)
+{
+    const int lane = __lane_id(), w = threadIdx.x / 64, nw = blockDim.x / 64;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) sh[w] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int i = 0; i < nw; ++i) { uint32_t t = sh[i]; sh[i] = run; run += t; }
+        sh[nw] = run;
+    }
+    __syncthreads();
+    uint32_t r = sh[w] + inc - v;
+    total = sh[nw];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_local(const uint32_t* __restrict__ in,
+                                                      uint32_t* __restrict__ out, int64_t n,
+                                                      uint32_t* __restrict__ sums)
+{
+    __shared__ uint32_t sh[16];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+    for (int j = 0; j < kScanItems; ++j) {
+        v[j] = (base + j < n) ? in[base + j] : 0u;
+        s += v[j];
+    }
+    uint32_t total;
+    uint32_t run = block_excl_scan(s, sh, total);
+    for (int j = 0; j < kScanItems; ++j) {
+        if (base + j < n) out[base + j] = run;
+        run += v[j];
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(uint32_t* sums, int64_t nb)
+{
+    __shared__ uint32_t sh[17];
+    uint32_t carry = 0;
+    for (int64_t base = 0; base < nb; base += 1024) {
+        const int64_t i = base + threadIdx.x;
+        uint32_t v = i < nb ? sums[i] : 0u;
+        uint32_t total;
+        uint32_t r = block_excl_scan(v, sh, total);
+        if (i < nb) sums[i] = r + carry;
+        carry += total;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* __restrict__ out, int64_t n,
+                                                    const uint32_t* __restrict__ sums)
+{
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    const uint32_t add = sums[blockIdx.x];
+    for (int j = threadIdx.x; j < kScanTile; j += kBlock)
+        if (base + j < n) out[base + j] += add;
+}
+
+// ------------------------------------------------------------ radix sort
+// 8-bit digits, tiles of 4096 keys per 256-thread block.  count: per-tile
+// digit histogram (LDS atomics); scan over (digit, tile); scatter: each wave
+// owns 1024 consecutive keys, ranks equal digits with 8 ballots, so the sort
+// is stable.
+constexpr int kRsTile = 4096;
+constexpr int kRsWaveKeys = kRsTile / 4;
+
+__global__ __launch_bounds__(kBlock) void k_rs_count(const uint64_t* __restrict__ keys, int64_t n,
+                                                    int shift, uint32_t* __restrict__ hist,
+                                                    int64_t nblk)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kRsTile;
+    for (int j = 0; j < kRsTile / kBlock; ++j) {
+        const int64_t i = base + j * kBlock + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint64_t* __restrict__ kin,
+                                                      const uint32_t* __restrict__ vin,
+                                                      uint64_t* __restrict__ kout,
+                                                      uint32_t* __restrict__ vout, int64_t n,
+                                                      int shift, const uint32_t* __restrict__ hist,
+                                                      int64_t nblk, int identity)
+{
+    __shared__ uint32_t cnt[4][256];
+    __shared__ uint32_t base[4][256];
+    const int w = threadIdx.x / 64;
+    const int lane = __lane_id();
+    for (int k = threadIdx.x; k < 4 * 256; k += kBlock) (&cnt[0][0])[k] = 0;
+    __syncthreads();
+    const int64_t wbase = (int64_t)blockIdx.x * kRsTile + (int64_t)w * kRsWaveKeys;
+    for (int c = 0; c < kRsWaveKeys / 64; ++c) {
+        const int64_t i = wbase + c * 64 + lane;
+        if (i < n) atomicAdd(&cnt[w][(kin[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    {
+        const int d = threadIdx.x;
+        uint32_t run = hist[(int64_t)d * nblk + blockIdx.x];
+        for (int ww = 0; ww < 4; ++ww) {
+            base[ww][d] = run;
+            run += cnt[ww][d];
+        }
+    }
+    __syncthreads();
+    const uint64_t lt = lanemask_lt();
+    for (int c = 0; c < kRsWaveKeys / 64; ++c) {
+        const int64_t i = wbase + c * 64 + lane;
+        const bool valid = i < n;
+        const uint64_t key = valid ? kin[i] : 0ull;
+        const uint32_t d = (uint32_t)(key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(valid && bit);
+            peers &= bit ? bb : ~bb;
+        }
+        uint32_t pos = 0;
+        if (valid) {
+            const uint32_t rank = (uint32_t)__popcll(peers & lt);
+            pos = base[w][d] + rank;
+            kout[pos] = key;
+            vout[pos] = identity ? (uint32_t)i : vin[i];
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (valid && (peers & lt) == 0ull) base[w][d] += (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// -------------------------------------------------- gather sorted points
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_gather_sorted(const float* __restrict__ resc,
+                                                         const uint32_t* __restrict__ perm,
+                                                         int64_t n, float4* __restrict__ pts,
+                                                         int32_t* __restrict__ inv)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t o = perm[s];
+    float4 p;
+    p.x = resc[(int64_t)o * D + 0];
+    p.y = resc[(int64_t)o * D + 1];
+    p.z = D == 3 ? resc[(int64_t)o * D + 2] : 0.f;
+    p.w = __uint_as_float(o);
+    pts[s] = p;
+    inv[o] = (int32_t)s;
+}
+
+// ---------------------------------------------------------------- tree
+__global__ __launch_bounds__(kBlock) void k_leaves(const float4* __restrict__ pts, int64_t n,
+                                                  int bucket, int64_t L, Node* __restrict__ nodes)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= L) return;
+    Node nd;
+    for (int k = 0; k < 3; ++k) { nd.lo[k] = INFINITY; nd.hi[k] = -INFINITY; }
+    nd.pad[0] = nd.pad[1] = 0.f;
+    const int64_t s0 = b * bucket;
+    const int64_t s1 = min(n, s0 + bucket);
+    for (int64_t s = s0; s < s1; ++s) {
+        const float4 p = pts[s];
+        nd.lo[0] = fminf(nd.lo[0], p.x); nd.hi[0] = fmaxf(nd.hi[0], p.x);
+        nd.lo[1] = fminf(nd.lo[1], p.y); nd.hi[1] = fmaxf(nd.hi[1], p.y);
+        nd.lo[2] = fminf(nd.lo[2], p.z); nd.hi[2] = fmaxf(nd.hi[2], p.z);
+    }
+    nodes[L + b] = nd;
+}
+
+__global__ __launch_bounds__(kBlock) void k_level(Node* __restrict__ nodes, int64_t lo_idx, int64_t cnt)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const int64_t i = lo_idx + t;
+    const Node a = nodes[2 * i], b = nodes[2 * i + 1];
+    Node nd;
+    for (int k = 0; k < 3; ++k) {
+        nd.lo[k] = fminf(a.lo[k], b.lo[k]);
+        nd.hi[k] = fmaxf(a.hi[k], b.hi[k]);
+    }
+    nd.pad[0] = nd.pad[1] = 0.f;
+    nodes[i] = nd;
+}
+
+// --------------------------------------------------------- cone search
+// One wave = 64 query points (consecutive in Morton order, or a query list).
+// The wave walks the binary hierarchy together (packet traversal): a node is
+// entered when any lane may still improve a cone from it; leaf points are
+// read with wave-uniform (scalar) loads and every lane tests them against its
+// own query.  Per lane, the best key (d2 bits << 32 | input id) of each cone
+// sits in LDS; the key order is (d2, id), which for a fixed query equals
+// EMST_isInf's (d2, min id, max id) order (macros_cuda.h:37).
+//
+// Cone membership follows NodeEMST.h:1204-1233 on the float difference
+// q - p: face (axis, sign) for every axis whose |d| equals the max (sign from
+// d >= 0, so -0 is '+'), and every slab of that face whose non-axis signs
+// agree, where a zero coordinate is in both halves.
+//
+// Node test: for each face, the smallest float d2 any point of the box can
+// have inside the face's pyramid (|d_axis| >= |d_other|), computed with the
+// same rounded operations on the box corners, so it never exceeds a point's
+// own d2; the face is kept while that bound is <= the face's worst cone best
+// (empty cone = +inf).  Equality is kept because a tie can still win on id.
+template <int D>
+struct ConeCfg {
+    static constexpr int NF = 2 * D;
+    static constexpr int NS = 1 << (D - 1);
+    static constexpr int K = NF * NS;
+};
+
+// allowed halves of one non-axis coordinate: bit0 = '+' half, bit1 = '-' half
+__device__ __forceinline__ uint32_t halves(float d)
+{
+    return d > 0.f ? 1u : (d < 0.f ? 2u : 3u);
+}
+
+template <int D>
+__device__ __forceinline__ uint32_t cone_mask(float dx, float dy, float dz)
+{
+    const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    const uint32_t hx = halves(dx), hy = halves(dy);
+    uint32_t mask = 0;
+    if (D == 2) {
+        const float m = fmaxf(ax, ay);
+        if (ax >= m) mask |= hy << (2 * (dx < 0.f ? 1 : 0));
+        if (ay >= m) mask |= hx << (2 * (dy < 0.f ? 3 : 2));
+    } else {
+        const uint32_t hz = halves(dz);
+        const float m = fmaxf(ax, fmaxf(ay, az));
+        // slab s = bit0 (first non-axis), bit1 (second); half code b -> bit value
+        // expand(h1, h2): slabs {s : (h1 >> (s&1)) & 1 && (h2 >> (s>>1)) & 1}
+        auto expand = [](uint32_t h1, uint32_t h2) -> uint32_t {
+            return ((h2 & 1u) ? h1 : 0u) | ((h2 & 2u) ? (h1 << 2) : 0u);
+        };
+        if (ax >= m) mask |= expand(hy, hz) << (4 * (dx < 0.f ? 1 : 0));
+        if (ay >= m) mask |= expand(hx, hz) << (4 * (dy < 0.f ? 3 : 2));
+        if (az >= m) mask |= expand(hx, hy) << (4 * (dz < 0.f ? 5 : 4));
+    }
+    return mask;
+}
+
+// smallest |d| over [lo, hi]
+__device__ __forceinline__ float mabs(float lo, float hi)
+{
+    return lo > 0.f ? lo : (hi < 0.f ? -hi : 0.f);
+}
+
+template <int D>
+__device__ __forceinline__ bool node_needed(const Node& nd, float px, float py, float pz,
+                                            const uint32_t* bound)
+{
+    const float lx = nd.lo[0] - px, hx = nd.hi[0] - px;
+    const float ly = nd.lo[1] - py, hy = nd.hi[1] - py;
+    const float mx = mabs(lx, hx), my = mabs(ly, hy);
+    bool need = false;
+    if (D == 2) {
+        // face +x: d_x >= |d_y|
+        {
+            const float a = fmaxf(lx, my);
+            need |= (hx >= my) && (__float_as_uint(d2_of(a, my)) <= bound[0]);
+        }
+        {
+            const float a = fmaxf(-hx, my);
+            need |= (-lx >= my) && (__float_as_uint(d2_of(a, my)) <= bound[1]);
+        }
+        {
+            const float a = fmaxf(ly, mx);
+            need |= (hy >= mx) && (__float_as_uint(d2_of(mx, a)) <= bound[2]);
+        }
+        {
+            const float a = fmaxf(-hy, mx);
+            need |= (-ly >= mx) && (__float_as_uint(d2_of(mx, a)) <= bound[3]);
+        }
+    } else {
+        const float lz = nd.lo[2] - pz, hz = nd.hi[2] - pz;
+        const float mz = mabs(lz, hz);
+        {
+            const float o = fmaxf(my, mz);
+            const float ap = fmaxf(lx, o), an = fmaxf(-hx, o);
+            need |= (hx >= o) && (__float_as_uint(d2_of(ap, my, mz)) <= bound[0]);
+            need |= (-lx >= o) && (__float_as_uint(d2_of(an, my, mz)) <= bound[1]);
+        }
+        {
+            const float o = fmaxf(mx, mz);
+            const float ap = fmaxf(ly, o), an = fmaxf(-hy, o);
+            need |= (hy >= o) && (__float_as_uint(d2_of(mx, ap, mz)) <= bound[2]);
+            need |= (-ly >= o) && (__float_as_uint(d2_of(mx, an, mz)) <= bound[3]);
+        }
+        {
+            const float o = fmaxf(mx, my);
+            const float ap = fmaxf(lz, o), an = fmaxf(-hz, o);
+            need |= (hz >= o) && (__float_as_uint(d2_of(mx, my, ap)) <= bound[4]);
+            need |= (-lz >= o) && (__float_as_uint(d2_of(mx, my, an)) <= bound[5]);
+        }
+    }
+    return need;
+}
+
+template <int D>
+__global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ pts,
+                                                   const Node* __restrict__ nodes, int64_t L,
+                                                   int bucket, int64_t n,
+                                                   const int32_t* __restrict__ qlist, int64_t nq,
+                                                   const int32_t* __restrict__ label,
+                                                   int check_labels,
+                                                   const int32_t* __restrict__ inv,
+                                                   uint64_t* __restrict__ cand_key,
+                                                   int32_t* __restrict__ cand_pos)
+{
+    using C = ConeCfg<D>;
+    __shared__ uint64_t best[C::K][64];
+    __shared__ int32_t stk[64];
+    const int lane = __lane_id();
+    const int64_t qi = (int64_t)blockIdx.x * 64 + lane;
+    const bool active = qi < nq;
+    const int64_t p = active ? (qlist ? (int64_t)qlist[qi] : qi) : 0;
+    const float4 me = pts[p];
+    const float px = me.x, py = me.y, pz = me.z;
+    const int32_t mylab = (check_labels && active) ? label[p] : 0;
+    for (int c = 0; c < C::K; ++c) best[c][lane] = kEmptyKey;
+    uint32_t bound[C::NF];
+    for (int f = 0; f < C::NF; ++f) bound[f] = active ? 0xffffffffu : 0u;
+
+    // packet centre (for near-first child order)
+    float cx = active ? px : 0.f, cy = active ? py : 0.f, cz = active ? pz : 0.f;
+    float na = active ? 1.f : 0.f;
+    for (int off = 32; off > 0; off >>= 1) {
+        cx += __shfl_xor(cx, off);
+        cy += __shfl_xor(cy, off);
+        cz += __shfl_xor(cz, off);
+        na += __shfl_xor(na, off);
+    }
+    cx /= na; cy /= na; cz /= na;
+
+    int sp = 0;
+    if (lane == 0) stk[0] = 1;
+    sp = 1;
+    __builtin_amdgcn_wave_barrier();
+    while (sp > 0) {
+        --sp;
+        const int node = __builtin_amdgcn_readfirstlane(stk[sp]);
+        const Node nd = nodes[node];
+        const bool need = active && node_needed<D>(nd, px, py, pz, bound);
+        if (__ballot(need) == 0ull) continue;
+        if (node >= L) {
+            const int64_t s0 = (int64_t)(node - L) * bucket;
+            const int64_t s1 = min(n, s0 + bucket);
+            for (int64_t s = s0; s < s1; ++s) {
+                const float4 q = pts[s];
+                const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
+                const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
+                bool take = need && s != p;
+                if (take && check_labels && d2 == 0.f) take = label[s] != mylab;
+                if (take) {
+                    uint32_t mask = cone_mask<D>(dx, dy, dz);
+                    const uint64_t key = ((uint64_t)__float_as_uint(d2) << 32) | __float_as_uint(q.w);
+                    while (mask) {
+                        const int c = __builtin_ctz(mask);
+                        mask &= mask - 1;
+                        if (key < best[c][lane]) best[c][lane] = key;
+                    }
+                }
+            }
+            if (need) {
+                for (int f = 0; f < C::NF; ++f) {
+                    uint32_t b = 0;
+                    for (int sl = 0; sl < C::NS; ++sl) {
+                        const uint64_t k = best[f * C::NS + sl][lane];
+                        const uint32_t hiw = (uint32_t)(k >> 32);
+                        b = max(b, hiw);
+                    }
+                    bound[f] = b;
+                }
+            }
+        } else {
+            const int a = 2 * node, b = 2 * node + 1;
+            const Node na_ = nodes[a], nb_ = nodes[b];
+            auto bd = [&](const Node& x) {
+                float s = 0.f;
+                const float dxx = fmaxf(fmaxf(x.lo[0] - cx, cx - x.hi[0]), 0.f);
+                const float dyy = fmaxf(fmaxf(x.lo[1] - cy, cy - x.hi[1]), 0.f);
+                const float dzz = fmaxf(fmaxf(x.lo[2] - cz, cz - x.hi[2]), 0.f);
+                s = dxx * dxx + dyy * dyy + dzz * dzz;
+                return s;
+            };
+            const bool afirst = bd(na_) <= bd(nb_);
+            if (lane == 0) {
+                stk[sp] = afirst ? b : a;
+                stk[sp + 1] = afirst ? a : b;
+            }
+            sp += 2;
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    if (!active) return;
+    uint64_t* ok = cand_key + p * C::K;
+    int32_t* op = cand_pos + p * C::K;
+    for (int c = 0; c < C::K; ++c) {
+        const uint64_t k = best[c][lane];
+        ok[c] = k;
+        op[c] = (k == kEmptyKey) ? -1 : inv[(uint32_t)k];
+    }
+}
+
+// ------------------------------------------------------- Boruvka rounds
+__global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __restrict__ label,
+                                                      int32_t* __restrict__ corr,
+                                                      uint64_t* __restrict__ corr_key,
+                                                      uint64_t* __restrict__ comp1,
+                                                      uint32_t* __restrict__ comp2, Counters* C)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) { C->edges = 0; C->research = 0; C->zero_or_empty = 0; C->unflat = 0; }
+    if (i >= n) return;
+    label[i] = (int32_t)i;          // K_EMST_initDisjointSet (EMSTOperators.h:330-343)
+    corr[i] = -1;                   // correspondenceMap reset (SolutionEMST.cu:464-466)
+    corr_key[i] = kEmptyKey;
+    comp1[i] = kEmptyKey;
+    comp2[i] = 0xffffffffu;
+}
+
+// Points that must search again next round: a zero-distance cone or no cone
+// at all (NodeEMST.h:1051-1077).
+__global__ __launch_bounds__(kBlock) void k_flag_research(int K, int64_t n,
+                                                         const uint64_t* __restrict__ cand_key,
+                                                         int32_t* __restrict__ list, Counters* C)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    bool zero = false, any = false;
+    for (int k = 0; k < K; ++k) {
+        const uint64_t key = cand_key[p * K + k];
+        if (key == kEmptyKey) continue;
+        any = true;
+        zero |= (key >> 32) == 0u;
+    }
+    if (zero || !any) {
+        const uint32_t slot = atomicAdd(&C->research, 1u);
+        list[slot] = (int32_t)p;
+        atomicAdd(&C->zero_or_empty, 1u);
+    }
+}
+
+// Wave pre-reduction: lanes sharing the wave leader's label merge into one
+// atomic; the rest issue their own.
+__device__ __forceinline__ void atomic_min_grouped(uint64_t* arr, int32_t lab, uint64_t key, bool valid)
+{
+    const uint64_t vm = __ballot(valid);
+    if (vm == 0ull) return;
+    const int leader = __builtin_ctzll(vm);
+    const int32_t llab = __shfl(lab, leader);
+    const bool grp = valid && lab == llab;
+    const uint64_t gm = __ballot(grp);
+    uint64_t v = grp ? key : kEmptyKey;
+    if (__popcll(gm) > 1) {
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint64_t o = (uint64_t)__shfl_xor((long long)v, off);
+            v = o < v ? o : v;
+        }
+        if (__lane_id() == leader) atomicMin((unsigned long long*)&arr[llab], (unsigned long long)v);
+        if (valid && !grp) atomicMin((unsigned long long*)&arr[lab], (unsigned long long)key);
+    } else if (valid) {
+        atomicMin((unsigned long long*)&arr[lab], (unsigned long long)key);
+    }
+}
+
+// find min 1 (NodeEMST.h:1303-1384) + first phase of the component minimum:
+// per point, keep the previous target while it is outside the component,
+// else the (d2, id)-smallest cone candidate with another label; then
+// atomicMin of (d2 bits, min id) on the component root.
+__global__ __launch_bounds__(kBlock) void k_round_min(int K, int64_t n, const float4* __restrict__ pts,
+                                                     const uint64_t* __restrict__ cand_key,
+                                                     const int32_t* __restrict__ cand_pos,
+                                                     const int32_t* __restrict__ label,
+                                                     int32_t* __restrict__ label2,
+                                                     int32_t* __restrict__ corr,
+                                                     uint64_t* __restrict__ corr_key,
+                                                     uint64_t* __restrict__ comp1,
+                                                     int32_t* __restrict__ fixed)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool inb = p < n;
+    int32_t lp = 0;
+    uint64_t key1 = kEmptyKey;
+    bool valid = false;
+    if (inb) {
+        lp = label[p];
+        label2[p] = lp;
+        fixed[p] = 0;
+        int32_t c = corr[p];
+        uint64_t ck = corr_key[p];
+        if (!(c != -1 && label[c] != lp)) {
+            c = -1;
+            ck = kEmptyKey;
+            for (int k = 0; k < K; ++k) {
+                const uint64_t key = cand_key[p * K + k];
+                if (key >= ck) continue;
+                const int32_t q = cand_pos[p * K + k];
+                if (label[q] == lp) continue;
+                ck = key;
+                c = q;
+            }
+            corr[p] = c;
+            corr_key[p] = ck;
+        }
+        if (c != -1) {
+            const uint32_t pid = __float_as_uint(pts[p].w);
+            const uint32_t qid = (uint32_t)ck;
+            key1 = (ck & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
+            valid = true;
+        }
+    }
+    atomic_min_grouped(comp1, lp, key1, valid);
+}
+
+// second phase: among members matching (d2, min id), the smallest max id.
+__global__ __launch_bounds__(kBlock) void k_round_comp2(int64_t n, const float4* __restrict__ pts,
+                                                       const int32_t* __restrict__ label,
+                                                       const int32_t* __restrict__ corr,
+                                                       const uint64_t* __restrict__ corr_key,
+                                                       const uint64_t* __restrict__ comp1,
+                                                       uint32_t* __restrict__ comp2)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || corr[p] == -1) return;
+    const int32_t lp = label[p];
+    const uint64_t ck = corr_key[p];
+    const uint32_t pid = __float_as_uint(pts[p].w), qid = (uint32_t)ck;
+    const uint64_t key1 = (ck & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
+    if (key1 == comp1[lp]) atomicMin(&comp2[lp], max(pid, qid));
+}
+
+// winner flag (fixedMap, NodeEMST.h:947-949)
+__global__ __launch_bounds__(kBlock) void k_round_winner(int64_t n, const float4* __restrict__ pts,
+                                                        const int32_t* __restrict__ label,
+                                                        const int32_t* __restrict__ corr,
+                                                        const uint64_t* __restrict__ corr_key,
+                                                        const uint64_t* __restrict__ comp1,
+                                                        const uint32_t* __restrict__ comp2,
+                                                        int32_t* __restrict__ fixed)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || corr[p] == -1) return;
+    const int32_t lp = label[p];
+    const uint64_t ck = corr_key[p];
+    const uint32_t pid = __float_as_uint(pts[p].w), qid = (uint32_t)ck;
+    const uint64_t key1 = (ck & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
+    if (key1 == comp1[lp] && max(pid, qid) == comp2[lp]) fixed[p] = 1;
+}
+
+// hook/union (EMSTOperators.h:1267-1303): roots read from the flattened
+// labels, written into label2.  Non-mutual: root(p) -> root(q).  Mutual pair:
+// the endpoint with the smaller id keeps its root.
+__global__ __launch_bounds__(kBlock) void k_round_hook(int64_t n, const float4* __restrict__ pts,
+                                                      const int32_t* __restrict__ label,
+                                                      int32_t* __restrict__ label2,
+                                                      const int32_t* __restrict__ corr,
+                                                      const uint64_t* __restrict__ corr_key,
+                                                      const int32_t* __restrict__ fixed,
+                                                      Counters* C, int32_t* __restrict__ eu,
+                                                      int32_t* __restrict__ ev,
+                                                      float* __restrict__ ed2,
+                                                      int32_t* __restrict__ eround, int round)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || !fixed[p]) return;
+    const int32_t q = corr[p];
+    const uint64_t ck = corr_key[p];
+    const uint32_t pid = __float_as_uint(pts[p].w), qid = (uint32_t)ck;
+    const int32_t rp = label[p], rq = label[q];
+    bool add = false;
+    if (!fixed[q] || corr[q] != (int32_t)p) {
+        label2[rp] = rq;
+        add = true;
+    } else if (pid < qid) {
+        label2[rq] = rp;
+        add = true;
+    }
+    if (add) {
+        const uint32_t slot = atomicAdd(&C->edges, 1u);
+        if ((int64_t)slot >= n) return;    // a forest never holds n edges; guards the arrays
+        eu[slot] = (int32_t)min(pid, qid);
+        ev[slot] = (int32_t)max(pid, qid);
+        ed2[slot] = __uint_as_float((uint32_t)(ck >> 32));
+        eround[slot] = round;
+    }
+}
+
+// flatten (EMSTOperators.h:412-433): chase label2 to the root, with path
+// halving on label2 (every write stores an ancestor, so concurrent chases stay
+// valid); reset the component minima for the next round.
+// Each launch takes at most kFlattenSteps steps per point; a point still off
+// its root raises C->unflat and the host launches the kernel again, so every
+// wave ends even if the hook forest were malformed.
+constexpr int kFlattenSteps = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_round_flatten(int64_t n, int32_t* __restrict__ label,
+                                                         int32_t* label2,
+                                                         uint64_t* __restrict__ comp1,
+                                                         uint32_t* __restrict__ comp2, Counters* C)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int32_t r = label2[p];
+    int32_t nx = label2[r];
+    int steps = 0;
+    while (nx != r && steps < kFlattenSteps) {
+        const int32_t nn = label2[nx];
+        label2[r] = nn;
+        r = nn;
+        nx = label2[r];
+        ++steps;
+    }
+    if (nx != r) atomicOr(&C->unflat, 1u);
+    label[p] = r;
+    comp1[p] = kEmptyKey;
+    comp2[p] = 0xffffffffu;
+}
+
+// ------------------------------------------------------------- weight
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_weight(const float* __restrict__ xyz,
+                                                  const int32_t* __restrict__ eu,
+                                                  const int32_t* __restrict__ ev, int64_t m,
+                                                  double* __restrict__ partial)
+{
+    __shared__ double sh[kBlock / 64];
+    double s = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        const float* a = xyz + (int64_t)eu[i] * D;
+        const float* b = xyz + (int64_t)ev[i] * D;
+        const float dx = a[0] - b[0], dy = a[1] - b[1];
+        const float d2 = D == 3 ? d2_of(dx, dy, a[2] - b[2]) : d2_of(dx, dy);
+        s += sqrt((double)d2);
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (__lane_id() == 0) sh[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < kBlock / 64; ++w) t += sh[w];
+        partial[blockIdx.x] = t;
+    }
+}
+
+// ------------------------------------------------------------ outputs
+__global__ __launch_bounds__(kBlock) void k_labels_out(int64_t n, const int32_t* __restrict__ label,
+                                                      const float4* __restrict__ pts,
+                                                      const int32_t* __restrict__ inv,
+                                                      int32_t* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = (int32_t)__float_as_uint(pts[label[inv[i]]].w);
+}
+
+__global__ __launch_bounds__(kBlock) void k_cands_out(int K, int64_t n, const uint64_t* __restrict__ ck,
+                                                     const int32_t* __restrict__ inv,
+                                                     int32_t* __restrict__ cand, float* __restrict__ cd2)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t s = inv[i];
+    for (int k = 0; k < K; ++k) {
+        const uint64_t key = ck[s * K + k];
+        cand[i * K + k] = key == kEmptyKey ? -1 : (int32_t)(uint32_t)key;
+        cd2[i * K + k] = key == kEmptyKey ? INFINITY : __uint_as_float((uint32_t)(key >> 32));
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_edge_keys(const int32_t* __restrict__ eu,
+                                                     const int32_t* __restrict__ ev, int64_t m,
+                                                     uint64_t* __restrict__ keys)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    keys[i] = ((uint64_t)(uint32_t)eu[i] << 32) | (uint32_t)ev[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_edges_out(const uint64_t* __restrict__ keys,
+                                                     const uint32_t* __restrict__ idx,
+                                                     const float* __restrict__ ed2, int64_t m,
+                                                     int32_t* __restrict__ u, int32_t* __restrict__ v,
+                                                     float* __restrict__ d2)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint64_t k = keys[i];
+    u[i] = (int32_t)(k >> 32);
+    v[i] = (int32_t)(uint32_t)k;
+    d2[i] = ed2[idx[i]];
+}
+
+// ------------------------------------------------ reference cell grid
+// findCell (ViewGrid.h:339-359): p1 = FEuclid(FDual(0)), size_slab = 2R;
+// li = (float)(p - p1) + R in double; cell = floor(li / 2R).
+__global__ __launch_bounds__(kBlock) void k_cell_grid(const float* __restrict__ resc, int64_t n, int dim,
+                                                     const int32_t* __restrict__ geo,
+                                                     int32_t* __restrict__ cell_of,
+                                                     uint64_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ cell_count)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int R = geo[9];
+    bool ok = true;
+    int32_t c[3] = {0, 0, 0};
+    for (int k = 0; k < dim; ++k) {
+        const int32_t PC = geo[3 + k], pc = geo[6 + k];
+        const float p1 = (float)((PC % 2) * R - PC * R + pc);
+        const float p2 = (float)((2 + PC % 2) * R - PC * R + pc);
+        const double size_slab = (double)(p2 - p1);
+        const float dif = resc[i * dim + k] - p1;
+        const double li = (double)dif + size_slab / 2;
+        c[k] = (int32_t)floor(li / size_slab);
+        ok = ok && c[k] >= 0 && c[k] < geo[k];
+    }
+    for (int k = 0; k < 3; ++k) cell_of[i * 3 + k] = ok ? c[k] : -1;
+    const int64_t ncell = (int64_t)geo[0] * geo[1] * geo[2];
+    const int64_t lin = ok ? (int64_t)c[0] + (int64_t)geo[0] * ((int64_t)c[1] + (int64_t)geo[1] * c[2]) : ncell;
+    keys[i] = (uint64_t)lin;
+    if (ok) atomicAdd(&cell_count[lin], 1u);
+}
+
+inline unsigned nblocks(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+// ================================================================ launchers
+void launch_bbox(const float* xyz, int64_t n, int dim, Params* P, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bbox_init, dim3(1), dim3(1), 0, s, P);
+    if (n <= 0) return;
+    const unsigned g = (unsigned)std::min<int64_t>(nblocks(n), 2048);
+    if (dim == 3) hipLaunchKernelGGL(k_bbox<3>, dim3(g), dim3(kBlock), 0, s, xyz, n, P);
+    else hipLaunchKernelGGL(k_bbox<2>, dim3(g), dim3(kBlock), 0, s, xyz, n, P);
+}
+
+void launch_params(Params* P, int64_t n, int dim, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_params, dim3(1), dim3(1), 0, s, P, n, dim);
+}
+
+void launch_rescale_morton(const float* xyz, int64_t n, int dim, const Params* P, float* resc,
+                           uint64_t* keys, hipStream_t s)
+{
+    if (n <= 0) return;
+    if (dim == 3) hipLaunchKernelGGL(k_rescale_morton<3>, dim3(nblocks(n)), dim3(kBlock), 0, s, xyz, n, P, resc, keys);
+    else hipLaunchKernelGGL(k_rescale_morton<2>, dim3(nblocks(n)), dim3(kBlock), 0, s, xyz, n, P, resc, keys);
+}
+
+size_t scan_tmp_words(int64_t n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 1; }
+
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp, hipStream_t s)
+{
+    if (n <= 0) return;
+    const int64_t nb = (n + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(kBlock), 0, s, in, out, n, tmp);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, tmp, nb);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(kBlock), 0, s, out, n, tmp);
+}
+
+size_t radix_hist_words(int64_t n)
+{
+    const int64_t nblk = (n + kRsTile - 1) / kRsTile;
+    return (size_t)(256 * std::max<int64_t>(nblk, 1));
+}
+
+void radix_sort(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, int64_t n, int bits,
+                bool vals_identity, uint32_t* hist, uint32_t* scan_tmp, hipStream_t s)
+{
+    if (n <= 0) return;
+    const int64_t nblk = (n + kRsTile - 1) / kRsTile;
+    uint64_t* ka = k0; uint32_t* va = v0;
+    uint64_t* kb = k1; uint32_t* vb = v1;
+    bool identity = vals_identity;
+    int passes = 0;
+    for (int shift = 0; shift < bits; shift += 8, ++passes) {
+        hipLaunchKernelGGL(k_rs_count, dim3((unsigned)nblk), dim3(kBlock), 0, s, ka, n, shift, hist, nblk);
+        exclusive_scan_u32(hist, hist, 256 * nblk, scan_tmp, s);
+        hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nblk), dim3(kBlock), 0, s, ka, va, kb, vb, n,
+                           shift, hist, nblk, identity ? 1 : 0);
+        identity = false;
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    if (ka != k0) {
+        hipMemcpyAsync(k0, ka, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, s);
+        hipMemcpyAsync(v0, va, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s);
+    } else if (passes == 0 && vals_identity) {
+        // no pass ran: values are the identity permutation
+        hipMemcpyAsync(v0, v0, 0, hipMemcpyDeviceToDevice, s);
+    }
+}
+
+void launch_gather_sorted(const float* resc, const uint32_t* perm, int64_t n, int dim, float4* pts,
+                          int32_t* inv, hipStream_t s)
+{
+    if (n <= 0) return;
+    if (dim == 3) hipLaunchKernelGGL(k_gather_sorted<3>, dim3(nblocks(n)), dim3(kBlock), 0, s, resc, perm, n, pts, inv);
+    else hipLaunchKernelGGL(k_gather_sorted<2>, dim3(nblocks(n)), dim3(kBlock), 0, s, resc, perm, n, pts, inv);
+}
+
+void launch_build_tree(const float4* pts, int64_t n, int bucket, int64_t L, Node* nodes, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_leaves, dim3(nblocks(L)), dim3(kBlock), 0, s, pts, n, bucket, L, nodes);
+    for (int64_t lo = L / 2; lo >= 1; lo /= 2)
+        hipLaunchKernelGGL(k_level, dim3(nblocks(lo)), dim3(kBlock), 0, s, nodes, lo, lo);
+}
+
+void launch_cone_search(int dim, const float4* pts, const Node* nodes, int64_t L, int bucket,
+                        int64_t n, const int32_t* qlist, int64_t nq, const int32_t* label,
+                        bool check_labels, const int32_t* inv, uint64_t* cand_key,
+                        int32_t* cand_pos, hipStream_t s)
+{
+    if (nq <= 0) return;
+    const unsigned g = nblocks(nq, 64);
+    if (dim == 3)
+        hipLaunchKernelGGL(k_cone_search<3>, dim3(g), dim3(64), 0, s, pts, nodes, L, bucket, n, qlist, nq,
+                           label, check_labels ? 1 : 0, inv, cand_key, cand_pos);
+    else
+        hipLaunchKernelGGL(k_cone_search<2>, dim3(g), dim3(64), 0, s, pts, nodes, L, bucket, n, qlist, nq,
+                           label, check_labels ? 1 : 0, inv, cand_key, cand_pos);
+}
+
+void launch_round_init(int64_t n, int32_t* label, int32_t* corr, uint64_t* corr_key,
+                       uint64_t* comp1, uint32_t* comp2, Counters* C, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_round_init, dim3(nblocks(std::max<int64_t>(n, 1))), dim3(kBlock), 0, s, n, label,
+                       corr, corr_key, comp1, comp2, C);
+}
+
+void launch_flag_research(int K, int64_t n, const uint64_t* cand_key, int32_t* list, Counters* C,
+                          hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_flag_research, dim3(nblocks(n)), dim3(kBlock), 0, s, K, n, cand_key, list, C);
+}
+
+void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const uint64_t* cand_key,
+                        const int32_t* cand_pos, int32_t* label, int32_t* label2, int32_t* corr,
+                        uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, int32_t* fixed,
+                        Counters* C, int32_t* eu, int32_t* ev, float* ed2, int32_t* eround,
+                        int round, hipStream_t s)
+{
+    if (n <= 0) return;
+    const unsigned g = nblocks(n);
+    switch (phase) {
+    case 0:
+        hipLaunchKernelGGL(k_round_min, dim3(g), dim3(kBlock), 0, s, K, n, pts, cand_key, cand_pos, label,
+                           label2, corr, corr_key, comp1, fixed);
+        break;
+    case 1:
+        hipLaunchKernelGGL(k_round_comp2, dim3(g), dim3(kBlock), 0, s, n, pts, label, corr, corr_key, comp1,
+                           comp2);
+        hipLaunchKernelGGL(k_round_winner, dim3(g), dim3(kBlock), 0, s, n, pts, label, corr, corr_key,
+                           comp1, comp2, fixed);
+        break;
+    case 2:
+        hipLaunchKernelGGL(k_round_hook, dim3(g), dim3(kBlock), 0, s, n, pts, label, label2, corr,
+                           corr_key, fixed, C, eu, ev, ed2, eround, round);
+        break;
+    case 3:
+        hipLaunchKernelGGL(k_round_flatten, dim3(g), dim3(kBlock), 0, s, n, label, label2, comp1, comp2, C);
+        break;
+    }
+}
+
+void launch_weight(const float* xyz, int dim, const int32_t* eu, const int32_t* ev, int64_t m,
+                   double* partial, int nblk, hipStream_t s)
+{
+    if (dim == 3) hipLaunchKernelGGL(k_weight<3>, dim3(nblk), dim3(kBlock), 0, s, xyz, eu, ev, m, partial);
+    else hipLaunchKernelGGL(k_weight<2>, dim3(nblk), dim3(kBlock), 0, s, xyz, eu, ev, m, partial);
+}
+
+void launch_labels_out(int64_t n, const int32_t* label, const float4* pts, const int32_t* inv,
+                       int32_t* out, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_labels_out, dim3(nblocks(n)), dim3(kBlock), 0, s, n, label, pts, inv, out);
+}
+
+void launch_cands_out(int K, int64_t n, const uint64_t* cand_key, const int32_t* inv, int32_t* cand,
+                      float* cd2, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cands_out, dim3(nblocks(n)), dim3(kBlock), 0, s, K, n, cand_key, inv, cand, cd2);
+}
+
+void launch_edge_keys(const int32_t* eu, const int32_t* ev, int64_t m, uint64_t* keys, hipStream_t s)
+{
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_edge_keys, dim3(nblocks(m)), dim3(kBlock), 0, s, eu, ev, m, keys);
+}
+
+void launch_edges_out(const uint64_t* keys, const uint32_t* idx, const float* ed2, int64_t m, int32_t* u,
+                      int32_t* v, float* d2, hipStream_t s)
+{
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_edges_out, dim3(nblocks(m)), dim3(kBlock), 0, s, keys, idx, ed2, m, u, v, d2);
+}
+
+void launch_cell_grid(const float* resc, int64_t n, int dim, const int32_t* geo, int32_t* cell_of,
+                      uint64_t* keys, uint32_t* cell_count, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cell_grid, dim3(nblocks(n)), dim3(kBlock), 0, s, resc, n, dim, geo, cell_of, keys,
+                       cell_count);
+}
+
+}  // namespace emst

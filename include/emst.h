@@ -1,0 +1,173 @@
+/*
+ * emst.h -- C ABI of the MI355X (gfx950) EMST engine.
+ *
+ * Drop-in boundary for the reference's EMST entry surface: a point array goes
+ * in; the spanning forest after any Boruvka round, or the finished tree, and
+ * its total weight come out.  Every entry point names the reference interface
+ * it replaces (paths relative to the reference repository root).
+ *
+ * Conventions
+ *   - Plain C types only; every call returns an emst_status (0 = EMST_OK).
+ *   - A context owns its device buffers and one HIP stream; contexts are
+ *     independent and may be used from different host threads (one thread
+ *     per context at a time).  No process globals, nothing ever exits.
+ *   - Point ids are input indices 0..n-1.  The reference stores byte offsets
+ *     (i * sizeof(node)), an order-preserving map of the same ids
+ *     (basic_components/include/GridOfNodes.h:454-465).
+ *   - Edge order is canonical: u < v, sorted by (u, v).
+ *   - Ties between squared distances break on (d2, min id, max id)
+ *     (basic_components/include/macros_cuda.h:37).
+ */
+#ifndef EMST_AMD_H
+#define EMST_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    EMST_OK = 0,
+    EMST_E_ARG = 1,         /* bad argument (NULL, dim, n < 0, size too small) */
+    EMST_E_HIP = 2,         /* HIP runtime or launch error */
+    EMST_E_OOM = 3,         /* device allocation failed */
+    EMST_E_NONFINITE = 4,   /* NaN or Inf coordinate */
+    EMST_E_STATE = 5,       /* call out of order (e.g. step before set_points) */
+    EMST_E_NODEVICE = 6,    /* no HIP device */
+    EMST_E_LIMIT = 7,       /* input beyond a supported limit (n >= 2^31) */
+    EMST_E_IO = 8           /* file read/write failure */
+} emst_status;
+
+/* Options.  Replaces the INI keys the EMST path reads through
+ * ConfigParamsCF (coalition_framework/src/config/ConfigParamsCF.cpp:131-144,
+ * basic_components/include/ConfigParams.h:313-321) and the compile-time
+ * switches (emst_coalition/src/SolutionEMSTOperators.cu:17-18,
+ * emst_coalition/include/SolutionEMST.h:55-56). */
+typedef struct {
+    int32_t dim;            /* point dimension: 2 or 3 */
+    int32_t level_radius;   /* R of the reference cell grid (levelRadius); cells are 2R wide */
+    int32_t max_rounds;     /* Boruvka round cap; the reference stops at 100 */
+    int32_t bucket_size;    /* points per leaf cell of the search hierarchy (16..64, 0 = auto) */
+    int32_t trace;          /* 1: record per-round forest size, length and phase times */
+    int32_t device;         /* HIP device ordinal */
+    int32_t radius_mode;    /* 0 plain; 1 learned seed radius (needs emst_set_radius_model) */
+    int32_t reserved[5];
+} emst_options;
+
+/* One row of the per-round trace: the columns of the reference's stats file
+ * (emst_coalition/src/SolutionEMSTRW.cu:100-165, TspResultInfo in
+ * emst_coalition/include/EMSTOperators.h:54-70).  Times in milliseconds. */
+typedef struct {
+    int32_t iteration;
+    int32_t size;               /* forest edges after this round */
+    double length;              /* forest weight after this round (FP64, input coords) */
+    float time_flatten;
+    float time_termination;
+    float time_find_next_closest;   /* cone search + per-point candidate min */
+    float time_find_min_pair;       /* per-component minimum */
+    float time_connect_union;       /* hooking */
+    int64_t searched;               /* points whose cone search ran this round */
+} emst_round_trace;
+
+typedef struct emst_ctx emst_ctx;
+
+/* Fills defaults: dim 3, level_radius 1, max_rounds 100, bucket_size 0 (auto),
+ * trace 0, device 0, radius_mode 0. */
+void emst_default_options(emst_options* opt);
+
+/* Replaces CalculateurEMST::initialize's static set-up and the device query
+ * of SolutionEMST::initialize() (emst_coalition/src/CalculateurEMST.cu:113-230,
+ * emst_coalition/src/SolutionEMST.cu:17-270).  Returns NULL on failure; the
+ * reason is then given by emst_last_error(NULL). */
+emst_ctx* emst_create(const emst_options* opt);
+int emst_destroy(emst_ctx* ctx);
+
+/* Text of the last error on this context (or of the last failed
+ * emst_create when ctx is NULL).  Never NULL. */
+const char* emst_last_error(const emst_ctx* ctx);
+
+/* Replaces SolutionEMST<DimP,DimCM>::initialize(NetLink&, PointEuclidean& pMin,
+ * PointEuclidean& pMax) (emst_coalition/src/SolutionEMST.cu:273-509): takes an
+ * in-memory point array (AoS, n x dim float32, like the reference's
+ * adaptiveMap of PointE<D>), computes the bounding box (InputRW.h:541-559),
+ * rescales (SolutionEMST.cu:305-379) and builds the search hierarchy.
+ * xyz_on_device = 1: xyz is a device pointer on the context's device and the
+ * context only reads it (it must stay valid until the next set_points or
+ * destroy).  Resets the forest to n singletons. */
+int emst_set_points(emst_ctx* ctx, const float* xyz, int64_t n, int32_t dim,
+                    int32_t xyz_on_device);
+
+/* Replaces SolutionEMST::run() (emst_coalition/src/SolutionEMSTOperators.cu:146-153):
+ * Boruvka rounds until one tree remains, no component has an outgoing
+ * candidate (a forest), or max_rounds is reached. */
+int emst_run(emst_ctx* ctx);
+
+/* Replaces SolutionEMST::activate() / CalculateurEMST::activate()
+ * (emst_coalition/src/SolutionEMSTOperators.cu:156-422,
+ * emst_coalition/src/CalculateurEMST.cu:330-429): one Boruvka round.
+ * *finished = 1 when the termination test holds before the round
+ * (the round is then not run). */
+int emst_step(emst_ctx* ctx, int32_t* finished);
+
+/* Replaces getLinks()/getLinks3() (emst_coalition/include/CalculateurEMST.h:52-58,
+ * CalculateurEMST.cu:97-111), which return the per-node adjacency: here the
+ * forest's edge list.  Pass NULL arrays to query *m only; otherwise *m is the
+ * capacity on entry and the count on return.  d2 = float32 squared distance
+ * in rescaled coordinates (the ordering key).  Any array may be NULL. */
+int emst_get_edges(emst_ctx* ctx, int32_t* u, int32_t* v, float* d2, int64_t* m);
+
+/* Replaces SolutionEMST::evaluate() (emst_coalition/src/SolutionEMST.cu:560-581,
+ * K_EMST_evaluate_ST emst_coalition/include/EMSTOperators.h:262-292): FP64 sum
+ * of sqrt(float d2) over the forest edges, on the input coordinates. */
+int emst_get_weight(emst_ctx* ctx, double* weight, int64_t* size);
+
+/* The disjoint-set labels (disjointSetMap, written by NeuralNetEMST::writeLinks
+ * emst_coalition/include/NeuralNetEMST.h:310-318): root point index per point,
+ * after the last round's flatten. */
+int emst_get_labels(emst_ctx* ctx, int32_t* labels);
+
+/* Replaces getAdaptiveMap()/getAdaptiveMap3() (CalculateurEMST.h:45-51): the
+ * rescaled coordinates, n x dim float32, input order; also unitX. */
+int emst_get_rescaled(emst_ctx* ctx, float* xyz, double* unit_x);
+
+/* Per-round trace rows (see emst_round_trace).  *nrows: capacity in, count out. */
+int emst_get_trace(emst_ctx* ctx, emst_round_trace* rows, int32_t* nrows);
+
+/* Stage output for parity: the cone candidates of the last cone search,
+ * n x K input ids (-1 = empty cone), K = 8 (2D) or 24 (3D), and their float32
+ * d2 (inf = empty).  Mirrors NodeComputeOctantMD::slabP/slabDP
+ * (emst_coalition/include/NodeEMST.h:960-1011). */
+int emst_get_candidates(emst_ctx* ctx, int32_t* cand, float* cand_d2, int32_t* k);
+
+/* The reference's cellular matrix for the current points: dual-cell extents
+ * and, per point, its dual cell (findCell, basic_components/include/ViewGrid.h:339-359)
+ * and the CSR listing of each cell (the cell insert of K_EMST_initializeSpiralSearch,
+ * emst_coalition/include/EMSTOperators.h:80-110, without the 32-point cap of
+ * basic_components/include/Cell.h:53).  ext_dual[3] out; cell_of n x 3 out
+ * (-1 when outside); cell_start (ncell+1) and cell_pts (n) out, ncell =
+ * ext_dual[0]*ext_dual[1]*ext_dual[2]; pass NULL cell_start/cell_pts to get
+ * only ext_dual.  Points of a cell are listed in increasing id. */
+int emst_get_cell_grid(emst_ctx* ctx, int32_t level_radius, int32_t* ext_dual,
+                       int32_t* cell_of, int64_t* cell_start, int32_t* cell_pts);
+
+/* Kernel times of the last emst_run (ms, device events on the context's
+ * stream): [0] build (bbox, rescale, sort, hierarchy), [1] cone search,
+ * [2] Boruvka rounds, [3] total.  n_times: capacity in, count out. */
+int emst_get_times(emst_ctx* ctx, float* times, int32_t* n_times);
+
+/* Synthetic clouds, identical on every host (no counterpart in the reference;
+ * used by the benchmark and tests).  SplitMix64-seeded xoshiro128+,
+ * u = (x >> 8) * 2^-24.  kind: 0 uniform [0,1)^dim; 1 Gaussian mixture
+ * (64 centres, sigma log-uniform 0.002-0.02, 10% uniform background, clamped
+ * to the unit cube); 2 power-law (16 centres, radial density r^-1.5,
+ * r in [1e-4, 0.5]).  out: n x dim float32 host memory. */
+int emst_synth(int32_t kind, int64_t n, int32_t dim, uint64_t seed, float* out);
+
+/* Library version string. */
+const char* emst_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EMST_AMD_H */
