@@ -24,7 +24,7 @@ EXPORTS = [
     "emst_default_options", "emst_create", "emst_destroy", "emst_last_error", "emst_set_points",
     "emst_run", "emst_step", "emst_get_edges", "emst_get_weight", "emst_get_labels",
     "emst_get_rescaled", "emst_get_trace", "emst_get_candidates", "emst_get_cell_grid",
-    "emst_get_times", "emst_synth", "emst_version",
+    "emst_get_times", "emst_get_search_stats", "emst_synth", "emst_version",
 ]
 
 
@@ -38,7 +38,8 @@ class Options(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int32), ("level_radius", ctypes.c_int32),
                 ("max_rounds", ctypes.c_int32), ("bucket_size", ctypes.c_int32),
                 ("trace", ctypes.c_int32), ("device", ctypes.c_int32),
-                ("radius_mode", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+                ("radius_mode", ctypes.c_int32), ("seed_scale", ctypes.c_float),
+                ("reserved", ctypes.c_int32 * 4)]
 
 
 class RoundTrace(ctypes.Structure):
@@ -81,6 +82,7 @@ def lib():
     L.emst_get_candidates.argtypes = [P, P, P, P]
     L.emst_get_cell_grid.argtypes = [P, i32, P, P, P, P]
     L.emst_get_times.argtypes = [P, P, P]
+    L.emst_get_search_stats.argtypes = [P, P, P]
     L.emst_synth.argtypes = [i32, i64, i32, ctypes.c_uint64, P]
     L.emst_version.restype = ctypes.c_char_p
     _lib = L
@@ -103,12 +105,14 @@ def synth(kind, n, dim, seed):
 class Calculateur:
     """One EMST context on one device (CalculateurEMST / SolutionEMST)."""
 
-    def __init__(self, dim=3, level_radius=1, max_rounds=100, bucket_size=0, trace=False, device=0):
+    def __init__(self, dim=3, level_radius=1, max_rounds=100, bucket_size=0, trace=False, device=0,
+                 seed_scale=0.0):
         L = lib()
         o = Options()
         L.emst_default_options(ctypes.byref(o))
         o.dim, o.level_radius, o.max_rounds = dim, level_radius, max_rounds
         o.bucket_size, o.trace, o.device = bucket_size, int(trace), device
+        o.seed_scale = seed_scale
         self._L = L
         self._c = L.emst_create(ctypes.byref(o))
         if not self._c:
@@ -217,6 +221,17 @@ class Calculateur:
         k = ctypes.c_int32(4)
         self._chk(self._L.emst_get_times(self._c, t, ctypes.byref(k)), "times")
         return list(t)[: k.value]
+
+
+def _stats_names():
+    return ["packets", "nodes", "leaves", "leaf_points", "pairs"]
+
+
+def search_stats(calc):
+    t = (ctypes.c_int64 * 5)()
+    k = ctypes.c_int32(5)
+    calc._chk(calc._L.emst_get_search_stats(calc._c, t, ctypes.byref(k)), "stats")
+    return dict(zip(_stats_names(), list(t)[: k.value]))
 
 
 def emst(points, **kw):

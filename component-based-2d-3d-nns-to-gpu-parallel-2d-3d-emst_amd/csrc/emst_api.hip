@@ -57,6 +57,8 @@ struct emst_ctx {
     uint32_t zero_or_empty = 0;
     std::vector<emst_round_trace> trace;
     float times[4] = {0, 0, 0, 0};
+    int64_t stats[5] = {0, 0, 0, 0, 0};
+    float seed_scale = 1.5f;          // plain seed: multiple of the leaf cell's mean spacing
     hipEvent_t ev[8] = {};
     // device buffers
     DevBuf<float> xyz_own, resc;
@@ -202,7 +204,7 @@ int build(emst_ctx* c)
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ev[1], s));
     launch_cone_search(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, nullptr, n, c->label.p, false,
-                       c->inv.p, c->cand_key.p, c->cand_pos.p, s);
+                       c->inv.p, nullptr, c->seed_scale, c->cand_key.p, c->cand_pos.p, c->C, s);
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ev[2], s));
     launch_flag_research(c->K, n, c->cand_key.p, c->list.p, c->C, s);
@@ -211,6 +213,11 @@ int build(emst_ctx* c)
     int rc = read_counters(c, &h);
     if (rc) return rc;
     c->zero_or_empty = h.zero_or_empty;
+    c->stats[0] = (int64_t)h.packets;
+    c->stats[1] = (int64_t)h.nodes;
+    c->stats[2] = (int64_t)h.leaves;
+    c->stats[3] = (int64_t)h.leaf_points;
+    c->stats[4] = (int64_t)h.pairs;
     c->times[0] = elapsed(c->ev[0], c->ev[1]);
     c->times[1] = elapsed(c->ev[1], c->ev[2]);
     c->built = true;
@@ -260,7 +267,8 @@ int round_once(emst_ctx* c, int32_t* finished)
         if (rc) return rc;
         if (h.research > 0)
             launch_cone_search(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, c->list.p, h.research,
-                               c->label.p, true, c->inv.p, c->cand_key.p, c->cand_pos.p, s);
+                               c->label.p, true, c->inv.p, nullptr, c->seed_scale, c->cand_key.p,
+                               c->cand_pos.p, nullptr, s);
         LAUNCHCHK(c);
         searched = h.research;
     }
@@ -358,6 +366,7 @@ emst_ctx* emst_create(const emst_options* opt)
     if (hipSetDevice(o.device) != hipSuccess) { g_create_error = "hipSetDevice failed"; return nullptr; }
     emst_ctx* c = new emst_ctx();
     c->opt = o;
+    if (o.seed_scale > 0.f) c->seed_scale = o.seed_scale;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         g_create_error = "hipStreamCreate failed";
         delete c;
@@ -636,6 +645,15 @@ int emst_get_times(emst_ctx* c, float* times, int32_t* n_times)
     const int32_t k = std::min<int32_t>(*n_times, 4);
     for (int i = 0; i < k; ++i) times[i] = c->times[i];
     *n_times = k;
+    return EMST_OK;
+}
+
+int emst_get_search_stats(emst_ctx* c, int64_t* stats, int32_t* n_stats)
+{
+    if (!c || !stats || !n_stats) return EMST_E_ARG;
+    const int32_t k = std::min<int32_t>(*n_stats, 5);
+    for (int i = 0; i < k; ++i) stats[i] = c->stats[i];
+    *n_stats = k;
     return EMST_OK;
 }
 

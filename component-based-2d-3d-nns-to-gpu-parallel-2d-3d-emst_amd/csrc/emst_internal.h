@@ -27,6 +27,9 @@ struct Counters {
     uint32_t research;      // points queued for a repeated cone search
     uint32_t zero_or_empty; // points holding a zero-distance or no candidate
     uint32_t unflat;        // flatten stopped at its step cap (run it again)
+    // cone-search counters of the last full search: packets (waves), nodes
+    // tested, leaves entered, leaf points read per packet, lane-point pairs
+    unsigned long long packets, nodes, leaves, leaf_points, pairs;
 };
 
 // Search-hierarchy node: tight box of a contiguous range of Morton-sorted points.
@@ -55,8 +58,8 @@ void launch_build_tree(const float4* pts, int64_t n, int bucket, int64_t L, Node
 // check_labels: skip zero-distance points of the query's own component.
 void launch_cone_search(int dim, const float4* pts, const Node* nodes, int64_t L, int bucket,
                         int64_t n, const int32_t* qlist, int64_t nq, const int32_t* label,
-                        bool check_labels, const int32_t* inv, uint64_t* cand_key,
-                        int32_t* cand_pos, hipStream_t s);
+                        bool check_labels, const int32_t* inv, const float* seed_r, float seed_scale,
+                        uint64_t* cand_key, int32_t* cand_pos, Counters* stats, hipStream_t s);
 void launch_round_init(int64_t n, int32_t* label, int32_t* corr, uint64_t* corr_key,
                        uint64_t* comp1, uint32_t* comp2, Counters* C, hipStream_t s);
 void launch_flag_research(int K, int64_t n, const uint64_t* cand_key, int32_t* list,

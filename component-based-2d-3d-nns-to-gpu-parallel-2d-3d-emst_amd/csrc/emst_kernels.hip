@@ -444,63 +444,90 @@ __device__ __forceinline__ uint32_t cone_mask(float dx, float dy, float dz)
     return mask;
 }
 
-// smallest |d| over [lo, hi]
-__device__ __forceinline__ float mabs(float lo, float hi)
-{
-    return lo > 0.f ? lo : (hi < 0.f ? -hi : 0.f);
-}
-
+// Node test.  For every point q of the box, |d_k| >= m_k on each axis, m_k the
+// smallest |d_k| over the box; a half-axis (d_k >= 0 or d_k <= 0) that the box
+// reaches always has that same minimum (0 when the box straddles p's
+// coordinate, else the one reachable half).  So inside face (axis a, sign) the
+// box's points satisfy |d_a| >= max(smallest |d_a| on the face's side, m_o1,
+// m_o2) and |d_oi| >= m_oi, one bound shared by all the face's slabs the box
+// reaches.  The float d2 formula is monotone in each |d|, so the same rounded
+// operations on these bounds never exceed any q's own d2.  The box is needed
+// when, for some face, that bound is <= the largest bound among the face's
+// reachable open slabs, and its farthest point lies beyond `inner` (points
+// with d2 <= inner were examined in an earlier pass).  Bounds are int32 d2
+// bit patterns: -1 = cone closed, INT_MAX = open without candidate.
 template <int D>
 __device__ __forceinline__ bool node_needed(const Node& nd, float px, float py, float pz,
-                                            const uint32_t* bound)
+                                            const int32_t* bound, int32_t maxb, int32_t inner)
 {
-    const float lx = nd.lo[0] - px, hx = nd.hi[0] - px;
-    const float ly = nd.lo[1] - py, hy = nd.hi[1] - py;
-    const float mx = mabs(lx, hx), my = mabs(ly, hy);
+    float l[3], h[3], m[3], u[3];
+    bool fp[3], fn[3];
+    l[0] = nd.lo[0] - px; h[0] = nd.hi[0] - px;
+    l[1] = nd.lo[1] - py; h[1] = nd.hi[1] - py;
+    if (D == 3) { l[2] = nd.lo[2] - pz; h[2] = nd.hi[2] - pz; }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        m[k] = fmaxf(fmaxf(l[k], -h[k]), 0.f);
+        u[k] = fmaxf(-l[k], h[k]);
+        fp[k] = h[k] >= 0.f;
+        fn[k] = l[k] <= 0.f;
+    }
+    {
+        const float lb = D == 3 ? d2_of(m[0], m[1], m[2]) : d2_of(m[0], m[1]);
+        if ((int32_t)__float_as_uint(lb) > maxb) return false;
+        const float far = D == 3 ? d2_of(u[0], u[1], u[2]) : d2_of(u[0], u[1]);
+        if ((int32_t)__float_as_uint(far) <= inner) return false;
+    }
     bool need = false;
-    if (D == 2) {
-        // face +x: d_x >= |d_y|
-        {
-            const float a = fmaxf(lx, my);
-            need |= (hx >= my) && (__float_as_uint(d2_of(a, my)) <= bound[0]);
+#pragma unroll
+    for (int f = 0; f < 2 * D; ++f) {
+        const int a = f / 2;
+        const bool neg = f & 1;
+        const int o1 = a == 0 ? 1 : 0;
+        const int o2 = a == 2 ? 1 : 2;
+        const float o = D == 3 ? fmaxf(m[o1], m[o2]) : m[o1];
+        const float reach = neg ? -l[a] : h[a];
+        const float am = fmaxf(neg ? -h[a] : l[a], o);
+        float v[3];
+        v[a] = am;
+        v[o1] = m[o1];
+        if (D == 3) v[o2] = m[o2];
+        const float lb = D == 3 ? d2_of(v[0], v[1], v[2]) : d2_of(v[0], v[1]);
+        const int32_t* b = bound + f * (1 << (D - 1));
+        int32_t bm;
+        if (D == 3) {
+            bm = max(max((fp[o1] && fp[o2]) ? b[0] : -1, (fn[o1] && fp[o2]) ? b[1] : -1),
+                     max((fp[o1] && fn[o2]) ? b[2] : -1, (fn[o1] && fn[o2]) ? b[3] : -1));
+        } else {
+            bm = max(fp[o1] ? b[0] : -1, fn[o1] ? b[1] : -1);
         }
-        {
-            const float a = fmaxf(-hx, my);
-            need |= (-lx >= my) && (__float_as_uint(d2_of(a, my)) <= bound[1]);
-        }
-        {
-            const float a = fmaxf(ly, mx);
-            need |= (hy >= mx) && (__float_as_uint(d2_of(mx, a)) <= bound[2]);
-        }
-        {
-            const float a = fmaxf(-hy, mx);
-            need |= (-ly >= mx) && (__float_as_uint(d2_of(mx, a)) <= bound[3]);
-        }
-    } else {
-        const float lz = nd.lo[2] - pz, hz = nd.hi[2] - pz;
-        const float mz = mabs(lz, hz);
-        {
-            const float o = fmaxf(my, mz);
-            const float ap = fmaxf(lx, o), an = fmaxf(-hx, o);
-            need |= (hx >= o) && (__float_as_uint(d2_of(ap, my, mz)) <= bound[0]);
-            need |= (-lx >= o) && (__float_as_uint(d2_of(an, my, mz)) <= bound[1]);
-        }
-        {
-            const float o = fmaxf(mx, mz);
-            const float ap = fmaxf(ly, o), an = fmaxf(-hy, o);
-            need |= (hy >= o) && (__float_as_uint(d2_of(mx, ap, mz)) <= bound[2]);
-            need |= (-ly >= o) && (__float_as_uint(d2_of(mx, an, mz)) <= bound[3]);
-        }
-        {
-            const float o = fmaxf(mx, my);
-            const float ap = fmaxf(lz, o), an = fmaxf(-hz, o);
-            need |= (hz >= o) && (__float_as_uint(d2_of(mx, my, ap)) <= bound[4]);
-            need |= (-lz >= o) && (__float_as_uint(d2_of(mx, my, an)) <= bound[5]);
-        }
+        need |= (reach >= o) && ((int32_t)__float_as_uint(lb) <= bm);
     }
     return need;
 }
 
+__device__ __forceinline__ float bcast(float v, int j)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
+// best key -> int32 bound pattern (empty cone = INT_MAX)
+__device__ __forceinline__ int32_t key_bound(unsigned long long k)
+{
+    const uint32_t hi = (uint32_t)(k >> 32);
+    return hi == 0xffffffffu ? 0x7fffffff : (int32_t)hi;
+}
+
+constexpr int kMaxCappedPasses = 6;
+
+// Cone search, exact, by iterative deepening (the tree form of the Elias ring
+// expansion).  Pass k caps every open cone's bound at R_k^2 (R_0 = the seed,
+// R_{k+1} = 2 R_k); after the pass a cone whose best d2 is <= R_k^2 is exact
+// and closes; the pass after kMaxCappedPasses has no cap, so every cone ends
+// exact.  The seed only decides how much work each pass does, never the result.
+//
+// Seeds: seed_r (per sorted point) when given, else seed_scale times the mean
+// spacing of the point's own leaf cell ((box volume / points)^(1/D)).
 template <int D>
 __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ pts,
                                                    const Node* __restrict__ nodes, int64_t L,
@@ -509,12 +536,18 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
                                                    const int32_t* __restrict__ label,
                                                    int check_labels,
                                                    const int32_t* __restrict__ inv,
+                                                   const float* __restrict__ seed_r,
+                                                   float seed_scale,
                                                    uint64_t* __restrict__ cand_key,
-                                                   int32_t* __restrict__ cand_pos)
+                                                   int32_t* __restrict__ cand_pos,
+                                                   Counters* __restrict__ stats)
 {
     using C = ConeCfg<D>;
-    __shared__ uint64_t best[C::K][64];
-    __shared__ int32_t stk[64];
+    __shared__ unsigned long long best[C::K][64];
+    __shared__ Node stk_box[64];
+    __shared__ int32_t stk_id[64];
+    __shared__ int32_t stk_ver[64];
+    __shared__ unsigned long long stk_mask[64];
     const int lane = __lane_id();
     const int64_t qi = (int64_t)blockIdx.x * 64 + lane;
     const bool active = qi < nq;
@@ -522,9 +555,32 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
     const float4 me = pts[p];
     const float px = me.x, py = me.y, pz = me.z;
     const int32_t mylab = (check_labels && active) ? label[p] : 0;
+#pragma unroll
     for (int c = 0; c < C::K; ++c) best[c][lane] = kEmptyKey;
-    uint32_t bound[C::NF];
-    for (int f = 0; f < C::NF; ++f) bound[f] = active ? 0xffffffffu : 0u;
+
+    // seed radius
+    float r;
+    if (seed_r) {
+        r = seed_r[p];
+    } else {
+        const Node own = nodes[L + p / bucket];
+        const int64_t s0 = (p / bucket) * bucket;
+        const float cnt = (float)(min(n, s0 + bucket) - s0);
+        float vol = 1.f, ext = 0.f;
+        for (int k = 0; k < D; ++k) {
+            const float e = own.hi[k] - own.lo[k];
+            vol *= e;
+            ext = fmaxf(ext, e);
+        }
+        float sp = vol > 0.f ? powf(vol / cnt, 1.f / D) : ext / cnt;
+        if (!(sp > 0.f)) sp = 1e-6f;
+        r = seed_scale * sp;
+    }
+    float r2 = r * r;
+    uint32_t open = active ? (uint32_t)((1ull << C::K) - 1) : 0u;
+    int32_t inner = -1;
+    int32_t bound[C::K];
+    int32_t maxb = -1;
 
     // packet centre (for near-first child order)
     float cx = active ? px : 0.f, cy = active ? py : 0.f, cz = active ? pz : 0.f;
@@ -536,66 +592,118 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
         na += __shfl_xor(na, off);
     }
     cx /= na; cy /= na; cz /= na;
+    auto cdist = [&](const Node& x) {
+        const float dxx = fmaxf(fmaxf(x.lo[0] - cx, cx - x.hi[0]), 0.f);
+        const float dyy = fmaxf(fmaxf(x.lo[1] - cy, cy - x.hi[1]), 0.f);
+        const float dzz = fmaxf(fmaxf(x.lo[2] - cz, cz - x.hi[2]), 0.f);
+        return dxx * dxx + dyy * dyy + dzz * dzz;
+    };
 
-    int sp = 0;
-    if (lane == 0) stk[0] = 1;
-    sp = 1;
-    __builtin_amdgcn_wave_barrier();
-    while (sp > 0) {
-        --sp;
-        const int node = __builtin_amdgcn_readfirstlane(stk[sp]);
-        const Node nd = nodes[node];
-        const bool need = active && node_needed<D>(nd, px, py, pz, bound);
-        if (__ballot(need) == 0ull) continue;
-        if (node >= L) {
-            const int64_t s0 = (int64_t)(node - L) * bucket;
-            const int64_t s1 = min(n, s0 + bucket);
-            for (int64_t s = s0; s < s1; ++s) {
-                const float4 q = pts[s];
-                const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
-                const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
-                bool take = need && s != p;
-                if (take && check_labels && d2 == 0.f) take = label[s] != mylab;
-                if (take) {
-                    uint32_t mask = cone_mask<D>(dx, dy, dz);
-                    const uint64_t key = ((uint64_t)__float_as_uint(d2) << 32) | __float_as_uint(q.w);
-                    while (mask) {
-                        const int c = __builtin_ctz(mask);
-                        mask &= mask - 1;
-                        if (key < best[c][lane]) best[c][lane] = key;
+    uint32_t st_nodes = 0, st_leaves = 0, st_pts = 0, st_pairs = 0;
+    for (int pass = 0; pass <= kMaxCappedPasses; ++pass) {
+        if (__ballot(open != 0u) == 0ull) break;
+        const bool last = pass == kMaxCappedPasses;
+        const int32_t cap = last ? 0x7fffffff : (int32_t)__float_as_uint(r2);
+        auto refresh = [&]() {
+            maxb = -1;
+#pragma unroll
+            for (int c = 0; c < C::K; ++c) {
+                const int32_t b = ((open >> c) & 1u) ? min(key_bound(best[c][lane]), cap) : -1;
+                bound[c] = b;
+                maxb = max(maxb, b);
+            }
+        };
+        refresh();
+        int sp = 0;
+        int ver = 0;        // bumps whenever a leaf tightened the bounds
+        if (lane == 0) { stk_box[0] = nodes[1]; stk_id[0] = 1; stk_ver[0] = -1; }
+        sp = 1;
+        __builtin_amdgcn_wave_barrier();
+        while (sp > 0) {
+            --sp;
+            const int node = __builtin_amdgcn_readfirstlane(stk_id[sp]);
+            const int pver = __builtin_amdgcn_readfirstlane(stk_ver[sp]);
+            const Node nd = stk_box[sp];
+            // a node tested against the current bounds when pushed keeps its lane mask
+            bool need;
+            if (pver == ver) {
+                need = (stk_mask[sp] >> lane) & 1ull;
+            } else {
+                need = open != 0u && node_needed<D>(nd, px, py, pz, bound, maxb, inner);
+                ++st_nodes;
+            }
+            const uint64_t needm = __ballot(need);
+            if (needm == 0ull) continue;
+            if (node >= L) {
+                const int64_t s0 = (int64_t)(node - L) * bucket;
+                const int cnt = (int)(min(n, s0 + bucket) - s0);
+                ++st_leaves;
+                st_pts += (uint32_t)cnt;
+                st_pairs += (uint32_t)cnt * (uint32_t)__popcll(needm);
+                for (int base = 0; base < cnt; base += 64) {
+                    const int m = min(64, cnt - base);
+                    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (lane < m) v = pts[s0 + base + lane];
+                    for (int j = 0; j < m; ++j) {
+                        const float qx = bcast(v.x, j), qy = bcast(v.y, j), qz = bcast(v.z, j);
+                        const uint32_t qid = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(v.w), j);
+                        const int64_t s = s0 + base + j;
+                        const float dx = qx - px, dy = qy - py, dz = qz - pz;
+                        const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
+                        const int32_t db = (int32_t)__float_as_uint(d2);
+                        bool take = need && s != p && db <= maxb && db > inner;
+                        if (check_labels && take && d2 == 0.f) take = label[s] != mylab;
+                        if (take) {
+                            uint32_t mask = cone_mask<D>(dx, dy, dz) & open;
+                            const unsigned long long key =
+                                ((unsigned long long)__float_as_uint(d2) << 32) | qid;
+                            while (mask) {
+                                const int c = __builtin_ctz(mask);
+                                mask &= mask - 1;
+                                atomicMin(&best[c][lane], key);
+                            }
+                        }
                     }
                 }
-            }
-            if (need) {
-                for (int f = 0; f < C::NF; ++f) {
-                    uint32_t b = 0;
-                    for (int sl = 0; sl < C::NS; ++sl) {
-                        const uint64_t k = best[f * C::NS + sl][lane];
-                        const uint32_t hiw = (uint32_t)(k >> 32);
-                        b = max(b, hiw);
+                if (need) refresh();
+                ++ver;
+            } else {
+                const int a = 2 * node, b = 2 * node + 1;
+                const Node boxa = nodes[a], boxb = nodes[b];
+                const unsigned long long ma =
+                    __ballot(open != 0u && node_needed<D>(boxa, px, py, pz, bound, maxb, inner));
+                const unsigned long long mb =
+                    __ballot(open != 0u && node_needed<D>(boxb, px, py, pz, bound, maxb, inner));
+                st_nodes += 2;
+                const bool ua = ma != 0ull, ub = mb != 0ull;
+                const bool afirst = cdist(boxa) <= cdist(boxb);
+                if (lane == 0) {
+                    int t = sp;
+                    if (afirst) {
+                        if (ub) { stk_box[t] = boxb; stk_id[t] = b; stk_ver[t] = ver; stk_mask[t] = mb; ++t; }
+                        if (ua) { stk_box[t] = boxa; stk_id[t] = a; stk_ver[t] = ver; stk_mask[t] = ma; }
+                    } else {
+                        if (ua) { stk_box[t] = boxa; stk_id[t] = a; stk_ver[t] = ver; stk_mask[t] = ma; ++t; }
+                        if (ub) { stk_box[t] = boxb; stk_id[t] = b; stk_ver[t] = ver; stk_mask[t] = mb; }
                     }
-                    bound[f] = b;
                 }
+                sp += (int)ua + (int)ub;
+                __builtin_amdgcn_wave_barrier();
             }
-        } else {
-            const int a = 2 * node, b = 2 * node + 1;
-            const Node na_ = nodes[a], nb_ = nodes[b];
-            auto bd = [&](const Node& x) {
-                float s = 0.f;
-                const float dxx = fmaxf(fmaxf(x.lo[0] - cx, cx - x.hi[0]), 0.f);
-                const float dyy = fmaxf(fmaxf(x.lo[1] - cy, cy - x.hi[1]), 0.f);
-                const float dzz = fmaxf(fmaxf(x.lo[2] - cz, cz - x.hi[2]), 0.f);
-                s = dxx * dxx + dyy * dyy + dzz * dzz;
-                return s;
-            };
-            const bool afirst = bd(na_) <= bd(nb_);
-            if (lane == 0) {
-                stk[sp] = afirst ? b : a;
-                stk[sp + 1] = afirst ? a : b;
-            }
-            sp += 2;
-            __builtin_amdgcn_wave_barrier();
         }
+        // close the cones whose best lies within this pass's radius
+#pragma unroll
+        for (int c = 0; c < C::K; ++c)
+            if (((open >> c) & 1u) && key_bound(best[c][lane]) <= cap) open &= ~(1u << c);
+        inner = cap;
+        r2 *= 4.f;
+    }
+    if (stats && lane == 0) {
+        atomicAdd(&stats->packets, 1ull);
+        atomicAdd(&stats->nodes, (unsigned long long)st_nodes);
+        atomicAdd(&stats->leaves, (unsigned long long)st_leaves);
+        atomicAdd(&stats->leaf_points, (unsigned long long)st_pts);
+        atomicAdd(&stats->pairs, (unsigned long long)st_pairs);
     }
     if (!active) return;
     uint64_t* ok = cand_key + p * C::K;
@@ -615,7 +723,10 @@ __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __res
                                                       uint32_t* __restrict__ comp2, Counters* C)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) { C->edges = 0; C->research = 0; C->zero_or_empty = 0; C->unflat = 0; }
+    if (i == 0) {
+        C->edges = 0; C->research = 0; C->zero_or_empty = 0; C->unflat = 0;
+        C->packets = C->nodes = C->leaves = C->leaf_points = C->pairs = 0;
+    }
     if (i >= n) return;
     label[i] = (int32_t)i;          // K_EMST_initDisjointSet (EMSTOperators.h:330-343)
     corr[i] = -1;                   // correspondenceMap reset (SolutionEMST.cu:464-466)
@@ -1016,17 +1127,17 @@ void launch_build_tree(const float4* pts, int64_t n, int bucket, int64_t L, Node
 
 void launch_cone_search(int dim, const float4* pts, const Node* nodes, int64_t L, int bucket,
                         int64_t n, const int32_t* qlist, int64_t nq, const int32_t* label,
-                        bool check_labels, const int32_t* inv, uint64_t* cand_key,
-                        int32_t* cand_pos, hipStream_t s)
+                        bool check_labels, const int32_t* inv, const float* seed_r, float seed_scale,
+                        uint64_t* cand_key, int32_t* cand_pos, Counters* stats, hipStream_t s)
 {
     if (nq <= 0) return;
     const unsigned g = nblocks(nq, 64);
     if (dim == 3)
         hipLaunchKernelGGL(k_cone_search<3>, dim3(g), dim3(64), 0, s, pts, nodes, L, bucket, n, qlist, nq,
-                           label, check_labels ? 1 : 0, inv, cand_key, cand_pos);
+                           label, check_labels ? 1 : 0, inv, seed_r, seed_scale, cand_key, cand_pos, stats);
     else
         hipLaunchKernelGGL(k_cone_search<2>, dim3(g), dim3(64), 0, s, pts, nodes, L, bucket, n, qlist, nq,
-                           label, check_labels ? 1 : 0, inv, cand_key, cand_pos);
+                           label, check_labels ? 1 : 0, inv, seed_r, seed_scale, cand_key, cand_pos, stats);
 }
 
 void launch_round_init(int64_t n, int32_t* label, int32_t* corr, uint64_t* corr_key,

@@ -52,7 +52,9 @@ typedef struct {
     int32_t trace;          /* 1: record per-round forest size, length and phase times */
     int32_t device;         /* HIP device ordinal */
     int32_t radius_mode;    /* 0 plain; 1 learned seed radius (needs emst_set_radius_model) */
-    int32_t reserved[5];
+    float seed_scale;       /* plain-form first search radius, in mean spacings of the point's
+                               leaf cell (0 = default 1.5); changes work, never results */
+    int32_t reserved[4];
 } emst_options;
 
 /* One row of the per-round trace: the columns of the reference's stats file
@@ -155,6 +157,14 @@ int emst_get_cell_grid(emst_ctx* ctx, int32_t level_radius, int32_t* ext_dual,
  * stream): [0] build (bbox, rescale, sort, hierarchy), [1] cone search,
  * [2] Boruvka rounds, [3] total.  n_times: capacity in, count out. */
 int emst_get_times(emst_ctx* ctx, float* times, int32_t* n_times);
+
+/* Counters of the last full cone search (the work behind its bytes):
+ * [0] packets (64-point waves), [1] hierarchy nodes tested, [2] leaf cells
+ * entered, [3] leaf points read (summed over packets), [4] point pairs
+ * evaluated (leaf points x lanes that needed the leaf).  No counterpart in
+ * the reference (its per-phase timers are in SolutionEMSTOperators.cu:166-373).
+ * n_stats: capacity in, count out. */
+int emst_get_search_stats(emst_ctx* ctx, int64_t* stats, int32_t* n_stats);
 
 /* Synthetic clouds, identical on every host (no counterpart in the reference;
  * used by the benchmark and tests).  SplitMix64-seeded xoshiro128+,
