@@ -200,11 +200,12 @@ int build(emst_ctx* c)
                c->hist.p, c->scan_tmp.p, s);
     launch_gather_sorted(c->resc.p, c->vals0.p, n, c->dim, c->pts.p, c->inv.p, s);
     launch_build_tree(c->pts.p, n, c->bucket, c->L, c->nodes.p, s);
-    launch_round_init(n, c->label.p, c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->C, s);
+    launch_round_init(n, c->label.p, c->eu.p, c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->C, s);
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ev[1], s));
     launch_cone_search(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, nullptr, n, c->label.p, false,
-                       c->inv.p, nullptr, c->seed_scale, c->cand_key.p, c->cand_pos.p, c->C, s);
+                       c->inv.p, nullptr, c->seed_scale, c->cand_key.p, c->cand_pos.p, nullptr, c->corr.p,
+                       c->corr_key.p, c->C, s);
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ev[2], s));
     launch_flag_research(c->K, n, c->cand_key.p, c->list.p, c->C, s);
@@ -268,7 +269,7 @@ int round_once(emst_ctx* c, int32_t* finished)
         if (h.research > 0)
             launch_cone_search(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, c->list.p, h.research,
                                c->label.p, true, c->inv.p, nullptr, c->seed_scale, c->cand_key.p,
-                               c->cand_pos.p, nullptr, s);
+                               c->cand_pos.p, c->corr.p, nullptr, nullptr, nullptr, s);
         LAUNCHCHK(c);
         searched = h.research;
     }
@@ -300,7 +301,7 @@ int round_once(emst_ctx* c, int32_t* finished)
         row.iteration = round;
         row.size = (int32_t)c->edges;
         double w = 0;
-        rc = weight_of(c, c->edges, &w);
+        rc = weight_of(c, n, &w);
         if (rc) return rc;
         row.length = w;
         row.time_find_next_closest = elapsed(c->ev[3], c->ev[4]);
@@ -486,8 +487,9 @@ int emst_get_edges(emst_ctx* c, int32_t* u, int32_t* v, float* d2, int64_t* m)
     if (cnt == 0) return EMST_OK;
     HIPCHK(c, hipSetDevice(c->opt.device));
     hipStream_t s = c->stream;
-    launch_edge_keys(c->eu.p, c->ev_.p, cnt, c->keys0.p, s);
-    radix_sort(c->keys0.p, c->vals0.p, c->keys1.p, c->vals1.p, cnt, 64, true, c->hist.p, c->scan_tmp.p, s);
+    // slots are indexed by absorbed root; empty slots sort last
+    launch_edge_keys(c->eu.p, c->ev_.p, c->n, c->keys0.p, s);
+    radix_sort(c->keys0.p, c->vals0.p, c->keys1.p, c->vals1.p, c->n, 64, true, c->hist.p, c->scan_tmp.p, s);
     // reuse label2/list as staging (free between rounds)
     int32_t* du = c->label2.p;
     int32_t* dv = c->list.p;
@@ -509,7 +511,7 @@ int emst_get_weight(emst_ctx* c, double* w, int64_t* size)
     HIPCHK(c, hipSetDevice(c->opt.device));
     const int64_t cnt = c->built ? (int64_t)c->edges : 0;
     if (size) *size = cnt;
-    return weight_of(c, cnt, w);
+    return weight_of(c, cnt > 0 ? c->n : 0, w);
 }
 
 int emst_get_labels(emst_ctx* c, int32_t* labels)

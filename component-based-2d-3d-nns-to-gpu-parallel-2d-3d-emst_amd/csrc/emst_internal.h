@@ -59,8 +59,9 @@ void launch_build_tree(const float4* pts, int64_t n, int bucket, int64_t L, Node
 void launch_cone_search(int dim, const float4* pts, const Node* nodes, int64_t L, int bucket,
                         int64_t n, const int32_t* qlist, int64_t nq, const int32_t* label,
                         bool check_labels, const int32_t* inv, const float* seed_r, float seed_scale,
-                        uint64_t* cand_key, int32_t* cand_pos, Counters* stats, hipStream_t s);
-void launch_round_init(int64_t n, int32_t* label, int32_t* corr, uint64_t* corr_key,
+                        uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
+                        uint64_t* first_key, Counters* stats, hipStream_t s);
+void launch_round_init(int64_t n, int32_t* label, int32_t* eu, int32_t* corr, uint64_t* corr_key,
                        uint64_t* comp1, uint32_t* comp2, Counters* C, hipStream_t s);
 void launch_flag_research(int K, int64_t n, const uint64_t* cand_key, int32_t* list,
                           Counters* C, hipStream_t s);

@@ -540,6 +540,9 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
                                                    float seed_scale,
                                                    uint64_t* __restrict__ cand_key,
                                                    int32_t* __restrict__ cand_pos,
+                                                   int32_t* __restrict__ revive,
+                                                   int32_t* __restrict__ first_corr,
+                                                   uint64_t* __restrict__ first_key,
                                                    Counters* __restrict__ stats)
 {
     using C = ConeCfg<D>;
@@ -706,17 +709,28 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
         atomicAdd(&stats->pairs, (unsigned long long)st_pairs);
     }
     if (!active) return;
-    uint64_t* ok = cand_key + p * C::K;
-    int32_t* op = cand_pos + p * C::K;
+    if (revive && revive[p] == -2) revive[p] = -1;
+    // cone-major layout [K][n]: lanes write consecutive words
+    uint64_t kmin = kEmptyKey;
+    int32_t pmin = -1;
     for (int c = 0; c < C::K; ++c) {
         const uint64_t k = best[c][lane];
-        ok[c] = k;
-        op[c] = (k == kEmptyKey) ? -1 : inv[(uint32_t)k];
+        const int32_t q = (k == kEmptyKey) ? -1 : inv[(uint32_t)k];
+        cand_key[(int64_t)c * n + p] = k;
+        cand_pos[(int64_t)c * n + p] = q;
+        if (k < kmin) { kmin = k; pmin = q; }
+    }
+    // first search (all points singletons): the round-1 target is the overall
+    // cone minimum (NodeEMST.h:1338-1381 with every label distinct)
+    if (first_corr) {
+        first_corr[p] = pmin >= 0 ? pmin : -2;
+        first_key[p] = kmin;
     }
 }
 
 // ------------------------------------------------------- Boruvka rounds
 __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __restrict__ label,
+                                                      int32_t* __restrict__ eu,
                                                       int32_t* __restrict__ corr,
                                                       uint64_t* __restrict__ corr_key,
                                                       uint64_t* __restrict__ comp1,
@@ -729,6 +743,7 @@ __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __res
     }
     if (i >= n) return;
     label[i] = (int32_t)i;          // K_EMST_initDisjointSet (EMSTOperators.h:330-343)
+    eu[i] = -1;                     // no edge absorbed this root yet
     corr[i] = -1;                   // correspondenceMap reset (SolutionEMST.cu:464-466)
     corr_key[i] = kEmptyKey;
     comp1[i] = kEmptyKey;
@@ -745,7 +760,7 @@ __global__ __launch_bounds__(kBlock) void k_flag_research(int K, int64_t n,
     if (p >= n) return;
     bool zero = false, any = false;
     for (int k = 0; k < K; ++k) {
-        const uint64_t key = cand_key[p * K + k];
+        const uint64_t key = cand_key[(int64_t)k * n + p];
         if (key == kEmptyKey) continue;
         any = true;
         zero |= (key >> 32) == 0u;
@@ -805,21 +820,24 @@ __global__ __launch_bounds__(kBlock) void k_round_min(int K, int64_t n, const fl
         fixed[p] = 0;
         int32_t c = corr[p];
         uint64_t ck = corr_key[p];
-        if (!(c != -1 && label[c] != lp)) {
+        // corr == -2: every cone candidate already inside the component; since
+        // components only merge, the point never has an outgoing candidate again
+        // (until a repeated cone search gives it new candidates)
+        if (c != -2 && !(c >= 0 && label[c] != lp)) {
             c = -1;
             ck = kEmptyKey;
             for (int k = 0; k < K; ++k) {
-                const uint64_t key = cand_key[p * K + k];
+                const uint64_t key = cand_key[(int64_t)k * n + p];
                 if (key >= ck) continue;
-                const int32_t q = cand_pos[p * K + k];
+                const int32_t q = cand_pos[(int64_t)k * n + p];
                 if (label[q] == lp) continue;
                 ck = key;
                 c = q;
             }
-            corr[p] = c;
+            corr[p] = c == -1 ? -2 : c;
             corr_key[p] = ck;
         }
-        if (c != -1) {
+        if (c >= 0) {
             const uint32_t pid = __float_as_uint(pts[p].w);
             const uint32_t qid = (uint32_t)ck;
             key1 = (ck & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
@@ -838,7 +856,7 @@ __global__ __launch_bounds__(kBlock) void k_round_comp2(int64_t n, const float4*
                                                        uint32_t* __restrict__ comp2)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n || corr[p] == -1) return;
+    if (p >= n || corr[p] < 0) return;
     const int32_t lp = label[p];
     const uint64_t ck = corr_key[p];
     const uint32_t pid = __float_as_uint(pts[p].w), qid = (uint32_t)ck;
@@ -856,7 +874,7 @@ __global__ __launch_bounds__(kBlock) void k_round_winner(int64_t n, const float4
                                                         int32_t* __restrict__ fixed)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n || corr[p] == -1) return;
+    if (p >= n || corr[p] < 0) return;
     const int32_t lp = label[p];
     const uint64_t ck = corr_key[p];
     const uint32_t pid = __float_as_uint(pts[p].w), qid = (uint32_t)ck;
@@ -866,40 +884,53 @@ __global__ __launch_bounds__(kBlock) void k_round_winner(int64_t n, const float4
 
 // hook/union (EMSTOperators.h:1267-1303): roots read from the flattened
 // labels, written into label2.  Non-mutual: root(p) -> root(q).  Mutual pair:
-// the endpoint with the smaller id keeps its root.
-__global__ __launch_bounds__(kBlock) void k_round_hook(int64_t n, const float4* __restrict__ pts,
-                                                      const int32_t* __restrict__ label,
-                                                      int32_t* __restrict__ label2,
-                                                      const int32_t* __restrict__ corr,
-                                                      const uint64_t* __restrict__ corr_key,
-                                                      const int32_t* __restrict__ fixed,
-                                                      Counters* C, int32_t* __restrict__ eu,
-                                                      int32_t* __restrict__ ev,
-                                                      float* __restrict__ ed2,
-                                                      int32_t* __restrict__ eround, int round)
+// the endpoint with the smaller id keeps its root.  Each hook absorbs exactly
+// one root and a root is absorbed at most once over the run, so the edge is
+// stored at the absorbed root's index (eu = -1 marks an empty slot); the edge
+// count is reduced per block.
+constexpr int kHookBlock = 1024;
+
+__global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const float4* __restrict__ pts,
+                                                          const int32_t* __restrict__ label,
+                                                          int32_t* __restrict__ label2,
+                                                          const int32_t* __restrict__ corr,
+                                                          const uint64_t* __restrict__ corr_key,
+                                                          const int32_t* __restrict__ fixed,
+                                                          Counters* C, int32_t* __restrict__ eu,
+                                                          int32_t* __restrict__ ev,
+                                                          float* __restrict__ ed2,
+                                                          int32_t* __restrict__ eround, int round)
 {
+    __shared__ uint32_t cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n || !fixed[p]) return;
-    const int32_t q = corr[p];
-    const uint64_t ck = corr_key[p];
-    const uint32_t pid = __float_as_uint(pts[p].w), qid = (uint32_t)ck;
-    const int32_t rp = label[p], rq = label[q];
     bool add = false;
-    if (!fixed[q] || corr[q] != (int32_t)p) {
-        label2[rp] = rq;
-        add = true;
-    } else if (pid < qid) {
-        label2[rq] = rp;
-        add = true;
+    if (p < n && fixed[p]) {
+        const int32_t q = corr[p];
+        const uint64_t ck = corr_key[p];
+        const uint32_t pid = __float_as_uint(pts[p].w), qid = (uint32_t)ck;
+        const int32_t rp = label[p], rq = label[q];
+        int32_t absorbed = -1;
+        if (!fixed[q] || corr[q] != (int32_t)p) {
+            label2[rp] = rq;
+            absorbed = rp;
+        } else if (pid < qid) {
+            label2[rq] = rp;
+            absorbed = rq;
+        }
+        if (absorbed >= 0) {
+            add = true;
+            eu[absorbed] = (int32_t)min(pid, qid);
+            ev[absorbed] = (int32_t)max(pid, qid);
+            ed2[absorbed] = __uint_as_float((uint32_t)(ck >> 32));
+            eround[absorbed] = round;
+        }
     }
-    if (add) {
-        const uint32_t slot = atomicAdd(&C->edges, 1u);
-        if ((int64_t)slot >= n) return;    // a forest never holds n edges; guards the arrays
-        eu[slot] = (int32_t)min(pid, qid);
-        ev[slot] = (int32_t)max(pid, qid);
-        ed2[slot] = __uint_as_float((uint32_t)(ck >> 32));
-        eround[slot] = round;
-    }
+    const uint64_t am = __ballot(add);
+    if (__lane_id() == 0 && am) atomicAdd(&cnt, (uint32_t)__popcll(am));
+    __syncthreads();
+    if (threadIdx.x == 0 && cnt) atomicAdd(&C->edges, cnt);
 }
 
 // flatten (EMSTOperators.h:412-433): chase label2 to the root, with path
@@ -944,6 +975,7 @@ __global__ __launch_bounds__(kBlock) void k_weight(const float* __restrict__ xyz
     double s = 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        if (eu[i] < 0) continue;
         const float* a = xyz + (int64_t)eu[i] * D;
         const float* b = xyz + (int64_t)ev[i] * D;
         const float dx = a[0] - b[0], dy = a[1] - b[1];
@@ -979,7 +1011,7 @@ __global__ __launch_bounds__(kBlock) void k_cands_out(int K, int64_t n, const ui
     if (i >= n) return;
     const int64_t s = inv[i];
     for (int k = 0; k < K; ++k) {
-        const uint64_t key = ck[s * K + k];
+        const uint64_t key = ck[(int64_t)k * n + s];
         cand[i * K + k] = key == kEmptyKey ? -1 : (int32_t)(uint32_t)key;
         cd2[i * K + k] = key == kEmptyKey ? INFINITY : __uint_as_float((uint32_t)(key >> 32));
     }
@@ -991,7 +1023,7 @@ __global__ __launch_bounds__(kBlock) void k_edge_keys(const int32_t* __restrict_
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
-    keys[i] = ((uint64_t)(uint32_t)eu[i] << 32) | (uint32_t)ev[i];
+    keys[i] = eu[i] < 0 ? ~0ull : (((uint64_t)(uint32_t)eu[i] << 32) | (uint32_t)ev[i]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_edges_out(const uint64_t* __restrict__ keys,
@@ -1128,23 +1160,26 @@ void launch_build_tree(const float4* pts, int64_t n, int bucket, int64_t L, Node
 void launch_cone_search(int dim, const float4* pts, const Node* nodes, int64_t L, int bucket,
                         int64_t n, const int32_t* qlist, int64_t nq, const int32_t* label,
                         bool check_labels, const int32_t* inv, const float* seed_r, float seed_scale,
-                        uint64_t* cand_key, int32_t* cand_pos, Counters* stats, hipStream_t s)
+                        uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
+                        uint64_t* first_key, Counters* stats, hipStream_t s)
 {
     if (nq <= 0) return;
     const unsigned g = nblocks(nq, 64);
     if (dim == 3)
         hipLaunchKernelGGL(k_cone_search<3>, dim3(g), dim3(64), 0, s, pts, nodes, L, bucket, n, qlist, nq,
-                           label, check_labels ? 1 : 0, inv, seed_r, seed_scale, cand_key, cand_pos, stats);
+                           label, check_labels ? 1 : 0, inv, seed_r, seed_scale, cand_key, cand_pos, revive,
+                           first_corr, first_key, stats);
     else
         hipLaunchKernelGGL(k_cone_search<2>, dim3(g), dim3(64), 0, s, pts, nodes, L, bucket, n, qlist, nq,
-                           label, check_labels ? 1 : 0, inv, seed_r, seed_scale, cand_key, cand_pos, stats);
+                           label, check_labels ? 1 : 0, inv, seed_r, seed_scale, cand_key, cand_pos, revive,
+                           first_corr, first_key, stats);
 }
 
-void launch_round_init(int64_t n, int32_t* label, int32_t* corr, uint64_t* corr_key,
+void launch_round_init(int64_t n, int32_t* label, int32_t* eu, int32_t* corr, uint64_t* corr_key,
                        uint64_t* comp1, uint32_t* comp2, Counters* C, hipStream_t s)
 {
     hipLaunchKernelGGL(k_round_init, dim3(nblocks(std::max<int64_t>(n, 1))), dim3(kBlock), 0, s, n, label,
-                       corr, corr_key, comp1, comp2, C);
+                       eu, corr, corr_key, comp1, comp2, C);
 }
 
 void launch_flag_research(int K, int64_t n, const uint64_t* cand_key, int32_t* list, Counters* C,
@@ -1174,8 +1209,8 @@ void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const ui
                            comp1, comp2, fixed);
         break;
     case 2:
-        hipLaunchKernelGGL(k_round_hook, dim3(g), dim3(kBlock), 0, s, n, pts, label, label2, corr,
-                           corr_key, fixed, C, eu, ev, ed2, eround, round);
+        hipLaunchKernelGGL(k_round_hook, dim3(nblocks(n, kHookBlock)), dim3(kHookBlock), 0, s, n, pts, label,
+                           label2, corr, corr_key, fixed, C, eu, ev, ed2, eround, round);
         break;
     case 3:
         hipLaunchKernelGGL(k_round_flatten, dim3(g), dim3(kBlock), 0, s, n, label, label2, comp1, comp2, C);
