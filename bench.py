@@ -20,6 +20,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 
+def profiled_traffic(args):
+    """HBM bytes per cone-search launch from the committed rocprofv3 summary of
+    this workload (FETCH_SIZE + WRITE_SIZE, each scaled by a plain copy), or
+    None when no summary covers it."""
+    tags = {(1, 10_000_000, 3): "gauss64_1e7_3d", (0, 1_000_000, 3): "uniform_1e6_3d"}
+    tag = tags.get((args.kind, args.n, args.dim))
+    if tag is None:
+        return None
+    path = os.path.join(ROOT, "profiles", f"k_cone_search_{tag}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":")[1]))
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -32,7 +49,19 @@ def main():
     ap.add_argument("--bucket", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=40_000,
                     help="points of the same distribution timed on the CPU oracle (0 = skip)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB mebibytes (scales rocprof byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        nbytes = args.copy_calibration << 20
+        src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
               "(the EMST engine runs one context per GPU)", file=sys.stderr)
@@ -81,7 +110,7 @@ def main():
     s_ms = sorted(search_ms)[len(search_ms) // 2]
     achieved = alg_bytes / (s_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-                "frac": round(achieved / 8000.0, 4), "traffic": None,
+                "frac": round(achieved / 8000.0, 4), "traffic": profiled_traffic(args),
                 "kernel": "k_cone_search", "kernel_ms": round(s_ms, 3)}
 
     cpu = None
