@@ -58,7 +58,7 @@ struct emst_ctx {
     std::vector<emst_round_trace> trace;
     float times[4] = {0, 0, 0, 0};
     int64_t stats[5] = {0, 0, 0, 0, 0};
-    float seed_scale = 1.5f;          // plain seed: multiple of the leaf cell's mean spacing
+    float seed_scale = 3.0f;          // plain seed: multiple of the leaf cell's mean spacing
     hipEvent_t ev[8] = {};
     // device buffers
     DevBuf<float> xyz_own, resc;
@@ -121,11 +121,15 @@ void release(DevBuf<T>& b)
     b.cap = 0;
 }
 
+// leaf cells hold a power of two of points in [8, 64] (the cone search maps
+// one leaf point per lane, 64 / bucket queries per pass of the wave)
 int choose_bucket(const emst_options& o, int dim)
 {
     int b = o.bucket_size;
-    if (b <= 0) b = dim == 3 ? 32 : 32;
-    return std::max(4, std::min(b, 256));
+    if (b <= 0) b = 64;
+    int r = 8;
+    while (r < b && r < 64) r <<= 1;
+    return r;
 }
 
 int alloc_all(emst_ctx* c)

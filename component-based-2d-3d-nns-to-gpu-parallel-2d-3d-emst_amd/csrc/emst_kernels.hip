@@ -643,28 +643,44 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
                 ++st_leaves;
                 st_pts += (uint32_t)cnt;
                 st_pairs += (uint32_t)cnt * (uint32_t)__popcll(needm);
-                for (int base = 0; base < cnt; base += 64) {
-                    const int m = min(64, cnt - base);
-                    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (lane < m) v = pts[s0 + base + lane];
-                    for (int j = 0; j < m; ++j) {
-                        const float qx = bcast(v.x, j), qy = bcast(v.y, j), qz = bcast(v.z, j);
-                        const uint32_t qid = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(v.w), j);
-                        const int64_t s = s0 + base + j;
-                        const float dx = qx - px, dy = qy - py, dz = qz - pz;
-                        const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
-                        const int32_t db = (int32_t)__float_as_uint(d2);
-                        bool take = need && s != p && db <= maxb && db > inner;
-                        if (check_labels && take && d2 == 0.f) take = label[s] != mylab;
-                        if (take) {
-                            uint32_t mask = cone_mask<D>(dx, dy, dz) & open;
-                            const unsigned long long key =
-                                ((unsigned long long)__float_as_uint(d2) << 32) | qid;
-                            while (mask) {
-                                const int c = __builtin_ctz(mask);
-                                mask &= mask - 1;
-                                atomicMin(&best[c][lane], key);
-                            }
+                // pair-parallel: lane = (query slot, leaf point); a chunk covers
+                // 64 / bucket needed queries x the leaf's points
+                const int j = lane & (bucket - 1);
+                const int qslot = lane / bucket;
+                const int qpc = 64 / bucket;
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (j < cnt) v = pts[s0 + j];
+                const int64_t s = s0 + j;
+                uint64_t rem = needm;
+                while (rem) {
+                    // the qslot-th remaining needed lane is this lane's query
+                    int myq = -1;
+                    for (int k = 0; k < qpc; ++k) {
+                        const int l = rem ? (int)__builtin_ctzll(rem) : -1;
+                        if (rem) rem &= rem - 1;
+                        if (k == qslot) myq = l;
+                    }
+                    const int src = myq >= 0 ? myq : 0;
+                    const float qx = __shfl(px, src), qy = __shfl(py, src), qz = __shfl(pz, src);
+                    const int32_t qmaxb = __shfl(maxb, src), qinner = __shfl(inner, src);
+                    const uint32_t qopen = (uint32_t)__shfl((int)open, src);
+                    const int32_t qpos = __shfl((int32_t)p, src);
+                    const float dx = v.x - qx, dy = v.y - qy, dz = v.z - qz;
+                    const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
+                    const int32_t db = (int32_t)__float_as_uint(d2);
+                    bool take = myq >= 0 && j < cnt && s != (int64_t)qpos && db <= qmaxb && db > qinner;
+                    if (check_labels) {
+                        const int32_t qlab = __shfl(mylab, src);
+                        if (take && d2 == 0.f) take = label[s] != qlab;
+                    }
+                    if (take) {
+                        uint32_t mask = cone_mask<D>(dx, dy, dz) & qopen;
+                        const unsigned long long key =
+                            ((unsigned long long)__float_as_uint(d2) << 32) | __float_as_uint(v.w);
+                        while (mask) {
+                            const int c = __builtin_ctz(mask);
+                            mask &= mask - 1;
+                            atomicMin(&best[c][myq], key);
                         }
                     }
                 }

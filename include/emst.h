@@ -48,12 +48,13 @@ typedef struct {
     int32_t dim;            /* point dimension: 2 or 3 */
     int32_t level_radius;   /* R of the reference cell grid (levelRadius); cells are 2R wide */
     int32_t max_rounds;     /* Boruvka round cap; the reference stops at 100 */
-    int32_t bucket_size;    /* points per leaf cell of the search hierarchy (16..64, 0 = auto) */
+    int32_t bucket_size;    /* points per leaf cell of the search hierarchy: a power of two in
+                               [8, 64], rounded up (0 = 64) */
     int32_t trace;          /* 1: record per-round forest size, length and phase times */
     int32_t device;         /* HIP device ordinal */
     int32_t radius_mode;    /* 0 plain; 1 learned seed radius (needs emst_set_radius_model) */
     float seed_scale;       /* plain-form first search radius, in mean spacings of the point's
-                               leaf cell (0 = default 1.5); changes work, never results */
+                               leaf cell (0 = default 3.0); changes work, never results */
     int32_t reserved[4];
 } emst_options;
 
