@@ -47,6 +47,7 @@ def main():
     ap.add_argument("--kind", type=int, default=1, help="0 uniform, 1 Gaussian mixture, 2 power-law")
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--bucket", type=int, default=0)
+    ap.add_argument("--plain", action="store_true", help="plain seed only (skip the learned form)")
     ap.add_argument("--cpu-sample", type=int, default=40_000,
                     help="points of the same distribution timed on the CPU oracle (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -79,35 +80,53 @@ def main():
     torch.cuda.synchronize()
 
     K = 2 * args.dim * (1 << (args.dim - 1))
-    ctx = pkg.Calculateur(dim=args.dim, bucket_size=args.bucket)
-    search_ms, total_ms = [], []
+    model_path = os.path.join(ge.PKG_DIR, "models", f"radius_{args.dim}d.npy")
+    model = np.load(model_path) if (os.path.exists(model_path) and not args.plain) else None
 
-    def step():
-        ctx.initialize_device(dev.data_ptr(), args.n, args.dim)
-        ctx.run()
-        w, m = ctx.evaluate()
-        return w, m
+    def timed(form_model):
+        ctx = pkg.Calculateur(dim=args.dim, bucket_size=args.bucket)
+        if form_model is not None:
+            ctx.set_radius_model(form_model)
+        search_ms, total_ms = [], []
 
-    for _ in range(args.warmup):
-        w, m = step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        w, m = step()
-        t = ctx.times()
-        search_ms.append(t[1])
-        total_ms.append(t[3])
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms_per_step = dt * 1e3 / args.steps
-    value = args.n * args.steps / dt / 1e6
-    assert m == args.n - 1, f"tree has {m} edges"
+        def step():
+            ctx.initialize_device(dev.data_ptr(), args.n, args.dim)
+            ctx.run()
+            return ctx.evaluate()
+
+        for _ in range(args.warmup):
+            w, m = step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            w, m = step()
+            t = ctx.times()
+            search_ms.append(t[1])
+            total_ms.append(t[3])
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        u, v, _ = ctx.getLinks()
+        ctx.close()
+        assert m == args.n - 1, f"tree has {m} edges"
+        med = lambda xs: sorted(xs)[len(xs) // 2]
+        return {"dt": dt, "w": w, "m": m, "search_ms": med(search_ms), "total_ms": med(total_ms),
+                "edges_digest": hash((u.tobytes(), v.tobytes()))}
+
+    plain = timed(None)
+    learned = timed(model) if model is not None else None
+    if learned is not None:
+        assert learned["edges_digest"] == plain["edges_digest"], "learned seeds changed the tree"
+        assert learned["w"] == plain["w"]
+    head = learned if learned is not None else plain
+    ms_per_step = head["dt"] * 1e3 / args.steps
+    value = args.n * args.steps / head["dt"] / 1e6
+    w, m = head["w"], head["m"]
 
     # dominant kernel: the round-1 cone search; algorithmic bytes per point
-    # = 8*D (own coordinates) + 4 + 8*K (one (id, d2) per cone out)
-    # + the sorted coordinates and ids read once; HIP events on the context stream.
+    # = 8*D (own coordinates) + 4 + 8*K (one (id, d2) per cone out);
+    # time from HIP events on the context stream
     alg_bytes = args.n * (8 * args.dim + 4 + 8 * K)
-    s_ms = sorted(search_ms)[len(search_ms) // 2]
+    s_ms = head["search_ms"]
     achieved = alg_bytes / (s_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                 "frac": round(achieved / 8000.0, 4), "traffic": profiled_traffic(args),
@@ -129,18 +148,24 @@ def main():
                          f"of the reference ring search (R=1), single thread, {ct:.1f} s"}
 
     out = {
-        "metric": "EMST Mpoints/s (plain form), 10^7-point 3D Gaussian mixture",
+        "metric": "EMST Mpoints/s at 10^7 points, learned-radius form (plain form in config)",
         "value": round(value, 3), "unit": "Mpoints/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (project generator, seeded)",
         "config": {"workload": f"EMST {args.n} pts {args.dim}D {names.get(args.kind)} seed {args.seed}",
-                   "n": args.n, "dim": args.dim, "form": "plain", "edges": int(m),
-                   "weight": w, "gpu_total_ms_median": round(sorted(total_ms)[len(total_ms) // 2], 3)},
+                   "n": args.n, "dim": args.dim, "form": "learned" if learned is not None else "plain",
+                   "edges": int(m), "weight": w,
+                   "gpu_total_ms_median": round(head["total_ms"], 3),
+                   "plain_mpts": round(args.n * args.steps / plain["dt"] / 1e6, 3),
+                   "plain_search_ms": round(plain["search_ms"], 3),
+                   "learned_mpts": round(args.n * args.steps / learned["dt"] / 1e6, 3) if learned else None,
+                   "learned_search_ms": round(learned["search_ms"], 3) if learned else None,
+                   "learned_vs_plain": round(plain["dt"] / learned["dt"], 3) if learned else None,
+                   "same_tree": learned is not None},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    ctx.close()
     print(json.dumps(out))
 
 

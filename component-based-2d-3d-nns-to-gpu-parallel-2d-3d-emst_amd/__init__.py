@@ -25,7 +25,10 @@ EXPORTS = [
     "emst_run", "emst_step", "emst_get_edges", "emst_get_weight", "emst_get_labels",
     "emst_get_rescaled", "emst_get_trace", "emst_get_candidates", "emst_get_cell_grid",
     "emst_get_times", "emst_get_search_stats", "emst_synth", "emst_version",
+    "emst_set_radius_model", "emst_get_radius_samples", "emst_mlp_train", "emst_mlp_eval", "emst_mlp_grad",
 ]
+
+MLP_PARAMS = 4818
 
 
 class EmstError(RuntimeError):
@@ -84,6 +87,11 @@ def lib():
     L.emst_get_times.argtypes = [P, P, P]
     L.emst_get_search_stats.argtypes = [P, P, P]
     L.emst_synth.argtypes = [i32, i64, i32, ctypes.c_uint64, P]
+    L.emst_set_radius_model.argtypes = [P, P]
+    L.emst_get_radius_samples.argtypes = [P, P, P, P]
+    L.emst_mlp_train.argtypes = [i32, P, P, i64, i32, i32, ctypes.c_float, ctypes.c_uint64, P, P]
+    L.emst_mlp_eval.argtypes = [i32, P, P, i64, P]
+    L.emst_mlp_grad.argtypes = [i32, P, P, P, i64, P, P]
     L.emst_version.restype = ctypes.c_char_p
     _lib = L
     return L
@@ -216,11 +224,51 @@ class Calculateur:
                                              _p(cpts)), "cell grid")
         return ext, cell_of, start, cpts[: self.n]
 
+    def set_radius_model(self, model):
+        """Learned seed radius for the next runs (None = plain seed)."""
+        if model is None:
+            self._chk(self._L.emst_set_radius_model(self._c, None), "radius model")
+            return
+        m = np.ascontiguousarray(model, dtype=np.float32)
+        assert m.size == MLP_PARAMS
+        self._chk(self._L.emst_set_radius_model(self._c, _p(m)), "radius model")
+
+    def radius_samples(self):
+        """(features n x 8, targets n) of the last cone search."""
+        m = ctypes.c_int64(0)
+        self._chk(self._L.emst_get_radius_samples(self._c, None, None, ctypes.byref(m)), "samples")
+        X = np.empty((m.value, 8), np.float32)
+        T = np.empty(m.value, np.float32)
+        self._chk(self._L.emst_get_radius_samples(self._c, _p(X), _p(T), ctypes.byref(m)), "samples")
+        return X, T
+
     def times(self):
         t = (ctypes.c_float * 4)()
         k = ctypes.c_int32(4)
         self._chk(self._L.emst_get_times(self._c, t, ctypes.byref(k)), "times")
         return list(t)[: k.value]
+
+
+def train_radius_model(X, T, epochs=8, batch=1024, lr=2e-3, seed=1, device=0):
+    """Train the seed-radius MLP on the GPU; returns (model, loss per epoch)."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    T = np.ascontiguousarray(T, dtype=np.float32)
+    model = np.empty(MLP_PARAMS, np.float32)
+    losses = np.empty(epochs, np.float32)
+    rc = lib().emst_mlp_train(device, _p(X), _p(T), len(T), epochs, batch, ctypes.c_float(lr), seed,
+                              _p(model), _p(losses))
+    if rc:
+        raise EmstError(rc, "emst_mlp_train")
+    return model, losses
+
+
+def mlp_eval(model, X, device=0):
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    out = np.empty(len(X), np.float32)
+    rc = lib().emst_mlp_eval(device, _p(np.ascontiguousarray(model, np.float32)), _p(X), len(X), _p(out))
+    if rc:
+        raise EmstError(rc, "emst_mlp_eval")
+    return out
 
 
 def _stats_names():

@@ -69,6 +69,8 @@ struct emst_ctx {
     DevBuf<Node> nodes;
     DevBuf<float> ed2;
     DevBuf<double> partial;
+    DevBuf<float> model, feat, pred, seed;     // learned seed radius
+    bool learned = false;
     Params* P = nullptr;
     Counters* C = nullptr;
 };
@@ -207,8 +209,18 @@ int build(emst_ctx* c)
     launch_round_init(n, c->label.p, c->eu.p, c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->C, s);
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ev[1], s));
+    const float* seed_r = nullptr;
+    if (c->learned) {
+        int rc2;
+        if ((rc2 = ensure(c, c->feat, (size_t)n * 8)) || (rc2 = ensure(c, c->pred, (size_t)n)) ||
+            (rc2 = ensure(c, c->seed, (size_t)n)))
+            return rc2;
+        launch_radius_features(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, c->P, c->feat.p, s);
+        launch_radius_seed(n, c->model.p, c->feat.p, c->pred.p, c->seed.p, s);
+        seed_r = c->seed.p;
+    }
     launch_cone_search(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, nullptr, n, c->label.p, false,
-                       c->inv.p, nullptr, c->seed_scale, c->cand_key.p, c->cand_pos.p, nullptr, c->corr.p,
+                       c->inv.p, seed_r, c->seed_scale, c->cand_key.p, c->cand_pos.p, nullptr, c->corr.p,
                        c->corr_key.p, c->C, s);
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ev[2], s));
@@ -398,6 +410,7 @@ int emst_destroy(emst_ctx* c)
     release(c->label); release(c->label2); release(c->corr); release(c->fixed); release(c->list);
     release(c->eu); release(c->ev_); release(c->eround); release(c->geo); release(c->nodes);
     release(c->ed2); release(c->partial);
+    release(c->model); release(c->feat); release(c->pred); release(c->seed);
     if (c->P) hipFree(c->P);
     if (c->C) hipFree(c->C);
     for (auto& e : c->ev) if (e) hipEventDestroy(e);
@@ -651,6 +664,45 @@ int emst_get_times(emst_ctx* c, float* times, int32_t* n_times)
     const int32_t k = std::min<int32_t>(*n_times, 4);
     for (int i = 0; i < k; ++i) times[i] = c->times[i];
     *n_times = k;
+    return EMST_OK;
+}
+
+int emst_set_radius_model(emst_ctx* c, const float* model)
+{
+    if (!c) return EMST_E_ARG;
+    HIPCHK(c, hipSetDevice(c->opt.device));
+    if (!model) {
+        c->learned = false;
+        c->built = false;
+        return EMST_OK;
+    }
+    int rc = ensure(c, c->model, EMST_MLP_PARAMS);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->model.p, model, sizeof(float) * EMST_MLP_PARAMS, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->learned = true;
+    c->built = false;       // the next run searches with the new seeds
+    return EMST_OK;
+}
+
+int emst_get_radius_samples(emst_ctx* c, float* feat, float* target, int64_t* m)
+{
+    if (!m) return fail(c, EMST_E_ARG, "m must not be NULL");
+    int rc = ensure_built(c);
+    if (rc) return rc;
+    const int64_t n = c->n;
+    if (!feat && !target) { *m = n; return EMST_OK; }
+    if (*m < n) { *m = n; return fail(c, EMST_E_ARG, "sample arrays too small"); }
+    *m = n;
+    if (n == 0) return EMST_OK;
+    if ((rc = ensure(c, c->feat, (size_t)n * 8)) || (rc = ensure(c, c->pred, (size_t)n))) return rc;
+    hipStream_t s = c->stream;
+    launch_radius_features(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, c->P, c->feat.p, s);
+    launch_radius_targets(c->K, n, c->cand_key.p, c->feat.p, c->pred.p, s);
+    LAUNCHCHK(c);
+    if (feat) HIPCHK(c, hipMemcpyAsync(feat, c->feat.p, sizeof(float) * n * 8, hipMemcpyDeviceToHost, s));
+    if (target) HIPCHK(c, hipMemcpyAsync(target, c->pred.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
     return EMST_OK;
 }
 

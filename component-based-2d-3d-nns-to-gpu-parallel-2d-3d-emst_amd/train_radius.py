@@ -1,0 +1,107 @@
+"""Train the seed-radius MLP from the plain method's own searches.
+
+Protocol: 2^20 (features, log true radius) pairs per dimension, logged from
+plain cone searches on the uniform clouds of the reference-path configs
+(2D: 10^4 points seed 0 and 10^6 points seed 1; 3D: 10^6 points seed 2),
+trained on the GPU with the library's forward/backward kernels
+(emst_mlp_train).  The seed offset (added to the predicted log radius) is
+then picked by timing the cone search on a held-out uniform cloud.
+
+    python train_radius.py --out <dir>      (needs a GPU; writes radius_2d.npy,
+                                             radius_3d.npy and train_radius.json)
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def load_pkg():
+    import importlib.util
+    if "emst_amd" in sys.modules:
+        return sys.modules["emst_amd"]
+    spec = importlib.util.spec_from_file_location("emst_amd", os.path.join(HERE, "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["emst_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+TRAIN_CLOUDS = {2: [(0, 10_000, 0), (0, 1_000_000, 1)], 3: [(0, 1_000_000, 2)]}
+VALID_CLOUD = {2: (0, 2_000_000, 101), 3: (0, 2_000_000, 102)}
+OFFSETS = (-0.25, 0.0, 0.25, 0.5, 0.75, 1.0)
+
+
+def samples(pkg, dim):
+    Xs, Ts = [], []
+    for kind, n, seed in TRAIN_CLOUDS[dim]:
+        pts = pkg.synth(kind, n, dim, seed)
+        with pkg.Calculateur(dim=dim) as c:
+            c.initialize(pts)
+            X, T = c.radius_samples()
+        keep = T > -19.0
+        Xs.append(X[keep])
+        Ts.append(T[keep])
+    X = np.concatenate(Xs)
+    T = np.concatenate(Ts)
+    rng = np.random.default_rng(dim)
+    idx = rng.choice(len(T), size=min(len(T), 1 << 20), replace=False)
+    return X[idx], T[idx]
+
+
+def search_ms(pkg, dim, pts, model, reps=3):
+    with pkg.Calculateur(dim=dim) as c:
+        c.initialize(pts)
+        if model is not None:
+            c.set_radius_model(model)
+        best = None
+        for _ in range(reps):
+            c.initialize(pts)
+            c.run()
+            t = c.times()[1]
+            best = t if best is None else min(best, t)
+    return best
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=os.path.join(HERE, "models"))
+    ap.add_argument("--epochs", type=int, default=8)
+    args = ap.parse_args()
+    os.makedirs(args.out, exist_ok=True)
+    pkg = load_pkg()
+    report = {}
+    for dim in (2, 3):
+        t0 = time.time()
+        X, T = samples(pkg, dim)
+        print(f"{dim}D: {len(T)} pairs, target mean {T.mean():.3f} var {T.var():.4f}", flush=True)
+        model, losses = pkg.train_radius_model(X, T, epochs=args.epochs, batch=1024, lr=2e-3, seed=dim)
+        print(f"{dim}D: loss per epoch {np.round(losses, 4).tolist()}", flush=True)
+        kind, n, seed = VALID_CLOUD[dim]
+        vpts = pkg.synth(kind, n, dim, seed)
+        plain = search_ms(pkg, dim, vpts, None)
+        timings = {}
+        for off in OFFSETS:
+            m = model.copy()
+            m[-1] = off
+            timings[off] = search_ms(pkg, dim, vpts, m)
+            print(f"{dim}D: offset {off:+.2f}: cone search {timings[off]:.2f} ms (plain {plain:.2f})", flush=True)
+        best = min(timings, key=timings.get)
+        model[-1] = best
+        np.save(os.path.join(args.out, f"radius_{dim}d.npy"), model)
+        report[f"{dim}d"] = {"pairs": int(len(T)), "target_var": float(T.var()),
+                             "loss_per_epoch": [float(x) for x in losses], "offset": best,
+                             "valid_search_ms": {"plain": plain, **{str(k): v for k, v in timings.items()}},
+                             "seconds": time.time() - t0}
+    with open(os.path.join(args.out, "train_radius.json"), "w") as f:
+        json.dump(report, f, indent=1)
+    print(json.dumps(report))
+
+
+if __name__ == "__main__":
+    main()

@@ -167,6 +167,48 @@ int emst_get_times(emst_ctx* ctx, float* times, int32_t* n_times);
  * n_stats: capacity in, count out. */
 int emst_get_search_stats(emst_ctx* ctx, int64_t* stats, int32_t* n_stats);
 
+/* ---- learned seed radius (no counterpart in the reference) -----------------
+ * A multilayer perceptron 8 -> 64 -> 64 -> 1 (ReLU; BF16 weights and
+ * activations, FP32 accumulation on the matrix cores) predicts, per point,
+ * log(R / s0): R = radius within which all the point's non-empty cones have
+ * their nearest point, s0 = mean point spacing of its leaf cell.  The cone
+ * search starts its iterative deepening at s0 * exp(prediction + offset); the
+ * result is the same for any seed, only the work changes.
+ * Model: EMST_MLP_PARAMS floats, W1[8][64] | b1[64] | W2[64][64] | b2[64] |
+ * w3[64] | b3 | feature mean[8] | feature 1/std[8] | seed offset.
+ * Features (per point): log s0; log(s1/s0), log(s2/s1), log(s3/s2) for the
+ * 1st..3rd enclosing cells; log aspect of the leaf box; distance to the leaf
+ * box boundary / s0; log(1 + distance to the cloud's box boundary / s0);
+ * log(1 + distance to the 2nd enclosing cell's boundary / s0). */
+#define EMST_MLP_PARAMS 4818
+
+/* Use `model` for the seed radius of this context's cone searches (copied);
+ * NULL returns to the plain seed (seed_scale x leaf spacing). */
+int emst_set_radius_model(emst_ctx* ctx, const float* model);
+
+/* Training pairs from the context's current points: features (m x 8) and
+ * targets (m) of every point, from its last cone search.  Pass NULL arrays to
+ * query *m; otherwise *m is the capacity in, count out. */
+int emst_get_radius_samples(emst_ctx* ctx, float* feat, float* target, int64_t* m);
+
+/* Trains a model on the GPU with the hand-written forward/backward kernels:
+ * mean squared error on targets, Adam (lr, betas 0.9/0.999), `epochs` passes
+ * over m samples in shuffled minibatches of `batch` (a multiple of 64), He
+ * initialisation from `seed`, feature normalisation from the data.  model:
+ * EMST_MLP_PARAMS floats out; loss_per_epoch (optional): mean squared error
+ * of each epoch. */
+int emst_mlp_train(int32_t device, const float* feat, const float* target, int64_t m, int32_t epochs,
+                   int32_t batch, float lr, uint64_t seed, float* model, float* loss_per_epoch);
+
+/* Forward pass on the GPU: out[i] = prediction for feat[i] (without offset). */
+int emst_mlp_eval(int32_t device, const float* model, const float* feat, int64_t m, float* out);
+
+/* Gradient of sum_i (prediction_i - target_i)^2 with respect to the model's
+ * trainable parameters (first 4801 floats; the rest are returned as 0), and
+ * that sum, from the GPU kernels. */
+int emst_mlp_grad(int32_t device, const float* model, const float* feat, const float* target, int64_t m,
+                  float* grad, double* loss);
+
 /* Synthetic clouds, identical on every host (no counterpart in the reference;
  * used by the benchmark and tests).  SplitMix64-seeded xoshiro128+,
  * u = (x >> 8) * 2^-24.  kind: 0 uniform [0,1)^dim; 1 Gaussian mixture
