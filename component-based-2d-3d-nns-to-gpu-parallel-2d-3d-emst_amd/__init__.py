@@ -272,12 +272,12 @@ def mlp_eval(model, X, device=0):
 
 
 def _stats_names():
-    return ["packets", "nodes", "leaves", "leaf_points", "pairs"]
+    return ["packets", "nodes", "leaves", "leaf_points", "pairs", "group_points", "grouped"]
 
 
 def search_stats(calc):
-    t = (ctypes.c_int64 * 5)()
-    k = ctypes.c_int32(5)
+    t = (ctypes.c_int64 * 7)()
+    k = ctypes.c_int32(7)
     calc._chk(calc._L.emst_get_search_stats(calc._c, t, ctypes.byref(k)), "stats")
     return dict(zip(_stats_names(), list(t)[: k.value]))
 

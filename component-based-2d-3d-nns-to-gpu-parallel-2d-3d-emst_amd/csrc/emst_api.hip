@@ -6,6 +6,7 @@
 // input points, like the reference's cuda_duree span plus its cell build.
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -57,7 +58,7 @@ struct emst_ctx {
     uint32_t zero_or_empty = 0;
     std::vector<emst_round_trace> trace;
     float times[4] = {0, 0, 0, 0};
-    int64_t stats[5] = {0, 0, 0, 0, 0};
+    int64_t stats[7] = {0, 0, 0, 0, 0, 0, 0};
     float seed_scale = 3.0f;          // plain seed: multiple of the leaf cell's mean spacing
     hipEvent_t ev[8] = {};
     // device buffers
@@ -71,6 +72,12 @@ struct emst_ctx {
     DevBuf<double> partial;
     DevBuf<float> model, feat, pred, seed;     // learned seed radius
     bool learned = false;
+    // coincident-point groups (DupState)
+    DevBuf<int32_t> dup_rep, dup_stamp, dup_list, dup_runs;
+    DevBuf<uint32_t> dup_m2;
+    bool no_grouping = false;         // set when distinct points met at float distance 0
+    bool grouped = false;
+    int epoch = 0;
     Params* P = nullptr;
     Counters* C = nullptr;
 };
@@ -167,6 +174,11 @@ int alloc_all(emst_ctx* c)
     ENS(ed2, n);
     ENS(partial, kWeightBlocks);
     ENS(geo, 16);
+    ENS(dup_rep, n);
+    ENS(dup_stamp, n);
+    ENS(dup_list, n);
+    ENS(dup_runs, n);
+    ENS(dup_m2, n);
 #undef ENS
     return EMST_OK;
 }
@@ -188,6 +200,24 @@ float elapsed(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
+static_assert(offsetof(Counters, pad2) + 3 * sizeof(uint32_t) - offsetof(Counters, dup_runs) ==
+                  8 * sizeof(uint32_t),
+              "the dup counters are cleared as 8 consecutive words");
+static_assert(offsetof(Counters, zero_or_empty) == offsetof(Counters, research) + sizeof(uint32_t),
+              "research and zero_or_empty are cleared together");
+
+DupState dup_state(emst_ctx* c)
+{
+    DupState d{};
+    if (c->no_grouping) return d;
+    d.rep = c->dup_rep.p;
+    d.stamp = c->dup_stamp.p;
+    d.m2 = c->dup_m2.p;
+    d.list = c->dup_list.p;
+    d.epoch = c->epoch;
+    return d;
+}
+
 // bbox, rescale, Morton sort, hierarchy, round-1 cone search
 int build(emst_ctx* c)
 {
@@ -205,7 +235,15 @@ int build(emst_ctx* c)
     radix_sort(c->keys0.p, c->vals0.p, c->keys1.p, c->vals1.p, n, c->dim == 3 ? 63 : 64, true,
                c->hist.p, c->scan_tmp.p, s);
     launch_gather_sorted(c->resc.p, c->vals0.p, n, c->dim, c->pts.p, c->inv.p, s);
-    launch_build_tree(c->pts.p, n, c->bucket, c->L, c->nodes.p, s);
+    // coincident-point groups: dup_runs .. pad2 start at zero
+    HIPCHK(c, hipMemsetAsync(&c->C->dup_runs, 0, 8 * sizeof(uint32_t), s));
+    c->epoch = 1;
+    const DupState dup = dup_state(c);
+    if (dup.rep) {
+        launch_dup_groups(c->keys0.p, c->pts.p, n, c->dup_rep.p, c->dup_runs.p, c->dup_list.p, c->C, s);
+        HIPCHK(c, hipMemsetAsync(c->dup_stamp.p, 0, sizeof(int32_t) * (size_t)n, s));
+    }
+    launch_build_tree(c->pts.p, n, c->bucket, c->L, c->nodes.p, dup.rep, c->C, s);
     launch_round_init(n, c->label.p, c->eu.p, c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->C, s);
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ev[1], s));
@@ -221,20 +259,31 @@ int build(emst_ctx* c)
     }
     launch_cone_search(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, nullptr, n, c->label.p, false,
                        c->inv.p, seed_r, c->seed_scale, c->cand_key.p, c->cand_pos.p, nullptr, c->corr.p,
-                       c->corr_key.p, c->C, s);
+                       c->corr_key.p, c->C, true, dup, s);
+    launch_dup_fill(c->dim, n, c->label.p, c->pts.p, c->inv.p, c->cand_key.p, c->cand_pos.p, nullptr, c->corr.p,
+                    c->corr_key.p, c->C, dup, s);
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ev[2], s));
-    launch_flag_research(c->K, n, c->cand_key.p, c->list.p, c->C, s);
+    launch_flag_research(c->K, n, c->cand_key.p, c->list.p, dup.rep, c->C, s);
     LAUNCHCHK(c);
     Counters h{};
     int rc = read_counters(c, &h);
     if (rc) return rc;
+    if (h.grouped && h.underflow) {
+        // distinct points at float distance 0: the groups would merge
+        // candidates that the per-point rule keeps apart; search ungrouped
+        c->no_grouping = true;
+        return build(c);
+    }
+    c->grouped = h.grouped != 0u;
     c->zero_or_empty = h.zero_or_empty;
     c->stats[0] = (int64_t)h.packets;
     c->stats[1] = (int64_t)h.nodes;
     c->stats[2] = (int64_t)h.leaves;
     c->stats[3] = (int64_t)h.leaf_points;
     c->stats[4] = (int64_t)h.pairs;
+    c->stats[5] = (int64_t)h.dup_points;
+    c->stats[6] = c->grouped ? 1 : 0;
     c->times[0] = elapsed(c->ev[0], c->ev[1]);
     c->times[1] = elapsed(c->ev[1], c->ev[2]);
     c->built = true;
@@ -276,18 +325,25 @@ int round_once(emst_ctx* c, int32_t* finished)
     int64_t searched = c->rounds == 0 ? n : 0;
     if (tr) HIPCHK(c, hipEventRecord(c->ev[3], s));
     if (c->rounds > 0 && c->zero_or_empty > 0) {
-        HIPCHK(c, hipMemsetAsync(&c->C->research, 0, sizeof(uint32_t), s));
-        launch_flag_research(c->K, n, c->cand_key.p, c->list.p, c->C, s);
+        // research and zero_or_empty restart for this round
+        HIPCHK(c, hipMemsetAsync(&c->C->research, 0, 2 * sizeof(uint32_t), s));
+        DupState dup{};
+        if (c->grouped) dup = dup_state(c);
+        launch_flag_research(c->K, n, c->cand_key.p, c->list.p, dup.rep, c->C, s);
         LAUNCHCHK(c);
         Counters h{};
         int rc = read_counters(c, &h);
         if (rc) return rc;
-        if (h.research > 0)
+        if (h.research > 0) {
+            dup.epoch = ++c->epoch;
             launch_cone_search(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, c->list.p, h.research,
                                c->label.p, true, c->inv.p, nullptr, c->seed_scale, c->cand_key.p,
-                               c->cand_pos.p, c->corr.p, nullptr, nullptr, nullptr, s);
+                               c->cand_pos.p, c->corr.p, nullptr, nullptr, c->C, false, dup, s);
+            launch_dup_fill(c->dim, n, c->label.p, c->pts.p, c->inv.p, c->cand_key.p, c->cand_pos.p, c->corr.p,
+                            nullptr, nullptr, c->C, dup, s);
+        }
         LAUNCHCHK(c);
-        searched = h.research;
+        searched = h.zero_or_empty;
     }
     const int round = c->rounds + 1;
     for (int ph = 0; ph < 4; ++ph) {
@@ -454,6 +510,7 @@ int emst_set_points(emst_ctx* c, const float* xyz, int64_t n, int32_t dim, int32
     }
     c->have_points = true;
     c->built = false;
+    c->no_grouping = false;
     c->finished = false;
     c->rounds = 0;
     c->edges = 0;
@@ -709,7 +766,7 @@ int emst_get_radius_samples(emst_ctx* c, float* feat, float* target, int64_t* m)
 int emst_get_search_stats(emst_ctx* c, int64_t* stats, int32_t* n_stats)
 {
     if (!c || !stats || !n_stats) return EMST_E_ARG;
-    const int32_t k = std::min<int32_t>(*n_stats, 5);
+    const int32_t k = std::min<int32_t>(*n_stats, 7);
     for (int i = 0; i < k; ++i) stats[i] = c->stats[i];
     *n_stats = k;
     return EMST_OK;

@@ -352,18 +352,130 @@ __global__ __launch_bounds__(kBlock) void k_gather_sorted(const float* __restric
     inv[o] = (int32_t)s;
 }
 
+// ------------------------------------------------ coincident-point groups
+// Identical points share a Morton key, so each group lies inside one run of
+// equal keys; the sort is stable over input order, so within a run positions
+// ascend with input id and a group's first position holds its smallest id.
+constexpr int kMixedRunMax = 1024;   // longest run holding >= 2 locations that is split
+
+__device__ __forceinline__ bool same_xyz(const float4& a, const float4& b)
+{
+    return a.x == b.x && a.y == b.y && a.z == b.z;
+}
+
+// append the lanes holding `take` to list, one atomic per wave
+__device__ __forceinline__ void wave_append(bool take, int32_t v, int32_t* list, uint32_t* counter)
+{
+    const uint64_t m = __ballot(take);
+    if (m == 0ull) return;
+    const int leader = __builtin_ctzll(m);
+    uint32_t base = 0;
+    if (__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, leader);
+    if (take) list[base + (uint32_t)__popcll(m & lanemask_lt())] = v;
+}
+
+// rep[] = -1 everywhere; the first position of every run of >= 2 equal keys is queued
+__global__ __launch_bounds__(kBlock) void k_dup_runs(const uint64_t* __restrict__ keys, int64_t n,
+                                                    int32_t* __restrict__ rep, int32_t* __restrict__ runs,
+                                                    Counters* __restrict__ C)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    rep[i] = -1;
+    const uint64_t k = keys[i];
+    const bool head = i == 0 || keys[i - 1] != k;
+    if (head && i + 1 < n && keys[i + 1] == k) runs[atomicAdd(&C->dup_runs, 1u)] = (int32_t)i;
+}
+
+// One wave per run (grid-stride over the queued runs).  A run at one location
+// is one group; a run over several locations (distinct points closer than a
+// Morton cell) of at most kMixedRunMax points is split by pairwise
+// comparison; a longer one sets dup_overflow, which turns grouping off.
+__global__ __launch_bounds__(kBlock) void k_dup_group(const uint64_t* __restrict__ keys,
+                                                     const float4* __restrict__ pts, int64_t n,
+                                                     int32_t* __restrict__ rep,
+                                                     const int32_t* __restrict__ runs,
+                                                     int32_t* __restrict__ list, Counters* __restrict__ C)
+{
+    const int lane = __lane_id();
+    const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
+    const int64_t nruns = C->dup_runs;
+    for (int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + threadIdx.x / 64; w < nruns; w += nwaves) {
+        const int64_t s = runs[w];
+        const uint64_t key = keys[s];
+        int64_t e = s + 1;
+        for (int64_t b = s + 1;; b += 64) {
+            const int64_t i = b + lane;
+            const uint64_t m = __ballot(i >= n || keys[i] != key);
+            if (m) {
+                e = b + __builtin_ctzll(m);
+                break;
+            }
+        }
+        const float4 head = pts[s];
+        bool mixed = false;
+        for (int64_t b = s; b < e; b += 64) {
+            const int64_t i = b + lane;
+            if (i < e) mixed |= !same_xyz(pts[i], head);
+        }
+        if (__ballot(mixed) == 0ull) {
+            for (int64_t b = s; b < e; b += 64) {
+                const int64_t i = b + lane;
+                if (i < e) rep[i] = (int32_t)s;
+                wave_append(i < e, (int32_t)i, list, &C->dup_points);
+            }
+        } else if (e - s <= kMixedRunMax) {
+            for (int64_t b = s; b < e; b += 64) {
+                const int64_t i = b + lane;
+                const float4 me = i < e ? pts[i] : head;
+                int64_t first = -1;
+                int twins = 0;
+                for (int64_t j = s; j < e; ++j) {
+                    if (same_xyz(pts[j], me)) {
+                        if (first < 0) first = j;
+                        ++twins;
+                    }
+                }
+                const bool grp = i < e && twins >= 2;
+                if (grp) rep[i] = (int32_t)first;
+                wave_append(grp, (int32_t)i, list, &C->dup_points);
+            }
+        } else if (lane == 0) {
+            atomicOr(&C->dup_overflow, 1u);
+        }
+    }
+}
+
+__global__ void k_dup_finish(Counters* C)
+{
+    C->grouped = (C->dup_points > 0u && C->dup_overflow == 0u) ? 1u : 0u;
+}
+
 // ---------------------------------------------------------------- tree
+// A leaf box covers the leaf's candidates: while coincident-point grouping is
+// on, group members are never candidates (their representative, the group's
+// smallest id, wins every tie at their common location), so they are left out
+// and a leaf of members only keeps the empty box (+inf, -inf), which no node
+// test accepts.
 __global__ __launch_bounds__(kBlock) void k_leaves(const float4* __restrict__ pts, int64_t n,
-                                                  int bucket, int64_t L, Node* __restrict__ nodes)
+                                                  int bucket, int64_t L, Node* __restrict__ nodes,
+                                                  const int32_t* __restrict__ rep,
+                                                  const Counters* __restrict__ C)
 {
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= L) return;
+    const bool grouped = rep != nullptr && C->grouped != 0u;
     Node nd;
     for (int k = 0; k < 3; ++k) { nd.lo[k] = INFINITY; nd.hi[k] = -INFINITY; }
     nd.pad[0] = nd.pad[1] = 0.f;
     const int64_t s0 = b * bucket;
     const int64_t s1 = min(n, s0 + bucket);
     for (int64_t s = s0; s < s1; ++s) {
+        if (grouped) {
+            const int32_t r = rep[s];
+            if (r >= 0 && r != (int32_t)s) continue;
+        }
         const float4 p = pts[s];
         nd.lo[0] = fminf(nd.lo[0], p.x); nd.hi[0] = fmaxf(nd.hi[0], p.x);
         nd.lo[1] = fminf(nd.lo[1], p.y); nd.hi[1] = fmaxf(nd.hi[1], p.y);
@@ -456,9 +568,15 @@ __device__ __forceinline__ uint32_t cone_mask(float dx, float dy, float dz)
 // reachable open slabs, and its farthest point lies beyond `inner` (points
 // with d2 <= inner were examined in an earlier pass).  Bounds are int32 d2
 // bit patterns: -1 = cone closed, INT_MAX = open without candidate.
+//
+// Faces are strict where the cone is: a '-' face holds d_a < 0 only, so the
+// box must reach below the query on that axis; a '+' face holds d_a > 0 or
+// the zero vector, and with strict_plus (coincident points never candidates)
+// d_a > 0 only.
 template <int D>
 __device__ __forceinline__ bool node_needed(const Node& nd, float px, float py, float pz,
-                                            const int32_t* bound, int32_t maxb, int32_t inner)
+                                            const int32_t* bound, int32_t maxb, int32_t inner,
+                                            bool strict_plus)
 {
     float l[3], h[3], m[3], u[3];
     bool fp[3], fn[3];
@@ -501,14 +619,10 @@ __device__ __forceinline__ bool node_needed(const Node& nd, float px, float py, 
         } else {
             bm = max(fp[o1] ? b[0] : -1, fn[o1] ? b[1] : -1);
         }
-        need |= (reach >= o) && ((int32_t)__float_as_uint(lb) <= bm);
+        const bool open_side = (neg || strict_plus) ? reach > 0.f : true;
+        need |= open_side && (reach >= o) && ((int32_t)__float_as_uint(lb) <= bm);
     }
     return need;
-}
-
-__device__ __forceinline__ float bcast(float v, int j)
-{
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
 }
 
 // best key -> int32 bound pattern (empty cone = INT_MAX)
@@ -543,7 +657,8 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
                                                    int32_t* __restrict__ revive,
                                                    int32_t* __restrict__ first_corr,
                                                    uint64_t* __restrict__ first_key,
-                                                   Counters* __restrict__ stats)
+                                                   Counters* __restrict__ ctr, int collect_stats,
+                                                   DupState dup)
 {
     using C = ConeCfg<D>;
     __shared__ unsigned long long best[C::K][64];
@@ -553,8 +668,19 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
     __shared__ unsigned long long stk_mask[64];
     const int lane = __lane_id();
     const int64_t qi = (int64_t)blockIdx.x * 64 + lane;
-    const bool active = qi < nq;
+    bool active = qi < nq;
     const int64_t p = active ? (qlist ? (int64_t)qlist[qi] : qi) : 0;
+    // grouping on: group members do not search (k_dup_fill gives them their
+    // representative's cones) and no point takes a candidate at its own
+    // location; otherwise only the point itself is skipped
+    const bool grouped = dup.rep != nullptr && ctr->grouped != 0u;
+    bool is_rep = false;
+    if (grouped && active) {
+        const int32_t r = dup.rep[p];
+        active = r < 0 || r == (int32_t)p;
+        is_rep = r == (int32_t)p;
+    }
+    bool underflow = false;
     const float4 me = pts[p];
     const float px = me.x, py = me.y, pz = me.z;
     const int32_t mylab = (check_labels && active) ? label[p] : 0;
@@ -632,7 +758,7 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
             if (pver == ver) {
                 need = (stk_mask[sp] >> lane) & 1ull;
             } else {
-                need = open != 0u && node_needed<D>(nd, px, py, pz, bound, maxb, inner);
+                need = open != 0u && node_needed<D>(nd, px, py, pz, bound, maxb, inner, grouped);
                 ++st_nodes;
             }
             const uint64_t needm = __ballot(need);
@@ -668,7 +794,12 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
                     const float dx = v.x - qx, dy = v.y - qy, dz = v.z - qz;
                     const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
                     const int32_t db = (int32_t)__float_as_uint(d2);
-                    bool take = myq >= 0 && j < cnt && s != (int64_t)qpos && db <= qmaxb && db > qinner;
+                    const bool here = dx == 0.f && dy == 0.f && dz == 0.f;
+                    bool take = myq >= 0 && j < cnt && (grouped ? !here : s != (int64_t)qpos) &&
+                                db <= qmaxb && db > qinner;
+                    // distinct points at float distance 0 (underflow) break the
+                    // grouping's premise; the caller then searches ungrouped
+                    if (grouped && take && d2 == 0.f) underflow = true;
                     if (check_labels) {
                         const int32_t qlab = __shfl(mylab, src);
                         if (take && d2 == 0.f) take = label[s] != qlab;
@@ -690,9 +821,9 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
                 const int a = 2 * node, b = 2 * node + 1;
                 const Node boxa = nodes[a], boxb = nodes[b];
                 const unsigned long long ma =
-                    __ballot(open != 0u && node_needed<D>(boxa, px, py, pz, bound, maxb, inner));
+                    __ballot(open != 0u && node_needed<D>(boxa, px, py, pz, bound, maxb, inner, grouped));
                 const unsigned long long mb =
-                    __ballot(open != 0u && node_needed<D>(boxb, px, py, pz, bound, maxb, inner));
+                    __ballot(open != 0u && node_needed<D>(boxb, px, py, pz, bound, maxb, inner, grouped));
                 st_nodes += 2;
                 const bool ua = ma != 0ull, ub = mb != 0ull;
                 const bool afirst = cdist(boxa) <= cdist(boxb);
@@ -717,15 +848,20 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
         inner = cap;
         r2 *= 4.f;
     }
-    if (stats && lane == 0) {
-        atomicAdd(&stats->packets, 1ull);
-        atomicAdd(&stats->nodes, (unsigned long long)st_nodes);
-        atomicAdd(&stats->leaves, (unsigned long long)st_leaves);
-        atomicAdd(&stats->leaf_points, (unsigned long long)st_pts);
-        atomicAdd(&stats->pairs, (unsigned long long)st_pairs);
+    if (collect_stats && lane == 0) {
+        atomicAdd(&ctr->packets, 1ull);
+        atomicAdd(&ctr->nodes, (unsigned long long)st_nodes);
+        atomicAdd(&ctr->leaves, (unsigned long long)st_leaves);
+        atomicAdd(&ctr->leaf_points, (unsigned long long)st_pts);
+        atomicAdd(&ctr->pairs, (unsigned long long)st_pairs);
     }
+    if (__ballot(underflow) != 0ull && lane == 0) atomicOr(&ctr->underflow, 1u);
     if (!active) return;
     if (revive && revive[p] == -2) revive[p] = -1;
+    if (is_rep) {
+        dup.stamp[p] = dup.epoch;
+        dup.m2[p] = 0xffffffffu;
+    }
     // cone-major layout [K][n]: lanes write consecutive words
     uint64_t kmin = kEmptyKey;
     int32_t pmin = -1;
@@ -741,6 +877,83 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
     if (first_corr) {
         first_corr[p] = pmin >= 0 ? pmin : -2;
         first_key[p] = kmin;
+    }
+}
+
+// Smallest member id per searched group among the members whose component
+// differs from the representative's (labels are distinct before round 1).
+__global__ __launch_bounds__(kBlock) void k_dup_m2(const int32_t* __restrict__ label,
+                                                  const float4* __restrict__ pts,
+                                                  const Counters* __restrict__ C, DupState dup)
+{
+    if (C->grouped == 0u) return;
+    const int64_t np = C->dup_points;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t p = dup.list[t];
+        const int32_t r = dup.rep[p];
+        if (dup.stamp[r] != dup.epoch || label[p] == label[r]) continue;
+        atomicMin(&dup.m2[r], __float_as_uint(pts[p].w));
+    }
+}
+
+// Group points after their representative searched (it skipped every point
+// at its own location).  A point at a location shared with another component
+// (first search: with any other point) has d2 = 0 to it; the zero vector lies
+// in exactly the '+'-face cones, where (0, smallest such id) beats every other
+// key: the smallest group id of another component is the representative's id
+// when p's component differs from the representative's, else m2.  The '-'
+// cones, and all cones when the group is one component, equal the
+// representative's (same location, same candidate set).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_dup_fill(int64_t n, const int32_t* __restrict__ label,
+                                                    const float4* __restrict__ pts,
+                                                    const int32_t* __restrict__ inv,
+                                                    uint64_t* __restrict__ cand_key,
+                                                    int32_t* __restrict__ cand_pos,
+                                                    int32_t* __restrict__ revive,
+                                                    int32_t* __restrict__ first_corr,
+                                                    uint64_t* __restrict__ first_key,
+                                                    const Counters* __restrict__ C, DupState dup)
+{
+    using CC = ConeCfg<D>;
+    if (C->grouped == 0u) return;
+    const int64_t np = C->dup_points;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t p = dup.list[t];
+        const int32_t r = dup.rep[p];
+        if (dup.stamp[r] != dup.epoch) continue;
+        const uint32_t m2 = dup.m2[r];
+        const bool zero = m2 != 0xffffffffu;
+        uint32_t z = 0;
+        int32_t zpos = -1;
+        if (zero) {
+            z = label[p] != label[r] ? __float_as_uint(pts[r].w) : m2;
+            zpos = inv[z];
+        }
+        // the representative's '-' cones are never rewritten, and its '+'
+        // cones only when every member rewrites its own: no read races a write
+        for (int c = 0; c < CC::K; ++c) {
+            const int64_t o = (int64_t)c * n;
+            if (zero && ((c / CC::NS) & 1) == 0) {
+                cand_key[o + p] = (uint64_t)z;
+                cand_pos[o + p] = zpos;
+            } else if (p != r) {
+                cand_key[o + p] = cand_key[o + r];
+                cand_pos[o + p] = cand_pos[o + r];
+            }
+        }
+        if (first_corr) {
+            if (zero) {
+                first_corr[p] = zpos;
+                first_key[p] = (uint64_t)z;
+            } else if (p != r) {
+                first_corr[p] = first_corr[r];
+                first_key[p] = first_key[r];
+            }
+        }
+        if (revive && revive[p] == -2) revive[p] = -1;
     }
 }
 
@@ -770,10 +983,12 @@ __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __res
 // at all (NodeEMST.h:1051-1077).
 __global__ __launch_bounds__(kBlock) void k_flag_research(int K, int64_t n,
                                                          const uint64_t* __restrict__ cand_key,
-                                                         int32_t* __restrict__ list, Counters* C)
+                                                         int32_t* __restrict__ list,
+                                                         const int32_t* __restrict__ rep, Counters* C)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    const bool grouped = rep != nullptr && C->grouped != 0u;
     bool zero = false, any = false;
     for (int k = 0; k < K; ++k) {
         const uint64_t key = cand_key[(int64_t)k * n + p];
@@ -782,9 +997,14 @@ __global__ __launch_bounds__(kBlock) void k_flag_research(int K, int64_t n,
         zero |= (key >> 32) == 0u;
     }
     if (zero || !any) {
-        const uint32_t slot = atomicAdd(&C->research, 1u);
-        list[slot] = (int32_t)p;
         atomicAdd(&C->zero_or_empty, 1u);
+        // a group's points share their flag (same cones up to the '+' cones,
+        // which are all zero or all the representative's): queue the representative
+        if (grouped) {
+            const int32_t r = rep[p];
+            if (r >= 0 && r != (int32_t)p) return;
+        }
+        list[atomicAdd(&C->research, 1u)] = (int32_t)p;
     }
 }
 
@@ -1166,9 +1386,20 @@ void launch_gather_sorted(const float* resc, const uint32_t* perm, int64_t n, in
     else hipLaunchKernelGGL(k_gather_sorted<2>, dim3(nblocks(n)), dim3(kBlock), 0, s, resc, perm, n, pts, inv);
 }
 
-void launch_build_tree(const float4* pts, int64_t n, int bucket, int64_t L, Node* nodes, hipStream_t s)
+void launch_dup_groups(const uint64_t* keys, const float4* pts, int64_t n, int32_t* rep,
+                       int32_t* runs, int32_t* list, Counters* C, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_leaves, dim3(nblocks(L)), dim3(kBlock), 0, s, pts, n, bucket, L, nodes);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_dup_runs, dim3(nblocks(n)), dim3(kBlock), 0, s, keys, n, rep, runs, C);
+    const unsigned g = (unsigned)std::min<int64_t>(nblocks(n), 1024);
+    hipLaunchKernelGGL(k_dup_group, dim3(g), dim3(kBlock), 0, s, keys, pts, n, rep, runs, list, C);
+    hipLaunchKernelGGL(k_dup_finish, dim3(1), dim3(1), 0, s, C);
+}
+
+void launch_build_tree(const float4* pts, int64_t n, int bucket, int64_t L, Node* nodes,
+                       const int32_t* rep, const Counters* C, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_leaves, dim3(nblocks(L)), dim3(kBlock), 0, s, pts, n, bucket, L, nodes, rep, C);
     for (int64_t lo = L / 2; lo >= 1; lo /= 2)
         hipLaunchKernelGGL(k_level, dim3(nblocks(lo)), dim3(kBlock), 0, s, nodes, lo, lo);
 }
@@ -1177,18 +1408,35 @@ void launch_cone_search(int dim, const float4* pts, const Node* nodes, int64_t L
                         int64_t n, const int32_t* qlist, int64_t nq, const int32_t* label,
                         bool check_labels, const int32_t* inv, const float* seed_r, float seed_scale,
                         uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
-                        uint64_t* first_key, Counters* stats, hipStream_t s)
+                        uint64_t* first_key, Counters* C, bool collect_stats, DupState dup,
+                        hipStream_t s)
 {
     if (nq <= 0) return;
     const unsigned g = nblocks(nq, 64);
+    const int cs = collect_stats ? 1 : 0;
     if (dim == 3)
         hipLaunchKernelGGL(k_cone_search<3>, dim3(g), dim3(64), 0, s, pts, nodes, L, bucket, n, qlist, nq,
                            label, check_labels ? 1 : 0, inv, seed_r, seed_scale, cand_key, cand_pos, revive,
-                           first_corr, first_key, stats);
+                           first_corr, first_key, C, cs, dup);
     else
         hipLaunchKernelGGL(k_cone_search<2>, dim3(g), dim3(64), 0, s, pts, nodes, L, bucket, n, qlist, nq,
                            label, check_labels ? 1 : 0, inv, seed_r, seed_scale, cand_key, cand_pos, revive,
-                           first_corr, first_key, stats);
+                           first_corr, first_key, C, cs, dup);
+}
+
+void launch_dup_fill(int dim, int64_t n, const int32_t* label, const float4* pts, const int32_t* inv,
+                     uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
+                     uint64_t* first_key, const Counters* C, DupState dup, hipStream_t s)
+{
+    if (n <= 0 || dup.rep == nullptr) return;
+    const unsigned g = (unsigned)std::min<int64_t>(nblocks(n), 512);
+    hipLaunchKernelGGL(k_dup_m2, dim3(g), dim3(kBlock), 0, s, label, pts, C, dup);
+    if (dim == 3)
+        hipLaunchKernelGGL(k_dup_fill<3>, dim3(g), dim3(kBlock), 0, s, n, label, pts, inv, cand_key, cand_pos,
+                           revive, first_corr, first_key, C, dup);
+    else
+        hipLaunchKernelGGL(k_dup_fill<2>, dim3(g), dim3(kBlock), 0, s, n, label, pts, inv, cand_key, cand_pos,
+                           revive, first_corr, first_key, C, dup);
 }
 
 void launch_round_init(int64_t n, int32_t* label, int32_t* eu, int32_t* corr, uint64_t* corr_key,
@@ -1198,11 +1446,11 @@ void launch_round_init(int64_t n, int32_t* label, int32_t* eu, int32_t* corr, ui
                        eu, corr, corr_key, comp1, comp2, C);
 }
 
-void launch_flag_research(int K, int64_t n, const uint64_t* cand_key, int32_t* list, Counters* C,
-                          hipStream_t s)
+void launch_flag_research(int K, int64_t n, const uint64_t* cand_key, int32_t* list, const int32_t* rep,
+                          Counters* C, hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_flag_research, dim3(nblocks(n)), dim3(kBlock), 0, s, K, n, cand_key, list, C);
+    hipLaunchKernelGGL(k_flag_research, dim3(nblocks(n)), dim3(kBlock), 0, s, K, n, cand_key, list, rep, C);
 }
 
 void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const uint64_t* cand_key,

@@ -162,9 +162,11 @@ int emst_get_times(emst_ctx* ctx, float* times, int32_t* n_times);
 /* Counters of the last full cone search (the work behind its bytes):
  * [0] packets (64-point waves), [1] hierarchy nodes tested, [2] leaf cells
  * entered, [3] leaf points read (summed over packets), [4] point pairs
- * evaluated (leaf points x lanes that needed the leaf).  No counterpart in
- * the reference (its per-phase timers are in SolutionEMSTOperators.cu:166-373).
- * n_stats: capacity in, count out. */
+ * evaluated (leaf points x lanes that needed the leaf), [5] points sharing
+ * their location with another point, [6] 1 when those were searched as
+ * groups (one search per location), 0 when every point searched itself.
+ * No counterpart in the reference (its per-phase timers are in
+ * SolutionEMSTOperators.cu:166-373).  n_stats: capacity in, count out. */
 int emst_get_search_stats(emst_ctx* ctx, int64_t* stats, int32_t* n_stats);
 
 /* ---- learned seed radius (no counterpart in the reference) -----------------

@@ -176,6 +176,80 @@ def test_degenerate_inputs(pkg, oracle_mod, case):
     assert np.array_equal(g["labels"], ref["labels"])
 
 
+def _morton_keys(resc):
+    """Host restatement of the Morton quantisation (per-axis cell index)."""
+    resc = np.asarray(resc, np.float32)
+    dim = resc.shape[1]
+    maxq = (1 << (21 if dim == 3 else 32)) - 1
+    mx = np.float32(resc.max())
+    qs = np.float32(maxq / np.float64(mx))
+    q = np.minimum((resc * qs).astype(np.float64), maxq).astype(np.uint64)
+    return [tuple(r) for r in q]
+
+
+def _coincident_case(case, oracle_mod):
+    rng = np.random.default_rng(17)
+    if case == "groups2":
+        base = rng.random((1500, 2), dtype=np.float32)
+        pts = np.repeat(base, rng.integers(1, 7, len(base)), axis=0)
+    elif case == "groups3":
+        base = rng.random((1200, 3), dtype=np.float32)
+        pts = np.repeat(base, rng.integers(1, 5, len(base)), axis=0)
+    elif case == "big_group":
+        pts = np.concatenate([rng.random((2000, 2), dtype=np.float32), np.full((3000, 2), 0.5, np.float32)])
+    elif case == "near_twins":
+        base = rng.random((800, 3), dtype=np.float32)
+        step = np.nextafter(base, np.float32(2))
+        pts = np.concatenate([base, step, base[:300], step[:200]])
+    elif case == "wide_run":
+        # 1400 points on two distinct locations of one Morton cell: too long
+        # to split, so grouping turns off
+        other = rng.random((500, 3), dtype=np.float32)
+        for a in np.linspace(0.3, 0.7, 41, dtype=np.float32):
+            b = np.float32(a + np.spacing(a))
+            pts = np.concatenate([other, np.full((700, 3), a, np.float32), np.full((700, 3), b, np.float32)])
+            rs = oracle_mod.rescale(pts)[0]
+            keys = _morton_keys(rs)
+            if not np.array_equal(rs[500], rs[1200]) and keys[500] == keys[1200]:
+                break
+        else:
+            pytest.skip("no two locations found in one Morton cell")
+    else:   # underflow: distinct points at float distance 0, plus copies
+        pts = np.concatenate([rng.random((200, 2), dtype=np.float32),
+                              np.zeros((3, 2), np.float32), np.array([[1e-30, 0.0]], np.float32)])
+    return rng.permutation(pts)
+
+
+@pytest.mark.parametrize("case,grouped", [("groups2", 1), ("groups3", 1), ("big_group", 1),
+                                          ("near_twins", 1), ("wide_run", 0), ("underflow", 0)])
+def test_coincident_groups(pkg, oracle_mod, case, grouped):
+    """Points sharing a location search once per location; their cones,
+    the tree, the per-round sizes and the labels equal the per-point oracle.
+    Long mixed runs and distinct points at float distance 0 search per point."""
+    pts = _coincident_case(case, oracle_mod)
+    print("case", case, pts.shape, flush=True)
+    with pkg.Calculateur(dim=pts.shape[1], trace=True) as c:
+        c.initialize(pts)
+        cand, cd2 = c.candidates()
+        st = pkg.search_stats(c)
+        resc, _ = c.getAdaptiveMap()
+        c.run()
+        u, v, _ = c.getLinks()
+        tr = c.trace()
+        lab = c.labels()
+    _, counts = np.unique(resc, axis=0, return_counts=True)
+    assert st["grouped"] == grouped
+    if grouped:
+        assert st["group_points"] == int(counts[counts > 1].sum())
+    ocid, ocd2, _ = oracle_mod.octants(resc, mode=0)
+    assert np.array_equal(cand, ocid)
+    assert np.array_equal(cd2.view(np.uint32), ocd2.view(np.uint32))
+    ref = oracle_mod.emst(pts, mode=0)
+    assert np.array_equal(np.stack([u, v], 1).astype(np.int64), oracle_mod.canon(ref["u"], ref["v"]))
+    assert [r["size"] for r in tr] == ref["cum_edges"]
+    assert np.array_equal(lab, ref["labels"])
+
+
 @pytest.mark.parametrize("n", [0, 1])
 def test_tiny(pkg, n):
     pts = np.zeros((n, 3), np.float32)
