@@ -73,11 +73,13 @@ struct emst_ctx {
     DevBuf<float> model, feat, pred, seed;     // learned seed radius
     bool learned = false;
     // coincident-point groups (DupState)
-    DevBuf<int32_t> dup_rep, dup_stamp, dup_list, dup_runs;
+    DevBuf<int32_t> dup_rep, dup_list, dup_runs, dup_gslot, dup_bpos;
     DevBuf<uint32_t> dup_m2;
+    DevBuf<uint64_t> dup_bkey;
     bool no_grouping = false;         // set when distinct points met at float distance 0
     bool grouped = false;
-    int epoch = 0;
+    bool groups_settled = false;      // no group spans two components any more
+    int64_t dup_points = 0, dup_groups = 0;
     Params* P = nullptr;
     Counters* C = nullptr;
 };
@@ -175,10 +177,9 @@ int alloc_all(emst_ctx* c)
     ENS(partial, kWeightBlocks);
     ENS(geo, 16);
     ENS(dup_rep, n);
-    ENS(dup_stamp, n);
     ENS(dup_list, n);
     ENS(dup_runs, n);
-    ENS(dup_m2, n);
+    ENS(dup_gslot, n);
 #undef ENS
     return EMST_OK;
 }
@@ -200,7 +201,7 @@ float elapsed(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
-static_assert(offsetof(Counters, pad2) + 3 * sizeof(uint32_t) - offsetof(Counters, dup_runs) ==
+static_assert(offsetof(Counters, pad2) + sizeof(uint32_t) - offsetof(Counters, dup_runs) ==
                   8 * sizeof(uint32_t),
               "the dup counters are cleared as 8 consecutive words");
 static_assert(offsetof(Counters, zero_or_empty) == offsetof(Counters, research) + sizeof(uint32_t),
@@ -209,12 +210,15 @@ static_assert(offsetof(Counters, zero_or_empty) == offsetof(Counters, research) 
 DupState dup_state(emst_ctx* c)
 {
     DupState d{};
-    if (c->no_grouping) return d;
+    if (!c->grouped) return d;
     d.rep = c->dup_rep.p;
-    d.stamp = c->dup_stamp.p;
-    d.m2 = c->dup_m2.p;
     d.list = c->dup_list.p;
-    d.epoch = c->epoch;
+    d.gslot = c->dup_gslot.p;
+    d.m2 = c->dup_m2.p;
+    d.bkey = c->dup_bkey.p;
+    d.bpos = c->dup_bpos.p;
+    d.npoints = c->dup_points;
+    d.ngroups = c->dup_groups;
     return d;
 }
 
@@ -237,13 +241,29 @@ int build(emst_ctx* c)
     launch_gather_sorted(c->resc.p, c->vals0.p, n, c->dim, c->pts.p, c->inv.p, s);
     // coincident-point groups: dup_runs .. pad2 start at zero
     HIPCHK(c, hipMemsetAsync(&c->C->dup_runs, 0, 8 * sizeof(uint32_t), s));
-    c->epoch = 1;
-    const DupState dup = dup_state(c);
-    if (dup.rep) {
-        launch_dup_groups(c->keys0.p, c->pts.p, n, c->dup_rep.p, c->dup_runs.p, c->dup_list.p, c->C, s);
-        HIPCHK(c, hipMemsetAsync(c->dup_stamp.p, 0, sizeof(int32_t) * (size_t)n, s));
+    c->grouped = false;
+    c->groups_settled = false;
+    c->dup_points = c->dup_groups = 0;
+    if (!c->no_grouping) {
+        launch_dup_groups(c->keys0.p, c->pts.p, n, c->dup_rep.p, c->dup_runs.p, c->dup_list.p,
+                          c->dup_gslot.p, c->C, s);
+        LAUNCHCHK(c);
+        Counters h{};
+        int rc0 = read_counters(c, &h);
+        if (rc0) return rc0;
+        if (h.grouped) {
+            const int K2 = c->K / 2;
+            if ((rc0 = ensure(c, c->dup_m2, h.dup_groups)) ||
+                (rc0 = ensure(c, c->dup_bkey, (size_t)K2 * h.dup_groups)) ||
+                (rc0 = ensure(c, c->dup_bpos, (size_t)K2 * h.dup_groups)))
+                return rc0;
+            c->grouped = true;
+            c->dup_points = h.dup_points;
+            c->dup_groups = h.dup_groups;
+        }
     }
-    launch_build_tree(c->pts.p, n, c->bucket, c->L, c->nodes.p, dup.rep, c->C, s);
+    const DupState dup = dup_state(c);
+    launch_build_tree(c->pts.p, n, c->bucket, c->L, c->nodes.p, dup.rep, s);
     launch_round_init(n, c->label.p, c->eu.p, c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->C, s);
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ev[1], s));
@@ -264,18 +284,20 @@ int build(emst_ctx* c)
                     c->corr_key.p, c->C, dup, s);
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ev[2], s));
-    launch_flag_research(c->K, n, c->cand_key.p, c->list.p, dup.rep, c->C, s);
+    // while grouping is on only group points can hold a zero-distance or no
+    // candidate, and k_dup_fill refreshes those; the queue serves the
+    // per-point form
+    if (!c->grouped) launch_flag_research(c->K, n, c->cand_key.p, c->list.p, c->C, s);
     LAUNCHCHK(c);
     Counters h{};
     int rc = read_counters(c, &h);
     if (rc) return rc;
-    if (h.grouped && h.underflow) {
+    if (c->grouped && h.underflow) {
         // distinct points at float distance 0: the groups would merge
         // candidates that the per-point rule keeps apart; search ungrouped
         c->no_grouping = true;
         return build(c);
     }
-    c->grouped = h.grouped != 0u;
     c->zero_or_empty = h.zero_or_empty;
     c->stats[0] = (int64_t)h.packets;
     c->stats[1] = (int64_t)h.nodes;
@@ -284,6 +306,7 @@ int build(emst_ctx* c)
     c->stats[4] = (int64_t)h.pairs;
     c->stats[5] = (int64_t)h.dup_points;
     c->stats[6] = c->grouped ? 1 : 0;
+    if (c->grouped) c->groups_settled = h.dup_split == 0u;
     c->times[0] = elapsed(c->ev[0], c->ev[1]);
     c->times[1] = elapsed(c->ev[1], c->ev[2]);
     c->built = true;
@@ -324,24 +347,30 @@ int round_once(emst_ctx* c, int32_t* finished)
     const bool tr = c->opt.trace != 0;
     int64_t searched = c->rounds == 0 ? n : 0;
     if (tr) HIPCHK(c, hipEventRecord(c->ev[3], s));
-    if (c->rounds > 0 && c->zero_or_empty > 0) {
+    bool refreshed = false;
+    if (c->rounds > 0 && c->grouped) {
+        // group points: zero-distance '+' cones while the group spans several
+        // components, the representative's own cones once it is one (the
+        // repeated search of NodeEMST.h:1051-1077, without searching again)
+        if (!c->groups_settled) {
+            launch_dup_fill(c->dim, n, c->label.p, c->pts.p, c->inv.p, c->cand_key.p, c->cand_pos.p, c->corr.p,
+                            nullptr, nullptr, c->C, dup_state(c), s);
+            LAUNCHCHK(c);
+            refreshed = true;
+            searched = c->dup_points;
+        }
+    } else if (c->rounds > 0 && c->zero_or_empty > 0) {
         // research and zero_or_empty restart for this round
         HIPCHK(c, hipMemsetAsync(&c->C->research, 0, 2 * sizeof(uint32_t), s));
-        DupState dup{};
-        if (c->grouped) dup = dup_state(c);
-        launch_flag_research(c->K, n, c->cand_key.p, c->list.p, dup.rep, c->C, s);
+        launch_flag_research(c->K, n, c->cand_key.p, c->list.p, c->C, s);
         LAUNCHCHK(c);
         Counters h{};
         int rc = read_counters(c, &h);
         if (rc) return rc;
-        if (h.research > 0) {
-            dup.epoch = ++c->epoch;
+        if (h.research > 0)
             launch_cone_search(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, c->list.p, h.research,
                                c->label.p, true, c->inv.p, nullptr, c->seed_scale, c->cand_key.p,
-                               c->cand_pos.p, c->corr.p, nullptr, nullptr, c->C, false, dup, s);
-            launch_dup_fill(c->dim, n, c->label.p, c->pts.p, c->inv.p, c->cand_key.p, c->cand_pos.p, c->corr.p,
-                            nullptr, nullptr, c->C, dup, s);
-        }
+                               c->cand_pos.p, c->corr.p, nullptr, nullptr, c->C, false, DupState{}, s);
         LAUNCHCHK(c);
         searched = h.zero_or_empty;
     }
@@ -367,6 +396,8 @@ int round_once(emst_ctx* c, int32_t* finished)
     }
     const uint32_t before = c->edges;
     c->edges = h.edges;
+    // components only merge: once no group spans two of them, none will again
+    if (refreshed && h.dup_split == 0u) c->groups_settled = true;
     c->rounds = round;
     if (tr) {
         emst_round_trace row{};

@@ -36,22 +36,29 @@ struct Counters {
     uint32_t dup_overflow;  // a run too long to split into groups: grouping off
     uint32_t grouped;       // grouping on (dup_points > 0 and no overflow)
     uint32_t underflow;     // two distinct points at float distance 0 seen: grouping must be off
-    uint32_t pad2[3];
+    uint32_t dup_groups;    // number of groups
+    uint32_t dup_split;     // set by k_dup_m2 when some group spans >= 2 components
+    uint32_t pad2;
 };
 
 // Coincident-point groups.  Points with identical rescaled coordinates form a
 // group whose representative is its smallest input id (its first sorted
 // position, the sort being stable).  rep[p] = representative's sorted
 // position, or -1 for a point without an identical twin.  While grouping is
-// on, only representatives and singletons search; members copy the
-// representative's cones and take their zero-distance '+' cones from the
-// group (k_dup_fill).
+// on, only representatives and singletons search, once, and no point takes a
+// candidate at its own location; k_dup_fill then gives every group point its
+// cones: the '-' cones are the representative's, the '+' cones either the
+// zero-distance candidate of the group (while the group spans >= 2
+// components) or the representative's own '+' cones kept in bkey/bpos.
 struct DupState {
-    const int32_t* rep;     // nullptr: grouping never on for this call
-    int32_t* stamp;         // search epoch of the representative's last search
-    uint32_t* m2;           // per representative: smallest member id whose label differs from its own
-    const int32_t* list;    // group points (members and representatives), C->dup_points of them
-    int epoch;
+    const int32_t* rep;     // nullptr: grouping off
+    const int32_t* list;    // the group points (members and representatives)
+    const int32_t* gslot;   // per representative position: its group index
+    uint32_t* m2;           // per group: smallest member id whose label differs from the representative's
+    uint64_t* bkey;         // per group: the representative's '+' cones, [D * 2^(D-1)][ngroups]
+    int32_t* bpos;
+    int64_t npoints;        // entries of list
+    int64_t ngroups;
 };
 
 // Search-hierarchy node: tight box of a contiguous range of Morton-sorted points.
@@ -75,13 +82,13 @@ size_t scan_tmp_words(int64_t n);
 void launch_gather_sorted(const float* resc, const uint32_t* perm, int64_t n, int dim, float4* pts,
                           int32_t* inv, hipStream_t s);
 // Coincident-point groups from the sorted keys and points: fills rep[] (all
-// n entries), list[] and C->dup_* / C->grouped.  runs: n words of scratch.
-// The dup counters must be zero on entry.
+// n entries), list[], gslot[] (representatives) and C->dup_* / C->grouped.
+// runs: n words of scratch.  The dup counters must be zero on entry.
 void launch_dup_groups(const uint64_t* keys, const float4* pts, int64_t n, int32_t* rep,
-                       int32_t* runs, int32_t* list, Counters* C, hipStream_t s);
-// Leaf boxes leave out group members while grouping is on (rep may be nullptr).
+                       int32_t* runs, int32_t* list, int32_t* gslot, Counters* C, hipStream_t s);
+// Leaf boxes leave out group members (rep == nullptr: grouping off).
 void launch_build_tree(const float4* pts, int64_t n, int bucket, int64_t L, Node* nodes,
-                       const int32_t* rep, const Counters* C, hipStream_t s);
+                       const int32_t* rep, hipStream_t s);
 // Cone search over queries [0,n) (qlist == nullptr) or qlist[0..nq).
 // check_labels: skip zero-distance points of the query's own component.
 // collect_stats: add the traversal counters to C.
@@ -91,17 +98,17 @@ void launch_cone_search(int dim, const float4* pts, const Node* nodes, int64_t L
                         uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
                         uint64_t* first_key, Counters* C, bool collect_stats, DupState dup,
                         hipStream_t s);
-// After a grouped cone search: members of the searched groups take their
-// representative's cones, and every group point its zero-distance '+' cones.
+// Cones of every group point for the current labels (after the grouped
+// cone search, and at the start of each round while a group may still span
+// several components).  Sets C->dup_split when one does.
 void launch_dup_fill(int dim, int64_t n, const int32_t* label, const float4* pts, const int32_t* inv,
                      uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
-                     uint64_t* first_key, const Counters* C, DupState dup, hipStream_t s);
+                     uint64_t* first_key, Counters* C, DupState dup, hipStream_t s);
 void launch_round_init(int64_t n, int32_t* label, int32_t* eu, int32_t* corr, uint64_t* corr_key,
                        uint64_t* comp1, uint32_t* comp2, Counters* C, hipStream_t s);
-// Queue the points holding a zero-distance or no candidate; while grouping is
-// on, group members are counted but not queued (their representative is).
+// Queue the points holding a zero-distance or no candidate.
 void launch_flag_research(int K, int64_t n, const uint64_t* cand_key, int32_t* list,
-                          const int32_t* rep, Counters* C, hipStream_t s);
+                          Counters* C, hipStream_t s);
 void launch_round(int K, int64_t n, const float4* pts, const uint64_t* cand_key,
                   const int32_t* cand_pos, int32_t* label, int32_t* label2, int32_t* corr,
                   uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, int32_t* fixed,

@@ -396,7 +396,8 @@ __global__ __launch_bounds__(kBlock) void k_dup_group(const uint64_t* __restrict
                                                      const float4* __restrict__ pts, int64_t n,
                                                      int32_t* __restrict__ rep,
                                                      const int32_t* __restrict__ runs,
-                                                     int32_t* __restrict__ list, Counters* __restrict__ C)
+                                                     int32_t* __restrict__ list, int32_t* __restrict__ gslot,
+                                                     Counters* __restrict__ C)
 {
     const int lane = __lane_id();
     const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
@@ -420,6 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_dup_group(const uint64_t* __restrict
             if (i < e) mixed |= !same_xyz(pts[i], head);
         }
         if (__ballot(mixed) == 0ull) {
+            if (lane == 0) gslot[s] = (int32_t)atomicAdd(&C->dup_groups, 1u);
             for (int64_t b = s; b < e; b += 64) {
                 const int64_t i = b + lane;
                 if (i < e) rep[i] = (int32_t)s;
@@ -439,6 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_dup_group(const uint64_t* __restrict
                 }
                 const bool grp = i < e && twins >= 2;
                 if (grp) rep[i] = (int32_t)first;
+                if (grp && first == i) gslot[i] = (int32_t)atomicAdd(&C->dup_groups, 1u);
                 wave_append(grp, (int32_t)i, list, &C->dup_points);
             }
         } else if (lane == 0) {
@@ -460,12 +463,11 @@ __global__ void k_dup_finish(Counters* C)
 // test accepts.
 __global__ __launch_bounds__(kBlock) void k_leaves(const float4* __restrict__ pts, int64_t n,
                                                   int bucket, int64_t L, Node* __restrict__ nodes,
-                                                  const int32_t* __restrict__ rep,
-                                                  const Counters* __restrict__ C)
+                                                  const int32_t* __restrict__ rep)
 {
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= L) return;
-    const bool grouped = rep != nullptr && C->grouped != 0u;
+    const bool grouped = rep != nullptr;
     Node nd;
     for (int k = 0; k < 3; ++k) { nd.lo[k] = INFINITY; nd.hi[k] = -INFINITY; }
     nd.pad[0] = nd.pad[1] = 0.f;
@@ -673,7 +675,7 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
     // grouping on: group members do not search (k_dup_fill gives them their
     // representative's cones) and no point takes a candidate at its own
     // location; otherwise only the point itself is skipped
-    const bool grouped = dup.rep != nullptr && ctr->grouped != 0u;
+    const bool grouped = dup.rep != nullptr;
     bool is_rep = false;
     if (grouped && active) {
         const int32_t r = dup.rep[p];
@@ -858,10 +860,7 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
     if (__ballot(underflow) != 0ull && lane == 0) atomicOr(&ctr->underflow, 1u);
     if (!active) return;
     if (revive && revive[p] == -2) revive[p] = -1;
-    if (is_rep) {
-        dup.stamp[p] = dup.epoch;
-        dup.m2[p] = 0xffffffffu;
-    }
+    const int64_t gs = is_rep ? (int64_t)dup.gslot[p] : 0;
     // cone-major layout [K][n]: lanes write consecutive words
     uint64_t kmin = kEmptyKey;
     int32_t pmin = -1;
@@ -871,6 +870,13 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
         cand_key[(int64_t)c * n + p] = k;
         cand_pos[(int64_t)c * n + p] = q;
         if (k < kmin) { kmin = k; pmin = q; }
+        // a representative keeps its '+' cones for the rounds in which its
+        // group has become one component (k_dup_fill)
+        if (is_rep && ((c / C::NS) & 1) == 0) {
+            const int64_t b = (int64_t)((c / C::NS / 2) * C::NS + c % C::NS) * dup.ngroups + gs;
+            dup.bkey[b] = k;
+            dup.bpos[b] = q;
+        }
     }
     // first search (all points singletons): the round-1 target is the overall
     // cone minimum (NodeEMST.h:1338-1381 with every label distinct)
@@ -880,31 +886,34 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
     }
 }
 
-// Smallest member id per searched group among the members whose component
-// differs from the representative's (labels are distinct before round 1).
+// Per group, the smallest member id whose component differs from the
+// representative's (labels are distinct before round 1); dup_split is set
+// when any group spans >= 2 components.  m2 holds ~0 on entry.
 __global__ __launch_bounds__(kBlock) void k_dup_m2(const int32_t* __restrict__ label,
                                                   const float4* __restrict__ pts,
-                                                  const Counters* __restrict__ C, DupState dup)
+                                                  Counters* __restrict__ C, DupState dup)
 {
-    if (C->grouped == 0u) return;
-    const int64_t np = C->dup_points;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np;
+    bool split = false;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < dup.npoints;
          t += (int64_t)gridDim.x * blockDim.x) {
         const int32_t p = dup.list[t];
         const int32_t r = dup.rep[p];
-        if (dup.stamp[r] != dup.epoch || label[p] == label[r]) continue;
-        atomicMin(&dup.m2[r], __float_as_uint(pts[p].w));
+        if (label[p] == label[r]) continue;
+        atomicMin(&dup.m2[dup.gslot[r]], __float_as_uint(pts[p].w));
+        split = true;
     }
+    if (split) atomicOr(&C->dup_split, 1u);
 }
 
-// Group points after their representative searched (it skipped every point
-// at its own location).  A point at a location shared with another component
-// (first search: with any other point) has d2 = 0 to it; the zero vector lies
-// in exactly the '+'-face cones, where (0, smallest such id) beats every other
-// key: the smallest group id of another component is the representative's id
-// when p's component differs from the representative's, else m2.  The '-'
-// cones, and all cones when the group is one component, equal the
-// representative's (same location, same candidate set).
+// Cones of the group points.  A point whose location is shared with another
+// component (before round 1: with any other point) has d2 = 0 to it; the zero
+// vector lies in exactly the '+'-face cones, where (0, smallest such id)
+// beats every other key.  The smallest group id of another component is the
+// representative's id when p's component differs from the representative's,
+// else m2.  Otherwise the '+' cones are the representative's own (kept at its
+// search; no point at the shared location is a candidate).  The '-' cones are
+// always the representative's: the zero vector is in no '-' cone.  first_corr
+// / first_key (round 1) take the smallest key over the point's cones.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_dup_fill(int64_t n, const int32_t* __restrict__ label,
                                                     const float4* __restrict__ pts,
@@ -913,18 +922,15 @@ __global__ __launch_bounds__(kBlock) void k_dup_fill(int64_t n, const int32_t* _
                                                     int32_t* __restrict__ cand_pos,
                                                     int32_t* __restrict__ revive,
                                                     int32_t* __restrict__ first_corr,
-                                                    uint64_t* __restrict__ first_key,
-                                                    const Counters* __restrict__ C, DupState dup)
+                                                    uint64_t* __restrict__ first_key, DupState dup)
 {
     using CC = ConeCfg<D>;
-    if (C->grouped == 0u) return;
-    const int64_t np = C->dup_points;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < dup.npoints;
          t += (int64_t)gridDim.x * blockDim.x) {
         const int32_t p = dup.list[t];
         const int32_t r = dup.rep[p];
-        if (dup.stamp[r] != dup.epoch) continue;
-        const uint32_t m2 = dup.m2[r];
+        const int64_t g = dup.gslot[r];
+        const uint32_t m2 = dup.m2[g];
         const bool zero = m2 != 0xffffffffu;
         uint32_t z = 0;
         int32_t zpos = -1;
@@ -932,26 +938,32 @@ __global__ __launch_bounds__(kBlock) void k_dup_fill(int64_t n, const int32_t* _
             z = label[p] != label[r] ? __float_as_uint(pts[r].w) : m2;
             zpos = inv[z];
         }
-        // the representative's '-' cones are never rewritten, and its '+'
-        // cones only when every member rewrites its own: no read races a write
+        uint64_t kmin = kEmptyKey;
+        int32_t pmin = -1;
+        // the representative's '-' cones are read, never written, here
         for (int c = 0; c < CC::K; ++c) {
             const int64_t o = (int64_t)c * n;
-            if (zero && ((c / CC::NS) & 1) == 0) {
-                cand_key[o + p] = (uint64_t)z;
-                cand_pos[o + p] = zpos;
-            } else if (p != r) {
-                cand_key[o + p] = cand_key[o + r];
-                cand_pos[o + p] = cand_pos[o + r];
+            uint64_t k;
+            int32_t q;
+            if (((c / CC::NS) & 1) == 0) {
+                const int64_t b = (int64_t)((c / CC::NS / 2) * CC::NS + c % CC::NS) * dup.ngroups + g;
+                k = zero ? (uint64_t)z : dup.bkey[b];
+                q = zero ? zpos : dup.bpos[b];
+                cand_key[o + p] = k;
+                cand_pos[o + p] = q;
+            } else {
+                k = cand_key[o + r];
+                q = cand_pos[o + r];
+                if (p != r) {
+                    cand_key[o + p] = k;
+                    cand_pos[o + p] = q;
+                }
             }
+            if (k < kmin) { kmin = k; pmin = q; }
         }
         if (first_corr) {
-            if (zero) {
-                first_corr[p] = zpos;
-                first_key[p] = (uint64_t)z;
-            } else if (p != r) {
-                first_corr[p] = first_corr[r];
-                first_key[p] = first_key[r];
-            }
+            first_corr[p] = pmin >= 0 ? pmin : -2;
+            first_key[p] = kmin;
         }
         if (revive && revive[p] == -2) revive[p] = -1;
     }
@@ -983,12 +995,10 @@ __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __res
 // at all (NodeEMST.h:1051-1077).
 __global__ __launch_bounds__(kBlock) void k_flag_research(int K, int64_t n,
                                                          const uint64_t* __restrict__ cand_key,
-                                                         int32_t* __restrict__ list,
-                                                         const int32_t* __restrict__ rep, Counters* C)
+                                                         int32_t* __restrict__ list, Counters* C)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    const bool grouped = rep != nullptr && C->grouped != 0u;
     bool zero = false, any = false;
     for (int k = 0; k < K; ++k) {
         const uint64_t key = cand_key[(int64_t)k * n + p];
@@ -998,12 +1008,6 @@ __global__ __launch_bounds__(kBlock) void k_flag_research(int K, int64_t n,
     }
     if (zero || !any) {
         atomicAdd(&C->zero_or_empty, 1u);
-        // a group's points share their flag (same cones up to the '+' cones,
-        // which are all zero or all the representative's): queue the representative
-        if (grouped) {
-            const int32_t r = rep[p];
-            if (r >= 0 && r != (int32_t)p) return;
-        }
         list[atomicAdd(&C->research, 1u)] = (int32_t)p;
     }
 }
@@ -1387,19 +1391,19 @@ void launch_gather_sorted(const float* resc, const uint32_t* perm, int64_t n, in
 }
 
 void launch_dup_groups(const uint64_t* keys, const float4* pts, int64_t n, int32_t* rep,
-                       int32_t* runs, int32_t* list, Counters* C, hipStream_t s)
+                       int32_t* runs, int32_t* list, int32_t* gslot, Counters* C, hipStream_t s)
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_dup_runs, dim3(nblocks(n)), dim3(kBlock), 0, s, keys, n, rep, runs, C);
     const unsigned g = (unsigned)std::min<int64_t>(nblocks(n), 1024);
-    hipLaunchKernelGGL(k_dup_group, dim3(g), dim3(kBlock), 0, s, keys, pts, n, rep, runs, list, C);
+    hipLaunchKernelGGL(k_dup_group, dim3(g), dim3(kBlock), 0, s, keys, pts, n, rep, runs, list, gslot, C);
     hipLaunchKernelGGL(k_dup_finish, dim3(1), dim3(1), 0, s, C);
 }
 
 void launch_build_tree(const float4* pts, int64_t n, int bucket, int64_t L, Node* nodes,
-                       const int32_t* rep, const Counters* C, hipStream_t s)
+                       const int32_t* rep, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_leaves, dim3(nblocks(L)), dim3(kBlock), 0, s, pts, n, bucket, L, nodes, rep, C);
+    hipLaunchKernelGGL(k_leaves, dim3(nblocks(L)), dim3(kBlock), 0, s, pts, n, bucket, L, nodes, rep);
     for (int64_t lo = L / 2; lo >= 1; lo /= 2)
         hipLaunchKernelGGL(k_level, dim3(nblocks(lo)), dim3(kBlock), 0, s, nodes, lo, lo);
 }
@@ -1426,17 +1430,19 @@ void launch_cone_search(int dim, const float4* pts, const Node* nodes, int64_t L
 
 void launch_dup_fill(int dim, int64_t n, const int32_t* label, const float4* pts, const int32_t* inv,
                      uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
-                     uint64_t* first_key, const Counters* C, DupState dup, hipStream_t s)
+                     uint64_t* first_key, Counters* C, DupState dup, hipStream_t s)
 {
-    if (n <= 0 || dup.rep == nullptr) return;
-    const unsigned g = (unsigned)std::min<int64_t>(nblocks(n), 512);
+    if (n <= 0 || dup.rep == nullptr || dup.npoints <= 0) return;
+    hipMemsetAsync(dup.m2, 0xff, sizeof(uint32_t) * (size_t)dup.ngroups, s);
+    hipMemsetAsync(&C->dup_split, 0, sizeof(uint32_t), s);
+    const unsigned g = (unsigned)std::min<int64_t>(nblocks(dup.npoints), 1024);
     hipLaunchKernelGGL(k_dup_m2, dim3(g), dim3(kBlock), 0, s, label, pts, C, dup);
     if (dim == 3)
         hipLaunchKernelGGL(k_dup_fill<3>, dim3(g), dim3(kBlock), 0, s, n, label, pts, inv, cand_key, cand_pos,
-                           revive, first_corr, first_key, C, dup);
+                           revive, first_corr, first_key, dup);
     else
         hipLaunchKernelGGL(k_dup_fill<2>, dim3(g), dim3(kBlock), 0, s, n, label, pts, inv, cand_key, cand_pos,
-                           revive, first_corr, first_key, C, dup);
+                           revive, first_corr, first_key, dup);
 }
 
 void launch_round_init(int64_t n, int32_t* label, int32_t* eu, int32_t* corr, uint64_t* corr_key,
@@ -1446,11 +1452,11 @@ void launch_round_init(int64_t n, int32_t* label, int32_t* eu, int32_t* corr, ui
                        eu, corr, corr_key, comp1, comp2, C);
 }
 
-void launch_flag_research(int K, int64_t n, const uint64_t* cand_key, int32_t* list, const int32_t* rep,
-                          Counters* C, hipStream_t s)
+void launch_flag_research(int K, int64_t n, const uint64_t* cand_key, int32_t* list, Counters* C,
+                          hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_flag_research, dim3(nblocks(n)), dim3(kBlock), 0, s, K, n, cand_key, list, rep, C);
+    hipLaunchKernelGGL(k_flag_research, dim3(nblocks(n)), dim3(kBlock), 0, s, K, n, cand_key, list, C);
 }
 
 void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const uint64_t* cand_key,
