@@ -82,6 +82,11 @@ def main():
     K = 2 * args.dim * (1 << (args.dim - 1))
     model_path = os.path.join(ge.PKG_DIR, "models", f"radius_{args.dim}d.npy")
     model = np.load(model_path) if (os.path.exists(model_path) and not args.plain) else None
+    model_clouds = None
+    if model is not None:
+        meta = os.path.join(ge.PKG_DIR, "models", "train_radius.json")
+        if os.path.exists(meta):
+            model_clouds = json.load(open(meta)).get("clouds")
 
     def timed(form_model):
         ctx = pkg.Calculateur(dim=args.dim, bucket_size=args.bucket)
@@ -162,6 +167,7 @@ def main():
                    "learned_mpts": round(args.n * args.steps / learned["dt"] / 1e6, 3) if learned else None,
                    "learned_search_ms": round(learned["search_ms"], 3) if learned else None,
                    "learned_vs_plain": round(plain["dt"] / learned["dt"], 3) if learned else None,
+                   "radius_model_training_clouds": model_clouds,
                    "same_tree": learned is not None},
         "roofline": roofline,
         "cpu_baseline": cpu,

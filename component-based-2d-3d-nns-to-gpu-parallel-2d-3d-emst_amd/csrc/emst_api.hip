@@ -70,7 +70,7 @@ struct emst_ctx {
     DevBuf<Node> nodes;
     DevBuf<float> ed2;
     DevBuf<double> partial;
-    DevBuf<float> model, feat, pred, seed;     // learned seed radius
+    DevBuf<float> model, feat, pred, seed, ls;     // learned seed radius
     bool learned = false;
     // coincident-point groups (DupState)
     DevBuf<int32_t> dup_rep, dup_list, dup_runs, dup_gslot, dup_bpos;
@@ -271,10 +271,10 @@ int build(emst_ctx* c)
     if (c->learned) {
         int rc2;
         if ((rc2 = ensure(c, c->feat, (size_t)n * 8)) || (rc2 = ensure(c, c->pred, (size_t)n)) ||
-            (rc2 = ensure(c, c->seed, (size_t)n)))
+            (rc2 = ensure(c, c->seed, (size_t)n)) || (rc2 = ensure(c, c->ls, (size_t)n)))
             return rc2;
-        launch_radius_features(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, c->P, c->feat.p, s);
-        launch_radius_seed(n, c->model.p, c->feat.p, c->pred.p, c->seed.p, s);
+        launch_radius_features(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, c->feat.p, c->ls.p, s);
+        launch_radius_seed(n, c->model.p, c->feat.p, c->ls.p, c->pred.p, c->seed.p, s);
         seed_r = c->seed.p;
     }
     launch_cone_search(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, nullptr, n, c->label.p, false,
@@ -783,10 +783,12 @@ int emst_get_radius_samples(emst_ctx* c, float* feat, float* target, int64_t* m)
     if (*m < n) { *m = n; return fail(c, EMST_E_ARG, "sample arrays too small"); }
     *m = n;
     if (n == 0) return EMST_OK;
-    if ((rc = ensure(c, c->feat, (size_t)n * 8)) || (rc = ensure(c, c->pred, (size_t)n))) return rc;
+    if ((rc = ensure(c, c->feat, (size_t)n * 8)) || (rc = ensure(c, c->pred, (size_t)n)) ||
+        (rc = ensure(c, c->ls, (size_t)n)))
+        return rc;
     hipStream_t s = c->stream;
-    launch_radius_features(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, c->P, c->feat.p, s);
-    launch_radius_targets(c->K, n, c->cand_key.p, c->feat.p, c->pred.p, s);
+    launch_radius_features(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, c->feat.p, c->ls.p, s);
+    launch_radius_targets(c->K, n, c->cand_key.p, c->ls.p, c->pred.p, s);
     LAUNCHCHK(c);
     if (feat) HIPCHK(c, hipMemcpyAsync(feat, c->feat.p, sizeof(float) * n * 8, hipMemcpyDeviceToHost, s));
     if (target) HIPCHK(c, hipMemcpyAsync(target, c->pred.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));

@@ -129,10 +129,12 @@ void launch_edges_out(const uint64_t* keys, const uint32_t* idx, const float* ed
 void launch_cell_grid(const float* resc, int64_t n, int dim, const int32_t* geo, int32_t* cell_of,
                       uint64_t* keys, uint32_t* cell_count, hipStream_t s);
 // learned seed radius (emst_mlp.hip)
+// X: n x 8 model features, LS: n x log(leaf spacing s0)
 void launch_radius_features(int dim, const float4* pts, const Node* nodes, int64_t L, int bucket, int64_t n,
-                            const Params* P, float* X, hipStream_t s);
-void launch_radius_targets(int K, int64_t n, const uint64_t* cand_key, const float* X, float* T, hipStream_t s);
-void launch_radius_seed(int64_t n, const float* model, const float* X, float* Y, float* R, hipStream_t s);
+                            float* X, float* LS, hipStream_t s);
+void launch_radius_targets(int K, int64_t n, const uint64_t* cand_key, const float* LS, float* T, hipStream_t s);
+void launch_radius_seed(int64_t n, const float* model, const float* X, const float* LS, float* Y, float* R,
+                        hipStream_t s);
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
                         hipStream_t s);
 

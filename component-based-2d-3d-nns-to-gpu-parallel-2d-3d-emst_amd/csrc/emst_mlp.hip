@@ -31,6 +31,7 @@ constexpr int oW1 = 0, oB1 = oW1 + kF * kH, oW2 = oB1 + kH, oB2 = oW2 + kH * kH,
               oB3 = oW3 + kH, oMean = oB3 + 1, oIstd = oMean + kF, oOff = oIstd + kF,
               kParams = oOff + 1;
 constexpr int kTrain = oMean;    // trainable parameters come first
+constexpr float kXClamp = 4.f;   // normalised features are clamped to [-4, 4]
 static_assert(kParams == EMST_MLP_PARAMS, "model layout");
 
 typedef short s4 __attribute__((ext_vector_type(4)));
@@ -83,7 +84,9 @@ __device__ __forceinline__ void zero(f4 (&acc)[MT][NT])
 }
 
 // Forward (and, when T != nullptr, backward) for 64 samples per block (one wave).
-// X: m x 8 raw features (normalised here with the model's mean / 1/std).
+// X: m x 8 raw features (normalised here with the model's mean / 1/std and
+// clamped to [-kXClamp, kXClamp], so a cloud unlike the training clouds
+// cannot drive the network outside the range it was fitted on).
 // Inference writes y (+ seed offset when add_offset).  Training accumulates
 // the gradient of sum((y - t)^2) * gscale into G and the squared error into loss.
 __global__ __launch_bounds__(64) void k_mlp(const float* __restrict__ X, const float* __restrict__ T,
@@ -111,7 +114,8 @@ __global__ __launch_bounds__(64) void k_mlp(const float* __restrict__ X, const f
     for (int i = l; i < kH * kH; i += 64) sW2[i] = f2bf(P[oW2 + i]);
     for (int f = 0; f < kFP; ++f) {
         float x = 0.f;
-        if (f < kF && valid) x = (X[(s0 + l) * kF + f] - P[oMean + f]) * P[oIstd + f];
+        if (f < kF && valid)
+            x = fminf(fmaxf((X[(s0 + l) * kF + f] - P[oMean + f]) * P[oIstd + f], -kXClamp), kXClamp);
         sXt[f * 64 + l] = f2bf(x);
     }
     __syncthreads();
@@ -261,7 +265,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ X, 
         const bool valid = s0 + l < m;
         for (int f = 0; f < kFP; ++f) {
             float x = 0.f;
-            if (f < kF && valid) x = (X[(s0 + l) * kF + f] - mean[f]) * istd[f];
+            if (f < kF && valid) x = fminf(fmaxf((X[(s0 + l) * kF + f] - mean[f]) * istd[f], -kXClamp), kXClamp);
             xt[f * 64 + l] = f2bf(x);
         }
         __builtin_amdgcn_wave_barrier();
@@ -317,11 +321,14 @@ __global__ void k_adam(float* __restrict__ P, float* __restrict__ G, float* __re
 
 }  // namespace
 
-// Features of every sorted point for the seed model (see emst.h for the list).
+// Features of every sorted point for the seed model, all free of the cloud's
+// scale (ratios of lengths, fractions), and LS = log s0, the leaf spacing the
+// target and the seed are measured in.  Neighbours are the 8 nearest sorted
+// positions (Morton order).
 __global__ __launch_bounds__(256) void k_radius_features(int dim, const float4* __restrict__ pts,
                                                         const Node* __restrict__ nodes, int64_t L,
-                                                        int bucket, int64_t n, const Params* __restrict__ P,
-                                                        float* __restrict__ X)
+                                                        int bucket, int64_t n, float* __restrict__ X,
+                                                        float* __restrict__ LS)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
@@ -329,11 +336,10 @@ __global__ __launch_bounds__(256) void k_radius_features(int dim, const float4* 
     const float pc[3] = {me.x, me.y, me.z};
     float sp[4];
     int64_t node = L + p / bucket;
-    Node leaf{}, lvl2{};
+    Node leaf{};
     for (int k = 0; k < 4; ++k) {
         const Node nd = nodes[node];
         if (k == 0) leaf = nd;
-        if (k == 2) lvl2 = nd;
         // points under this node: its leaf range clipped to n
         int64_t first = node, last = node;
         while (first < L) { first = 2 * first; last = 2 * last + 1; }
@@ -349,30 +355,60 @@ __global__ __launch_bounds__(256) void k_radius_features(int dim, const float4* 
         sp[k] = fmaxf(s, 1e-6f);
         if (node > 1) node >>= 1;
     }
-    float emin = INFINITY, emax = 0.f, fd = INFINITY, gd = INFINITY, pd = INFINITY;
+    float emin = INFINITY, emax = 0.f, fd = INFINITY;
     for (int d = 0; d < dim; ++d) {
-        const float e = leaf.hi[d] - leaf.lo[d];
+        const float e = fmaxf(leaf.hi[d] - leaf.lo[d], 0.f);
         emin = fminf(emin, e);
         emax = fmaxf(emax, e);
         fd = fminf(fd, fminf(pc[d] - leaf.lo[d], leaf.hi[d] - pc[d]));
-        gd = fminf(gd, fminf(pc[d], P->newmax[d] - pc[d]));
-        pd = fminf(pd, fminf(pc[d] - lvl2.lo[d], lvl2.hi[d] - pc[d]));
     }
+    // sorted neighbours: distances (smallest, median), one-sidedness
+    float dd[8];
+    float dsum = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;
+    int nd = 0;
+#pragma unroll
+    for (int o = -4; o <= 4; ++o) {
+        if (o == 0) continue;
+        const int64_t q = p + o;
+        float dist = INFINITY;
+        if (q >= 0 && q < n) {
+            const float4 w = pts[q];
+            const float dx = w.x - me.x, dy = w.y - me.y, dz = dim == 3 ? w.z - me.z : 0.f;
+            dist = sqrtf(dx * dx + dy * dy + dz * dz);
+            dsum += dist;
+            vx += dx; vy += dy; vz += dz;
+            ++nd;
+        }
+        dd[o < 0 ? o + 4 : o + 3] = dist;
+    }
+    // sort the 8 distances (missing neighbours at the array ends are +inf)
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 7 - i; ++j) {
+            const float a = dd[j], b = dd[j + 1];
+            dd[j] = fminf(a, b);
+            dd[j + 1] = fmaxf(a, b);
+        }
+    const float dmin = dd[0];
+    const float dmed = dd[(nd - 1) / 2];
+    const float s0 = sp[0];
     float* x = X + p * 8;
-    x[0] = logf(sp[0]);
+    x[0] = logf(fmaxf(dmin, 1e-3f * s0) / s0);
     x[1] = logf(sp[1] / sp[0]);
     x[2] = logf(sp[2] / sp[1]);
     x[3] = logf(sp[3] / sp[2]);
     x[4] = logf(fmaxf(emax, 1e-6f) / fmaxf(emin, 1e-6f * fmaxf(emax, 1e-6f)));
-    x[5] = fmaxf(fd, 0.f) / sp[0];
-    x[6] = logf(1.f + fmaxf(gd, 0.f) / sp[0]);
-    x[7] = logf(1.f + fmaxf(pd, 0.f) / sp[0]);
+    x[5] = fmaxf(fd, 0.f) / s0;
+    x[6] = logf(fmaxf(dmed, 1e-3f * s0) / s0);
+    x[7] = dsum > 0.f ? sqrtf(vx * vx + vy * vy + vz * vz) / dsum : 0.f;
+    LS[p] = logf(s0);
 }
 
 // Target: log(R / s0), R = sqrt(largest non-empty cone best d2) of the point,
-// s0 = exp(feature 0) its leaf spacing.  Points with no candidate get -20.
+// s0 = exp(LS) its leaf spacing.  Points with no candidate get -20.
 __global__ __launch_bounds__(256) void k_radius_targets(int K, int64_t n, const uint64_t* __restrict__ cand_key,
-                                                       const float* __restrict__ X, float* __restrict__ T)
+                                                       const float* __restrict__ LS, float* __restrict__ T)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
@@ -382,38 +418,39 @@ __global__ __launch_bounds__(256) void k_radius_targets(int K, int64_t n, const 
         if (key != kEmptyKey) mx = fmaxf(mx, __uint_as_float((uint32_t)(key >> 32)));
     }
     const float r = mx > 0.f ? sqrtf(mx) : 0.f;
-    T[p] = r > 0.f ? fmaxf(logf(r) - X[p * 8], -20.f) : -20.f;
+    T[p] = r > 0.f ? fmaxf(logf(r) - LS[p], -20.f) : -20.f;
 }
 
 // Seed radius per sorted point from the model: s0 * exp(y + offset).
-__global__ __launch_bounds__(256) void k_radius_seed(int64_t n, const float* __restrict__ X,
+__global__ __launch_bounds__(256) void k_radius_seed(int64_t n, const float* __restrict__ LS,
                                                     const float* __restrict__ Y, float* __restrict__ R)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    R[p] = expf(X[p * 8] + fminf(Y[p], 20.f));
+    R[p] = expf(LS[p] + fminf(Y[p], 20.f));
 }
 
 void launch_radius_features(int dim, const float4* pts, const Node* nodes, int64_t L, int bucket, int64_t n,
-                            const Params* P, float* X, hipStream_t s)
+                            float* X, float* LS, hipStream_t s)
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_radius_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dim, pts, nodes, L,
-                       bucket, n, P, X);
+                       bucket, n, X, LS);
 }
 
-void launch_radius_targets(int K, int64_t n, const uint64_t* cand_key, const float* X, float* T, hipStream_t s)
+void launch_radius_targets(int K, int64_t n, const uint64_t* cand_key, const float* LS, float* T, hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_radius_targets, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, K, n, cand_key, X, T);
+    hipLaunchKernelGGL(k_radius_targets, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, K, n, cand_key, LS, T);
 }
 
-void launch_radius_seed(int64_t n, const float* model, const float* X, float* Y, float* R, hipStream_t s)
+void launch_radius_seed(int64_t n, const float* model, const float* X, const float* LS, float* Y, float* R,
+                        hipStream_t s)
 {
     if (n <= 0) return;
     const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
     hipLaunchKernelGGL(k_mlp_infer, dim3(g), dim3(256), 0, s, X, n, model, Y, 1);
-    hipLaunchKernelGGL(k_radius_seed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, X, Y, R);
+    hipLaunchKernelGGL(k_radius_seed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, LS, Y, R);
 }
 
 }  // namespace emst

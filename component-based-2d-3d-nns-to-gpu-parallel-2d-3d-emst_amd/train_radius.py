@@ -1,14 +1,18 @@
 """Train the seed-radius MLP from the plain method's own searches.
 
-Protocol: 2^20 (features, log true radius) pairs per dimension, logged from
-plain cone searches on the uniform clouds of the reference-path configs
-(2D: 10^4 points seed 0 and 10^6 points seed 1; 3D: 10^6 points seed 2),
-trained on the GPU with the library's forward/backward kernels
-(emst_mlp_train).  The seed offset (added to the predicted log radius) is
-then picked by timing the cone search on a held-out uniform cloud.
+Two training sets (--clouds), 2^20 (features, log true radius) pairs per
+dimension, logged from plain cone searches:
+  uniform  the uniform clouds of the reference-path configs only (2D: 10^4
+           points seed 0 and 10^6 points seed 1; 3D: 10^6 points seed 2);
+  mixed    those plus 10^6-point clouds of the clustered and power-law
+           generators at seeds the benchmark does not use (11-16).
+Training runs on the GPU with the library's forward/backward kernels
+(emst_mlp_train).  The seed offset (added to the predicted log radius) is then
+picked by timing the cone search on held-out clouds (uniform; for mixed also
+one clustered and one power-law cloud, again at unused seeds).
 
-    python train_radius.py --out <dir>      (needs a GPU; writes radius_2d.npy,
-                                             radius_3d.npy and train_radius.json)
+    python train_radius.py --out <dir> [--clouds mixed]   (needs a GPU; writes
+        radius_2d.npy, radius_3d.npy and train_radius.json)
 """
 import argparse
 import json
@@ -32,14 +36,21 @@ def load_pkg():
     return mod
 
 
-TRAIN_CLOUDS = {2: [(0, 10_000, 0), (0, 1_000_000, 1)], 3: [(0, 1_000_000, 2)]}
-VALID_CLOUD = {2: (0, 2_000_000, 101), 3: (0, 2_000_000, 102)}
+TRAIN_CLOUDS = {
+    "uniform": {2: [(0, 10_000, 0), (0, 1_000_000, 1)], 3: [(0, 1_000_000, 2)]},
+    "mixed": {2: [(0, 10_000, 0), (0, 1_000_000, 1), (2, 1_000_000, 11), (1, 1_000_000, 12)],
+              3: [(0, 1_000_000, 2), (1, 1_000_000, 13), (2, 1_000_000, 14)]},
+}
+VALID_CLOUDS = {
+    "uniform": {2: [(0, 2_000_000, 101)], 3: [(0, 2_000_000, 102)]},
+    "mixed": {2: [(0, 2_000_000, 101), (2, 2_000_000, 15)], 3: [(0, 2_000_000, 102), (1, 2_000_000, 16)]},
+}
 OFFSETS = (-0.25, 0.0, 0.25, 0.5, 0.75, 1.0)
 
 
-def samples(pkg, dim):
+def samples(pkg, dim, clouds):
     Xs, Ts = [], []
-    for kind, n, seed in TRAIN_CLOUDS[dim]:
+    for kind, n, seed in TRAIN_CLOUDS[clouds][dim]:
         pts = pkg.synth(kind, n, dim, seed)
         with pkg.Calculateur(dim=dim) as c:
             c.initialize(pts)
@@ -72,24 +83,24 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "models"))
     ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--clouds", choices=sorted(TRAIN_CLOUDS), default="uniform")
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
     pkg = load_pkg()
-    report = {}
+    report = {"clouds": args.clouds}
     for dim in (2, 3):
         t0 = time.time()
-        X, T = samples(pkg, dim)
+        X, T = samples(pkg, dim, args.clouds)
         print(f"{dim}D: {len(T)} pairs, target mean {T.mean():.3f} var {T.var():.4f}", flush=True)
         model, losses = pkg.train_radius_model(X, T, epochs=args.epochs, batch=1024, lr=2e-3, seed=dim)
         print(f"{dim}D: loss per epoch {np.round(losses, 4).tolist()}", flush=True)
-        kind, n, seed = VALID_CLOUD[dim]
-        vpts = pkg.synth(kind, n, dim, seed)
-        plain = search_ms(pkg, dim, vpts, None)
+        vclouds = [pkg.synth(kind, n, dim, seed) for kind, n, seed in VALID_CLOUDS[args.clouds][dim]]
+        plain = sum(search_ms(pkg, dim, v, None) for v in vclouds)
         timings = {}
         for off in OFFSETS:
             m = model.copy()
             m[-1] = off
-            timings[off] = search_ms(pkg, dim, vpts, m)
+            timings[off] = sum(search_ms(pkg, dim, v, m) for v in vclouds)
             print(f"{dim}D: offset {off:+.2f}: cone search {timings[off]:.2f} ms (plain {plain:.2f})", flush=True)
         best = min(timings, key=timings.get)
         model[-1] = best

@@ -178,10 +178,13 @@ int emst_get_search_stats(emst_ctx* ctx, int64_t* stats, int32_t* n_stats);
  * result is the same for any seed, only the work changes.
  * Model: EMST_MLP_PARAMS floats, W1[8][64] | b1[64] | W2[64][64] | b2[64] |
  * w3[64] | b3 | feature mean[8] | feature 1/std[8] | seed offset.
- * Features (per point): log s0; log(s1/s0), log(s2/s1), log(s3/s2) for the
- * 1st..3rd enclosing cells; log aspect of the leaf box; distance to the leaf
- * box boundary / s0; log(1 + distance to the cloud's box boundary / s0);
- * log(1 + distance to the 2nd enclosing cell's boundary / s0). */
+ * Features (per point, all independent of the cloud's scale): log(d / s0),
+ * d = distance to the nearest of the 8 nearest sorted (Morton-order)
+ * positions; log(s1/s0), log(s2/s1), log(s3/s2) for the 1st..3rd enclosing
+ * cells; log aspect of the leaf box; distance to the leaf box boundary / s0;
+ * log(median distance to those 8 neighbours / s0); length of the sum of the 8
+ * neighbour offsets / sum of their lengths.  Normalised features are clamped
+ * to [-4, 4] before the first layer. */
 #define EMST_MLP_PARAMS 4818
 
 /* Use `model` for the seed radius of this context's cone searches (copied);

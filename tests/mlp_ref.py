@@ -18,6 +18,7 @@ oIstd = oMean + F
 oOff = oIstd + F
 NPARAM = oOff + 1
 NTRAIN = oMean
+XCLAMP = 4.0      # normalised features are clamped to [-4, 4]
 
 
 def bf16(a):
@@ -39,7 +40,7 @@ def forward(P, X, bf16_on=True):
     r = bf16 if bf16_on else (lambda a: a)
     dt = np.float32 if bf16_on else np.float64
     X = np.asarray(X, dt)
-    xn = r(((X - mean.astype(dt)) * istd.astype(dt)).astype(dt))
+    xn = r(np.clip((X - mean.astype(dt)) * istd.astype(dt), -XCLAMP, XCLAMP).astype(dt))
     z1 = xn @ r(W1.astype(dt)) + b1.astype(dt)
     h1 = r(np.maximum(z1, 0).astype(dt))
     z2 = h1 @ r(W2.astype(dt)) + b2.astype(dt)
