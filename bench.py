@@ -52,6 +52,10 @@ def main():
                     help="points of the same distribution timed on the CPU oracle (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (scales rocprof byte counters)")
+    ap.add_argument("--batch", type=int, default=0, metavar="CLOUDS",
+                    help="batched workload instead: CLOUDS clouds, sizes log-uniform 1e2-1e5, half 2D half "
+                         "3D, each uniform or 8 Gaussians (generator seed --batch-seed)")
+    ap.add_argument("--batch-seed", type=int, default=5)
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -74,6 +78,9 @@ def main():
 
     pkg = ge.load_package()
     pkg.lib()
+    if args.batch:
+        run_batch(args, pkg)
+        return
     names = {0: "uniform", 1: "gauss64+10%bg", 2: "powerlaw16"}
     pts = pkg.synth(args.kind, args.n, args.dim, args.seed)
     dev = torch.from_numpy(pts).to("cuda:0")
@@ -171,6 +178,96 @@ def main():
                    "same_tree": learned is not None},
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def run_batch(args, pkg):
+    """Batched EMST: the 2D clouds and the 3D clouds each run as one batch
+    context (emst_set_clouds), the two contexts concurrently from two host
+    threads on their own streams.  value = all points / wall time of a step;
+    per-cloud latency = build + cone search + the rounds up to the cloud's
+    completion round, from the contexts' HIP events in a traced step."""
+    import threading
+
+    import numpy as np
+    import torch
+
+    rng = np.random.default_rng(args.batch_seed)
+    B = args.batch
+    sizes = np.rint(np.exp(rng.uniform(np.log(1e2), np.log(1e5), B))).astype(np.int64)
+    dims = np.array([2] * (B // 2) + [3] * (B - B // 2))
+    rng.shuffle(dims)
+    kinds = np.where(rng.random(B) < 0.5, 0, 3)
+    groups = []
+    for d in (2, 3):
+        idx = np.nonzero(dims == d)[0]
+        clouds = [pkg.synth(int(kinds[i]), int(sizes[i]), d, args.batch_seed * 1_000_003 + int(i)) for i in idx]
+        packed = torch.from_numpy(np.concatenate(clouds)).to("cuda:0")
+        groups.append({"dim": d, "idx": idx, "sizes": sizes[idx], "dev": packed})
+    torch.cuda.synchronize()
+    total = int(sizes.sum())
+
+    def make(trace):
+        return [pkg.Calculateur(dim=g["dim"], trace=trace) for g in groups]
+
+    def step(ctxs):
+        errs = []
+
+        def work(ctx, g):
+            try:
+                ctx.initialize_clouds_device(g["dev"].data_ptr(), g["sizes"])
+                ctx.run()
+            except Exception as e:      # surfaced after join
+                errs.append(e)
+
+        ts = [threading.Thread(target=work, args=(c, g)) for c, g in zip(ctxs, groups)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        if errs:
+            raise errs[0]
+
+    ctxs = make(False)
+    for _ in range(args.warmup):
+        step(ctxs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(ctxs)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    edges = 0
+    for c, g in zip(ctxs, groups):
+        w, e, r = c.cloud_results()
+        assert np.array_equal(e, np.maximum(g["sizes"] - 1, 0)), "a cloud did not finish its tree"
+        edges += int(e.sum())
+        c.close()
+    # latency from a traced step
+    tctx = make(True)
+    step(tctx)
+    lat = np.zeros(B)
+    for c, g in zip(tctx, groups):
+        t = c.times()
+        rows = c.trace()
+        per_round = [r["time_find_next_closest"] + r["time_find_min_pair"] + r["time_connect_union"] +
+                     r["time_flatten"] for r in rows]
+        cum = np.concatenate([[0.0], np.cumsum(per_round)])
+        _, _, last = c.cloud_results()
+        lat[g["idx"]] = t[0] + t[1] + cum[np.minimum(last, len(per_round))]
+        c.close()
+    out = {
+        "metric": f"batched EMST Mpoints/s ({B} clouds)", "value": round(total * args.steps / dt / 1e6, 3),
+        "unit": "Mpoints/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3 / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (project generator, seeded)",
+        "config": {"workload": f"{B} clouds, sizes log-uniform 1e2-1e5, half 2D half 3D, uniform or 8 Gaussians",
+                   "clouds": B, "points": total, "edges": edges,
+                   "clouds_2d": int((dims == 2).sum()), "points_2d": int(sizes[dims == 2].sum()),
+                   "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+                   "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+                   "latency_ms_max": round(float(lat.max()), 3)},
     }
     print(json.dumps(out))
 

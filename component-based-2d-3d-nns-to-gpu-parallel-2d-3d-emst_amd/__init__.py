@@ -26,6 +26,7 @@ EXPORTS = [
     "emst_get_rescaled", "emst_get_trace", "emst_get_candidates", "emst_get_cell_grid",
     "emst_get_times", "emst_get_search_stats", "emst_synth", "emst_version",
     "emst_set_radius_model", "emst_get_radius_samples", "emst_mlp_train", "emst_mlp_eval", "emst_mlp_grad",
+    "emst_set_clouds", "emst_get_cloud_results",
 ]
 
 MLP_PARAMS = 4818
@@ -92,6 +93,8 @@ def lib():
     L.emst_mlp_train.argtypes = [i32, P, P, i64, i32, i32, ctypes.c_float, ctypes.c_uint64, P, P]
     L.emst_mlp_eval.argtypes = [i32, P, P, i64, P]
     L.emst_mlp_grad.argtypes = [i32, P, P, P, i64, P, P]
+    L.emst_set_clouds.argtypes = [P, i32, P, P, i32, i32]
+    L.emst_get_cloud_results.argtypes = [P, P, P, P]
     L.emst_version.restype = ctypes.c_char_p
     _lib = L
     return L
@@ -159,6 +162,36 @@ class Calculateur:
         """Same, for points already resident on the device (AoS float32)."""
         self._chk(self._L.emst_set_points(self._c, ctypes.c_void_p(dev_ptr), n, dim, 1), "set_points")
         self.n, self.dim = n, dim
+
+    def initialize_clouds(self, clouds):
+        """A batch of independent clouds of this context's dimension (list of
+        n_i x dim arrays), packed back to back; edges then carry global ids
+        (cloud offset + local id)."""
+        pts = [np.ascontiguousarray(c, dtype=np.float32).reshape(-1, self.dim) for c in clouds]
+        sizes = np.array([len(c) for c in pts], np.int64)
+        packed = np.ascontiguousarray(np.concatenate(pts) if pts else np.zeros((0, self.dim), np.float32))
+        self._keep = packed
+        self._chk(self._L.emst_set_clouds(self._c, len(pts), _p(sizes), _p(packed), self.dim, 0), "set_clouds")
+        self.n = int(sizes.sum())
+        self.offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+
+    def initialize_clouds_device(self, dev_ptr, sizes):
+        """Same, for a packed batch already resident on the device."""
+        sizes = np.ascontiguousarray(sizes, dtype=np.int64)
+        self._sizes = sizes
+        self._chk(self._L.emst_set_clouds(self._c, len(sizes), _p(sizes), ctypes.c_void_p(dev_ptr), self.dim, 1),
+                  "set_clouds")
+        self.n = int(sizes.sum())
+        self.offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+
+    def cloud_results(self):
+        """Per cloud: (FP64 weight, edge count, completion round)."""
+        k = len(self.offsets) - 1
+        w = np.empty(k, np.float64)
+        e = np.empty(k, np.int32)
+        r = np.empty(k, np.int32)
+        self._chk(self._L.emst_get_cloud_results(self._c, _p(w), _p(e), _p(r)), "cloud_results")
+        return w, e, r
 
     def run(self):
         self._chk(self._L.emst_run(self._c), "run")

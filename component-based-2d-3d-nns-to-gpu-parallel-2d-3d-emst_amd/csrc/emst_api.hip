@@ -31,10 +31,18 @@ thread_local std::string g_create_error;
 
 constexpr int kWeightBlocks = 512;
 
+// Device buffer owned by a context (freed with it).
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
 };
 
 }  // namespace
@@ -43,12 +51,17 @@ struct emst_ctx {
     emst_options opt{};
     hipStream_t stream = nullptr;
     std::string err;
-    // point set
+    // point set: ncl clouds (one for emst_set_points), n points in all
     int64_t n = 0;
     int dim = 0;
     int K = 0;
     int bucket = 32;
-    int64_t L = 0;
+    int64_t L = 0;                    // leaves of cloud 0 (the single-cloud hierarchy)
+    int ncl = 0;
+    std::vector<Cloud> hcl;
+    int64_t nleaves = 0, Lmax = 0;
+    int mbits = 21;                   // Morton bits per axis
+    int64_t target_edges = 0;         // sum over clouds of (points - 1)
     const float* xyz = nullptr;       // input points on the device (AoS)
     bool have_points = false;
     bool built = false;
@@ -71,6 +84,10 @@ struct emst_ctx {
     DevBuf<float> ed2;
     DevBuf<double> partial;
     DevBuf<float> model, feat, pred, seed, ls;     // learned seed radius
+    DevBuf<Cloud> dcl;
+    DevBuf<Params> params;
+    DevBuf<double> cloud_w;
+    DevBuf<int32_t> cloud_e, cloud_r;
     bool learned = false;
     // coincident-point groups (DupState)
     DevBuf<int32_t> dup_rep, dup_list, dup_runs, dup_gslot, dup_bpos;
@@ -159,7 +176,7 @@ int alloc_all(emst_ctx* c)
     ENS(scan_tmp, scan_tmp_words((int64_t)radix_hist_words(n)) + scan_tmp_words(n) + 16);
     ENS(pts, n);
     ENS(inv, n);
-    ENS(nodes, 2 * c->L);
+    ENS(nodes, 2 * c->nleaves);
     ENS(cand_key, n * K);
     ENS(cand_pos, n * K);
     ENS(label, n);
@@ -228,16 +245,27 @@ int build(emst_ctx* c)
     const int64_t n = c->n;
     hipStream_t s = c->stream;
     HIPCHK(c, hipEventRecord(c->ev[0], s));
-    launch_bbox(c->xyz, n, c->dim, c->P, s);
-    launch_params(c->P, n, c->dim, s);
+    launch_bbox(c->xyz, c->dcl.p, c->ncl, n, c->dim, c->P, s);
+    launch_params(c->P, c->dcl.p, c->ncl, c->dim, c->mbits, s);
     LAUNCHCHK(c);
-    uint32_t nonfinite = 0;
-    HIPCHK(c, hipMemcpyAsync(&nonfinite, &c->P->nonfinite, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipStreamSynchronize(s));
-    if (nonfinite) return fail(c, EMST_E_NONFINITE, "input holds a NaN or Inf coordinate");
-    launch_rescale_morton(c->xyz, n, c->dim, c->P, c->resc.p, c->keys0.p, s);
-    radix_sort(c->keys0.p, c->vals0.p, c->keys1.p, c->vals1.p, n, c->dim == 3 ? 63 : 64, true,
-               c->hist.p, c->scan_tmp.p, s);
+    {
+        std::vector<Params> hp(c->ncl);
+        HIPCHK(c, hipMemcpyAsync(hp.data(), c->P, sizeof(Params) * c->ncl, hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        for (int k = 0; k < c->ncl; ++k)
+            if (hp[k].nonfinite)
+                return fail(c, EMST_E_NONFINITE,
+                            c->ncl == 1 ? std::string("input holds a NaN or Inf coordinate")
+                                        : "cloud " + std::to_string(k) + " holds a NaN or Inf coordinate");
+    }
+    launch_rescale_morton(c->xyz, c->dcl.p, c->ncl, n, c->dim, c->mbits, c->P, c->resc.p, c->keys0.p, s);
+    int key_bits = c->mbits * c->dim;
+    if (c->ncl > 1) {
+        int cb = 0;
+        while ((1ll << cb) < c->ncl) ++cb;
+        key_bits += cb;
+    }
+    radix_sort(c->keys0.p, c->vals0.p, c->keys1.p, c->vals1.p, n, key_bits, true, c->hist.p, c->scan_tmp.p, s);
     launch_gather_sorted(c->resc.p, c->vals0.p, n, c->dim, c->pts.p, c->inv.p, s);
     // coincident-point groups: dup_runs .. pad2 start at zero
     HIPCHK(c, hipMemsetAsync(&c->C->dup_runs, 0, 8 * sizeof(uint32_t), s));
@@ -263,12 +291,12 @@ int build(emst_ctx* c)
         }
     }
     const DupState dup = dup_state(c);
-    launch_build_tree(c->pts.p, n, c->bucket, c->L, c->nodes.p, dup.rep, s);
+    launch_build_tree(c->pts.p, c->dcl.p, c->ncl, c->nleaves, c->Lmax, c->bucket, c->nodes.p, dup.rep, s);
     launch_round_init(n, c->label.p, c->eu.p, c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->C, s);
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ev[1], s));
     const float* seed_r = nullptr;
-    if (c->learned) {
+    if (c->learned && c->ncl == 1) {
         int rc2;
         if ((rc2 = ensure(c, c->feat, (size_t)n * 8)) || (rc2 = ensure(c, c->pred, (size_t)n)) ||
             (rc2 = ensure(c, c->seed, (size_t)n)) || (rc2 = ensure(c, c->ls, (size_t)n)))
@@ -277,7 +305,7 @@ int build(emst_ctx* c)
         launch_radius_seed(n, c->model.p, c->feat.p, c->ls.p, c->pred.p, c->seed.p, s);
         seed_r = c->seed.p;
     }
-    launch_cone_search(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, nullptr, n, c->label.p, false,
+    launch_cone_search(c->dim, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, n, nullptr, n, c->label.p, false,
                        c->inv.p, seed_r, c->seed_scale, c->cand_key.p, c->cand_pos.p, nullptr, c->corr.p,
                        c->corr_key.p, c->C, true, dup, s);
     launch_dup_fill(c->dim, n, c->label.p, c->pts.p, c->inv.p, c->cand_key.p, c->cand_pos.p, nullptr, c->corr.p,
@@ -312,7 +340,7 @@ int build(emst_ctx* c)
     c->built = true;
     c->rounds = 0;
     c->edges = 0;
-    c->finished = n <= 1;
+    c->finished = c->target_edges == 0;
     c->trace.clear();
     return EMST_OK;
 }
@@ -338,7 +366,7 @@ int round_once(emst_ctx* c, int32_t* finished)
 {
     const int64_t n = c->n;
     hipStream_t s = c->stream;
-    if (c->finished || (int64_t)c->edges >= n - 1 || c->rounds >= c->opt.max_rounds) {
+    if (c->finished || (int64_t)c->edges >= c->target_edges || c->rounds >= c->opt.max_rounds) {
         c->finished = true;
         *finished = 1;
         return EMST_OK;
@@ -368,7 +396,7 @@ int round_once(emst_ctx* c, int32_t* finished)
         int rc = read_counters(c, &h);
         if (rc) return rc;
         if (h.research > 0)
-            launch_cone_search(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, c->list.p, h.research,
+            launch_cone_search(c->dim, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, n, c->list.p, h.research,
                                c->label.p, true, c->inv.p, nullptr, c->seed_scale, c->cand_key.p,
                                c->cand_pos.p, c->corr.p, nullptr, nullptr, c->C, false, DupState{}, s);
         LAUNCHCHK(c);
@@ -477,8 +505,7 @@ emst_ctx* emst_create(const emst_options* opt)
         return nullptr;
     }
     for (auto& e : c->ev) hipEventCreate(&e);
-    if (hipMalloc((void**)&c->P, sizeof(Params)) != hipSuccess ||
-        hipMalloc((void**)&c->C, sizeof(Counters)) != hipSuccess) {
+    if (hipMalloc((void**)&c->C, sizeof(Counters)) != hipSuccess) {
         g_create_error = "hipMalloc of context state failed";
         emst_destroy(c);
         return nullptr;
@@ -491,14 +518,6 @@ int emst_destroy(emst_ctx* c)
     if (!c) return EMST_E_ARG;
     hipSetDevice(c->opt.device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    release(c->xyz_own); release(c->resc); release(c->keys0); release(c->keys1); release(c->cand_key);
-    release(c->corr_key); release(c->comp1); release(c->vals0); release(c->vals1); release(c->hist);
-    release(c->scan_tmp); release(c->comp2); release(c->pts); release(c->inv); release(c->cand_pos);
-    release(c->label); release(c->label2); release(c->corr); release(c->fixed); release(c->list);
-    release(c->eu); release(c->ev_); release(c->eround); release(c->geo); release(c->nodes);
-    release(c->ed2); release(c->partial);
-    release(c->model); release(c->feat); release(c->pred); release(c->seed);
-    if (c->P) hipFree(c->P);
     if (c->C) hipFree(c->C);
     for (auto& e : c->ev) if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -512,23 +531,51 @@ const char* emst_last_error(const emst_ctx* c)
     return c->err.c_str();
 }
 
-int emst_set_points(emst_ctx* c, const float* xyz, int64_t n, int32_t dim, int32_t on_device)
+int emst_set_clouds(emst_ctx* c, int32_t ncl, const int64_t* npts, const float* xyz, int32_t dim,
+                    int32_t on_device)
 {
     if (!c) return EMST_E_ARG;
-    if (n < 0 || (n > 0 && !xyz)) return fail(c, EMST_E_ARG, "n < 0 or NULL points");
+    if (ncl <= 0 || !npts) return fail(c, EMST_E_ARG, "need at least one cloud");
     if (dim != 2 && dim != 3) return fail(c, EMST_E_ARG, "dim must be 2 or 3");
-    if (n >= ((int64_t)1 << 31) - 1) return fail(c, EMST_E_LIMIT, "n must be below 2^31 - 1");
+    int64_t n = 0;
+    for (int32_t k = 0; k < ncl; ++k) {
+        if (npts[k] < 0) return fail(c, EMST_E_ARG, "negative cloud size");
+        n += npts[k];
+    }
+    if (n > 0 && !xyz) return fail(c, EMST_E_ARG, "NULL points");
+    if (n >= ((int64_t)1 << 31) - 1) return fail(c, EMST_E_LIMIT, "total points must be below 2^31 - 1");
     HIPCHK(c, hipSetDevice(c->opt.device));
     c->n = n;
     c->dim = dim;
     c->K = 2 * dim * (1 << (dim - 1));
     c->bucket = choose_bucket(c->opt, dim);
-    const int64_t nb = std::max<int64_t>(1, (n + c->bucket - 1) / c->bucket);
-    int64_t L = 1;
-    while (L < nb) L <<= 1;
-    c->L = L;
-    int rc = alloc_all(c);
-    if (rc) return rc;
+    c->ncl = ncl;
+    c->hcl.assign(ncl, Cloud{});
+    int64_t pos = 0, lbase = 0, Lmax = 1, target = 0;
+    for (int32_t k = 0; k < ncl; ++k) {
+        const int64_t nb = std::max<int64_t>(1, (npts[k] + c->bucket - 1) / c->bucket);
+        int64_t L = 1;
+        while (L < nb) L <<= 1;
+        c->hcl[k] = Cloud{pos, npts[k], lbase, L};
+        pos += npts[k];
+        lbase += L;
+        Lmax = std::max(Lmax, L);
+        target += std::max<int64_t>(npts[k] - 1, 0);
+    }
+    c->nleaves = lbase;
+    c->Lmax = Lmax;
+    c->L = c->hcl[0].L;
+    c->target_edges = target;
+    // Morton bits per axis: the cloud index takes the top key bits of a batch
+    int cb = 0;
+    if (ncl > 1)
+        while ((1ll << cb) < ncl) ++cb;
+    c->mbits = std::min(dim == 3 ? 21 : 32, (64 - cb) / dim);
+    int rc;
+    if ((rc = alloc_all(c)) || (rc = ensure(c, c->dcl, (size_t)ncl)) || (rc = ensure(c, c->params, (size_t)ncl)))
+        return rc;
+    c->P = c->params.p;
+    HIPCHK(c, hipMemcpyAsync(c->dcl.p, c->hcl.data(), sizeof(Cloud) * ncl, hipMemcpyHostToDevice, c->stream));
     if (on_device) {
         c->xyz = xyz;
     } else {
@@ -536,9 +583,9 @@ int emst_set_points(emst_ctx* c, const float* xyz, int64_t n, int32_t dim, int32
         if (rc) return rc;
         if (n > 0)
             HIPCHK(c, hipMemcpyAsync(c->xyz_own.p, xyz, sizeof(float) * n * dim, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
         c->xyz = c->xyz_own.p;
     }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_points = true;
     c->built = false;
     c->no_grouping = false;
@@ -546,6 +593,35 @@ int emst_set_points(emst_ctx* c, const float* xyz, int64_t n, int32_t dim, int32
     c->rounds = 0;
     c->edges = 0;
     c->trace.clear();
+    return EMST_OK;
+}
+
+int emst_set_points(emst_ctx* c, const float* xyz, int64_t n, int32_t dim, int32_t on_device)
+{
+    if (!c) return EMST_E_ARG;
+    if (n < 0 || (n > 0 && !xyz)) return fail(c, EMST_E_ARG, "n < 0 or NULL points");
+    return emst_set_clouds(c, 1, &n, xyz, dim, on_device);
+}
+
+int emst_get_cloud_results(emst_ctx* c, double* weight, int32_t* edges, int32_t* last_round)
+{
+    int rc = need_points(c);
+    if (rc) return rc;
+    if (!c->built) return fail(c, EMST_E_STATE, "no run yet");
+    HIPCHK(c, hipSetDevice(c->opt.device));
+    const int ncl = c->ncl;
+    if ((rc = ensure(c, c->cloud_w, (size_t)ncl)) || (rc = ensure(c, c->cloud_e, (size_t)ncl)) ||
+        (rc = ensure(c, c->cloud_r, (size_t)ncl)))
+        return rc;
+    hipStream_t s = c->stream;
+    launch_cloud_results(c->xyz, c->dim, c->dcl.p, ncl, c->eu.p, c->ev_.p, c->eround.p, c->cloud_w.p,
+                         c->cloud_e.p, c->cloud_r.p, s);
+    LAUNCHCHK(c);
+    if (weight) HIPCHK(c, hipMemcpyAsync(weight, c->cloud_w.p, sizeof(double) * ncl, hipMemcpyDeviceToHost, s));
+    if (edges) HIPCHK(c, hipMemcpyAsync(edges, c->cloud_e.p, sizeof(int32_t) * ncl, hipMemcpyDeviceToHost, s));
+    if (last_round)
+        HIPCHK(c, hipMemcpyAsync(last_round, c->cloud_r.p, sizeof(int32_t) * ncl, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
     return EMST_OK;
 }
 
@@ -685,6 +761,7 @@ int emst_get_cell_grid(emst_ctx* c, int32_t R, int32_t* ext_dual, int32_t* cell_
     if (R <= 0) return fail(c, EMST_E_ARG, "level radius must be positive");
     int rc = ensure_built(c);
     if (rc) return rc;
+    if (c->ncl != 1) return fail(c, EMST_E_STATE, "the reference cell grid is defined for one cloud");
     Params hp{};
     HIPCHK(c, hipMemcpyAsync(&hp, c->P, sizeof(Params), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -778,6 +855,7 @@ int emst_get_radius_samples(emst_ctx* c, float* feat, float* target, int64_t* m)
     if (!m) return fail(c, EMST_E_ARG, "m must not be NULL");
     int rc = ensure_built(c);
     if (rc) return rc;
+    if (c->ncl != 1) return fail(c, EMST_E_STATE, "radius samples come from a single-cloud context");
     const int64_t n = c->n;
     if (!feat && !target) { *m = n; return EMST_OK; }
     if (*m < n) { *m = n; return fail(c, EMST_E_ARG, "sample arrays too small"); }

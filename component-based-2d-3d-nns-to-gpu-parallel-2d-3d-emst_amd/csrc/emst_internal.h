@@ -61,6 +61,20 @@ struct DupState {
     int64_t ngroups;
 };
 
+// One point cloud of a context (a context holds one cloud, or a batch of
+// independent clouds of one dimension run together, packed back to back in
+// one input array).  Cloud c owns the global ids and sorted positions
+// [pos, pos + n) (ids keep input order inside the cloud, so every tie breaks
+// as in a run of the cloud alone; its input coordinates start at float
+// pos * dim) and its own implicit binary hierarchy of L leaves (a power of
+// two) at nodes[2 * lbase + i], 1 <= i < 2L.
+struct Cloud {
+    int64_t pos;
+    int64_t n;
+    int64_t lbase;
+    int64_t L;
+};
+
 // Search-hierarchy node: tight box of a contiguous range of Morton-sorted points.
 struct alignas(32) Node {
     float lo[3];
@@ -69,10 +83,13 @@ struct alignas(32) Node {
 };
 
 // ---- launch wrappers (emst_kernels.hip); all asynchronous on `s` ----
-void launch_bbox(const float* xyz, int64_t n, int dim, Params* P, hipStream_t s);
-void launch_params(Params* P, int64_t n, int dim, hipStream_t s);
-void launch_rescale_morton(const float* xyz, int64_t n, int dim, const Params* P, float* resc,
-                           uint64_t* keys, hipStream_t s);
+// Per cloud: bounding box (A1), rescale parameters (A2); per point: rescaled
+// coordinates and the sort key (cloud index above mbits-per-axis Morton bits;
+// no cloud bits for a single cloud).  N = total points, P[ncl].
+void launch_bbox(const float* xyz, const Cloud* cl, int ncl, int64_t N, int dim, Params* P, hipStream_t s);
+void launch_params(Params* P, const Cloud* cl, int ncl, int dim, int mbits, hipStream_t s);
+void launch_rescale_morton(const float* xyz, const Cloud* cl, int ncl, int64_t N, int dim, int mbits,
+                           const Params* P, float* resc, uint64_t* keys, hipStream_t s);
 // LSD radix sort of (keys, vals) by bits [0, bits); result lands in (k0, v0).
 // vals_identity: values start as 0..n-1 (v0 is not read).
 void radix_sort(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, int64_t n, int bits,
@@ -87,12 +104,13 @@ void launch_gather_sorted(const float* resc, const uint32_t* perm, int64_t n, in
 void launch_dup_groups(const uint64_t* keys, const float4* pts, int64_t n, int32_t* rep,
                        int32_t* runs, int32_t* list, int32_t* gslot, Counters* C, hipStream_t s);
 // Leaf boxes leave out group members (rep == nullptr: grouping off).
-void launch_build_tree(const float4* pts, int64_t n, int bucket, int64_t L, Node* nodes,
-                       const int32_t* rep, hipStream_t s);
+// nleaves = sum of the clouds' L, Lmax = largest L.
+void launch_build_tree(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, int64_t Lmax, int bucket,
+                       Node* nodes, const int32_t* rep, hipStream_t s);
 // Cone search over queries [0,n) (qlist == nullptr) or qlist[0..nq).
 // check_labels: skip zero-distance points of the query's own component.
 // collect_stats: add the traversal counters to C.
-void launch_cone_search(int dim, const float4* pts, const Node* nodes, int64_t L, int bucket,
+void launch_cone_search(int dim, const float4* pts, const Node* nodes, const Cloud* cl, int ncl, int bucket,
                         int64_t n, const int32_t* qlist, int64_t nq, const int32_t* label,
                         bool check_labels, const int32_t* inv, const float* seed_r, float seed_scale,
                         uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
@@ -116,6 +134,11 @@ void launch_round(int K, int64_t n, const float4* pts, const uint64_t* cand_key,
                   int round, float* t_ms, hipEvent_t* ev_marks, hipStream_t s);
 void launch_weight(const float* xyz, int dim, const int32_t* eu, const int32_t* ev, int64_t m,
                    double* partial, int nblocks, hipStream_t s);
+// Per cloud: FP64 weight of its forest edges (input coordinates), its edge
+// count and the last round that added one of them.
+void launch_cloud_results(const float* xyz, int dim, const Cloud* cl, int ncl, const int32_t* eu,
+                          const int32_t* ev, const int32_t* eround, double* weight, int32_t* edges,
+                          int32_t* last_round, hipStream_t s);
 void launch_labels_out(int64_t n, const int32_t* label, const float4* pts, const int32_t* inv,
                        int32_t* out, hipStream_t s);
 void launch_cands_out(int K, int64_t n, const uint64_t* cand_key, const int32_t* inv,

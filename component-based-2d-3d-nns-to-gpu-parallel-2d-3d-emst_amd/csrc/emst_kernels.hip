@@ -49,59 +49,120 @@ __device__ __forceinline__ uint64_t lanemask_lt()
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// ------------------------------------------------------------ bbox (A1)
-// Params.pmin/pmax hold order-mapped uints during the reduction.
-__global__ void k_bbox_init(Params* P)
+// ------------------------------------------------------------ clouds
+// largest c with cl[c].pos <= p (the cloud holding global point p)
+__device__ __forceinline__ int find_cloud(const Cloud* cl, int ncl, int64_t p)
 {
-    uint32_t* lo = reinterpret_cast<uint32_t*>(P->pmin);
-    uint32_t* hi = reinterpret_cast<uint32_t*>(P->pmax);
-    for (int k = 0; k < 3; ++k) { lo[k] = 0xffffffffu; hi[k] = 0u; }
-    P->nonfinite = 0;
+    int lo = 0, hi = ncl - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cl[mid].pos <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_bbox(const float* __restrict__ xyz, int64_t n, Params* P)
+// largest c with cl[c].lbase <= g (the cloud holding global leaf g)
+__device__ __forceinline__ int find_cloud_leaf(const Cloud* cl, int ncl, int64_t g)
 {
+    int lo = 0, hi = ncl - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cl[mid].lbase <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ------------------------------------------------------------ bbox (A1)
+// Params.pmin/pmax hold order-mapped uints during the reduction.
+__global__ void k_bbox_init(Params* P, int ncl)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncl) return;
+    uint32_t* lo = reinterpret_cast<uint32_t*>(P[c].pmin);
+    uint32_t* hi = reinterpret_cast<uint32_t*>(P[c].pmax);
+    for (int k = 0; k < 3; ++k) { lo[k] = 0xffffffffu; hi[k] = 0u; }
+    P[c].nonfinite = 0;
+}
+
+// Each wave walks a contiguous chunk of global points with per-lane
+// accumulators for the chunk's current cloud, merged into P[cloud] (wave
+// reduction, then order-mapped atomics) whenever the cloud changes.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_bbox(const float* __restrict__ xyz, const Cloud* __restrict__ cl,
+                                                int ncl, int64_t N, Params* __restrict__ P)
+{
+    const int lane = __lane_id();
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / 64;
+    const int64_t chunk = ((N + nwaves - 1) / nwaves + 63) / 64 * 64;
+    const int64_t a = wave * chunk, b = min(N, a + chunk);
     uint32_t lo[D], hi[D];
     for (int k = 0; k < D; ++k) { lo[k] = 0xffffffffu; hi[k] = 0u; }
     uint32_t bad = 0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        for (int k = 0; k < D; ++k) {
-            float c = xyz[i * D + k];
-            bad |= !__builtin_isfinite(c);
-            uint32_t o = f2ord(c);
-            lo[k] = min(lo[k], o);
-            hi[k] = max(hi[k], o);
+    int cur = -1;
+    auto flush = [&]() {
+        for (int k = 0; k < D; ++k)
+            for (int off = 32; off > 0; off >>= 1) {
+                lo[k] = min(lo[k], (uint32_t)__shfl_xor((int)lo[k], off));
+                hi[k] = max(hi[k], (uint32_t)__shfl_xor((int)hi[k], off));
+            }
+        const bool any_bad = __any(bad);
+        if (lane == 0) {
+            uint32_t* plo = reinterpret_cast<uint32_t*>(P[cur].pmin);
+            uint32_t* phi = reinterpret_cast<uint32_t*>(P[cur].pmax);
+            for (int k = 0; k < D; ++k) {
+                atomicMin(&plo[k], lo[k]);
+                atomicMax(&phi[k], hi[k]);
+            }
+            if (any_bad) atomicOr(&P[cur].nonfinite, 1u);
+        }
+        for (int k = 0; k < D; ++k) { lo[k] = 0xffffffffu; hi[k] = 0u; }
+        bad = 0;
+    };
+    for (int64_t i0 = a; i0 < b; i0 += 64) {
+        const int64_t i = i0 + lane;
+        const int c = i < b ? find_cloud(cl, ncl, i) : -1;
+        uint64_t todo = __ballot(c >= 0);
+        while (todo) {
+            const int cc = __builtin_amdgcn_readfirstlane(__shfl(c, (int)__builtin_ctzll(todo)));
+            if (cc != cur) {
+                if (cur >= 0) flush();
+                cur = cc;
+            }
+            const bool mine = c == cc;
+            todo &= ~__ballot(mine);
+            if (mine) {
+                const float* q = xyz + i * D;
+                for (int k = 0; k < D; ++k) {
+                    const float v = q[k];
+                    bad |= !__builtin_isfinite(v);
+                    const uint32_t o = f2ord(v);
+                    lo[k] = min(lo[k], o);
+                    hi[k] = max(hi[k], o);
+                }
+            }
         }
     }
-    for (int k = 0; k < D; ++k)
-        for (int off = 32; off > 0; off >>= 1) {
-            lo[k] = min(lo[k], (uint32_t)__shfl_xor((int)lo[k], off));
-            hi[k] = max(hi[k], (uint32_t)__shfl_xor((int)hi[k], off));
-        }
-    bad = __any(bad);
-    if (__lane_id() == 0) {
-        uint32_t* plo = reinterpret_cast<uint32_t*>(P->pmin);
-        uint32_t* phi = reinterpret_cast<uint32_t*>(P->pmax);
-        for (int k = 0; k < D; ++k) {
-            atomicMin(&plo[k], lo[k]);
-            atomicMax(&phi[k], hi[k]);
-        }
-        if (bad) atomicOr(&P->nonfinite, 1u);
-    }
+    if (cur >= 0) flush();
 }
 
 // ---------------------------------------------------------- params (A2)
 // wg = (double)(pMax.x - pMin.x) etc. are float subtractions widened to double,
 // _wnd / unitX in double (SolutionEMST.cu:305-342).  The rescaled extremes are
 // the rescaled box corners: the map p -> (p - pMin) * unitXf is monotone.
-__global__ void k_params(Params* P, int64_t n, int dim)
+// One thread per cloud; Morton quantisation uses mbits bits per axis.
+__global__ void k_params(Params* Pa, const Cloud* __restrict__ cl, int ncl, int dim, int mbits)
 {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncl) return;
+    Params* P = Pa + c;
     uint32_t* plo = reinterpret_cast<uint32_t*>(P->pmin);
     uint32_t* phi = reinterpret_cast<uint32_t*>(P->pmax);
+    const int64_t n = cl[c].n;
     for (int k = 0; k < 3; ++k) {
-        if (k < dim) {
+        if (k < dim && n > 0) {
             P->pmin[k] = ord2f(plo[k]);
             P->pmax[k] = ord2f(phi[k]);
         } else {
@@ -126,8 +187,7 @@ __global__ void k_params(Params* P, int64_t n, int dim)
         P->newmax[k] = k < dim ? (P->pmax[k] - P->pmin[k]) * fu : 0.f;
         mx = fmaxf(mx, P->newmax[k]);
     }
-    const int bits = dim == 3 ? 21 : 32;
-    const double maxq = (double)((1ull << bits) - 1);
+    const double maxq = (double)((1ull << mbits) - 1);
     P->qscale = mx > 0 ? (float)(maxq / (double)mx) : 0.f;
 }
 
@@ -154,25 +214,31 @@ __device__ __forceinline__ uint64_t spread2(uint64_t x)
 }
 
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_rescale_morton(const float* __restrict__ xyz, int64_t n,
-                                                           const Params* __restrict__ P,
+__global__ __launch_bounds__(kBlock) void k_rescale_morton(const float* __restrict__ xyz,
+                                                           const Cloud* __restrict__ cl, int ncl,
+                                                           int64_t N, int mbits,
+                                                           const Params* __restrict__ Pa,
                                                            float* __restrict__ resc,
                                                            uint64_t* __restrict__ keys)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= N) return;
+    const int c = find_cloud(cl, ncl, i);
+    const Params* P = Pa + c;
+    const float* q = xyz + i * D;
     const float fu = P->unit_xf;
     const float qs = P->qscale;
-    const uint32_t maxq = D == 3 ? 0x1fffffu : 0xffffffffu;
+    const uint32_t maxq = (uint32_t)((1ull << mbits) - 1);
     uint64_t code = 0;
     for (int k = 0; k < D; ++k) {
-        float d = xyz[i * D + k] - P->pmin[k];     // PointE::operator- (Node.h:574-580)
+        float d = q[k] - P->pmin[k];                // PointE::operator- (Node.h:574-580)
         float v = d * fu;                           // PointE::operator*(float) (Node.h:588-593)
         resc[i * D + k] = v;
         float qf = v * qs;
-        uint32_t q = qf >= (float)maxq ? maxq : (uint32_t)qf;
-        code |= (D == 3 ? spread3(q) : spread2(q)) << k;
+        uint32_t qq = qf >= (float)maxq ? maxq : (uint32_t)qf;
+        code |= (D == 3 ? spread3(qq) : spread2(qq)) << k;
     }
+    if (ncl > 1) code |= (uint64_t)c << (mbits * D);
     keys[i] = code;
 }
 
@@ -461,18 +527,21 @@ __global__ void k_dup_finish(Counters* C)
 // smallest id, wins every tie at their common location), so they are left out
 // and a leaf of members only keeps the empty box (+inf, -inf), which no node
 // test accepts.
-__global__ __launch_bounds__(kBlock) void k_leaves(const float4* __restrict__ pts, int64_t n,
-                                                  int bucket, int64_t L, Node* __restrict__ nodes,
+__global__ __launch_bounds__(kBlock) void k_leaves(const float4* __restrict__ pts,
+                                                  const Cloud* __restrict__ cl, int ncl, int64_t nleaves,
+                                                  int bucket, Node* __restrict__ nodes,
                                                   const int32_t* __restrict__ rep)
 {
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= L) return;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nleaves) return;
+    const int c = find_cloud_leaf(cl, ncl, g);
+    const int64_t j = g - cl[c].lbase;          // the cloud's leaf j covers its points [j*bucket, ...)
     const bool grouped = rep != nullptr;
     Node nd;
     for (int k = 0; k < 3; ++k) { nd.lo[k] = INFINITY; nd.hi[k] = -INFINITY; }
     nd.pad[0] = nd.pad[1] = 0.f;
-    const int64_t s0 = b * bucket;
-    const int64_t s1 = min(n, s0 + bucket);
+    const int64_t s0 = cl[c].pos + j * bucket;
+    const int64_t s1 = min(cl[c].pos + cl[c].n, s0 + bucket);
     for (int64_t s = s0; s < s1; ++s) {
         if (grouped) {
             const int32_t r = rep[s];
@@ -483,22 +552,27 @@ __global__ __launch_bounds__(kBlock) void k_leaves(const float4* __restrict__ pt
         nd.lo[1] = fminf(nd.lo[1], p.y); nd.hi[1] = fmaxf(nd.hi[1], p.y);
         nd.lo[2] = fminf(nd.lo[2], p.z); nd.hi[2] = fmaxf(nd.hi[2], p.z);
     }
-    nodes[L + b] = nd;
+    nodes[2 * cl[c].lbase + cl[c].L + j] = nd;
 }
 
-__global__ __launch_bounds__(kBlock) void k_level(Node* __restrict__ nodes, int64_t lo_idx, int64_t cnt)
+// heap level [lo, 2 lo) of every cloud with more than lo leaves
+__global__ __launch_bounds__(kBlock) void k_level(Node* __restrict__ nodes, const Cloud* __restrict__ cl,
+                                                 int ncl, int64_t lo)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cnt) return;
-    const int64_t i = lo_idx + t;
-    const Node a = nodes[2 * i], b = nodes[2 * i + 1];
+    if (t >= (int64_t)ncl * lo) return;
+    const int c = (int)(t / lo);
+    if (lo >= cl[c].L) return;
+    Node* base = nodes + 2 * cl[c].lbase;
+    const int64_t i = lo + t % lo;
+    const Node a = base[2 * i], b = base[2 * i + 1];
     Node nd;
     for (int k = 0; k < 3; ++k) {
         nd.lo[k] = fminf(a.lo[k], b.lo[k]);
         nd.hi[k] = fmaxf(a.hi[k], b.hi[k]);
     }
     nd.pad[0] = nd.pad[1] = 0.f;
-    nodes[i] = nd;
+    base[i] = nd;
 }
 
 // --------------------------------------------------------- cone search
@@ -646,7 +720,8 @@ constexpr int kMaxCappedPasses = 6;
 // spacing of the point's own leaf cell ((box volume / points)^(1/D)).
 template <int D>
 __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ pts,
-                                                   const Node* __restrict__ nodes, int64_t L,
+                                                   const Node* __restrict__ nodes,
+                                                   const Cloud* __restrict__ clouds, int ncl,
                                                    int bucket, int64_t n,
                                                    const int32_t* __restrict__ qlist, int64_t nq,
                                                    const int32_t* __restrict__ label,
@@ -689,166 +764,182 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
 #pragma unroll
     for (int c = 0; c < C::K; ++c) best[c][lane] = kEmptyKey;
 
-    // seed radius
-    float r;
-    if (seed_r) {
-        r = seed_r[p];
-    } else {
-        const Node own = nodes[L + p / bucket];
-        const int64_t s0 = (p / bucket) * bucket;
-        const float cnt = (float)(min(n, s0 + bucket) - s0);
-        float vol = 1.f, ext = 0.f;
-        for (int k = 0; k < D; ++k) {
-            const float e = own.hi[k] - own.lo[k];
-            vol *= e;
-            ext = fmaxf(ext, e);
-        }
-        float sp = vol > 0.f ? powf(vol / cnt, 1.f / D) : ext / cnt;
-        if (!(sp > 0.f)) sp = 1e-6f;
-        r = seed_scale * sp;
-    }
-    float r2 = r * r;
-    uint32_t open = active ? (uint32_t)((1ull << C::K) - 1) : 0u;
-    int32_t inner = -1;
-    int32_t bound[C::K];
-    int32_t maxb = -1;
-
-    // packet centre (for near-first child order)
-    float cx = active ? px : 0.f, cy = active ? py : 0.f, cz = active ? pz : 0.f;
-    float na = active ? 1.f : 0.f;
-    for (int off = 32; off > 0; off >>= 1) {
-        cx += __shfl_xor(cx, off);
-        cy += __shfl_xor(cy, off);
-        cz += __shfl_xor(cz, off);
-        na += __shfl_xor(na, off);
-    }
-    cx /= na; cy /= na; cz /= na;
-    auto cdist = [&](const Node& x) {
-        const float dxx = fmaxf(fmaxf(x.lo[0] - cx, cx - x.hi[0]), 0.f);
-        const float dyy = fmaxf(fmaxf(x.lo[1] - cy, cy - x.hi[1]), 0.f);
-        const float dzz = fmaxf(fmaxf(x.lo[2] - cz, cz - x.hi[2]), 0.f);
-        return dxx * dxx + dyy * dyy + dzz * dzz;
-    };
-
     uint32_t st_nodes = 0, st_leaves = 0, st_pts = 0, st_pairs = 0;
-    for (int pass = 0; pass <= kMaxCappedPasses; ++pass) {
-        if (__ballot(open != 0u) == 0ull) break;
-        const bool last = pass == kMaxCappedPasses;
-        const int32_t cap = last ? 0x7fffffff : (int32_t)__float_as_uint(r2);
-        auto refresh = [&]() {
-            maxb = -1;
-#pragma unroll
-            for (int c = 0; c < C::K; ++c) {
-                const int32_t b = ((open >> c) & 1u) ? min(key_bound(best[c][lane]), cap) : -1;
-                bound[c] = b;
-                maxb = max(maxb, b);
+    // one traversal per cloud among the wave's queries (a packet of a single
+    // cloud, the usual case, makes one)
+    const int my_cl = active ? find_cloud(clouds, ncl, p) : -1;
+    uint64_t pending = __ballot(active);
+    while (pending) {
+        const int cl = __builtin_amdgcn_readfirstlane(__shfl(my_cl, (int)__builtin_ctzll(pending)));
+        const bool mine = my_cl == cl;
+        pending &= ~__ballot(mine);
+        const int64_t cpos = clouds[cl].pos, cend = clouds[cl].pos + clouds[cl].n;
+        const int64_t L = clouds[cl].L;
+        const Node* __restrict__ tree = nodes + 2 * clouds[cl].lbase;
+
+        // seed radius
+        float r = 1.f;
+        if (!mine) {
+            // not this traversal's query
+        } else if (seed_r) {
+            r = seed_r[p];
+        } else {
+            const int64_t lp = p - cpos;
+            const Node own = tree[L + lp / bucket];
+            const int64_t s0 = cpos + (lp / bucket) * bucket;
+            const float cnt = (float)(min(cend, s0 + bucket) - s0);
+            float vol = 1.f, ext = 0.f;
+            for (int k = 0; k < D; ++k) {
+                const float e = own.hi[k] - own.lo[k];
+                vol *= e;
+                ext = fmaxf(ext, e);
             }
+            float sp = vol > 0.f ? powf(vol / cnt, 1.f / D) : ext / cnt;
+            if (!(sp > 0.f)) sp = 1e-6f;
+            r = seed_scale * sp;
+        }
+        float r2 = r * r;
+        uint32_t open = mine ? (uint32_t)((1ull << C::K) - 1) : 0u;
+        int32_t inner = -1;
+        int32_t bound[C::K];
+        int32_t maxb = -1;
+
+        // packet centre (for near-first child order)
+        float cx = mine ? px : 0.f, cy = mine ? py : 0.f, cz = mine ? pz : 0.f;
+        float na = mine ? 1.f : 0.f;
+        for (int off = 32; off > 0; off >>= 1) {
+            cx += __shfl_xor(cx, off);
+            cy += __shfl_xor(cy, off);
+            cz += __shfl_xor(cz, off);
+            na += __shfl_xor(na, off);
+        }
+        cx /= na; cy /= na; cz /= na;
+        auto cdist = [&](const Node& x) {
+            const float dxx = fmaxf(fmaxf(x.lo[0] - cx, cx - x.hi[0]), 0.f);
+            const float dyy = fmaxf(fmaxf(x.lo[1] - cy, cy - x.hi[1]), 0.f);
+            const float dzz = fmaxf(fmaxf(x.lo[2] - cz, cz - x.hi[2]), 0.f);
+            return dxx * dxx + dyy * dyy + dzz * dzz;
         };
-        refresh();
-        int sp = 0;
-        int ver = 0;        // bumps whenever a leaf tightened the bounds
-        if (lane == 0) { stk_box[0] = nodes[1]; stk_id[0] = 1; stk_ver[0] = -1; }
-        sp = 1;
-        __builtin_amdgcn_wave_barrier();
-        while (sp > 0) {
-            --sp;
-            const int node = __builtin_amdgcn_readfirstlane(stk_id[sp]);
-            const int pver = __builtin_amdgcn_readfirstlane(stk_ver[sp]);
-            const Node nd = stk_box[sp];
-            // a node tested against the current bounds when pushed keeps its lane mask
-            bool need;
-            if (pver == ver) {
-                need = (stk_mask[sp] >> lane) & 1ull;
-            } else {
-                need = open != 0u && node_needed<D>(nd, px, py, pz, bound, maxb, inner, grouped);
-                ++st_nodes;
-            }
-            const uint64_t needm = __ballot(need);
-            if (needm == 0ull) continue;
-            if (node >= L) {
-                const int64_t s0 = (int64_t)(node - L) * bucket;
-                const int cnt = (int)(min(n, s0 + bucket) - s0);
-                ++st_leaves;
-                st_pts += (uint32_t)cnt;
-                st_pairs += (uint32_t)cnt * (uint32_t)__popcll(needm);
-                // pair-parallel: lane = (query slot, leaf point); a chunk covers
-                // 64 / bucket needed queries x the leaf's points
-                const int j = lane & (bucket - 1);
-                const int qslot = lane / bucket;
-                const int qpc = 64 / bucket;
-                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (j < cnt) v = pts[s0 + j];
-                const int64_t s = s0 + j;
-                uint64_t rem = needm;
-                while (rem) {
-                    // the qslot-th remaining needed lane is this lane's query
-                    int myq = -1;
-                    for (int k = 0; k < qpc; ++k) {
-                        const int l = rem ? (int)__builtin_ctzll(rem) : -1;
-                        if (rem) rem &= rem - 1;
-                        if (k == qslot) myq = l;
-                    }
-                    const int src = myq >= 0 ? myq : 0;
-                    const float qx = __shfl(px, src), qy = __shfl(py, src), qz = __shfl(pz, src);
-                    const int32_t qmaxb = __shfl(maxb, src), qinner = __shfl(inner, src);
-                    const uint32_t qopen = (uint32_t)__shfl((int)open, src);
-                    const int32_t qpos = __shfl((int32_t)p, src);
-                    const float dx = v.x - qx, dy = v.y - qy, dz = v.z - qz;
-                    const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
-                    const int32_t db = (int32_t)__float_as_uint(d2);
-                    const bool here = dx == 0.f && dy == 0.f && dz == 0.f;
-                    bool take = myq >= 0 && j < cnt && (grouped ? !here : s != (int64_t)qpos) &&
-                                db <= qmaxb && db > qinner;
-                    // distinct points at float distance 0 (underflow) break the
-                    // grouping's premise; the caller then searches ungrouped
-                    if (grouped && take && d2 == 0.f) underflow = true;
-                    if (check_labels) {
-                        const int32_t qlab = __shfl(mylab, src);
-                        if (take && d2 == 0.f) take = label[s] != qlab;
-                    }
-                    if (take) {
-                        uint32_t mask = cone_mask<D>(dx, dy, dz) & qopen;
-                        const unsigned long long key =
-                            ((unsigned long long)__float_as_uint(d2) << 32) | __float_as_uint(v.w);
-                        while (mask) {
-                            const int c = __builtin_ctz(mask);
-                            mask &= mask - 1;
-                            atomicMin(&best[c][myq], key);
+
+        for (int pass = 0; pass <= kMaxCappedPasses; ++pass) {
+            if (__ballot(open != 0u) == 0ull) break;
+            const bool last = pass == kMaxCappedPasses;
+            const int32_t cap = last ? 0x7fffffff : (int32_t)__float_as_uint(r2);
+            auto refresh = [&]() {
+                maxb = -1;
+#pragma unroll
+                for (int c = 0; c < C::K; ++c) {
+                    const int32_t b = ((open >> c) & 1u) ? min(key_bound(best[c][lane]), cap) : -1;
+                    bound[c] = b;
+                    maxb = max(maxb, b);
+                }
+            };
+            refresh();
+            int sp = 0;
+            int ver = 0;        // bumps whenever a leaf tightened the bounds
+            if (lane == 0) { stk_box[0] = tree[1]; stk_id[0] = 1; stk_ver[0] = -1; }
+            sp = 1;
+            __builtin_amdgcn_wave_barrier();
+            while (sp > 0) {
+                --sp;
+                const int node = __builtin_amdgcn_readfirstlane(stk_id[sp]);
+                const int pver = __builtin_amdgcn_readfirstlane(stk_ver[sp]);
+                const Node nd = stk_box[sp];
+                // a node tested against the current bounds when pushed keeps its lane mask
+                bool need;
+                if (pver == ver) {
+                    need = (stk_mask[sp] >> lane) & 1ull;
+                } else {
+                    need = open != 0u && node_needed<D>(nd, px, py, pz, bound, maxb, inner, grouped);
+                    ++st_nodes;
+                }
+                const uint64_t needm = __ballot(need);
+                if (needm == 0ull) continue;
+                if (node >= L) {
+                    const int64_t s0 = cpos + (int64_t)(node - L) * bucket;
+                    const int cnt = (int)max<int64_t>(min(cend, s0 + bucket) - s0, 0);
+                    ++st_leaves;
+                    st_pts += (uint32_t)cnt;
+                    st_pairs += (uint32_t)cnt * (uint32_t)__popcll(needm);
+                    // pair-parallel: lane = (query slot, leaf point); a chunk covers
+                    // 64 / bucket needed queries x the leaf's points
+                    const int j = lane & (bucket - 1);
+                    const int qslot = lane / bucket;
+                    const int qpc = 64 / bucket;
+                    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (j < cnt) v = pts[s0 + j];
+                    const int64_t s = s0 + j;
+                    uint64_t rem = needm;
+                    while (rem) {
+                        // the qslot-th remaining needed lane is this lane's query
+                        int myq = -1;
+                        for (int k = 0; k < qpc; ++k) {
+                            const int l = rem ? (int)__builtin_ctzll(rem) : -1;
+                            if (rem) rem &= rem - 1;
+                            if (k == qslot) myq = l;
+                        }
+                        const int src = myq >= 0 ? myq : 0;
+                        const float qx = __shfl(px, src), qy = __shfl(py, src), qz = __shfl(pz, src);
+                        const int32_t qmaxb = __shfl(maxb, src), qinner = __shfl(inner, src);
+                        const uint32_t qopen = (uint32_t)__shfl((int)open, src);
+                        const int32_t qpos = __shfl((int32_t)p, src);
+                        const float dx = v.x - qx, dy = v.y - qy, dz = v.z - qz;
+                        const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
+                        const int32_t db = (int32_t)__float_as_uint(d2);
+                        const bool here = dx == 0.f && dy == 0.f && dz == 0.f;
+                        bool take = myq >= 0 && j < cnt && (grouped ? !here : s != (int64_t)qpos) &&
+                                    db <= qmaxb && db > qinner;
+                        // distinct points at float distance 0 (underflow) break the
+                        // grouping's premise; the caller then searches ungrouped
+                        if (grouped && take && d2 == 0.f) underflow = true;
+                        if (check_labels) {
+                            const int32_t qlab = __shfl(mylab, src);
+                            if (take && d2 == 0.f) take = label[s] != qlab;
+                        }
+                        if (take) {
+                            uint32_t mask = cone_mask<D>(dx, dy, dz) & qopen;
+                            const unsigned long long key =
+                                ((unsigned long long)__float_as_uint(d2) << 32) | __float_as_uint(v.w);
+                            while (mask) {
+                                const int c = __builtin_ctz(mask);
+                                mask &= mask - 1;
+                                atomicMin(&best[c][myq], key);
+                            }
                         }
                     }
-                }
-                if (need) refresh();
-                ++ver;
-            } else {
-                const int a = 2 * node, b = 2 * node + 1;
-                const Node boxa = nodes[a], boxb = nodes[b];
-                const unsigned long long ma =
-                    __ballot(open != 0u && node_needed<D>(boxa, px, py, pz, bound, maxb, inner, grouped));
-                const unsigned long long mb =
-                    __ballot(open != 0u && node_needed<D>(boxb, px, py, pz, bound, maxb, inner, grouped));
-                st_nodes += 2;
-                const bool ua = ma != 0ull, ub = mb != 0ull;
-                const bool afirst = cdist(boxa) <= cdist(boxb);
-                if (lane == 0) {
-                    int t = sp;
-                    if (afirst) {
-                        if (ub) { stk_box[t] = boxb; stk_id[t] = b; stk_ver[t] = ver; stk_mask[t] = mb; ++t; }
-                        if (ua) { stk_box[t] = boxa; stk_id[t] = a; stk_ver[t] = ver; stk_mask[t] = ma; }
-                    } else {
-                        if (ua) { stk_box[t] = boxa; stk_id[t] = a; stk_ver[t] = ver; stk_mask[t] = ma; ++t; }
-                        if (ub) { stk_box[t] = boxb; stk_id[t] = b; stk_ver[t] = ver; stk_mask[t] = mb; }
+                    if (need) refresh();
+                    ++ver;
+                } else {
+                    const int a = 2 * node, b = 2 * node + 1;
+                    const Node boxa = tree[a], boxb = tree[b];
+                    const unsigned long long ma =
+                        __ballot(open != 0u && node_needed<D>(boxa, px, py, pz, bound, maxb, inner, grouped));
+                    const unsigned long long mb =
+                        __ballot(open != 0u && node_needed<D>(boxb, px, py, pz, bound, maxb, inner, grouped));
+                    st_nodes += 2;
+                    const bool ua = ma != 0ull, ub = mb != 0ull;
+                    const bool afirst = cdist(boxa) <= cdist(boxb);
+                    if (lane == 0) {
+                        int t = sp;
+                        if (afirst) {
+                            if (ub) { stk_box[t] = boxb; stk_id[t] = b; stk_ver[t] = ver; stk_mask[t] = mb; ++t; }
+                            if (ua) { stk_box[t] = boxa; stk_id[t] = a; stk_ver[t] = ver; stk_mask[t] = ma; }
+                        } else {
+                            if (ua) { stk_box[t] = boxa; stk_id[t] = a; stk_ver[t] = ver; stk_mask[t] = ma; ++t; }
+                            if (ub) { stk_box[t] = boxb; stk_id[t] = b; stk_ver[t] = ver; stk_mask[t] = mb; }
+                        }
                     }
+                    sp += (int)ua + (int)ub;
+                    __builtin_amdgcn_wave_barrier();
                 }
-                sp += (int)ua + (int)ub;
-                __builtin_amdgcn_wave_barrier();
             }
-        }
-        // close the cones whose best lies within this pass's radius
+            // close the cones whose best lies within this pass's radius
 #pragma unroll
-        for (int c = 0; c < C::K; ++c)
-            if (((open >> c) & 1u) && key_bound(best[c][lane]) <= cap) open &= ~(1u << c);
-        inner = cap;
-        r2 *= 4.f;
+            for (int c = 0; c < C::K; ++c)
+                if (((open >> c) & 1u) && key_bound(best[c][lane]) <= cap) open &= ~(1u << c);
+            inner = cap;
+            r2 *= 4.f;
+        }
     }
     if (collect_stats && lane == 0) {
         atomicAdd(&ctr->packets, 1ull);
@@ -1232,6 +1323,60 @@ __global__ __launch_bounds__(kBlock) void k_weight(const float* __restrict__ xyz
     }
 }
 
+// Per cloud (one block each): FP64 weight of its edges in input coordinates,
+// edge count, last round that added an edge.  Edges sit at the sorted
+// position of the root they absorbed, inside the cloud's range.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_cloud_results(const float* __restrict__ xyz,
+                                                         const Cloud* __restrict__ cl,
+                                                         const int32_t* __restrict__ eu,
+                                                         const int32_t* __restrict__ ev,
+                                                         const int32_t* __restrict__ eround,
+                                                         double* __restrict__ weight,
+                                                         int32_t* __restrict__ edges,
+                                                         int32_t* __restrict__ last_round)
+{
+    __shared__ double shw[kBlock / 64];
+    __shared__ int32_t shc[kBlock / 64], shr[kBlock / 64];
+    const int c = blockIdx.x;
+    const int64_t a = cl[c].pos, b = a + cl[c].n;
+    double s = 0;
+    int32_t cnt = 0, lr = 0;
+    for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) {
+        if (eu[i] < 0) continue;
+        const float* p = xyz + (int64_t)eu[i] * D;
+        const float* q = xyz + (int64_t)ev[i] * D;
+        const float dx = p[0] - q[0], dy = p[1] - q[1];
+        const float d2 = D == 3 ? d2_of(dx, dy, p[2] - q[2]) : d2_of(dx, dy);
+        s += sqrt((double)d2);
+        ++cnt;
+        lr = max(lr, eround[i]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off);
+        cnt += __shfl_xor(cnt, off);
+        lr = max(lr, __shfl_xor(lr, off));
+    }
+    if (__lane_id() == 0) {
+        shw[threadIdx.x / 64] = s;
+        shc[threadIdx.x / 64] = cnt;
+        shr[threadIdx.x / 64] = lr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        int32_t tc = 0, tr = 0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            t += shw[w];
+            tc += shc[w];
+            tr = max(tr, shr[w]);
+        }
+        weight[c] = t;
+        edges[c] = tc;
+        last_round[c] = tr;
+    }
+}
+
 // ------------------------------------------------------------ outputs
 __global__ __launch_bounds__(kBlock) void k_labels_out(int64_t n, const int32_t* __restrict__ label,
                                                       const float4* __restrict__ pts,
@@ -1316,26 +1461,30 @@ inline unsigned nblocks(int64_t n, int b = kBlock) { return (unsigned)((n + b - 
 }  // namespace
 
 // ================================================================ launchers
-void launch_bbox(const float* xyz, int64_t n, int dim, Params* P, hipStream_t s)
+void launch_bbox(const float* xyz, const Cloud* cl, int ncl, int64_t N, int dim, Params* P, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_bbox_init, dim3(1), dim3(1), 0, s, P);
-    if (n <= 0) return;
-    const unsigned g = (unsigned)std::min<int64_t>(nblocks(n), 2048);
-    if (dim == 3) hipLaunchKernelGGL(k_bbox<3>, dim3(g), dim3(kBlock), 0, s, xyz, n, P);
-    else hipLaunchKernelGGL(k_bbox<2>, dim3(g), dim3(kBlock), 0, s, xyz, n, P);
+    hipLaunchKernelGGL(k_bbox_init, dim3(nblocks(ncl)), dim3(kBlock), 0, s, P, ncl);
+    if (N <= 0) return;
+    const unsigned g = (unsigned)std::min<int64_t>(nblocks(N), 2048);
+    if (dim == 3) hipLaunchKernelGGL(k_bbox<3>, dim3(g), dim3(kBlock), 0, s, xyz, cl, ncl, N, P);
+    else hipLaunchKernelGGL(k_bbox<2>, dim3(g), dim3(kBlock), 0, s, xyz, cl, ncl, N, P);
 }
 
-void launch_params(Params* P, int64_t n, int dim, hipStream_t s)
+void launch_params(Params* P, const Cloud* cl, int ncl, int dim, int mbits, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_params, dim3(1), dim3(1), 0, s, P, n, dim);
+    hipLaunchKernelGGL(k_params, dim3(nblocks(ncl)), dim3(kBlock), 0, s, P, cl, ncl, dim, mbits);
 }
 
-void launch_rescale_morton(const float* xyz, int64_t n, int dim, const Params* P, float* resc,
-                           uint64_t* keys, hipStream_t s)
+void launch_rescale_morton(const float* xyz, const Cloud* cl, int ncl, int64_t N, int dim, int mbits,
+                           const Params* P, float* resc, uint64_t* keys, hipStream_t s)
 {
-    if (n <= 0) return;
-    if (dim == 3) hipLaunchKernelGGL(k_rescale_morton<3>, dim3(nblocks(n)), dim3(kBlock), 0, s, xyz, n, P, resc, keys);
-    else hipLaunchKernelGGL(k_rescale_morton<2>, dim3(nblocks(n)), dim3(kBlock), 0, s, xyz, n, P, resc, keys);
+    if (N <= 0) return;
+    if (dim == 3)
+        hipLaunchKernelGGL(k_rescale_morton<3>, dim3(nblocks(N)), dim3(kBlock), 0, s, xyz, cl, ncl, N, mbits, P,
+                           resc, keys);
+    else
+        hipLaunchKernelGGL(k_rescale_morton<2>, dim3(nblocks(N)), dim3(kBlock), 0, s, xyz, cl, ncl, N, mbits, P,
+                           resc, keys);
 }
 
 size_t scan_tmp_words(int64_t n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 1; }
@@ -1400,15 +1549,16 @@ void launch_dup_groups(const uint64_t* keys, const float4* pts, int64_t n, int32
     hipLaunchKernelGGL(k_dup_finish, dim3(1), dim3(1), 0, s, C);
 }
 
-void launch_build_tree(const float4* pts, int64_t n, int bucket, int64_t L, Node* nodes,
-                       const int32_t* rep, hipStream_t s)
+void launch_build_tree(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, int64_t Lmax, int bucket,
+                       Node* nodes, const int32_t* rep, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_leaves, dim3(nblocks(L)), dim3(kBlock), 0, s, pts, n, bucket, L, nodes, rep);
-    for (int64_t lo = L / 2; lo >= 1; lo /= 2)
-        hipLaunchKernelGGL(k_level, dim3(nblocks(lo)), dim3(kBlock), 0, s, nodes, lo, lo);
+    hipLaunchKernelGGL(k_leaves, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, pts, cl, ncl, nleaves, bucket, nodes,
+                       rep);
+    for (int64_t lo = Lmax / 2; lo >= 1; lo /= 2)
+        hipLaunchKernelGGL(k_level, dim3(nblocks((int64_t)ncl * lo)), dim3(kBlock), 0, s, nodes, cl, ncl, lo);
 }
 
-void launch_cone_search(int dim, const float4* pts, const Node* nodes, int64_t L, int bucket,
+void launch_cone_search(int dim, const float4* pts, const Node* nodes, const Cloud* cl, int ncl, int bucket,
                         int64_t n, const int32_t* qlist, int64_t nq, const int32_t* label,
                         bool check_labels, const int32_t* inv, const float* seed_r, float seed_scale,
                         uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
@@ -1419,11 +1569,11 @@ void launch_cone_search(int dim, const float4* pts, const Node* nodes, int64_t L
     const unsigned g = nblocks(nq, 64);
     const int cs = collect_stats ? 1 : 0;
     if (dim == 3)
-        hipLaunchKernelGGL(k_cone_search<3>, dim3(g), dim3(64), 0, s, pts, nodes, L, bucket, n, qlist, nq,
+        hipLaunchKernelGGL(k_cone_search<3>, dim3(g), dim3(64), 0, s, pts, nodes, cl, ncl, bucket, n, qlist, nq,
                            label, check_labels ? 1 : 0, inv, seed_r, seed_scale, cand_key, cand_pos, revive,
                            first_corr, first_key, C, cs, dup);
     else
-        hipLaunchKernelGGL(k_cone_search<2>, dim3(g), dim3(64), 0, s, pts, nodes, L, bucket, n, qlist, nq,
+        hipLaunchKernelGGL(k_cone_search<2>, dim3(g), dim3(64), 0, s, pts, nodes, cl, ncl, bucket, n, qlist, nq,
                            label, check_labels ? 1 : 0, inv, seed_r, seed_scale, cand_key, cand_pos, revive,
                            first_corr, first_key, C, cs, dup);
 }
@@ -1493,6 +1643,19 @@ void launch_weight(const float* xyz, int dim, const int32_t* eu, const int32_t* 
 {
     if (dim == 3) hipLaunchKernelGGL(k_weight<3>, dim3(nblk), dim3(kBlock), 0, s, xyz, eu, ev, m, partial);
     else hipLaunchKernelGGL(k_weight<2>, dim3(nblk), dim3(kBlock), 0, s, xyz, eu, ev, m, partial);
+}
+
+void launch_cloud_results(const float* xyz, int dim, const Cloud* cl, int ncl, const int32_t* eu,
+                          const int32_t* ev, const int32_t* eround, double* weight, int32_t* edges,
+                          int32_t* last_round, hipStream_t s)
+{
+    if (ncl <= 0) return;
+    if (dim == 3)
+        hipLaunchKernelGGL(k_cloud_results<3>, dim3(ncl), dim3(kBlock), 0, s, xyz, cl, eu, ev, eround, weight,
+                           edges, last_round);
+    else
+        hipLaunchKernelGGL(k_cloud_results<2>, dim3(ncl), dim3(kBlock), 0, s, xyz, cl, eu, ev, eround, weight,
+                           edges, last_round);
 }
 
 void launch_labels_out(int64_t n, const int32_t* label, const float4* pts, const int32_t* inv,

@@ -66,9 +66,11 @@ extern "C" int emst_synth(int32_t kind, int64_t n, int32_t dim, uint64_t seed, f
         for (int64_t i = 0; i < n * dim; ++i) out[i] = g.u();
         return EMST_OK;
     }
-    if (kind == 1) {
-        // 64 isotropic Gaussians, sigma log-uniform in [0.002, 0.02], 10% uniform background
-        const int nc = 64;
+    if (kind == 1 || kind == 3) {
+        // kind 1: 64 isotropic Gaussians, sigma log-uniform in [0.002, 0.02],
+        // 10% uniform background; kind 3: 8 such Gaussians, no background
+        const int nc = kind == 1 ? 64 : 8;
+        const float bg = kind == 1 ? 0.1f : 0.f;
         double cen[64][3], sig[64];
         for (int c = 0; c < nc; ++c) {
             for (int k = 0; k < dim; ++k) cen[c][k] = g.u();
@@ -77,7 +79,7 @@ extern "C" int emst_synth(int32_t kind, int64_t n, int32_t dim, uint64_t seed, f
         }
         for (int64_t i = 0; i < n; ++i) {
             float* p = out + i * dim;
-            if (g.u() < 0.1f) {
+            if (g.u() < bg) {
                 for (int k = 0; k < dim; ++k) p[k] = g.u();
             } else {
                 int c = (int)(g.u() * nc);

@@ -101,6 +101,24 @@ const char* emst_last_error(const emst_ctx* ctx);
 int emst_set_points(emst_ctx* ctx, const float* xyz, int64_t n, int32_t dim,
                     int32_t xyz_on_device);
 
+/* Batched form of emst_set_points (the batched entry of the drop-in surface;
+ * no counterpart in the reference, which runs one cloud per process):
+ * ncl independent clouds of one dimension, packed back to back in xyz (cloud
+ * c's npts[c] points follow cloud c-1's, AoS).  Global point ids run over
+ * the packed array (cloud c's point j has id offset_c + j, offset_c = sum of
+ * npts[<c]); every cloud is rescaled, searched and hooked on its own, so its
+ * forest, weight and per-round edges equal a run of that cloud alone.
+ * emst_run / emst_step advance all clouds together; emst_get_edges returns
+ * global ids; emst_get_cloud_results the per-cloud figures.  The learned
+ * seed radius and the reference cell grid are single-cloud only. */
+int emst_set_clouds(emst_ctx* ctx, int32_t ncl, const int64_t* npts, const float* xyz, int32_t dim,
+                    int32_t xyz_on_device);
+
+/* Per cloud of the last set_clouds / set_points: FP64 forest weight (input
+ * coordinates), forest edge count, and the last round that added one of its
+ * edges (its completion round).  Arrays of ncl entries; any may be NULL. */
+int emst_get_cloud_results(emst_ctx* ctx, double* weight, int32_t* edges, int32_t* last_round);
+
 /* Replaces SolutionEMST::run() (emst_coalition/src/SolutionEMSTOperators.cu:146-153):
  * Boruvka rounds until one tree remains, no component has an outgoing
  * candidate (a forest), or max_rounds is reached. */
@@ -219,7 +237,8 @@ int emst_mlp_grad(int32_t device, const float* model, const float* feat, const f
  * u = (x >> 8) * 2^-24.  kind: 0 uniform [0,1)^dim; 1 Gaussian mixture
  * (64 centres, sigma log-uniform 0.002-0.02, 10% uniform background, clamped
  * to the unit cube); 2 power-law (16 centres, radial density r^-1.5,
- * r in [1e-4, 0.5]).  out: n x dim float32 host memory. */
+ * r in [1e-4, 0.5]); 3 eight Gaussians (sigma as kind 1, no background).
+ * out: n x dim float32 host memory. */
 int emst_synth(int32_t kind, int64_t n, int32_t dim, uint64_t seed, float* out);
 
 /* Library version string. */
