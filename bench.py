@@ -56,6 +56,7 @@ def main():
                     help="batched workload instead: CLOUDS clouds, sizes log-uniform 1e2-1e5, half 2D half "
                          "3D, each uniform or 8 Gaussians (generator seed --batch-seed)")
     ap.add_argument("--batch-seed", type=int, default=5)
+    ap.add_argument("--no-autotune", action="store_true", help="batch: keep the default leaf size and seed")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -208,8 +209,11 @@ def run_batch(args, pkg):
     torch.cuda.synchronize()
     total = int(sizes.sum())
 
+    tuned = [(0, 0.0)] * len(groups)
+
     def make(trace):
-        return [pkg.Calculateur(dim=g["dim"], trace=trace) for g in groups]
+        return [pkg.Calculateur(dim=g["dim"], trace=trace, bucket_size=b, seed_scale=sc)
+                for g, (b, sc) in zip(groups, tuned)]
 
     def step(ctxs):
         errs = []
@@ -230,6 +234,12 @@ def run_batch(args, pkg):
             raise errs[0]
 
     ctxs = make(False)
+    if not args.no_autotune:
+        # untimed: leaf size and seed scale per dimension from the autotuner
+        for i, (c, g) in enumerate(zip(ctxs, groups)):
+            c.initialize_clouds_device(g["dev"].data_ptr(), g["sizes"])
+            b, sc, _ = c.autotune()
+            tuned[i] = (b, sc)
     for _ in range(args.warmup):
         step(ctxs)
     torch.cuda.synchronize()
@@ -265,6 +275,7 @@ def run_batch(args, pkg):
         "config": {"workload": f"{B} clouds, sizes log-uniform 1e2-1e5, half 2D half 3D, uniform or 8 Gaussians",
                    "clouds": B, "points": total, "edges": edges,
                    "clouds_2d": int((dims == 2).sum()), "points_2d": int(sizes[dims == 2].sum()),
+                   "autotuned_bucket_scale": {f"{g['dim']}d": list(t) for g, t in zip(groups, tuned)},
                    "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
                    "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
                    "latency_ms_max": round(float(lat.max()), 3)},

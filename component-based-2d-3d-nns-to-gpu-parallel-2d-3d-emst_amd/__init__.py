@@ -26,7 +26,7 @@ EXPORTS = [
     "emst_get_rescaled", "emst_get_trace", "emst_get_candidates", "emst_get_cell_grid",
     "emst_get_times", "emst_get_search_stats", "emst_synth", "emst_version",
     "emst_set_radius_model", "emst_get_radius_samples", "emst_mlp_train", "emst_mlp_eval", "emst_mlp_grad",
-    "emst_set_clouds", "emst_get_cloud_results",
+    "emst_set_clouds", "emst_get_cloud_results", "emst_autotune",
 ]
 
 MLP_PARAMS = 4818
@@ -95,6 +95,7 @@ def lib():
     L.emst_mlp_grad.argtypes = [i32, P, P, P, i64, P, P]
     L.emst_set_clouds.argtypes = [P, i32, P, P, i32, i32]
     L.emst_get_cloud_results.argtypes = [P, P, P, P]
+    L.emst_autotune.argtypes = [P, P, P, P]
     L.emst_version.restype = ctypes.c_char_p
     _lib = L
     return L
@@ -183,6 +184,15 @@ class Calculateur:
                   "set_clouds")
         self.n = int(sizes.sum())
         self.offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+
+    def autotune(self):
+        """Leaf size and seed scale picked by timing the current points; kept
+        for later runs.  Returns (bucket, scale, build + search ms)."""
+        b = ctypes.c_int32(0)
+        sc = ctypes.c_float(0)
+        ms = ctypes.c_float(0)
+        self._chk(self._L.emst_autotune(self._c, ctypes.byref(b), ctypes.byref(sc), ctypes.byref(ms)), "autotune")
+        return b.value, sc.value, ms.value
 
     def cloud_results(self):
         """Per cloud: (FP64 weight, edge count, completion round)."""

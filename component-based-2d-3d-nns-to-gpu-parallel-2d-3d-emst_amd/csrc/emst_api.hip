@@ -15,14 +15,6 @@
 #include "../../include/emst.h"
 #include "emst_internal.h"
 
-namespace emst {
-void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const uint64_t* cand_key,
-                        const int32_t* cand_pos, int32_t* label, int32_t* label2, int32_t* corr,
-                        uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, int32_t* fixed,
-                        Counters* C, int32_t* eu, int32_t* ev, float* ed2, int32_t* eround,
-                        int round, hipStream_t s);
-}
-
 using namespace emst;
 
 namespace {
@@ -96,6 +88,8 @@ struct emst_ctx {
     bool no_grouping = false;         // set when distinct points met at float distance 0
     bool grouped = false;
     bool groups_settled = false;      // no group spans two components any more
+    hipGraphExec_t graph = nullptr;   // captured rounds (run_graph)
+    std::vector<uintptr_t> graph_key;
     int64_t dup_points = 0, dup_groups = 0;
     Params* P = nullptr;
     Counters* C = nullptr;
@@ -224,6 +218,42 @@ static_assert(offsetof(Counters, pad2) + sizeof(uint32_t) - offsetof(Counters, d
 static_assert(offsetof(Counters, zero_or_empty) == offsetof(Counters, research) + sizeof(uint32_t),
               "research and zero_or_empty are cleared together");
 
+// Cloud positions, leaf counts and hierarchy offsets for leaves of `bucket`
+// points (hcl[k].n already set); sizes the buffers.
+int layout_clouds(emst_ctx* c, int bucket)
+{
+    c->bucket = bucket;
+    int64_t pos = 0, lbase = 0, Lmax = 1, target = 0;
+    for (Cloud& cl : c->hcl) {
+        const int64_t nb = std::max<int64_t>(1, (cl.n + bucket - 1) / bucket);
+        int64_t L = 1;
+        while (L < nb) L <<= 1;
+        cl.pos = pos;
+        cl.lbase = lbase;
+        cl.L = L;
+        pos += cl.n;
+        lbase += L;
+        Lmax = std::max(Lmax, L);
+        target += std::max<int64_t>(cl.n - 1, 0);
+    }
+    c->nleaves = lbase;
+    c->Lmax = Lmax;
+    c->L = c->hcl[0].L;
+    c->target_edges = target;
+    // Morton bits per axis: the cloud index takes the top key bits of a batch
+    int cb = 0;
+    if (c->ncl > 1)
+        while ((1ll << cb) < c->ncl) ++cb;
+    c->mbits = std::min(c->dim == 3 ? 21 : 32, (64 - cb) / c->dim);
+    int rc;
+    if ((rc = alloc_all(c)) || (rc = ensure(c, c->dcl, (size_t)c->ncl)) ||
+        (rc = ensure(c, c->params, (size_t)c->ncl)))
+        return rc;
+    c->P = c->params.p;
+    HIPCHK(c, hipMemcpyAsync(c->dcl.p, c->hcl.data(), sizeof(Cloud) * c->ncl, hipMemcpyHostToDevice, c->stream));
+    return EMST_OK;
+}
+
 DupState dup_state(emst_ctx* c)
 {
     DupState d{};
@@ -292,7 +322,8 @@ int build(emst_ctx* c)
     }
     const DupState dup = dup_state(c);
     launch_build_tree(c->pts.p, c->dcl.p, c->ncl, c->nleaves, c->Lmax, c->bucket, c->nodes.p, dup.rep, s);
-    launch_round_init(n, c->label.p, c->eu.p, c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->C, s);
+    launch_round_init(n, c->label.p, c->eu.p, c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->C,
+                      (uint32_t)c->target_edges, (uint32_t)c->opt.max_rounds, s);
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ev[1], s));
     const float* seed_r = nullptr;
@@ -361,7 +392,60 @@ int weight_of(emst_ctx* c, int64_t m, double* w)
     return EMST_OK;
 }
 
-// One Boruvka round (SolutionEMST::activate).  Termination is tested first.
+// The kernels of one Boruvka round (SolutionEMST::activate).  Round control
+// lives on the device (k_round_begin / k_round_end), so a round issued after
+// the run has finished does nothing.  trace: events after each phase.
+void issue_round(emst_ctx* c, bool tr)
+{
+    hipStream_t s = c->stream;
+    launch_round_begin(c->C, s);
+    if (tr) (void)hipEventRecord(c->ev[3], s);
+    // group points: zero-distance '+' cones while the group spans several
+    // components, the representative's own cones once it is one (the
+    // repeated search of NodeEMST.h:1051-1077, without searching again); the
+    // kernels skip themselves once the groups have settled
+    if (c->grouped)
+        launch_dup_fill(c->dim, c->n, c->label.p, c->pts.p, c->inv.p, c->cand_key.p, c->cand_pos.p, c->corr.p,
+                        nullptr, nullptr, c->C, dup_state(c), s);
+    for (int ph = 0; ph < 4; ++ph) {
+        launch_round_phase(ph, c->K, c->n, c->pts.p, c->cand_key.p, c->cand_pos.p, c->label.p, c->label2.p,
+                           c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->fixed.p, c->C, c->eu.p,
+                           c->ev_.p, c->ed2.p, c->eround.p, s);
+        if (tr) (void)hipEventRecord(c->ev[4 + ph], s);
+    }
+    launch_round_end(c->C, c->grouped, s);
+}
+
+// A flatten that stopped at its step cap paused the rounds (done == 2):
+// launch it until the labels are flat, then end the round again.
+int resume_flatten(emst_ctx* c, Counters* h)
+{
+    hipStream_t s = c->stream;
+    int rc;
+    for (int again = 0; h->unflat; ++again) {
+        if (again == 64) return fail(c, EMST_E_HIP, "flatten did not converge (cycle in the hook forest)");
+        HIPCHK(c, hipMemsetAsync(&c->C->unflat, 0, sizeof(uint32_t), s));
+        launch_round_phase(3, c->K, c->n, c->pts.p, c->cand_key.p, c->cand_pos.p, c->label.p, c->label2.p,
+                           c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->fixed.p, c->C, c->eu.p,
+                           c->ev_.p, c->ed2.p, c->eround.p, s);
+        LAUNCHCHK(c);
+        if ((rc = read_counters(c, h))) return rc;
+    }
+    HIPCHK(c, hipMemsetAsync(&c->C->done, 0, sizeof(uint32_t), s));
+    launch_round_end(c->C, c->grouped, s);
+    LAUNCHCHK(c);
+    return read_counters(c, h);
+}
+
+void absorb_counters(emst_ctx* c, const Counters& h)
+{
+    c->edges = h.edges;
+    c->rounds = (int)h.round;
+    if (h.done == 1u) c->finished = true;
+}
+
+// One round from the host (emst_step, traced runs, and runs that repeat cone
+// searches).  Termination is tested first.
 int round_once(emst_ctx* c, int32_t* finished)
 {
     const int64_t n = c->n;
@@ -374,19 +458,8 @@ int round_once(emst_ctx* c, int32_t* finished)
     *finished = 0;
     const bool tr = c->opt.trace != 0;
     int64_t searched = c->rounds == 0 ? n : 0;
-    if (tr) HIPCHK(c, hipEventRecord(c->ev[3], s));
-    bool refreshed = false;
     if (c->rounds > 0 && c->grouped) {
-        // group points: zero-distance '+' cones while the group spans several
-        // components, the representative's own cones once it is one (the
-        // repeated search of NodeEMST.h:1051-1077, without searching again)
-        if (!c->groups_settled) {
-            launch_dup_fill(c->dim, n, c->label.p, c->pts.p, c->inv.p, c->cand_key.p, c->cand_pos.p, c->corr.p,
-                            nullptr, nullptr, c->C, dup_state(c), s);
-            LAUNCHCHK(c);
-            refreshed = true;
-            searched = c->dup_points;
-        }
+        if (!c->groups_settled) searched = c->dup_points;
     } else if (c->rounds > 0 && c->zero_or_empty > 0) {
         // research and zero_or_empty restart for this round
         HIPCHK(c, hipMemsetAsync(&c->C->research, 0, 2 * sizeof(uint32_t), s));
@@ -402,34 +475,17 @@ int round_once(emst_ctx* c, int32_t* finished)
         LAUNCHCHK(c);
         searched = h.zero_or_empty;
     }
-    const int round = c->rounds + 1;
-    for (int ph = 0; ph < 4; ++ph) {
-        launch_round_phase(ph, c->K, n, c->pts.p, c->cand_key.p, c->cand_pos.p, c->label.p, c->label2.p,
-                           c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->fixed.p, c->C, c->eu.p,
-                           c->ev_.p, c->ed2.p, c->eround.p, round, s);
-        if (tr) HIPCHK(c, hipEventRecord(c->ev[4 + ph], s));
-    }
+    issue_round(c, tr);
     LAUNCHCHK(c);
     Counters h{};
     int rc = read_counters(c, &h);
     if (rc) return rc;
-    for (int again = 0; h.unflat; ++again) {
-        if (again == 64) return fail(c, EMST_E_HIP, "flatten did not converge (cycle in the hook forest)");
-        HIPCHK(c, hipMemsetAsync(&c->C->unflat, 0, sizeof(uint32_t), s));
-        launch_round_phase(3, c->K, n, c->pts.p, c->cand_key.p, c->cand_pos.p, c->label.p, c->label2.p,
-                           c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->fixed.p, c->C, c->eu.p,
-                           c->ev_.p, c->ed2.p, c->eround.p, round, s);
-        LAUNCHCHK(c);
-        if ((rc = read_counters(c, &h))) return rc;
-    }
-    const uint32_t before = c->edges;
-    c->edges = h.edges;
-    // components only merge: once no group spans two of them, none will again
-    if (refreshed && h.dup_split == 0u) c->groups_settled = true;
-    c->rounds = round;
+    if (h.done == 2u && (rc = resume_flatten(c, &h))) return rc;
+    absorb_counters(c, h);
+    c->groups_settled = h.dup_settled != 0u;
     if (tr) {
         emst_round_trace row{};
-        row.iteration = round;
+        row.iteration = c->rounds;
         row.size = (int32_t)c->edges;
         double w = 0;
         rc = weight_of(c, n, &w);
@@ -443,7 +499,57 @@ int round_once(emst_ctx* c, int32_t* finished)
         row.searched = searched;
         c->trace.push_back(row);
     }
-    if (c->edges == before) c->finished = true;     // no component has an outgoing candidate
+    return EMST_OK;
+}
+
+// Untraced runs without repeated cone searches: kGraphRounds rounds captured
+// once in a graph and replayed, one host synchronisation per replay; rounds
+// past the end do nothing.  The graph is kept while the buffers and sizes it
+// captured stay the same.
+constexpr int kGraphRounds = 4;
+
+std::vector<uintptr_t> graph_key(emst_ctx* c)
+{
+    const DupState d = dup_state(c);
+    return {(uintptr_t)c->n, (uintptr_t)c->K, (uintptr_t)c->dim, (uintptr_t)c->grouped, (uintptr_t)d.npoints,
+            (uintptr_t)d.ngroups, (uintptr_t)c->C, (uintptr_t)c->pts.p, (uintptr_t)c->cand_key.p,
+            (uintptr_t)c->cand_pos.p, (uintptr_t)c->label.p, (uintptr_t)c->label2.p, (uintptr_t)c->corr.p,
+            (uintptr_t)c->corr_key.p, (uintptr_t)c->comp1.p, (uintptr_t)c->comp2.p, (uintptr_t)c->fixed.p,
+            (uintptr_t)c->eu.p, (uintptr_t)c->ev_.p, (uintptr_t)c->ed2.p, (uintptr_t)c->eround.p,
+            (uintptr_t)c->inv.p, (uintptr_t)d.rep, (uintptr_t)d.list, (uintptr_t)d.gslot, (uintptr_t)d.m2,
+            (uintptr_t)d.bkey, (uintptr_t)d.bpos};
+}
+
+int run_graph(emst_ctx* c)
+{
+    hipStream_t s = c->stream;
+    const std::vector<uintptr_t> key = graph_key(c);
+    if (!c->graph || key != c->graph_key) {
+        if (c->graph) {
+            (void)hipGraphExecDestroy(c->graph);
+            c->graph = nullptr;
+        }
+        HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int r = 0; r < kGraphRounds; ++r) issue_round(c, false);
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamEndCapture(s, &g));
+        hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            c->graph = nullptr;
+            return fail(c, EMST_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        c->graph_key = key;
+    }
+    while (!c->finished) {
+        HIPCHK(c, hipGraphLaunch(c->graph, s));
+        Counters h{};
+        int rc = read_counters(c, &h);
+        if (rc) return rc;
+        if (h.done == 2u && (rc = resume_flatten(c, &h))) return rc;
+        absorb_counters(c, h);
+        c->groups_settled = h.dup_settled != 0u;
+    }
     return EMST_OK;
 }
 
@@ -518,6 +624,7 @@ int emst_destroy(emst_ctx* c)
     if (!c) return EMST_E_ARG;
     hipSetDevice(c->opt.device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->graph) hipGraphExecDestroy(c->graph);
     if (c->C) hipFree(c->C);
     for (auto& e : c->ev) if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -548,34 +655,11 @@ int emst_set_clouds(emst_ctx* c, int32_t ncl, const int64_t* npts, const float* 
     c->n = n;
     c->dim = dim;
     c->K = 2 * dim * (1 << (dim - 1));
-    c->bucket = choose_bucket(c->opt, dim);
     c->ncl = ncl;
     c->hcl.assign(ncl, Cloud{});
-    int64_t pos = 0, lbase = 0, Lmax = 1, target = 0;
-    for (int32_t k = 0; k < ncl; ++k) {
-        const int64_t nb = std::max<int64_t>(1, (npts[k] + c->bucket - 1) / c->bucket);
-        int64_t L = 1;
-        while (L < nb) L <<= 1;
-        c->hcl[k] = Cloud{pos, npts[k], lbase, L};
-        pos += npts[k];
-        lbase += L;
-        Lmax = std::max(Lmax, L);
-        target += std::max<int64_t>(npts[k] - 1, 0);
-    }
-    c->nleaves = lbase;
-    c->Lmax = Lmax;
-    c->L = c->hcl[0].L;
-    c->target_edges = target;
-    // Morton bits per axis: the cloud index takes the top key bits of a batch
-    int cb = 0;
-    if (ncl > 1)
-        while ((1ll << cb) < ncl) ++cb;
-    c->mbits = std::min(dim == 3 ? 21 : 32, (64 - cb) / dim);
-    int rc;
-    if ((rc = alloc_all(c)) || (rc = ensure(c, c->dcl, (size_t)ncl)) || (rc = ensure(c, c->params, (size_t)ncl)))
-        return rc;
-    c->P = c->params.p;
-    HIPCHK(c, hipMemcpyAsync(c->dcl.p, c->hcl.data(), sizeof(Cloud) * ncl, hipMemcpyHostToDevice, c->stream));
+    for (int32_t k = 0; k < ncl; ++k) c->hcl[k].n = npts[k];
+    int rc = layout_clouds(c, choose_bucket(c->opt, dim));
+    if (rc) return rc;
     if (on_device) {
         c->xyz = xyz;
     } else {
@@ -601,6 +685,45 @@ int emst_set_points(emst_ctx* c, const float* xyz, int64_t n, int32_t dim, int32
     if (!c) return EMST_E_ARG;
     if (n < 0 || (n > 0 && !xyz)) return fail(c, EMST_E_ARG, "n < 0 or NULL points");
     return emst_set_clouds(c, 1, &n, xyz, dim, on_device);
+}
+
+int emst_autotune(emst_ctx* c, int32_t* best_bucket, float* best_scale, float* best_ms)
+{
+    int rc = need_points(c);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->opt.device));
+    const bool learned = c->learned;
+    c->learned = false;                 // tunes the plain seed
+    float best = 0.f;
+    int bb = 64;
+    float bs = c->seed_scale;
+    for (int b : {16, 32, 64})
+        for (float sc : {2.f, 3.f, 4.f}) {
+            if ((rc = layout_clouds(c, b))) break;
+            c->seed_scale = sc;
+            c->no_grouping = false;
+            if ((rc = build(c))) break;
+            const float t = c->times[0] + c->times[1];
+            if (best == 0.f || t < best) {
+                best = t;
+                bb = b;
+                bs = sc;
+            }
+        }
+    c->learned = learned;
+    if (rc) return rc;
+    c->opt.bucket_size = bb;
+    c->seed_scale = bs;
+    if ((rc = layout_clouds(c, bb))) return rc;
+    c->built = false;
+    c->finished = false;
+    c->rounds = 0;
+    c->edges = 0;
+    c->trace.clear();
+    if (best_bucket) *best_bucket = bb;
+    if (best_scale) *best_scale = bs;
+    if (best_ms) *best_ms = best;
+    return EMST_OK;
 }
 
 int emst_get_cloud_results(emst_ctx* c, double* weight, int32_t* edges, int32_t* last_round)
@@ -633,10 +756,15 @@ int emst_run(emst_ctx* c)
     rc = ensure_built(c);
     if (rc) return rc;
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    int32_t fin = 0;
-    while (!fin) {
-        rc = round_once(c, &fin);
+    if (!c->opt.trace && (c->grouped || c->zero_or_empty == 0)) {
+        rc = run_graph(c);
         if (rc) return rc;
+    } else {
+        int32_t fin = 0;
+        while (!fin) {
+            rc = round_once(c, &fin);
+            if (rc) return rc;
+        }
     }
     HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev[3]));

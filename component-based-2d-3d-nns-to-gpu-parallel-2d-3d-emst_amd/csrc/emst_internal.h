@@ -39,6 +39,15 @@ struct Counters {
     uint32_t dup_groups;    // number of groups
     uint32_t dup_split;     // set by k_dup_m2 when some group spans >= 2 components
     uint32_t pad2;
+    // round control, on the device so that rounds can run back to back
+    // (captured in a graph) and stop by themselves
+    uint32_t round;         // rounds run so far
+    uint32_t done;          // 0 running, 1 finished, 2 paused: flatten needs another launch
+    uint32_t edges_prev;    // edges before the current round
+    uint32_t target;        // sum over clouds of (points - 1)
+    uint32_t max_rounds;
+    uint32_t dup_settled;   // no coincident group spans two components any more
+    uint32_t pad3[2];
 };
 
 // Coincident-point groups.  Points with identical rescaled coordinates form a
@@ -123,15 +132,22 @@ void launch_dup_fill(int dim, int64_t n, const int32_t* label, const float4* pts
                      uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
                      uint64_t* first_key, Counters* C, DupState dup, hipStream_t s);
 void launch_round_init(int64_t n, int32_t* label, int32_t* eu, int32_t* corr, uint64_t* corr_key,
-                       uint64_t* comp1, uint32_t* comp2, Counters* C, hipStream_t s);
+                       uint64_t* comp1, uint32_t* comp2, Counters* C, uint32_t target, uint32_t max_rounds,
+                       hipStream_t s);
+// Phase of a round: 0 per-point target + component min (first key), 1 second
+// key + winners, 2 hook, 3 flatten.
+void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const uint64_t* cand_key,
+                        const int32_t* cand_pos, int32_t* label, int32_t* label2, int32_t* corr,
+                        uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, int32_t* fixed,
+                        Counters* C, int32_t* eu, int32_t* ev, float* ed2, int32_t* eround,
+                        hipStream_t s);
+// Round control: begin (termination test, round += 1) and end (no progress
+// -> finished; flatten unfinished -> paused; groups settled).
+void launch_round_begin(Counters* C, hipStream_t s);
+void launch_round_end(Counters* C, bool grouped, hipStream_t s);
 // Queue the points holding a zero-distance or no candidate.
 void launch_flag_research(int K, int64_t n, const uint64_t* cand_key, int32_t* list,
                           Counters* C, hipStream_t s);
-void launch_round(int K, int64_t n, const float4* pts, const uint64_t* cand_key,
-                  const int32_t* cand_pos, int32_t* label, int32_t* label2, int32_t* corr,
-                  uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, int32_t* fixed,
-                  Counters* C, int32_t* eu, int32_t* ev, float* ed2, int32_t* eround,
-                  int round, float* t_ms, hipEvent_t* ev_marks, hipStream_t s);
 void launch_weight(const float* xyz, int dim, const int32_t* eu, const int32_t* ev, int64_t m,
                    double* partial, int nblocks, hipStream_t s);
 // Per cloud: FP64 weight of its forest edges (input coordinates), its edge

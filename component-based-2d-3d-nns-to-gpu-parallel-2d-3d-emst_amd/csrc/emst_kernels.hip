@@ -977,6 +977,16 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
     }
 }
 
+// Start of a group refresh: m2 = ~0 for every group, dup_split = 0.
+__global__ __launch_bounds__(kBlock) void k_dup_reset(Counters* __restrict__ C, DupState dup)
+{
+    if (C->done || C->dup_settled) return;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t0 == 0) C->dup_split = 0;
+    for (int64_t g = t0; g < dup.ngroups; g += stride) dup.m2[g] = 0xffffffffu;
+}
+
 // Per group, the smallest member id whose component differs from the
 // representative's (labels are distinct before round 1); dup_split is set
 // when any group spans >= 2 components.  m2 holds ~0 on entry.
@@ -984,6 +994,7 @@ __global__ __launch_bounds__(kBlock) void k_dup_m2(const int32_t* __restrict__ l
                                                   const float4* __restrict__ pts,
                                                   Counters* __restrict__ C, DupState dup)
 {
+    if (C->done || C->dup_settled) return;
     bool split = false;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < dup.npoints;
          t += (int64_t)gridDim.x * blockDim.x) {
@@ -1013,9 +1024,11 @@ __global__ __launch_bounds__(kBlock) void k_dup_fill(int64_t n, const int32_t* _
                                                     int32_t* __restrict__ cand_pos,
                                                     int32_t* __restrict__ revive,
                                                     int32_t* __restrict__ first_corr,
-                                                    uint64_t* __restrict__ first_key, DupState dup)
+                                                    uint64_t* __restrict__ first_key,
+                                                    const Counters* __restrict__ C, DupState dup)
 {
     using CC = ConeCfg<D>;
+    if (C->done || C->dup_settled) return;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < dup.npoints;
          t += (int64_t)gridDim.x * blockDim.x) {
         const int32_t p = dup.list[t];
@@ -1066,12 +1079,15 @@ __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __res
                                                       int32_t* __restrict__ corr,
                                                       uint64_t* __restrict__ corr_key,
                                                       uint64_t* __restrict__ comp1,
-                                                      uint32_t* __restrict__ comp2, Counters* C)
+                                                      uint32_t* __restrict__ comp2, Counters* C,
+                                                      uint32_t target, uint32_t max_rounds)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         C->edges = 0; C->research = 0; C->zero_or_empty = 0; C->unflat = 0;
         C->packets = C->nodes = C->leaves = C->leaf_points = C->pairs = 0;
+        C->round = 0; C->done = 0; C->edges_prev = 0; C->dup_settled = 0;
+        C->target = target; C->max_rounds = max_rounds;
     }
     if (i >= n) return;
     label[i] = (int32_t)i;          // K_EMST_initDisjointSet (EMSTOperators.h:330-343)
@@ -1080,6 +1096,36 @@ __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __res
     corr_key[i] = kEmptyKey;
     comp1[i] = kEmptyKey;
     comp2[i] = 0xffffffffu;
+}
+
+// Termination is tested before a round (SolutionEMSTOperators.cu:194-227 and
+// the 100-round cap): the tree is complete or the cap is reached.  While
+// done != 0 every round kernel returns at once, so rounds can be issued
+// ahead (a captured graph of several rounds) and stop by themselves.
+__global__ void k_round_begin(Counters* C)
+{
+    if (C->done) return;
+    if (C->edges >= C->target || C->round >= C->max_rounds) {
+        C->done = 1;
+        return;
+    }
+    C->round += 1;
+    C->edges_prev = C->edges;
+}
+
+// A round that added no edge leaves a forest with no outgoing candidate: done.
+// A flatten that hit its step cap pauses the rounds until the host has
+// finished it.  Coincident groups stop being refreshed once none spans two
+// components (components only merge).
+__global__ void k_round_end(Counters* C, int grouped)
+{
+    if (C->done) return;
+    if (C->unflat) {
+        C->done = 2;
+        return;
+    }
+    if (grouped && !C->dup_settled && C->dup_split == 0u) C->dup_settled = 1;
+    if (C->edges == C->edges_prev) C->done = 1;
 }
 
 // Points that must search again next round: a zero-distance cone or no cone
@@ -1138,8 +1184,9 @@ __global__ __launch_bounds__(kBlock) void k_round_min(int K, int64_t n, const fl
                                                      int32_t* __restrict__ corr,
                                                      uint64_t* __restrict__ corr_key,
                                                      uint64_t* __restrict__ comp1,
-                                                     int32_t* __restrict__ fixed)
+                                                     int32_t* __restrict__ fixed, const Counters* C)
 {
+    if (C->done) return;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool inb = p < n;
     int32_t lp = 0;
@@ -1184,8 +1231,9 @@ __global__ __launch_bounds__(kBlock) void k_round_comp2(int64_t n, const float4*
                                                        const int32_t* __restrict__ corr,
                                                        const uint64_t* __restrict__ corr_key,
                                                        const uint64_t* __restrict__ comp1,
-                                                       uint32_t* __restrict__ comp2)
+                                                       uint32_t* __restrict__ comp2, const Counters* C)
 {
+    if (C->done) return;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n || corr[p] < 0) return;
     const int32_t lp = label[p];
@@ -1202,8 +1250,9 @@ __global__ __launch_bounds__(kBlock) void k_round_winner(int64_t n, const float4
                                                         const uint64_t* __restrict__ corr_key,
                                                         const uint64_t* __restrict__ comp1,
                                                         const uint32_t* __restrict__ comp2,
-                                                        int32_t* __restrict__ fixed)
+                                                        int32_t* __restrict__ fixed, const Counters* C)
 {
+    if (C->done) return;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n || corr[p] < 0) return;
     const int32_t lp = label[p];
@@ -1230,8 +1279,10 @@ __global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const floa
                                                           Counters* C, int32_t* __restrict__ eu,
                                                           int32_t* __restrict__ ev,
                                                           float* __restrict__ ed2,
-                                                          int32_t* __restrict__ eround, int round)
+                                                          int32_t* __restrict__ eround)
 {
+    if (C->done) return;
+    const int round = (int)C->round;
     __shared__ uint32_t cnt;
     if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
@@ -1277,6 +1328,7 @@ __global__ __launch_bounds__(kBlock) void k_round_flatten(int64_t n, int32_t* __
                                                          uint64_t* __restrict__ comp1,
                                                          uint32_t* __restrict__ comp2, Counters* C)
 {
+    if (C->done == 1u) return;      // a paused round (done == 2) still flattens
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     int32_t r = label2[p];
@@ -1583,23 +1635,34 @@ void launch_dup_fill(int dim, int64_t n, const int32_t* label, const float4* pts
                      uint64_t* first_key, Counters* C, DupState dup, hipStream_t s)
 {
     if (n <= 0 || dup.rep == nullptr || dup.npoints <= 0) return;
-    hipMemsetAsync(dup.m2, 0xff, sizeof(uint32_t) * (size_t)dup.ngroups, s);
-    hipMemsetAsync(&C->dup_split, 0, sizeof(uint32_t), s);
+    hipLaunchKernelGGL(k_dup_reset, dim3((unsigned)std::min<int64_t>(nblocks(dup.ngroups), 256)), dim3(kBlock), 0,
+                       s, C, dup);
     const unsigned g = (unsigned)std::min<int64_t>(nblocks(dup.npoints), 1024);
     hipLaunchKernelGGL(k_dup_m2, dim3(g), dim3(kBlock), 0, s, label, pts, C, dup);
     if (dim == 3)
         hipLaunchKernelGGL(k_dup_fill<3>, dim3(g), dim3(kBlock), 0, s, n, label, pts, inv, cand_key, cand_pos,
-                           revive, first_corr, first_key, dup);
+                           revive, first_corr, first_key, C, dup);
     else
         hipLaunchKernelGGL(k_dup_fill<2>, dim3(g), dim3(kBlock), 0, s, n, label, pts, inv, cand_key, cand_pos,
-                           revive, first_corr, first_key, dup);
+                           revive, first_corr, first_key, C, dup);
 }
 
 void launch_round_init(int64_t n, int32_t* label, int32_t* eu, int32_t* corr, uint64_t* corr_key,
-                       uint64_t* comp1, uint32_t* comp2, Counters* C, hipStream_t s)
+                       uint64_t* comp1, uint32_t* comp2, Counters* C, uint32_t target, uint32_t max_rounds,
+                       hipStream_t s)
 {
     hipLaunchKernelGGL(k_round_init, dim3(nblocks(std::max<int64_t>(n, 1))), dim3(kBlock), 0, s, n, label,
-                       eu, corr, corr_key, comp1, comp2, C);
+                       eu, corr, corr_key, comp1, comp2, C, target, max_rounds);
+}
+
+void launch_round_begin(Counters* C, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(1), 0, s, C);
+}
+
+void launch_round_end(Counters* C, bool grouped, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_round_end, dim3(1), dim3(1), 0, s, C, grouped ? 1 : 0);
 }
 
 void launch_flag_research(int K, int64_t n, const uint64_t* cand_key, int32_t* list, Counters* C,
@@ -1613,24 +1676,24 @@ void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const ui
                         const int32_t* cand_pos, int32_t* label, int32_t* label2, int32_t* corr,
                         uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, int32_t* fixed,
                         Counters* C, int32_t* eu, int32_t* ev, float* ed2, int32_t* eround,
-                        int round, hipStream_t s)
+                        hipStream_t s)
 {
     if (n <= 0) return;
     const unsigned g = nblocks(n);
     switch (phase) {
     case 0:
         hipLaunchKernelGGL(k_round_min, dim3(g), dim3(kBlock), 0, s, K, n, pts, cand_key, cand_pos, label,
-                           label2, corr, corr_key, comp1, fixed);
+                           label2, corr, corr_key, comp1, fixed, C);
         break;
     case 1:
         hipLaunchKernelGGL(k_round_comp2, dim3(g), dim3(kBlock), 0, s, n, pts, label, corr, corr_key, comp1,
-                           comp2);
+                           comp2, C);
         hipLaunchKernelGGL(k_round_winner, dim3(g), dim3(kBlock), 0, s, n, pts, label, corr, corr_key,
-                           comp1, comp2, fixed);
+                           comp1, comp2, fixed, C);
         break;
     case 2:
         hipLaunchKernelGGL(k_round_hook, dim3(nblocks(n, kHookBlock)), dim3(kHookBlock), 0, s, n, pts, label,
-                           label2, corr, corr_key, fixed, C, eu, ev, ed2, eround, round);
+                           label2, corr, corr_key, fixed, C, eu, ev, ed2, eround);
         break;
     case 3:
         hipLaunchKernelGGL(k_round_flatten, dim3(g), dim3(kBlock), 0, s, n, label, label2, comp1, comp2, C);

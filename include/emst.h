@@ -119,6 +119,15 @@ int emst_set_clouds(emst_ctx* ctx, int32_t ncl, const int64_t* npts, const float
  * edges (its completion round).  Arrays of ncl entries; any may be NULL. */
 int emst_get_cloud_results(emst_ctx* ctx, double* weight, int32_t* edges, int32_t* last_round);
 
+/* Autotuner (no counterpart in the reference, whose cell size is the fixed
+ * level radius): times the build and cone search of the context's current
+ * points on the device for every leaf size (points per leaf cell: 16, 32,
+ * 64) and plain seed scale (2, 3, 4 leaf spacings), keeps the fastest pair in
+ * the context (used by every later set_points / set_clouds / run) and returns
+ * it; best_ms = its build + search time.  The result of a run never depends
+ * on either; any output may be NULL. */
+int emst_autotune(emst_ctx* ctx, int32_t* best_bucket, float* best_scale, float* best_ms);
+
 /* Replaces SolutionEMST::run() (emst_coalition/src/SolutionEMSTOperators.cu:146-153):
  * Boruvka rounds until one tree remains, no component has an outgoing
  * candidate (a forest), or max_rounds is reached. */
