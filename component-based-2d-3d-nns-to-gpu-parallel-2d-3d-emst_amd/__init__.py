@@ -26,8 +26,10 @@ EXPORTS = [
     "emst_get_rescaled", "emst_get_trace", "emst_get_candidates", "emst_get_cell_grid",
     "emst_get_times", "emst_get_search_stats", "emst_synth", "emst_version",
     "emst_set_radius_model", "emst_get_radius_samples", "emst_mlp_train", "emst_mlp_eval", "emst_mlp_grad",
-    "emst_set_clouds", "emst_get_cloud_results", "emst_autotune",
+    "emst_set_clouds", "emst_get_cloud_results", "emst_autotune", "emst_get_shape", "emst_read_points",
+    "emst_write_solution",
 ]
+CLI_PATH = os.path.join(_HERE, "emst_cli")
 
 MLP_PARAMS = 4818
 
@@ -96,6 +98,9 @@ def lib():
     L.emst_set_clouds.argtypes = [P, i32, P, P, i32, i32]
     L.emst_get_cloud_results.argtypes = [P, P, P, P]
     L.emst_autotune.argtypes = [P, P, P, P]
+    L.emst_get_shape.argtypes = [P, P, P]
+    L.emst_read_points.argtypes = [ctypes.c_char_p, i32, P, P]
+    L.emst_write_solution.argtypes = [P, ctypes.c_char_p]
     L.emst_version.restype = ctypes.c_char_p
     _lib = L
     return L
@@ -111,6 +116,22 @@ def synth(kind, n, dim, seed):
     rc = lib().emst_synth(kind, n, dim, seed, _p(out))
     if rc:
         raise EmstError(rc, "emst_synth")
+    return out
+
+
+def read_points(path, dim):
+    """InputRW::readFile: points of a .2d / .3d / .tsp / .points file as an
+    n x dim float32 array (host only)."""
+    L = lib()
+    n = ctypes.c_int64(0)
+    rc = L.emst_read_points(os.fsencode(path), dim, None, ctypes.byref(n))
+    if rc:
+        raise EmstError(rc, f"emst_read_points({path})")
+    out = np.empty((n.value, dim), np.float32)
+    if n.value:
+        rc = L.emst_read_points(os.fsencode(path), dim, _p(out), ctypes.byref(n))
+        if rc:
+            raise EmstError(rc, f"emst_read_points({path})")
     return out
 
 
@@ -184,6 +205,11 @@ class Calculateur:
                   "set_clouds")
         self.n = int(sizes.sum())
         self.offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+
+    def writeSolution(self, path):
+        """SolutionEMST::writeSolution: .links, .correspondenceMap,
+        .disjointSetMap and .points next to `path` (cut at its first '.' or '_')."""
+        self._chk(self._L.emst_write_solution(self._c, os.fsencode(path)), "write_solution")
 
     def autotune(self):
         """Leaf size and seed scale picked by timing the current points; kept

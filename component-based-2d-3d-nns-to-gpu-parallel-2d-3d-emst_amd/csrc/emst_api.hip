@@ -951,6 +951,15 @@ int emst_get_cell_grid(emst_ctx* c, int32_t R, int32_t* ext_dual, int32_t* cell_
     return EMST_OK;
 }
 
+int emst_get_shape(emst_ctx* c, int64_t* n, int32_t* dim)
+{
+    int rc = need_points(c);
+    if (rc) return rc;
+    if (n) *n = c->n;
+    if (dim) *dim = c->dim;
+    return EMST_OK;
+}
+
 int emst_get_times(emst_ctx* c, float* times, int32_t* n_times)
 {
     if (!c || !times || !n_times) return EMST_E_ARG;

@@ -241,6 +241,31 @@ int emst_mlp_eval(int32_t device, const float* model, const float* feat, int64_t
 int emst_mlp_grad(int32_t device, const float* model, const float* feat, const float* target, int64_t m,
                   float* grad, double* loss);
 
+/* Points and dimension of the context's current point set. */
+int emst_get_shape(emst_ctx* ctx, int64_t* n, int32_t* dim);
+
+/* ---- on-disk formats (host only; no GPU needed for the reader) ------------
+ * Replaces InputRW::readFile (emst_coalition/include/InputRW.h:44-623): the
+ * file name (after the last '/') is cut at its first '.' or '_' (getPos); the
+ * rest picks the reader,
+ * which opens base + its suffix: ".points" ("Width = W Height = H", then W*H
+ * points), ".3d" / ".2d" (first token N, then N points), anything else
+ * TSPLIB (NAME on line 1, "DIMENSION : N" on line 4, "index x y" from line
+ * 7, up to N cities or an earlier "EOF").  dim: the solution's point dimension (2D data read by a 3D solution get
+ * z = 0; a .3d file needs dim 3).  xyz NULL: *n = point count only;
+ * otherwise *n is the capacity in, count out.  EMST_E_IO on a missing or
+ * malformed file. */
+int emst_read_points(const char* path, int32_t dim, float* xyz, int64_t* n);
+
+/* Replaces SolutionEMST::writeSolution / NeuralNetEMST::writeLinks
+ * (emst_coalition/src/SolutionEMSTRW.cu:38-62, NeuralNetEMST.h:277-326,
+ * BufferLink.h:266-279): writes base.links (every node with its forest
+ * neighbours, ascending), base.correspondenceMap ((-1, -1) per point, the
+ * host copy the reference writes), base.disjointSetMap (4 x root index per
+ * point) and base.points (rescaled coordinates as 3D points), base = path cut
+ * at its first '.' or '_'. */
+int emst_write_solution(emst_ctx* ctx, const char* path);
+
 /* Synthetic clouds, identical on every host (no counterpart in the reference;
  * used by the benchmark and tests).  SplitMix64-seeded xoshiro128+,
  * u = (x >> 8) * 2^-24.  kind: 0 uniform [0,1)^dim; 1 Gaussian mixture

@@ -10,11 +10,15 @@ Sources (relative to the reference root):
       per-round rows "iteration ... benchMark size length ..." written by
       SolutionEMSTRW.cu:133-165 (size = cumulative forest edges, length =
       forest length as float, 6 significant digits)
-  emst_coalition/test_emst/test_2D_2/outypu.links / .points / .disjointSetMap
+  emst_coalition/test_emst/test_2D_2/outypu.links / .points / .disjointSetMap /
+      .correspondenceMap
       full output of a 3000randomCity.2d run (NeuralNetEMST.h:275-326)
   emst_coalition/test_emst/test_*_2/*randomCity.{2d,3d}, *.tsp  input clouds
+  emst_coalition/test_emst/test_2D_2/*.tsp  all 79 TSPLIB instances, gzipped
+      into tests/golden/tsplib/ (the sweep's inputs)
 """
 import collections
+import gzip
 import json
 import os
 import re
@@ -82,7 +86,19 @@ def main():
         "edges": sorted(edges),
         "disjoint_set_values": sorted(set(int(x) for x in dsm)),
     }
-    shutil.copyfile(os.path.join(T, "test_2D_2", "outypu.points"), os.path.join(DATA, "outypu.points"))
+    # the stats header line as the reference writes it (SolutionEMSTRW.cu:104-128)
+    fx["stats_header"] = open(os.path.join(T, "test_2D_2", "output")).readline().rstrip("\n")
+    # the run's written maps, for byte comparisons of the solution writers
+    for sfx in ("points", "links", "disjointSetMap", "correspondenceMap"):
+        shutil.copyfile(os.path.join(T, "test_2D_2", "outypu." + sfx), os.path.join(DATA, "outypu." + sfx))
+    # the TSPLIB instances of the reference's 2D tests, gzip-compressed
+    tsp_dir = os.path.join(HERE, "tsplib")
+    os.makedirs(tsp_dir, exist_ok=True)
+    for name in sorted(os.listdir(os.path.join(T, "test_2D_2"))):
+        if name.endswith(".tsp"):
+            with open(os.path.join(T, "test_2D_2", name), "rb") as fi, \
+                    gzip.GzipFile(os.path.join(tsp_dir, name + ".gz"), "wb", mtime=0) as fo:
+                fo.write(fi.read())
     with open(os.path.join(HERE, "fixtures.json"), "w") as f:
         json.dump(fx, f, indent=0)
     print("traces:", sorted(fx["traces"]))
