@@ -77,6 +77,7 @@ struct emst_ctx {
     DevBuf<double> partial;
     DevBuf<float> model, feat, pred, seed, ls;     // learned seed radius
     DevBuf<Cloud> dcl;
+    DevBuf<uint8_t> perm, cursor;      // cones ranked by key, per-point scan position (k_round_min)
     DevBuf<Params> params;
     DevBuf<double> cloud_w;
     DevBuf<int32_t> cloud_e, cloud_r;
@@ -187,6 +188,8 @@ int alloc_all(emst_ctx* c)
     ENS(ed2, n);
     ENS(partial, kWeightBlocks);
     ENS(geo, 16);
+    ENS(perm, n * K);
+    ENS(cursor, n);
     ENS(dup_rep, n);
     ENS(dup_list, n);
     ENS(dup_runs, n);
@@ -341,6 +344,7 @@ int build(emst_ctx* c)
                        c->corr_key.p, c->C, true, dup, s);
     launch_dup_fill(c->dim, n, c->label.p, c->pts.p, c->inv.p, c->cand_key.p, c->cand_pos.p, nullptr, c->corr.p,
                     c->corr_key.p, c->C, dup, s);
+    launch_rank_cones(c->dim, n, c->cand_key.p, nullptr, n, c->perm.p, c->cursor.p, nullptr, s);
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ev[2], s));
     // while grouping is on only group points can hold a zero-distance or no
@@ -404,11 +408,15 @@ void issue_round(emst_ctx* c, bool tr)
     // components, the representative's own cones once it is one (the
     // repeated search of NodeEMST.h:1051-1077, without searching again); the
     // kernels skip themselves once the groups have settled
-    if (c->grouped)
+    if (c->grouped) {
+        const DupState d = dup_state(c);
         launch_dup_fill(c->dim, c->n, c->label.p, c->pts.p, c->inv.p, c->cand_key.p, c->cand_pos.p, c->corr.p,
-                        nullptr, nullptr, c->C, dup_state(c), s);
+                        nullptr, nullptr, c->C, d, s);
+        launch_rank_cones(c->dim, c->n, c->cand_key.p, d.list, d.npoints, c->perm.p, c->cursor.p, c->C, s);
+    }
     for (int ph = 0; ph < 4; ++ph) {
-        launch_round_phase(ph, c->K, c->n, c->pts.p, c->cand_key.p, c->cand_pos.p, c->label.p, c->label2.p,
+        launch_round_phase(ph, c->K, c->n, c->pts.p, c->cand_key.p, c->cand_pos.p, c->perm.p, c->cursor.p,
+                           c->label.p, c->label2.p,
                            c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->fixed.p, c->C, c->eu.p,
                            c->ev_.p, c->ed2.p, c->eround.p, s);
         if (tr) (void)hipEventRecord(c->ev[4 + ph], s);
@@ -425,7 +433,8 @@ int resume_flatten(emst_ctx* c, Counters* h)
     for (int again = 0; h->unflat; ++again) {
         if (again == 64) return fail(c, EMST_E_HIP, "flatten did not converge (cycle in the hook forest)");
         HIPCHK(c, hipMemsetAsync(&c->C->unflat, 0, sizeof(uint32_t), s));
-        launch_round_phase(3, c->K, c->n, c->pts.p, c->cand_key.p, c->cand_pos.p, c->label.p, c->label2.p,
+        launch_round_phase(3, c->K, c->n, c->pts.p, c->cand_key.p, c->cand_pos.p, c->perm.p, c->cursor.p,
+                           c->label.p, c->label2.p,
                            c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->fixed.p, c->C, c->eu.p,
                            c->ev_.p, c->ed2.p, c->eround.p, s);
         LAUNCHCHK(c);
@@ -468,10 +477,12 @@ int round_once(emst_ctx* c, int32_t* finished)
         Counters h{};
         int rc = read_counters(c, &h);
         if (rc) return rc;
-        if (h.research > 0)
+        if (h.research > 0) {
             launch_cone_search(c->dim, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, n, c->list.p, h.research,
                                c->label.p, true, c->inv.p, nullptr, c->seed_scale, c->cand_key.p,
                                c->cand_pos.p, c->corr.p, nullptr, nullptr, c->C, false, DupState{}, s);
+            launch_rank_cones(c->dim, n, c->cand_key.p, c->list.p, h.research, c->perm.p, c->cursor.p, nullptr, s);
+        }
         LAUNCHCHK(c);
         searched = h.zero_or_empty;
     }
@@ -516,7 +527,7 @@ std::vector<uintptr_t> graph_key(emst_ctx* c)
             (uintptr_t)c->cand_pos.p, (uintptr_t)c->label.p, (uintptr_t)c->label2.p, (uintptr_t)c->corr.p,
             (uintptr_t)c->corr_key.p, (uintptr_t)c->comp1.p, (uintptr_t)c->comp2.p, (uintptr_t)c->fixed.p,
             (uintptr_t)c->eu.p, (uintptr_t)c->ev_.p, (uintptr_t)c->ed2.p, (uintptr_t)c->eround.p,
-            (uintptr_t)c->inv.p, (uintptr_t)d.rep, (uintptr_t)d.list, (uintptr_t)d.gslot, (uintptr_t)d.m2,
+            (uintptr_t)c->inv.p, (uintptr_t)c->perm.p, (uintptr_t)c->cursor.p, (uintptr_t)d.rep, (uintptr_t)d.list, (uintptr_t)d.gslot, (uintptr_t)d.m2,
             (uintptr_t)d.bkey, (uintptr_t)d.bpos};
 }
 

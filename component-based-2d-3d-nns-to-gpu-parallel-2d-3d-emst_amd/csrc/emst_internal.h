@@ -136,8 +136,13 @@ void launch_round_init(int64_t n, int32_t* label, int32_t* eu, int32_t* corr, ui
                        hipStream_t s);
 // Phase of a round: 0 per-point target + component min (first key), 1 second
 // key + winners, 2 hook, 3 flatten.
+// Ranks each point's cones by key for k_round_min (all points, or the qlist
+// points); C != nullptr: skipped once the rounds are done or groups settled.
+void launch_rank_cones(int dim, int64_t n, const uint64_t* cand_key, const int32_t* qlist, int64_t nq,
+                       uint8_t* perm, uint8_t* cursor, const Counters* C, hipStream_t s);
 void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const uint64_t* cand_key,
-                        const int32_t* cand_pos, int32_t* label, int32_t* label2, int32_t* corr,
+                        const int32_t* cand_pos, const uint8_t* perm, uint8_t* cursor, int32_t* label,
+                        int32_t* label2, int32_t* corr,
                         uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, int32_t* fixed,
                         Counters* C, int32_t* eu, int32_t* ev, float* ed2, int32_t* eround,
                         hipStream_t s);

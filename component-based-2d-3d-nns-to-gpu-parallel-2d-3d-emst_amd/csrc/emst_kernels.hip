@@ -869,20 +869,10 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
                     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
                     if (j < cnt) v = pts[s0 + j];
                     const int64_t s = s0 + j;
-                    uint64_t rem = needm;
-                    while (rem) {
-                        // the qslot-th remaining needed lane is this lane's query
-                        int myq = -1;
-                        for (int k = 0; k < qpc; ++k) {
-                            const int l = rem ? (int)__builtin_ctzll(rem) : -1;
-                            if (rem) rem &= rem - 1;
-                            if (k == qslot) myq = l;
-                        }
-                        const int src = myq >= 0 ? myq : 0;
-                        const float qx = __shfl(px, src), qy = __shfl(py, src), qz = __shfl(pz, src);
-                        const int32_t qmaxb = __shfl(maxb, src), qinner = __shfl(inner, src);
-                        const uint32_t qopen = (uint32_t)__shfl((int)open, src);
-                        const int32_t qpos = __shfl((int32_t)p, src);
+                    // one (query, leaf point) pair per lane: distance, cone, and
+                    // an LDS atomic into the query's cone best
+                    auto pair = [&](int myq, float qx, float qy, float qz, int32_t qmaxb, int32_t qinner,
+                                    uint32_t qopen, int32_t qpos, int32_t qlab) {
                         const float dx = v.x - qx, dy = v.y - qy, dz = v.z - qz;
                         const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
                         const int32_t db = (int32_t)__float_as_uint(d2);
@@ -892,10 +882,7 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
                         // distinct points at float distance 0 (underflow) break the
                         // grouping's premise; the caller then searches ungrouped
                         if (grouped && take && d2 == 0.f) underflow = true;
-                        if (check_labels) {
-                            const int32_t qlab = __shfl(mylab, src);
-                            if (take && d2 == 0.f) take = label[s] != qlab;
-                        }
+                        if (check_labels && take && d2 == 0.f) take = label[s] != qlab;
                         if (take) {
                             uint32_t mask = cone_mask<D>(dx, dy, dz) & qopen;
                             const unsigned long long key =
@@ -905,6 +892,33 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
                                 mask &= mask - 1;
                                 atomicMin(&best[c][myq], key);
                             }
+                        }
+                    };
+                    uint64_t rem = needm;
+                    if (qpc == 1) {
+                        // every lane works on the same query: broadcast its state with
+                        // v_readlane (a scalar register) instead of an LDS permute
+                        while (rem) {
+                            const int q = (int)__builtin_ctzll(rem);
+                            rem &= rem - 1;
+                            auto rl = [&](int x) { return __builtin_amdgcn_readlane(x, q); };
+                            pair(q, __int_as_float(rl(__float_as_int(px))), __int_as_float(rl(__float_as_int(py))),
+                                 __int_as_float(rl(__float_as_int(pz))), rl(maxb), rl(inner), (uint32_t)rl((int)open),
+                                 rl((int32_t)p), check_labels ? rl(mylab) : 0);
+                        }
+                    } else {
+                        while (rem) {
+                            // the qslot-th remaining needed lane is this lane's query
+                            int myq = -1;
+                            for (int k = 0; k < qpc; ++k) {
+                                const int l = rem ? (int)__builtin_ctzll(rem) : -1;
+                                if (rem) rem &= rem - 1;
+                                if (k == qslot) myq = l;
+                            }
+                            const int src = myq >= 0 ? myq : 0;
+                            pair(myq, __shfl(px, src), __shfl(py, src), __shfl(pz, src), __shfl(maxb, src),
+                                 __shfl(inner, src), (uint32_t)__shfl((int)open, src), __shfl((int32_t)p, src),
+                                 check_labels ? __shfl(mylab, src) : 0);
                         }
                     }
                     if (need) refresh();
@@ -1151,6 +1165,15 @@ __global__ __launch_bounds__(kBlock) void k_flag_research(int K, int64_t n,
 
 // Wave pre-reduction: lanes sharing the wave leader's label merge into one
 // atomic; the rest issue their own.
+// The minima only decrease, so a value read (even a stale one) that is
+// already <= the key makes the atomic redundant: late rounds, whose few large
+// components would otherwise take one atomic per wave on the same address,
+// mostly skip it.
+__device__ __forceinline__ void atomic_min_if_lower(uint64_t* a, uint64_t key)
+{
+    if (__atomic_load_n(a, __ATOMIC_RELAXED) > key) atomicMin((unsigned long long*)a, (unsigned long long)key);
+}
+
 __device__ __forceinline__ void atomic_min_grouped(uint64_t* arr, int32_t lab, uint64_t key, bool valid)
 {
     const uint64_t vm = __ballot(valid);
@@ -1165,20 +1188,74 @@ __device__ __forceinline__ void atomic_min_grouped(uint64_t* arr, int32_t lab, u
             const uint64_t o = (uint64_t)__shfl_xor((long long)v, off);
             v = o < v ? o : v;
         }
-        if (__lane_id() == leader) atomicMin((unsigned long long*)&arr[llab], (unsigned long long)v);
-        if (valid && !grp) atomicMin((unsigned long long*)&arr[lab], (unsigned long long)key);
+        if (__lane_id() == leader) atomic_min_if_lower(&arr[llab], v);
+        if (valid && !grp) atomic_min_if_lower(&arr[lab], key);
     } else if (valid) {
-        atomicMin((unsigned long long*)&arr[lab], (unsigned long long)key);
+        atomic_min_if_lower(&arr[lab], key);
     }
+}
+
+// Rank of each point's cones by key (bitonic network over K padded to a power
+// of two, all indices static so the keys stay in registers): perm[r][p] =
+// cone of rank r; cursor[p] = 0.  The order is (key, cone): padding entries
+// (key ~0, index >= K) rank after every real cone, empty ones included, so
+// ranks 0..K-1 are exactly the K cones.  Points: [0, n) or qlist[0..nq); with C the
+// kernel does nothing once the rounds are done or the groups have settled
+// (it follows a group refresh in a replayed round).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_rank_cones(int64_t n, const uint64_t* __restrict__ cand_key,
+                                                      const int32_t* __restrict__ qlist, int64_t nq,
+                                                      uint8_t* __restrict__ perm, uint8_t* __restrict__ cursor,
+                                                      const Counters* __restrict__ C)
+{
+    using CC = ConeCfg<D>;
+    constexpr int P = CC::K <= 8 ? 8 : 32;
+    if (C && (C->done || C->dup_settled)) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq) return;
+    const int64_t p = qlist ? (int64_t)qlist[t] : t;
+    uint64_t key[P];
+    uint32_t idx[P];
+#pragma unroll
+    for (int c = 0; c < P; ++c) {
+        key[c] = c < CC::K ? cand_key[(int64_t)c * n + p] : kEmptyKey;
+        idx[c] = (uint32_t)c;
+    }
+#pragma unroll
+    for (int k = 2; k <= P; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const bool up = (i & k) == 0;
+                    const bool gt = key[i] > key[l] || (key[i] == key[l] && idx[i] > idx[l]);
+                    if (gt == up) {
+                        const uint64_t tk = key[i]; key[i] = key[l]; key[l] = tk;
+                        const uint32_t ti = idx[i]; idx[i] = idx[l]; idx[l] = ti;
+                    }
+                }
+            }
+#pragma unroll
+    for (int r = 0; r < CC::K; ++r) perm[(int64_t)r * n + p] = (uint8_t)idx[r];
+    cursor[p] = 0;
 }
 
 // find min 1 (NodeEMST.h:1303-1384) + first phase of the component minimum:
 // per point, keep the previous target while it is outside the component,
 // else the (d2, id)-smallest cone candidate with another label; then
 // atomicMin of (d2 bits, min id) on the component root.
+//
+// The smallest outgoing candidate is the first, in key order, whose label
+// differs: the cones are ranked once (k_rank_cones) and cursor[p] skips the
+// ones already found inside the component, which never leave it (components
+// only merge), so each candidate is looked at about once over all rounds.
 __global__ __launch_bounds__(kBlock) void k_round_min(int K, int64_t n, const float4* __restrict__ pts,
                                                      const uint64_t* __restrict__ cand_key,
                                                      const int32_t* __restrict__ cand_pos,
+                                                     const uint8_t* __restrict__ perm,
+                                                     uint8_t* __restrict__ cursor,
                                                      const int32_t* __restrict__ label,
                                                      int32_t* __restrict__ label2,
                                                      int32_t* __restrict__ corr,
@@ -1204,14 +1281,44 @@ __global__ __launch_bounds__(kBlock) void k_round_min(int K, int64_t n, const fl
         if (c != -2 && !(c >= 0 && label[c] != lp)) {
             c = -1;
             ck = kEmptyKey;
-            for (int k = 0; k < K; ++k) {
-                const uint64_t key = cand_key[(int64_t)k * n + p];
-                if (key >= ck) continue;
-                const int32_t q = cand_pos[(int64_t)k * n + p];
-                if (label[q] == lp) continue;
-                ck = key;
-                c = q;
+            // kStep ranks per step, their loads independent of each other, so
+            // a long run of candidates inside the component costs few round
+            // trips to memory
+            constexpr int kStep = 4;
+            int r = cursor[p];
+            bool more = r < K;
+            while (more) {
+                uint64_t key[kStep];
+                int32_t q[kStep], lq[kStep];
+#pragma unroll
+                for (int i = 0; i < kStep; ++i) {
+                    const int cone = r + i < K ? (int)perm[(int64_t)(r + i) * n + p] : K;
+                    key[i] = kEmptyKey;
+                    q[i] = -1;
+                    if (cone < K) {             // anything else is not a cone index: the end
+                        key[i] = cand_key[(int64_t)cone * n + p];
+                        q[i] = cand_pos[(int64_t)cone * n + p];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < kStep; ++i) lq[i] = key[i] != kEmptyKey ? label[q[i]] : lp;
+#pragma unroll
+                for (int i = 0; i < kStep; ++i) {
+                    if (!more) continue;
+                    if (key[i] == kEmptyKey) {      // ranked last: no outgoing candidate left
+                        r = K;
+                        more = false;
+                    } else if (lq[i] != lp) {
+                        ck = key[i];
+                        c = q[i];
+                        more = false;
+                    } else {
+                        ++r;
+                    }
+                }
+                more = more && r < K;
             }
+            cursor[p] = (uint8_t)r;
             corr[p] = c == -1 ? -2 : c;
             corr_key[p] = ck;
         }
@@ -1672,8 +1779,21 @@ void launch_flag_research(int K, int64_t n, const uint64_t* cand_key, int32_t* l
     hipLaunchKernelGGL(k_flag_research, dim3(nblocks(n)), dim3(kBlock), 0, s, K, n, cand_key, list, C);
 }
 
+void launch_rank_cones(int dim, int64_t n, const uint64_t* cand_key, const int32_t* qlist, int64_t nq,
+                       uint8_t* perm, uint8_t* cursor, const Counters* C, hipStream_t s)
+{
+    if (nq <= 0) return;
+    if (dim == 3)
+        hipLaunchKernelGGL(k_rank_cones<3>, dim3(nblocks(nq)), dim3(kBlock), 0, s, n, cand_key, qlist, nq, perm,
+                           cursor, C);
+    else
+        hipLaunchKernelGGL(k_rank_cones<2>, dim3(nblocks(nq)), dim3(kBlock), 0, s, n, cand_key, qlist, nq, perm,
+                           cursor, C);
+}
+
 void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const uint64_t* cand_key,
-                        const int32_t* cand_pos, int32_t* label, int32_t* label2, int32_t* corr,
+                        const int32_t* cand_pos, const uint8_t* perm, uint8_t* cursor, int32_t* label,
+                        int32_t* label2, int32_t* corr,
                         uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, int32_t* fixed,
                         Counters* C, int32_t* eu, int32_t* ev, float* ed2, int32_t* eround,
                         hipStream_t s)
@@ -1682,8 +1802,8 @@ void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const ui
     const unsigned g = nblocks(n);
     switch (phase) {
     case 0:
-        hipLaunchKernelGGL(k_round_min, dim3(g), dim3(kBlock), 0, s, K, n, pts, cand_key, cand_pos, label,
-                           label2, corr, corr_key, comp1, fixed, C);
+        hipLaunchKernelGGL(k_round_min, dim3(g), dim3(kBlock), 0, s, K, n, pts, cand_key, cand_pos, perm, cursor,
+                           label, label2, corr, corr_key, comp1, fixed, C);
         break;
     case 1:
         hipLaunchKernelGGL(k_round_comp2, dim3(g), dim3(kBlock), 0, s, n, pts, label, corr, corr_key, comp1,
