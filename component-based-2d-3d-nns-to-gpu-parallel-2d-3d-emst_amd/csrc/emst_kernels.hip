@@ -739,7 +739,6 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
 {
     using C = ConeCfg<D>;
     __shared__ unsigned long long best[C::K][64];
-    __shared__ Node stk_box[64];
     __shared__ int32_t stk_id[64];
     __shared__ int32_t stk_ver[64];
     __shared__ unsigned long long stk_mask[64];
@@ -837,19 +836,20 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
             refresh();
             int sp = 0;
             int ver = 0;        // bumps whenever a leaf tightened the bounds
-            if (lane == 0) { stk_box[0] = tree[1]; stk_id[0] = 1; stk_ver[0] = -1; }
+            if (lane == 0) { stk_id[0] = 1; stk_ver[0] = -1; }
             sp = 1;
             __builtin_amdgcn_wave_barrier();
             while (sp > 0) {
                 --sp;
                 const int node = __builtin_amdgcn_readfirstlane(stk_id[sp]);
                 const int pver = __builtin_amdgcn_readfirstlane(stk_ver[sp]);
-                const Node nd = stk_box[sp];
-                // a node tested against the current bounds when pushed keeps its lane mask
+                // a node tested against the current bounds when pushed keeps its
+                // lane mask; after a leaf tightened the bounds it is tested again
                 bool need;
                 if (pver == ver) {
                     need = (stk_mask[sp] >> lane) & 1ull;
                 } else {
+                    const Node nd = tree[node];
                     need = open != 0u && node_needed<D>(nd, px, py, pz, bound, maxb, inner, grouped);
                     ++st_nodes;
                 }
@@ -924,26 +924,45 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
                     if (need) refresh();
                     ++ver;
                 } else {
-                    const int a = 2 * node, b = 2 * node + 1;
-                    const Node boxa = tree[a], boxb = tree[b];
-                    const unsigned long long ma =
-                        __ballot(open != 0u && node_needed<D>(boxa, px, py, pz, bound, maxb, inner, grouped));
-                    const unsigned long long mb =
-                        __ballot(open != 0u && node_needed<D>(boxb, px, py, pz, bound, maxb, inner, grouped));
-                    st_nodes += 2;
-                    const bool ua = ma != 0ull, ub = mb != 0ull;
-                    const bool afirst = cdist(boxa) <= cdist(boxb);
+                    // the next boxes down: the children when they are leaves, else
+                    // the four grandchildren (a descent then waits on half as many
+                    // dependent loads; the four loads are independent)
+                    const bool leafkids = 2 * (int64_t)node >= L;
+                    const int first = leafkids ? 2 * node : 4 * node;
+                    const int nk = leafkids ? 2 : 4;
+                    Node bx[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (i < nk) bx[i] = tree[first + i];
+                    uint64_t mk[4];
+                    float dk[4];
+                    int id[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        mk[i] = i < nk ? __ballot(open != 0u && node_needed<D>(bx[i], px, py, pz, bound, maxb,
+                                                                                 inner, grouped))
+                                       : 0ull;
+                        dk[i] = mk[i] ? cdist(bx[i]) : -1.f;
+                        id[i] = first + i;
+                    }
+                    st_nodes += (uint32_t)nk;
+                    // farthest pushed first, so the nearest is popped next
+                    auto ce = [&](int a, int b) {
+                        if (dk[a] < dk[b]) {
+                            const float td = dk[a]; dk[a] = dk[b]; dk[b] = td;
+                            const uint64_t tm = mk[a]; mk[a] = mk[b]; mk[b] = tm;
+                            const int ti = id[a]; id[a] = id[b]; id[b] = ti;
+                        }
+                    };
+                    ce(0, 1); ce(2, 3); ce(0, 2); ce(1, 3); ce(1, 2);
                     if (lane == 0) {
                         int t = sp;
-                        if (afirst) {
-                            if (ub) { stk_box[t] = boxb; stk_id[t] = b; stk_ver[t] = ver; stk_mask[t] = mb; ++t; }
-                            if (ua) { stk_box[t] = boxa; stk_id[t] = a; stk_ver[t] = ver; stk_mask[t] = ma; }
-                        } else {
-                            if (ua) { stk_box[t] = boxa; stk_id[t] = a; stk_ver[t] = ver; stk_mask[t] = ma; ++t; }
-                            if (ub) { stk_box[t] = boxb; stk_id[t] = b; stk_ver[t] = ver; stk_mask[t] = mb; }
-                        }
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            if (mk[i]) { stk_id[t] = id[i]; stk_ver[t] = ver; stk_mask[t] = mk[i]; ++t; }
                     }
-                    sp += (int)ua + (int)ub;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) sp += mk[i] != 0ull ? 1 : 0;
                     __builtin_amdgcn_wave_barrier();
                 }
             }
