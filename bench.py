@@ -119,11 +119,13 @@ def main():
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         u, v, _ = ctx.getLinks()
+        st = pkg.search_stats(ctx)
+        samples = ctx.radius_samples() if form_model is not None else None
         ctx.close()
         assert m == args.n - 1, f"tree has {m} edges"
         med = lambda xs: sorted(xs)[len(xs) // 2]
         return {"dt": dt, "w": w, "m": m, "search_ms": med(search_ms), "total_ms": med(total_ms),
-                "edges_digest": hash((u.tobytes(), v.tobytes()))}
+                "edges_digest": hash((u.tobytes(), v.tobytes())), "stats": st, "samples": samples}
 
     plain = timed(None)
     learned = timed(model) if model is not None else None
@@ -131,6 +133,22 @@ def main():
         assert learned["edges_digest"] == plain["edges_digest"], "learned seeds changed the tree"
         assert learned["w"] == plain["w"]
     head = learned if learned is not None else plain
+    seed_report = None
+    if learned is not None:
+        # hit rate: points whose seed already covers every non-empty cone's
+        # nearest point (pass 0 closes them all); savings: traversal work
+        X, T = learned["samples"]
+        has = T > -19.0
+        y = pkg.mlp_eval(model, X) + model[-1]
+        ps, ls = plain["stats"], learned["stats"]
+        seed_report = {
+            "hit_rate_learned": round(float(np.mean((y >= T)[has])), 4),
+            "hit_rate_plain": round(float(np.mean((np.log(3.0) >= T)[has])), 4),
+            "nodes_tested_per_packet": [round(ps["nodes"] / ps["packets"], 1), round(ls["nodes"] / ls["packets"], 1)],
+            "pairs_per_point": [round(ps["pairs"] / args.n, 1), round(ls["pairs"] / args.n, 1)],
+            "node_test_savings": round(1 - ls["nodes"] / ps["nodes"], 4),
+            "pair_savings": round(1 - ls["pairs"] / ps["pairs"], 4),
+        }
     ms_per_step = head["dt"] * 1e3 / args.steps
     value = args.n * args.steps / head["dt"] / 1e6
     w, m = head["w"], head["m"]
@@ -176,6 +194,7 @@ def main():
                    "learned_search_ms": round(learned["search_ms"], 3) if learned else None,
                    "learned_vs_plain": round(plain["dt"] / learned["dt"], 3) if learned else None,
                    "radius_model_training_clouds": model_clouds,
+                   "learned_seed": seed_report,
                    "same_tree": learned is not None},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -110,11 +110,13 @@ __global__ __launch_bounds__(kBlock) void k_bbox(const float* __restrict__ xyz, 
             }
         const bool any_bad = __any(bad);
         if (lane == 0) {
+            // the extremes only move outwards: a value already at least as far
+            // out makes the atomic redundant (so for most waves once a few have run)
             uint32_t* plo = reinterpret_cast<uint32_t*>(P[cur].pmin);
             uint32_t* phi = reinterpret_cast<uint32_t*>(P[cur].pmax);
             for (int k = 0; k < D; ++k) {
-                atomicMin(&plo[k], lo[k]);
-                atomicMax(&phi[k], hi[k]);
+                if (__atomic_load_n(&plo[k], __ATOMIC_RELAXED) > lo[k]) atomicMin(&plo[k], lo[k]);
+                if (__atomic_load_n(&phi[k], __ATOMIC_RELAXED) < hi[k]) atomicMax(&phi[k], hi[k]);
             }
             if (any_bad) atomicOr(&P[cur].nonfinite, 1u);
         }
@@ -471,27 +473,35 @@ __global__ __launch_bounds__(kBlock) void k_dup_group(const uint64_t* __restrict
     for (int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + threadIdx.x / 64; w < nruns; w += nwaves) {
         const int64_t s = runs[w];
         const uint64_t key = keys[s];
-        int64_t e = s + 1;
-        for (int64_t b = s + 1;; b += 64) {
-            const int64_t i = b + lane;
-            const uint64_t m = __ballot(i >= n || keys[i] != key);
-            if (m) {
-                e = b + __builtin_ctzll(m);
-                break;
-            }
+        // run end: the first larger key (the keys are sorted)
+        int64_t lo = s + 1, hi = n;
+        while (lo < hi) {
+            const int64_t mid = lo + (hi - lo) / 2;
+            if (keys[mid] == key) lo = mid + 1;
+            else hi = mid;
         }
+        const int64_t e = lo;
         const float4 head = pts[s];
         bool mixed = false;
-        for (int64_t b = s; b < e; b += 64) {
-            const int64_t i = b + lane;
-            if (i < e) mixed |= !same_xyz(pts[i], head);
+        // four independent loads per lane per trip
+        for (int64_t b = s; b < e; b += 256) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t i = b + 64 * u + lane;
+                if (i < e) mixed |= !same_xyz(pts[i], head);
+            }
         }
         if (__ballot(mixed) == 0ull) {
-            if (lane == 0) gslot[s] = (int32_t)atomicAdd(&C->dup_groups, 1u);
-            for (int64_t b = s; b < e; b += 64) {
-                const int64_t i = b + lane;
-                if (i < e) rep[i] = (int32_t)s;
-                wave_append(i < e, (int32_t)i, list, &C->dup_points);
+            // one group: its list entries are reserved in one step
+            uint32_t base = 0;
+            if (lane == 0) {
+                gslot[s] = (int32_t)atomicAdd(&C->dup_groups, 1u);
+                base = atomicAdd(&C->dup_points, (uint32_t)(e - s));
+            }
+            base = (uint32_t)__shfl((int)base, 0);
+            for (int64_t i = s + lane; i < e; i += 64) {
+                rep[i] = (int32_t)s;
+                list[base + (uint32_t)(i - s)] = (int32_t)i;
             }
         } else if (e - s <= kMixedRunMax) {
             for (int64_t b = s; b < e; b += 64) {
@@ -1029,15 +1039,38 @@ __global__ __launch_bounds__(kBlock) void k_dup_m2(const int32_t* __restrict__ l
 {
     if (C->done || C->dup_settled) return;
     bool split = false;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < dup.npoints;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t p = dup.list[t];
-        const int32_t r = dup.rep[p];
-        if (label[p] == label[r]) continue;
-        atomicMin(&dup.m2[dup.gslot[r]], __float_as_uint(pts[p].w));
-        split = true;
+    // every lane walks the same number of trips, so the wave-wide steps below
+    // see all 64 lanes
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t0 = (int64_t)blockIdx.x * blockDim.x; t0 < dup.npoints; t0 += stride) {
+        const int64_t t = t0 + threadIdx.x;
+        bool valid = false;
+        int32_t g = 0;
+        uint32_t id = 0xffffffffu;
+        if (t < dup.npoints) {
+            const int32_t p = dup.list[t];
+            const int32_t r = dup.rep[p];
+            if (label[p] != label[r]) {
+                valid = true;
+                g = dup.gslot[r];
+                id = __float_as_uint(pts[p].w);
+            }
+        }
+        split |= valid;
+        // lanes of one group (list entries of a group are mostly adjacent)
+        // merge into one atomic; a lower value already there skips it
+        const uint64_t vm = __ballot(valid);
+        if (vm == 0ull) continue;
+        const int32_t lg = __shfl(g, (int)__builtin_ctzll(vm));
+        const bool grp = valid && g == lg;
+        uint32_t v = grp ? id : 0xffffffffu;
+        for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off));
+        if (__lane_id() == (int)__builtin_ctzll(vm) && __atomic_load_n(&dup.m2[lg], __ATOMIC_RELAXED) > v)
+            atomicMin(&dup.m2[lg], v);
+        if (valid && !grp && __atomic_load_n(&dup.m2[g], __ATOMIC_RELAXED) > id) atomicMin(&dup.m2[g], id);
     }
-    if (split) atomicOr(&C->dup_split, 1u);
+    if (__ballot(split) != 0ull && __lane_id() == 0 && __atomic_load_n(&C->dup_split, __ATOMIC_RELAXED) == 0u)
+        atomicOr(&C->dup_split, 1u);
 }
 
 // Cones of the group points.  A point whose location is shared with another
