@@ -45,7 +45,7 @@ class Options(ctypes.Structure):
                 ("max_rounds", ctypes.c_int32), ("bucket_size", ctypes.c_int32),
                 ("trace", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("radius_mode", ctypes.c_int32), ("seed_scale", ctypes.c_float),
-                ("reserved", ctypes.c_int32 * 4)]
+                ("cone_dim", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
 
 
 class RoundTrace(ctypes.Structure):
@@ -139,13 +139,16 @@ class Calculateur:
     """One EMST context on one device (CalculateurEMST / SolutionEMST)."""
 
     def __init__(self, dim=3, level_radius=1, max_rounds=100, bucket_size=0, trace=False, device=0,
-                 seed_scale=0.0):
+                 seed_scale=0.0, cone_dim=0):
+        """cone_dim=2 with dim=3: the 2.5D mode (SolutionEMST<3,2>, 3D points,
+        wedges over the xy plane)."""
         L = lib()
         o = Options()
         L.emst_default_options(ctypes.byref(o))
         o.dim, o.level_radius, o.max_rounds = dim, level_radius, max_rounds
         o.bucket_size, o.trace, o.device = bucket_size, int(trace), device
         o.seed_scale = seed_scale
+        o.cone_dim = cone_dim
         self._L = L
         self._c = L.emst_create(ctypes.byref(o))
         if not self._c:
@@ -341,12 +344,12 @@ def mlp_eval(model, X, device=0):
 
 
 def _stats_names():
-    return ["packets", "nodes", "leaves", "leaf_points", "pairs", "group_points", "grouped"]
+    return ["packets", "nodes", "leaves", "leaf_points", "pairs", "group_points", "grouped", "cycles_cut"]
 
 
 def search_stats(calc):
-    t = (ctypes.c_int64 * 7)()
-    k = ctypes.c_int32(7)
+    t = (ctypes.c_int64 * 8)()
+    k = ctypes.c_int32(8)
     calc._chk(calc._L.emst_get_search_stats(calc._c, t, ctypes.byref(k)), "stats")
     return dict(zip(_stats_names(), list(t)[: k.value]))
 

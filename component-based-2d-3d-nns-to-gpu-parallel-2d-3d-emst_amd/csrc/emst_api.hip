@@ -46,6 +46,7 @@ struct emst_ctx {
     // point set: ncl clouds (one for emst_set_points), n points in all
     int64_t n = 0;
     int dim = 0;
+    int cdim = 0;                     // cone dimension: dim, or 2 over 3D points (2.5D mode)
     int K = 0;
     int bucket = 32;
     int64_t L = 0;                    // leaves of cloud 0 (the single-cloud hierarchy)
@@ -63,7 +64,7 @@ struct emst_ctx {
     uint32_t zero_or_empty = 0;
     std::vector<emst_round_trace> trace;
     float times[4] = {0, 0, 0, 0};
-    int64_t stats[7] = {0, 0, 0, 0, 0, 0, 0};
+    int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     float seed_scale = 3.0f;          // plain seed: multiple of the leaf cell's mean spacing
     hipEvent_t ev[8] = {};
     // device buffers
@@ -86,6 +87,11 @@ struct emst_ctx {
     DevBuf<int32_t> dup_rep, dup_list, dup_runs, dup_gslot, dup_bpos;
     DevBuf<uint32_t> dup_m2;
     DevBuf<uint64_t> dup_bkey;
+    // cycle rule of the 2.5D mode (CycleState)
+    DevBuf<int32_t> cy_j0, cy_j;
+    DevBuf<uint64_t> cy_kh;
+    DevBuf<uint32_t> cy_kl;
+    DevBuf<uint8_t> cy_onc, cy_cut;
     bool no_grouping = false;         // set when distinct points met at float distance 0
     bool grouped = false;
     bool groups_settled = false;      // no group spans two components any more
@@ -194,6 +200,14 @@ int alloc_all(emst_ctx* c)
     ENS(dup_list, n);
     ENS(dup_runs, n);
     ENS(dup_gslot, n);
+    if (c->cdim < c->dim) {
+        ENS(cy_j0, n);
+        ENS(cy_j, 2 * n);
+        ENS(cy_kh, 2 * n);
+        ENS(cy_kl, 2 * n);
+        ENS(cy_onc, n);
+        ENS(cy_cut, n);
+    }
 #undef ENS
     return EMST_OK;
 }
@@ -215,9 +229,9 @@ float elapsed(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
-static_assert(offsetof(Counters, pad2) + sizeof(uint32_t) - offsetof(Counters, dup_runs) ==
+static_assert(offsetof(Counters, cycles_cut) + sizeof(uint32_t) - offsetof(Counters, dup_runs) ==
                   8 * sizeof(uint32_t),
-              "the dup counters are cleared as 8 consecutive words");
+              "the dup counters and cycles_cut are cleared as 8 consecutive words");
 static_assert(offsetof(Counters, zero_or_empty) == offsetof(Counters, research) + sizeof(uint32_t),
               "research and zero_or_empty are cleared together");
 
@@ -255,6 +269,27 @@ int layout_clouds(emst_ctx* c, int bucket)
     c->P = c->params.p;
     HIPCHK(c, hipMemcpyAsync(c->dcl.p, c->hcl.data(), sizeof(Cloud) * c->ncl, hipMemcpyHostToDevice, c->stream));
     return EMST_OK;
+}
+
+CycleState cycle_state(emst_ctx* c)
+{
+    CycleState y{};
+    if (c->cdim >= c->dim) return y;
+    const int64_t n = c->n;
+    y.j0 = c->cy_j0.p;
+    y.j[0] = c->cy_j.p;
+    y.j[1] = c->cy_j.p + n;
+    y.kh[0] = c->cy_kh.p;
+    y.kh[1] = c->cy_kh.p + n;
+    y.kl[0] = c->cy_kl.p;
+    y.kl[1] = c->cy_kl.p + n;
+    y.onc = c->cy_onc.p;
+    y.cut = c->cy_cut.p;
+    y.inv = c->inv.p;
+    int steps = 1;
+    while (steps < 62 && ((int64_t)1 << steps) < n) ++steps;
+    y.steps = steps;
+    return y;
 }
 
 DupState dup_state(emst_ctx* c)
@@ -300,7 +335,7 @@ int build(emst_ctx* c)
     }
     radix_sort(c->keys0.p, c->vals0.p, c->keys1.p, c->vals1.p, n, key_bits, true, c->hist.p, c->scan_tmp.p, s);
     launch_gather_sorted(c->resc.p, c->vals0.p, n, c->dim, c->pts.p, c->inv.p, s);
-    // coincident-point groups: dup_runs .. pad2 start at zero
+    // coincident-point groups: dup_runs .. cycles_cut start at zero
     HIPCHK(c, hipMemsetAsync(&c->C->dup_runs, 0, 8 * sizeof(uint32_t), s));
     c->grouped = false;
     c->groups_settled = false;
@@ -330,7 +365,7 @@ int build(emst_ctx* c)
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ev[1], s));
     const float* seed_r = nullptr;
-    if (c->learned && c->ncl == 1) {
+    if (c->learned && c->ncl == 1 && c->cdim == c->dim) {
         int rc2;
         if ((rc2 = ensure(c, c->feat, (size_t)n * 8)) || (rc2 = ensure(c, c->pred, (size_t)n)) ||
             (rc2 = ensure(c, c->seed, (size_t)n)) || (rc2 = ensure(c, c->ls, (size_t)n)))
@@ -339,12 +374,12 @@ int build(emst_ctx* c)
         launch_radius_seed(n, c->model.p, c->feat.p, c->ls.p, c->pred.p, c->seed.p, s);
         seed_r = c->seed.p;
     }
-    launch_cone_search(c->dim, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, n, nullptr, n, c->label.p, false,
-                       c->inv.p, seed_r, c->seed_scale, c->cand_key.p, c->cand_pos.p, nullptr, c->corr.p,
+    launch_cone_search(c->dim, c->cdim, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, n, nullptr, n,
+                       c->label.p, false, c->inv.p, seed_r, c->seed_scale, c->cand_key.p, c->cand_pos.p, nullptr, c->corr.p,
                        c->corr_key.p, c->C, true, dup, s);
-    launch_dup_fill(c->dim, n, c->label.p, c->pts.p, c->inv.p, c->cand_key.p, c->cand_pos.p, nullptr, c->corr.p,
+    launch_dup_fill(c->cdim, n, c->label.p, c->pts.p, c->inv.p, c->cand_key.p, c->cand_pos.p, nullptr, c->corr.p,
                     c->corr_key.p, c->C, dup, s);
-    launch_rank_cones(c->dim, n, c->cand_key.p, nullptr, n, c->perm.p, c->cursor.p, nullptr, s);
+    launch_rank_cones(c->cdim, n, c->cand_key.p, nullptr, n, c->perm.p, c->cursor.p, nullptr, s);
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ev[2], s));
     // while grouping is on only group points can hold a zero-distance or no
@@ -369,6 +404,7 @@ int build(emst_ctx* c)
     c->stats[4] = (int64_t)h.pairs;
     c->stats[5] = (int64_t)h.dup_points;
     c->stats[6] = c->grouped ? 1 : 0;
+    c->stats[7] = 0;
     if (c->grouped) c->groups_settled = h.dup_split == 0u;
     c->times[0] = elapsed(c->ev[0], c->ev[1]);
     c->times[1] = elapsed(c->ev[1], c->ev[2]);
@@ -410,15 +446,16 @@ void issue_round(emst_ctx* c, bool tr)
     // kernels skip themselves once the groups have settled
     if (c->grouped) {
         const DupState d = dup_state(c);
-        launch_dup_fill(c->dim, c->n, c->label.p, c->pts.p, c->inv.p, c->cand_key.p, c->cand_pos.p, c->corr.p,
+        launch_dup_fill(c->cdim, c->n, c->label.p, c->pts.p, c->inv.p, c->cand_key.p, c->cand_pos.p, c->corr.p,
                         nullptr, nullptr, c->C, d, s);
-        launch_rank_cones(c->dim, c->n, c->cand_key.p, d.list, d.npoints, c->perm.p, c->cursor.p, c->C, s);
+        launch_rank_cones(c->cdim, c->n, c->cand_key.p, d.list, d.npoints, c->perm.p, c->cursor.p, c->C, s);
     }
+    const CycleState cy = cycle_state(c);
     for (int ph = 0; ph < 4; ++ph) {
         launch_round_phase(ph, c->K, c->n, c->pts.p, c->cand_key.p, c->cand_pos.p, c->perm.p, c->cursor.p,
                            c->label.p, c->label2.p,
                            c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->fixed.p, c->C, c->eu.p,
-                           c->ev_.p, c->ed2.p, c->eround.p, s);
+                           c->ev_.p, c->ed2.p, c->eround.p, cy, s);
         if (tr) (void)hipEventRecord(c->ev[4 + ph], s);
     }
     launch_round_end(c->C, c->grouped, s);
@@ -436,7 +473,7 @@ int resume_flatten(emst_ctx* c, Counters* h)
         launch_round_phase(3, c->K, c->n, c->pts.p, c->cand_key.p, c->cand_pos.p, c->perm.p, c->cursor.p,
                            c->label.p, c->label2.p,
                            c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->fixed.p, c->C, c->eu.p,
-                           c->ev_.p, c->ed2.p, c->eround.p, s);
+                           c->ev_.p, c->ed2.p, c->eround.p, CycleState{}, s);
         LAUNCHCHK(c);
         if ((rc = read_counters(c, h))) return rc;
     }
@@ -450,6 +487,7 @@ void absorb_counters(emst_ctx* c, const Counters& h)
 {
     c->edges = h.edges;
     c->rounds = (int)h.round;
+    c->stats[7] = (int64_t)h.cycles_cut;
     if (h.done == 1u) c->finished = true;
 }
 
@@ -478,10 +516,10 @@ int round_once(emst_ctx* c, int32_t* finished)
         int rc = read_counters(c, &h);
         if (rc) return rc;
         if (h.research > 0) {
-            launch_cone_search(c->dim, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, n, c->list.p, h.research,
-                               c->label.p, true, c->inv.p, nullptr, c->seed_scale, c->cand_key.p,
+            launch_cone_search(c->dim, c->cdim, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, n, c->list.p,
+                               h.research, c->label.p, true, c->inv.p, nullptr, c->seed_scale, c->cand_key.p,
                                c->cand_pos.p, c->corr.p, nullptr, nullptr, c->C, false, DupState{}, s);
-            launch_rank_cones(c->dim, n, c->cand_key.p, c->list.p, h.research, c->perm.p, c->cursor.p, nullptr, s);
+            launch_rank_cones(c->cdim, n, c->cand_key.p, c->list.p, h.research, c->perm.p, c->cursor.p, nullptr, s);
         }
         LAUNCHCHK(c);
         searched = h.zero_or_empty;
@@ -522,7 +560,10 @@ constexpr int kGraphRounds = 4;
 std::vector<uintptr_t> graph_key(emst_ctx* c)
 {
     const DupState d = dup_state(c);
-    return {(uintptr_t)c->n, (uintptr_t)c->K, (uintptr_t)c->dim, (uintptr_t)c->grouped, (uintptr_t)d.npoints,
+    const CycleState y = cycle_state(c);
+    return {(uintptr_t)c->n, (uintptr_t)c->K, (uintptr_t)c->dim, (uintptr_t)c->cdim, (uintptr_t)y.j0,
+            (uintptr_t)y.j[0], (uintptr_t)y.kh[0], (uintptr_t)y.kl[0], (uintptr_t)y.onc, (uintptr_t)y.cut,
+            (uintptr_t)c->grouped, (uintptr_t)d.npoints,
             (uintptr_t)d.ngroups, (uintptr_t)c->C, (uintptr_t)c->pts.p, (uintptr_t)c->cand_key.p,
             (uintptr_t)c->cand_pos.p, (uintptr_t)c->label.p, (uintptr_t)c->label2.p, (uintptr_t)c->corr.p,
             (uintptr_t)c->corr_key.p, (uintptr_t)c->comp1.p, (uintptr_t)c->comp2.p, (uintptr_t)c->fixed.p,
@@ -603,6 +644,14 @@ emst_ctx* emst_create(const emst_options* opt)
     if (opt) o = *opt;
     else emst_default_options(&o);
     if (o.dim != 2 && o.dim != 3) { g_create_error = "dim must be 2 or 3"; return nullptr; }
+    if (o.cone_dim != 0 && o.cone_dim != 2 && o.cone_dim != 3) {
+        g_create_error = "cone_dim must be 0 (the point dimension), 2 or 3";
+        return nullptr;
+    }
+    if (o.cone_dim == 2 && o.dim == 3 && o.radius_mode != 0) {
+        g_create_error = "the learned seed radius is not defined for the 2.5D mode";
+        return nullptr;
+    }
     if (o.max_rounds <= 0) o.max_rounds = 100;
     if (o.level_radius <= 0) o.level_radius = 1;
     int ndev = 0;
@@ -662,10 +711,13 @@ int emst_set_clouds(emst_ctx* c, int32_t ncl, const int64_t* npts, const float* 
     }
     if (n > 0 && !xyz) return fail(c, EMST_E_ARG, "NULL points");
     if (n >= ((int64_t)1 << 31) - 1) return fail(c, EMST_E_LIMIT, "total points must be below 2^31 - 1");
+    const int cdim = c->opt.cone_dim == 0 ? dim : c->opt.cone_dim;
+    if (cdim > dim) return fail(c, EMST_E_ARG, "cone_dim must not exceed the point dimension");
     HIPCHK(c, hipSetDevice(c->opt.device));
     c->n = n;
     c->dim = dim;
-    c->K = 2 * dim * (1 << (dim - 1));
+    c->cdim = cdim;
+    c->K = 2 * cdim * (1 << (cdim - 1));
     c->ncl = ncl;
     c->hcl.assign(ncl, Cloud{});
     for (int32_t k = 0; k < ncl; ++k) c->hcl[k].n = npts[k];
@@ -909,7 +961,7 @@ int emst_get_cell_grid(emst_ctx* c, int32_t R, int32_t* ext_dual, int32_t* cell_
     int32_t geo[16] = {0};
     for (int k = 0; k < 3; ++k) {
         int32_t ext = 1, pc = 0, PC = 0, ed = 1;
-        if (k < c->dim) {
+        if (k < c->cdim) {
             ext = (int32_t)std::ceil((double)hp.newmax[k] - 0.0) + 1;
             pc = ext / 2;
             PC = (pc + (R - 1)) / R;
@@ -922,7 +974,7 @@ int emst_get_cell_grid(emst_ctx* c, int32_t R, int32_t* ext_dual, int32_t* cell_
         geo[6 + k] = pc;
     }
     geo[9] = R;
-    geo[10] = c->dim;
+    geo[10] = c->cdim;
     for (int k = 0; k < 3; ++k) ext_dual[k] = geo[k];
     if (!cell_of && !cell_start && !cell_pts) return EMST_OK;
     const int64_t ncell = (int64_t)geo[0] * geo[1] * geo[2];
@@ -939,7 +991,7 @@ int emst_get_cell_grid(emst_ctx* c, int32_t R, int32_t* ext_dual, int32_t* cell_
     hipError_t e = hipMemcpyAsync(c->geo.p, geo, sizeof(geo), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemsetAsync(dcount.p, 0, sizeof(uint32_t) * (ncell + 1), s);
     if (e == hipSuccess) {
-        launch_cell_grid(c->resc.p, n, c->dim, c->geo.p, dcell.p, c->keys0.p, dcount.p, s);
+        launch_cell_grid(c->resc.p, n, c->cdim, c->dim, c->geo.p, dcell.p, c->keys0.p, dcount.p, s);
         exclusive_scan_u32(dcount.p, dstart.p, ncell + 1, tmp.p, s);
         int bits = 8;
         while (bits < 64 && ((uint64_t)ncell >> bits) != 0) bits += 8;
@@ -989,6 +1041,8 @@ int emst_set_radius_model(emst_ctx* c, const float* model)
         c->built = false;
         return EMST_OK;
     }
+    if (c->opt.cone_dim == 2 && c->opt.dim == 3)
+        return fail(c, EMST_E_ARG, "the learned seed radius is not defined for the 2.5D mode");
     int rc = ensure(c, c->model, EMST_MLP_PARAMS);
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(c->model.p, model, sizeof(float) * EMST_MLP_PARAMS, hipMemcpyHostToDevice, c->stream));
@@ -1004,6 +1058,7 @@ int emst_get_radius_samples(emst_ctx* c, float* feat, float* target, int64_t* m)
     int rc = ensure_built(c);
     if (rc) return rc;
     if (c->ncl != 1) return fail(c, EMST_E_STATE, "radius samples come from a single-cloud context");
+    if (c->cdim != c->dim) return fail(c, EMST_E_STATE, "radius samples come from full-dimensional cones");
     const int64_t n = c->n;
     if (!feat && !target) { *m = n; return EMST_OK; }
     if (*m < n) { *m = n; return fail(c, EMST_E_ARG, "sample arrays too small"); }
@@ -1025,7 +1080,7 @@ int emst_get_radius_samples(emst_ctx* c, float* feat, float* target, int64_t* m)
 int emst_get_search_stats(emst_ctx* c, int64_t* stats, int32_t* n_stats)
 {
     if (!c || !stats || !n_stats) return EMST_E_ARG;
-    const int32_t k = std::min<int32_t>(*n_stats, 7);
+    const int32_t k = std::min<int32_t>(*n_stats, 8);
     for (int i = 0; i < k; ++i) stats[i] = c->stats[i];
     *n_stats = k;
     return EMST_OK;

@@ -7,8 +7,9 @@
 //
 // The config file is the reference's sectioned "key = value" format; read
 // here: [coalition_global_param] functionModeChoice (4 RUN: 2D points,
-// 5 RUN_3D: 3D points; every other mode is reported unsupported, as the
-// reference does for modes it does not build), [global_param] traceActive
+// 5 RUN_3D: 3D points, 6 RUN_2DPOINT5: 3D points with cones over the xy
+// plane, CalculateurEMST.h:40; every other mode is reported unsupported, as
+// the reference does for modes it does not build), [global_param] traceActive
 // (one stats row per round) and levelRadius.  Without a config file the mode
 // follows the data suffix (.3d -> 3D, otherwise 2D) instead of the
 // reference's default mode 2, which runs no EMST.
@@ -155,11 +156,11 @@ int main(int argc, char* argv[])
         mode = is3d ? 5 : 4;
         if (!have_cfg) std::cout << "no config file: mode " << mode << " from the data suffix" << std::endl;
     }
-    if (mode != 4 && mode != 5) {
+    if (mode != 4 && mode != 5 && mode != 6) {
         std::cout << "UNSUPPORTED FUNCTIONMODE=" << mode << " !! " << std::endl;
         return 0;
     }
-    const int dim = mode == 5 ? 3 : 2;
+    const int dim = mode == 4 ? 2 : 3;
     const bool trace = get_int(kv, "global_param.traceActive", 1) != 0;
 
     int64_t n = 0;
@@ -178,6 +179,7 @@ int main(int argc, char* argv[])
     emst_options o;
     emst_default_options(&o);
     o.dim = dim;
+    o.cone_dim = mode == 6 ? 2 : 0;
     o.trace = 1;            // per-round times and lengths for the stats rows
     o.level_radius = get_int(kv, "global_param.levelRadius", 1);
     emst_ctx* c = emst_create(&o);

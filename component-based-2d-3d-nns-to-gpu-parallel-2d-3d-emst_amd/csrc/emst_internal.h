@@ -38,7 +38,7 @@ struct Counters {
     uint32_t underflow;     // two distinct points at float distance 0 seen: grouping must be off
     uint32_t dup_groups;    // number of groups
     uint32_t dup_split;     // set by k_dup_m2 when some group spans >= 2 components
-    uint32_t pad2;
+    uint32_t cycles_cut;    // hooks withheld by the cycle rule (2.5D mode), over the run
     // round control, on the device so that rounds can run back to back
     // (captured in a graph) and stop by themselves
     uint32_t round;         // rounds run so far
@@ -68,6 +68,29 @@ struct DupState {
     int32_t* bpos;
     int64_t npoints;        // entries of list
     int64_t ngroups;
+};
+
+// Cycle rule of the 2.5D mode (wedges over the xy plane, 3D distances).
+// There a component's chosen edge need not be the other side's smallest, and
+// the choices can close a cycle of components, of length >= 3 or of length 2
+// with two different edges; hooking along it would leave a cyclic label
+// chain.  Per round, over the component roots: j0 = the root that the chosen
+// edge leads to (the root itself when it has no outgoing candidate or chose
+// the same edge as that root: a mutual pair, which the hook resolves);
+// pointer doubling (j, kh/kl ping-pong, `steps` steps with 2^steps >= points)
+// gives the largest chosen edge (kh = d2 bits << 32 | min id, kl = max id)
+// on the path of 2^steps hops, which for a root on a cycle is the cycle's
+// largest; onc marks the roots on cycles, cut the root whose own edge is its
+// cycle's largest: that root does not hook this round.  j0 == nullptr: off.
+struct CycleState {
+    int32_t* j0;
+    int32_t* j[2];
+    uint64_t* kh[2];
+    uint32_t* kl[2];
+    uint8_t* onc;
+    uint8_t* cut;
+    const int32_t* inv;     // global id -> sorted position
+    int steps;
 };
 
 // One point cloud of a context (a context holds one cloud, or a batch of
@@ -119,13 +142,15 @@ void launch_build_tree(const float4* pts, const Cloud* cl, int ncl, int64_t nlea
 // Cone search over queries [0,n) (qlist == nullptr) or qlist[0..nq).
 // check_labels: skip zero-distance points of the query's own component.
 // collect_stats: add the traversal counters to C.
-void launch_cone_search(int dim, const float4* pts, const Node* nodes, const Cloud* cl, int ncl, int bucket,
+// dim: point dimension (distances); cdim: cone dimension (cdim = 2 with dim =
+// 3 is the 2.5D mode: wedges over the xy plane).
+void launch_cone_search(int dim, int cdim, const float4* pts, const Node* nodes, const Cloud* cl, int ncl, int bucket,
                         int64_t n, const int32_t* qlist, int64_t nq, const int32_t* label,
                         bool check_labels, const int32_t* inv, const float* seed_r, float seed_scale,
                         uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
                         uint64_t* first_key, Counters* C, bool collect_stats, DupState dup,
                         hipStream_t s);
-// Cones of every group point for the current labels (after the grouped
+// Cones of every group point for the current labels (dim: the cone dimension) (after the grouped
 // cone search, and at the start of each round while a group may still span
 // several components).  Sets C->dup_split when one does.
 void launch_dup_fill(int dim, int64_t n, const int32_t* label, const float4* pts, const int32_t* inv,
@@ -138,6 +163,7 @@ void launch_round_init(int64_t n, int32_t* label, int32_t* eu, int32_t* corr, ui
 // key + winners, 2 hook, 3 flatten.
 // Ranks each point's cones by key for k_round_min (all points, or the qlist
 // points); C != nullptr: skipped once the rounds are done or groups settled.
+// dim: the cone dimension.
 void launch_rank_cones(int dim, int64_t n, const uint64_t* cand_key, const int32_t* qlist, int64_t nq,
                        uint8_t* perm, uint8_t* cursor, const Counters* C, hipStream_t s);
 void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const uint64_t* cand_key,
@@ -145,7 +171,7 @@ void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const ui
                         int32_t* label2, int32_t* corr,
                         uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, int32_t* fixed,
                         Counters* C, int32_t* eu, int32_t* ev, float* ed2, int32_t* eround,
-                        hipStream_t s);
+                        const CycleState& cy, hipStream_t s);
 // Round control: begin (termination test, round += 1) and end (no progress
 // -> finished; flatten unfinished -> paused; groups settled).
 void launch_round_begin(Counters* C, hipStream_t s);
@@ -170,7 +196,8 @@ void launch_edges_out(const uint64_t* keys, const uint32_t* idx, const float* ed
                       int32_t* u, int32_t* v, float* d2, hipStream_t s);
 // Reference cellular matrix (findCell + CSR).
 // geo: int32[16] = ext_dual[3], PC[3], pc[3], R, dim (host-computed, see emst_api.hip).
-void launch_cell_grid(const float* resc, int64_t n, int dim, const int32_t* geo, int32_t* cell_of,
+// dim = grid dimension, pstride = floats per point (3 for the 2.5D mode's 2D grid).
+void launch_cell_grid(const float* resc, int64_t n, int dim, int pstride, const int32_t* geo, int32_t* cell_of,
                       uint64_t* keys, uint32_t* cell_count, hipStream_t s);
 // learned seed radius (emst_mlp.hip)
 // X: n x 8 model features, LS: n x log(leaf spacing s0)

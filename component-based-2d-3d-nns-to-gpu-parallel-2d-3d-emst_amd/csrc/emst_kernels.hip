@@ -659,7 +659,11 @@ __device__ __forceinline__ uint32_t cone_mask(float dx, float dy, float dz)
 // box must reach below the query on that axis; a '+' face holds d_a > 0 or
 // the zero vector, and with strict_plus (coincident points never candidates)
 // d_a > 0 only.
-template <int D>
+//
+// D = distance dimension, DC = cone dimension.  DC = 2 with D = 3 (the 2.5D
+// mode) classifies by the xy offset only, and every face bound also carries
+// the box's smallest |dz|.
+template <int D, int DC>
 __device__ __forceinline__ bool node_needed(const Node& nd, float px, float py, float pz,
                                             const int32_t* bound, int32_t maxb, int32_t inner,
                                             bool strict_plus)
@@ -684,22 +688,22 @@ __device__ __forceinline__ bool node_needed(const Node& nd, float px, float py, 
     }
     bool need = false;
 #pragma unroll
-    for (int f = 0; f < 2 * D; ++f) {
+    for (int f = 0; f < 2 * DC; ++f) {
         const int a = f / 2;
         const bool neg = f & 1;
         const int o1 = a == 0 ? 1 : 0;
         const int o2 = a == 2 ? 1 : 2;
-        const float o = D == 3 ? fmaxf(m[o1], m[o2]) : m[o1];
+        const float o = DC == 3 ? fmaxf(m[o1], m[o2]) : m[o1];
         const float reach = neg ? -l[a] : h[a];
         const float am = fmaxf(neg ? -h[a] : l[a], o);
         float v[3];
         v[a] = am;
         v[o1] = m[o1];
-        if (D == 3) v[o2] = m[o2];
+        if (D == 3) v[o2] = m[o2];      // DC = 2: o2 is the z axis, outside the cone
         const float lb = D == 3 ? d2_of(v[0], v[1], v[2]) : d2_of(v[0], v[1]);
-        const int32_t* b = bound + f * (1 << (D - 1));
+        const int32_t* b = bound + f * (1 << (DC - 1));
         int32_t bm;
-        if (D == 3) {
+        if (DC == 3) {
             bm = max(max((fp[o1] && fp[o2]) ? b[0] : -1, (fn[o1] && fp[o2]) ? b[1] : -1),
                      max((fp[o1] && fn[o2]) ? b[2] : -1, (fn[o1] && fn[o2]) ? b[3] : -1));
         } else {
@@ -728,7 +732,7 @@ constexpr int kMaxCappedPasses = 6;
 //
 // Seeds: seed_r (per sorted point) when given, else seed_scale times the mean
 // spacing of the point's own leaf cell ((box volume / points)^(1/D)).
-template <int D>
+template <int D, int DC>
 __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ pts,
                                                    const Node* __restrict__ nodes,
                                                    const Cloud* __restrict__ clouds, int ncl,
@@ -747,7 +751,7 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
                                                    Counters* __restrict__ ctr, int collect_stats,
                                                    DupState dup)
 {
-    using C = ConeCfg<D>;
+    using C = ConeCfg<DC>;
     __shared__ unsigned long long best[C::K][64];
     __shared__ int32_t stk_id[64];
     __shared__ int32_t stk_ver[64];
@@ -760,6 +764,10 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
     // representative's cones) and no point takes a candidate at its own
     // location; otherwise only the point itself is skipped
     const bool grouped = dup.rep != nullptr;
+    // strict '+' faces need the '+' cones to hold no point but the query's
+    // own location; with xy wedges over 3D points they also hold the points
+    // straight above and below it
+    const bool strict_plus = grouped && DC == D;
     bool is_rep = false;
     if (grouped && active) {
         const int32_t r = dup.rep[p];
@@ -860,7 +868,7 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
                     need = (stk_mask[sp] >> lane) & 1ull;
                 } else {
                     const Node nd = tree[node];
-                    need = open != 0u && node_needed<D>(nd, px, py, pz, bound, maxb, inner, grouped);
+                    need = open != 0u && node_needed<D, DC>(nd, px, py, pz, bound, maxb, inner, strict_plus);
                     ++st_nodes;
                 }
                 const uint64_t needm = __ballot(need);
@@ -894,7 +902,7 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
                         if (grouped && take && d2 == 0.f) underflow = true;
                         if (check_labels && take && d2 == 0.f) take = label[s] != qlab;
                         if (take) {
-                            uint32_t mask = cone_mask<D>(dx, dy, dz) & qopen;
+                            uint32_t mask = cone_mask<DC>(dx, dy, dz) & qopen;
                             const unsigned long long key =
                                 ((unsigned long long)__float_as_uint(d2) << 32) | __float_as_uint(v.w);
                             while (mask) {
@@ -949,8 +957,8 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
                     int id[4];
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        mk[i] = i < nk ? __ballot(open != 0u && node_needed<D>(bx[i], px, py, pz, bound, maxb,
-                                                                                 inner, grouped))
+                        mk[i] = i < nk ? __ballot(open != 0u && node_needed<D, DC>(bx[i], px, py, pz, bound,
+                                                                                     maxb, inner, strict_plus))
                                        : 0ull;
                         dk[i] = mk[i] ? cdist(bx[i]) : -1.f;
                         id[i] = first + i;
@@ -1438,7 +1446,8 @@ __global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const floa
                                                           Counters* C, int32_t* __restrict__ eu,
                                                           int32_t* __restrict__ ev,
                                                           float* __restrict__ ed2,
-                                                          int32_t* __restrict__ eround)
+                                                          int32_t* __restrict__ eround,
+                                                          const uint8_t* __restrict__ cut)
 {
     if (C->done) return;
     const int round = (int)C->round;
@@ -1447,7 +1456,7 @@ __global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const floa
     __syncthreads();
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool add = false;
-    if (p < n && fixed[p]) {
+    if (p < n && fixed[p] && !(cut && cut[label[p]])) {
         const int32_t q = corr[p];
         const uint64_t ck = corr_key[p];
         const uint32_t pid = __float_as_uint(pts[p].w), qid = (uint32_t)ck;
@@ -1472,6 +1481,79 @@ __global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const floa
     if (__lane_id() == 0 && am) atomicAdd(&cnt, (uint32_t)__popcll(am));
     __syncthreads();
     if (threadIdx.x == 0 && cnt) atomicAdd(&C->edges, cnt);
+}
+
+// ---- cycle rule of the 2.5D mode (CycleState).  Roots are the points whose
+// flattened label is their own position; every kernel visits roots only.
+__global__ __launch_bounds__(kBlock) void k_cycle_init(int64_t n, const int32_t* __restrict__ label,
+                                                      const uint64_t* __restrict__ comp1,
+                                                      const uint32_t* __restrict__ comp2, CycleState cy,
+                                                      const Counters* C)
+{
+    if (C->done) return;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n || label[r] != (int32_t)r) return;
+    int32_t t = (int32_t)r;
+    const uint64_t k1 = comp1[r];
+    const uint32_t k2 = comp2[r];
+    if (k1 != kEmptyKey) {
+        // the chosen edge (d2, min id, max id): its end outside the component
+        const int32_t la = label[cy.inv[(uint32_t)k1]], lb = label[cy.inv[k2]];
+        const int32_t o = la == (int32_t)r ? lb : la;
+        if (!(comp1[o] == k1 && comp2[o] == k2)) t = o;     // else a mutual pair
+    }
+    cy.j0[r] = t;
+    cy.j[0][r] = t;
+    cy.kh[0][r] = t == (int32_t)r ? 0ull : k1;
+    cy.kl[0][r] = t == (int32_t)r ? 0u : k2;
+    cy.onc[r] = 0;
+}
+
+// one doubling step: hop 2^k further, keep the largest edge key on the way
+__global__ __launch_bounds__(kBlock) void k_cycle_step(int64_t n, const int32_t* __restrict__ label,
+                                                      CycleState cy, int src, const Counters* C)
+{
+    if (C->done) return;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n || label[r] != (int32_t)r) return;
+    const int dst = src ^ 1;
+    const int32_t j = cy.j[src][r];
+    uint64_t h = cy.kh[src][r];
+    uint32_t l = cy.kl[src][r];
+    const uint64_t h2 = cy.kh[src][j];
+    const uint32_t l2 = cy.kl[src][j];
+    if (h2 > h || (h2 == h && l2 > l)) {
+        h = h2;
+        l = l2;
+    }
+    cy.j[dst][r] = cy.j[src][j];
+    cy.kh[dst][r] = h;
+    cy.kl[dst][r] = l;
+}
+
+// 2^steps hops from any root end on a cycle (or at a root that leads nowhere),
+// and every root of a cycle is such an end
+__global__ __launch_bounds__(kBlock) void k_cycle_mark(int64_t n, const int32_t* __restrict__ label,
+                                                      CycleState cy, int fin, const Counters* C)
+{
+    if (C->done) return;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n || label[r] != (int32_t)r) return;
+    cy.onc[cy.j[fin][r]] = 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cycle_cut(int64_t n, const int32_t* __restrict__ label,
+                                                     const uint64_t* __restrict__ comp1,
+                                                     const uint32_t* __restrict__ comp2, CycleState cy,
+                                                     int fin, Counters* C)
+{
+    if (C->done) return;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n || label[r] != (int32_t)r) return;
+    const bool cut = cy.onc[r] && cy.j0[r] != (int32_t)r && comp1[r] == cy.kh[fin][r] &&
+                     comp2[r] == cy.kl[fin][r];
+    cy.cut[r] = cut ? 1 : 0;
+    if (cut) atomicAdd(&C->cycles_cut, 1u);
 }
 
 // flatten (EMSTOperators.h:412-433): chase label2 to the root, with path
@@ -1640,7 +1722,7 @@ __global__ __launch_bounds__(kBlock) void k_edges_out(const uint64_t* __restrict
 // findCell (ViewGrid.h:339-359): p1 = FEuclid(FDual(0)), size_slab = 2R;
 // li = (float)(p - p1) + R in double; cell = floor(li / 2R).
 __global__ __launch_bounds__(kBlock) void k_cell_grid(const float* __restrict__ resc, int64_t n, int dim,
-                                                     const int32_t* __restrict__ geo,
+                                                     int pstride, const int32_t* __restrict__ geo,
                                                      int32_t* __restrict__ cell_of,
                                                      uint64_t* __restrict__ keys,
                                                      uint32_t* __restrict__ cell_count)
@@ -1655,7 +1737,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_grid(const float* __restrict__ 
         const float p1 = (float)((PC % 2) * R - PC * R + pc);
         const float p2 = (float)((2 + PC % 2) * R - PC * R + pc);
         const double size_slab = (double)(p2 - p1);
-        const float dif = resc[i * dim + k] - p1;
+        const float dif = resc[i * pstride + k] - p1;
         const double li = (double)dif + size_slab / 2;
         c[k] = (int32_t)floor(li / size_slab);
         ok = ok && c[k] >= 0 && c[k] < geo[k];
@@ -1769,8 +1851,8 @@ void launch_build_tree(const float4* pts, const Cloud* cl, int ncl, int64_t nlea
         hipLaunchKernelGGL(k_level, dim3(nblocks((int64_t)ncl * lo)), dim3(kBlock), 0, s, nodes, cl, ncl, lo);
 }
 
-void launch_cone_search(int dim, const float4* pts, const Node* nodes, const Cloud* cl, int ncl, int bucket,
-                        int64_t n, const int32_t* qlist, int64_t nq, const int32_t* label,
+void launch_cone_search(int dim, int cdim, const float4* pts, const Node* nodes, const Cloud* cl, int ncl,
+                        int bucket, int64_t n, const int32_t* qlist, int64_t nq, const int32_t* label,
                         bool check_labels, const int32_t* inv, const float* seed_r, float seed_scale,
                         uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
                         uint64_t* first_key, Counters* C, bool collect_stats, DupState dup,
@@ -1779,14 +1861,10 @@ void launch_cone_search(int dim, const float4* pts, const Node* nodes, const Clo
     if (nq <= 0) return;
     const unsigned g = nblocks(nq, 64);
     const int cs = collect_stats ? 1 : 0;
-    if (dim == 3)
-        hipLaunchKernelGGL(k_cone_search<3>, dim3(g), dim3(64), 0, s, pts, nodes, cl, ncl, bucket, n, qlist, nq,
-                           label, check_labels ? 1 : 0, inv, seed_r, seed_scale, cand_key, cand_pos, revive,
-                           first_corr, first_key, C, cs, dup);
-    else
-        hipLaunchKernelGGL(k_cone_search<2>, dim3(g), dim3(64), 0, s, pts, nodes, cl, ncl, bucket, n, qlist, nq,
-                           label, check_labels ? 1 : 0, inv, seed_r, seed_scale, cand_key, cand_pos, revive,
-                           first_corr, first_key, C, cs, dup);
+    auto kern = dim == 3 ? (cdim == 2 ? k_cone_search<3, 2> : k_cone_search<3, 3>) : k_cone_search<2, 2>;
+    hipLaunchKernelGGL(kern, dim3(g), dim3(64), 0, s, pts, nodes, cl, ncl, bucket, n, qlist, nq, label,
+                       check_labels ? 1 : 0, inv, seed_r, seed_scale, cand_key, cand_pos, revive, first_corr,
+                       first_key, C, cs, dup);
 }
 
 void launch_dup_fill(int dim, int64_t n, const int32_t* label, const float4* pts, const int32_t* inv,
@@ -1848,7 +1926,7 @@ void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const ui
                         int32_t* label2, int32_t* corr,
                         uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, int32_t* fixed,
                         Counters* C, int32_t* eu, int32_t* ev, float* ed2, int32_t* eround,
-                        hipStream_t s)
+                        const CycleState& cy, hipStream_t s)
 {
     if (n <= 0) return;
     const unsigned g = nblocks(n);
@@ -1864,8 +1942,16 @@ void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const ui
                            comp1, comp2, fixed, C);
         break;
     case 2:
+        if (cy.j0) {
+            hipLaunchKernelGGL(k_cycle_init, dim3(g), dim3(kBlock), 0, s, n, label, comp1, comp2, cy, C);
+            for (int k = 0; k < cy.steps; ++k)
+                hipLaunchKernelGGL(k_cycle_step, dim3(g), dim3(kBlock), 0, s, n, label, cy, k & 1, C);
+            hipLaunchKernelGGL(k_cycle_mark, dim3(g), dim3(kBlock), 0, s, n, label, cy, cy.steps & 1, C);
+            hipLaunchKernelGGL(k_cycle_cut, dim3(g), dim3(kBlock), 0, s, n, label, comp1, comp2, cy,
+                               cy.steps & 1, C);
+        }
         hipLaunchKernelGGL(k_round_hook, dim3(nblocks(n, kHookBlock)), dim3(kHookBlock), 0, s, n, pts, label,
-                           label2, corr, corr_key, fixed, C, eu, ev, ed2, eround);
+                           label2, corr, corr_key, fixed, C, eu, ev, ed2, eround, cy.j0 ? cy.cut : nullptr);
         break;
     case 3:
         hipLaunchKernelGGL(k_round_flatten, dim3(g), dim3(kBlock), 0, s, n, label, label2, comp1, comp2, C);
@@ -1920,12 +2006,12 @@ void launch_edges_out(const uint64_t* keys, const uint32_t* idx, const float* ed
     hipLaunchKernelGGL(k_edges_out, dim3(nblocks(m)), dim3(kBlock), 0, s, keys, idx, ed2, m, u, v, d2);
 }
 
-void launch_cell_grid(const float* resc, int64_t n, int dim, const int32_t* geo, int32_t* cell_of,
+void launch_cell_grid(const float* resc, int64_t n, int dim, int pstride, const int32_t* geo, int32_t* cell_of,
                       uint64_t* keys, uint32_t* cell_count, hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_cell_grid, dim3(nblocks(n)), dim3(kBlock), 0, s, resc, n, dim, geo, cell_of, keys,
-                       cell_count);
+    hipLaunchKernelGGL(k_cell_grid, dim3(nblocks(n)), dim3(kBlock), 0, s, resc, n, dim, pstride, geo, cell_of,
+                       keys, cell_count);
 }
 
 }  // namespace emst

@@ -55,7 +55,16 @@ typedef struct {
     int32_t radius_mode;    /* 0 plain; 1 learned seed radius (needs emst_set_radius_model) */
     float seed_scale;       /* plain-form first search radius, in mean spacings of the point's
                                leaf cell (0 = default 3.0); changes work, never results */
-    int32_t reserved[4];
+    int32_t cone_dim;       /* dimension of the cones (DimCM): 0 = the point dimension; 2 with
+                               3D points is the 2.5D mode, SolutionEMST<3,2>
+                               (emst_coalition/src/CalculateurEMST.cu:19,154-160,249-258):
+                               wedges over the xy plane, 3D distances, the 3D rescale.  There a
+                               component's chosen edges can close a cycle of components, on
+                               which the reference's flatten does not end; here the component
+                               whose edge is the cycle's largest under (d2, min id, max id)
+                               skips that round's hook (emst_get_search_stats entry 7 counts
+                               these).  Not with the learned seed radius. */
+    int32_t reserved[3];
 } emst_options;
 
 /* One row of the per-round trace: the columns of the reference's stats file
@@ -191,7 +200,8 @@ int emst_get_times(emst_ctx* ctx, float* times, int32_t* n_times);
  * entered, [3] leaf points read (summed over packets), [4] point pairs
  * evaluated (leaf points x lanes that needed the leaf), [5] points sharing
  * their location with another point, [6] 1 when those were searched as
- * groups (one search per location), 0 when every point searched itself.
+ * groups (one search per location), 0 when every point searched itself,
+ * [7] hooks withheld by the 2.5D mode's cycle rule over the rounds run so far.
  * No counterpart in the reference (its per-phase timers are in
  * SolutionEMSTOperators.cu:166-373).  n_stats: capacity in, count out. */
 int emst_get_search_stats(emst_ctx* ctx, int64_t* stats, int32_t* n_stats);

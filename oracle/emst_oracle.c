@@ -106,6 +106,16 @@ void or_grid_init(or_grid* g, int dim, int R, const double* newmin, const double
     if (dd > dm) dm = dd;
     g->d_max = dm;
     g->ncell = (int64_t)g->ext_dual[0] * g->ext_dual[1] * g->ext_dual[2];
+    g->pdim = dim;
+}
+
+/* 2.5D (SolutionEMST<3,2>, emst_coalition/src/CalculateurEMST.cu:19,154-160):
+ * the grid covers the first dim axes of pdim-dimensional points
+ * (extents from x and y only, SolutionEMST.cu:422-430 with DimCM = 2). */
+void or_grid_init_pd(or_grid* g, int dim, int pdim, int R, const double* newmin, const double* newmax)
+{
+    or_grid_init(g, dim, R, newmin, newmax);
+    g->pdim = pdim;
 }
 
 /* findCell (ViewGrid.h:339-359): size_slab = p2[dir]-p1[dir] (= 2R);
@@ -189,7 +199,7 @@ int64_t or_cell_build(const or_grid* g, const float* pts, int64_t n,
     memset(cell_start, 0, sizeof(int64_t) * (nc + 1));
     for (int64_t i = 0; i < n; ++i) {
         int32_t* c = cell_of + 3 * i;
-        if (!or_find_cell(g, pts + i * g->dim, c)) { outside++; continue; }
+        if (!or_find_cell(g, pts + i * g->pdim, c)) { outside++; continue; }
         cell_start[or_cell_linear(g, c) + 1]++;
     }
     for (int64_t c = 0; c < nc; ++c) cell_start[c + 1] += cell_start[c];
@@ -273,7 +283,7 @@ int or_octants_ref_point(const or_grid* g, const float* pts, const int32_t* cell
                          const int32_t* label, int32_t p, int32_t* cid_out, float* cd2_out,
                          int64_t* examined)
 {
-    const int dim = g->dim;
+    const int dim = g->dim, pdim = g->pdim;
     const int nface = 2 * dim, nslab = 1 << (dim - 1), K = nface * nslab;
     int32_t cid[OR_MAXK];
     float cd2[OR_MAXK];
@@ -281,7 +291,7 @@ int or_octants_ref_point(const or_grid* g, const float* pts, const int32_t* cell
     memcpy(cd2, cd2_out, sizeof(float) * K);
     if (!octants_entry(K, cid, cd2)) return 0;
 
-    const float* pp = pts + (int64_t)p * dim;
+    const float* pp = pts + (int64_t)p * pdim;
     const int32_t* pc = cell_of + 3 * (int64_t)p;
     int faceV[6] = {0, 0, 0, 0, 0, 0};
     int slabC[6][4];
@@ -341,8 +351,8 @@ int or_octants_ref_point(const or_grid* g, const float* pts, const int32_t* cell
                             for (int64_t j = cell_start[lc]; j < cell_start[lc + 1]; ++j) {
                                 int32_t q = cell_pts[j];
                                 if (q == p) continue;
-                                const float* qq = pts + (int64_t)q * dim;
-                                double v = or_d2(pp, qq, dim);
+                                const float* qq = pts + (int64_t)q * pdim;
+                                double v = or_d2(pp, qq, pdim);
                                 ex++;
                                 if (v != 0 || label[q] != label[p])
                                     classify_update(dim, p, pp, q, qq, v, cid, cd2, &modified);
@@ -361,10 +371,10 @@ int or_octants_ref_point(const or_grid* g, const float* pts, const int32_t* cell
 
 /* Exact form of the same contract: every q != p (with the same duplicate
  * rule) is examined, so each cone holds its true minimum. */
-int or_octants_exact_point(const float* pts, int64_t n, int dim, const int32_t* label,
+int or_octants_exact_point(const float* pts, int64_t n, int dim, int cdim, const int32_t* label,
                            int32_t p, int32_t* cid_out, float* cd2_out, int64_t* examined)
 {
-    const int K = 2 * dim * (1 << (dim - 1));
+    const int K = 2 * cdim * (1 << (cdim - 1));
     int32_t cid[OR_MAXK];
     float cd2[OR_MAXK];
     memcpy(cid, cid_out, sizeof(int32_t) * K);
@@ -377,7 +387,7 @@ int or_octants_exact_point(const float* pts, int64_t n, int dim, const int32_t* 
         const float* qq = pts + q * dim;
         double v = or_d2(pp, qq, dim);
         if (v != 0 || label[q] != label[p])
-            classify_update(dim, p, pp, (int32_t)q, qq, v, cid, cd2, &modified);
+            classify_update(cdim, p, pp, (int32_t)q, qq, v, cid, cd2, &modified);
     }
     if (examined) *examined += n - 1;
     if (modified) {
@@ -394,17 +404,17 @@ typedef struct {
 } or_cells;
 
 static int64_t octants_round(int mode, const or_grid* g, const or_cells* cs, const float* pts,
-                             int64_t n, int dim, const int32_t* label, int32_t* cid, float* cd2,
-                             int64_t* examined)
+                             int64_t n, int dim, int cdim, const int32_t* label, int32_t* cid,
+                             float* cd2, int64_t* examined)
 {
-    const int K = 2 * dim * (1 << (dim - 1));
+    const int K = 2 * cdim * (1 << (cdim - 1));
     int64_t searched = 0;
     for (int64_t p = 0; p < n; ++p) {
         if (mode == 1)
             searched += or_octants_ref_point(g, pts, cs->cell_of, cs->cell_start, cs->cell_pts,
                                              label, (int32_t)p, cid + p * K, cd2 + p * K, examined);
         else
-            searched += or_octants_exact_point(pts, n, dim, label, (int32_t)p, cid + p * K,
+            searched += or_octants_exact_point(pts, n, dim, cdim, label, (int32_t)p, cid + p * K,
                                                cd2 + p * K, examined);
     }
     return searched;
@@ -431,13 +441,67 @@ static void cells_free(or_cells* cs)
     free(cs->cell_pts);
 }
 
-void or_octants_all(int mode, const or_grid* g, const float* pts, int64_t n, int dim,
+void or_octants_all(int mode, const or_grid* g, const float* pts, int64_t n, int dim, int cdim,
                     const int32_t* label, int32_t* cid, float* cd2, int64_t* examined)
 {
     or_cells cs = {0, 0, 0};
     if (mode == 1) cells_alloc(g, pts, n, &cs, 0);
-    octants_round(mode, g, &cs, pts, n, dim, label, cid, cd2, examined);
+    octants_round(mode, g, &cs, pts, n, dim, cdim, label, cid, cd2, examined);
     if (mode == 1) cells_free(&cs);
+}
+
+/* Cycle rule (an addition: the reference has no counterpart).  Each
+ * component root r with a winner w (best[r]) points at the component of w's
+ * target; a mutual pair (the two winners chose the same edge) is resolved by
+ * the hook itself.  With cones over every axis of the distance, a
+ * component's chosen edge is its shortest outgoing edge, and the choices can
+ * close no other cycle.  In the 2.5D mode (xy wedges, 3D distances) they can:
+ * a cycle of >= 3 components, or of 2 components with two different edges,
+ * on which the reference's hook (EMSTOperators.h:1267-1303) would build a
+ * cyclic label chain that its flatten never leaves.  On every such cycle the
+ * component whose chosen edge is the largest under (d2, min id, max id) does
+ * not hook this round: cut[r] = 1 for its root.  Every other hook is the
+ * reference's, so where no such cycle forms the result is unchanged. */
+static void cycle_rule(int64_t n, const int32_t* label, const int32_t* corr, const float* dist,
+                       const int32_t* best, const int32_t* fixed, int32_t* tgt, int32_t* state,
+                       int32_t* cut, int32_t* path)
+{
+    for (int64_t r = 0; r < n; ++r) {
+        cut[r] = 0;
+        state[r] = 0;
+        tgt[r] = -1;
+        int32_t w = best[r];
+        if (w == -1) continue;
+        int32_t q = corr[w];
+        if (fixed[q] && corr[q] == w) continue;     /* mutual pair */
+        tgt[r] = label[q];
+    }
+    for (int64_t r0 = 0; r0 < n; ++r0) {
+        if (tgt[r0] == -1 || state[r0] != 0) continue;
+        int64_t len = 0;
+        int32_t r = (int32_t)r0;
+        while (r != -1 && state[r] == 0) {
+            state[r] = 1;
+            path[len++] = r;
+            r = tgt[r];
+        }
+        if (r != -1 && state[r] == 1) {
+            /* the path closed on itself: the cycle runs from r to the path's end */
+            int64_t k = 0;
+            while (path[k] != r) ++k;
+            int32_t worst = -1;
+            for (int64_t i = k; i < len; ++i) {
+                int32_t c = path[i], w = best[c], q = corr[w];
+                if (worst == -1) { worst = c; continue; }
+                int32_t v = best[worst], t = corr[v];
+                if (or_edge_less(dist[v], v < t ? v : t, v < t ? t : v,
+                                 dist[w], w < q ? w : q, w < q ? q : w))
+                    worst = c;
+            }
+            cut[worst] = 1;
+        }
+        for (int64_t i = 0; i < len; ++i) state[path[i]] = 2;
+    }
 }
 
 /* ------------------------------------------------------------ A8-A14 --
@@ -451,11 +515,11 @@ void or_octants_all(int mode, const or_grid* g, const float* pts, int64_t n, int
  *   hook/union (K_EMST_connectComponentAndUnion_2, EMSTOperators.h:1267-1303,
  *     parent = label snapshot, launcher :1802-1814).
  * Labels are point indices (the reference stores 4*i; same order). */
-int64_t or_boruvka(int mode, const or_grid* g, const float* pts, int64_t n, int dim,
+int64_t or_boruvka(int mode, const or_grid* g, const float* pts, int64_t n, int dim, int cdim,
                    int max_rounds, int32_t* eu, int32_t* ev, int32_t* eround,
                    int32_t* label_out, int32_t* cand_out, or_trace* tr)
 {
-    const int K = 2 * dim * (1 << (dim - 1));
+    const int K = 2 * cdim * (1 << (cdim - 1));
     if (max_rounds > OR_MAX_ROUNDS) max_rounds = OR_MAX_ROUNDS;
     or_trace local;
     if (!tr) tr = &local;
@@ -469,6 +533,10 @@ int64_t or_boruvka(int mode, const or_grid* g, const float* pts, int64_t n, int 
     int32_t* best = (int32_t*)malloc(sizeof(int32_t) * n);
     int32_t* cid = (int32_t*)malloc(sizeof(int32_t) * n * K);
     float* cd2 = (float*)malloc(sizeof(float) * n * K);
+    int32_t* tgt = (int32_t*)malloc(sizeof(int32_t) * n);
+    int32_t* state = (int32_t*)malloc(sizeof(int32_t) * n);
+    int32_t* cut = (int32_t*)malloc(sizeof(int32_t) * n);
+    int32_t* path = (int32_t*)malloc(sizeof(int32_t) * n);
     or_cells cs = {0, 0, 0};
     if (mode == 1) cells_alloc(g, pts, n, &cs, &tr->overfull_cells);
     for (int64_t i = 0; i < n; ++i) {
@@ -496,7 +564,7 @@ int64_t or_boruvka(int mode, const or_grid* g, const float* pts, int64_t n, int 
         iteration++;
         /* find min 1: computeOctants then search */
         int64_t ex = 0;
-        tr->searched[iteration - 1] = octants_round(mode, g, &cs, pts, n, dim, label, cid, cd2, &ex);
+        tr->searched[iteration - 1] = octants_round(mode, g, &cs, pts, n, dim, cdim, label, cid, cd2, &ex);
         tr->examined[iteration - 1] = ex;
         for (int64_t p = 0; p < n; ++p) {
             int32_t pcorr = corr[p];
@@ -527,10 +595,12 @@ int64_t or_boruvka(int mode, const or_grid* g, const float* pts, int64_t n, int 
         }
         for (int64_t r = 0; r < n; ++r)
             if (best[r] != -1) fixed[best[r]] = 1;
+        cycle_rule(n, label, corr, dist, best, fixed, tgt, state, cut, path);
         /* hook / union */
         memcpy(parent, label, sizeof(int32_t) * n);
         for (int64_t p = 0; p < n; ++p) {
             if (!fixed[p]) continue;
+            if (cut[parent[p]]) { tr->cycles_cut++; continue; }
             int32_t q = corr[p];
             int32_t rp = parent[p], rq = parent[q];
             if (!fixed[q] || corr[q] != p) {
@@ -550,7 +620,7 @@ int64_t or_boruvka(int mode, const or_grid* g, const float* pts, int64_t n, int 
         if (eu[i] > ev[i]) { int32_t t = eu[i]; eu[i] = ev[i]; ev[i] = t; }
     if (mode == 1) cells_free(&cs);
     free(label); free(parent); free(corr); free(dist); free(fixed); free(best);
-    free(cid); free(cd2);
+    free(cid); free(cd2); free(tgt); free(state); free(cut); free(path);
     return m;
 }
 

@@ -23,18 +23,20 @@ extern "C" {
 #endif
 
 #define OR_MAXD 3
-#define OR_MAXK 24          /* cones per point: 2D faces x 2^(D-1) slabs */
+#define OR_MAXK 24          /* cones per point: 2D faces x 2^(D-1) slabs (D = cone dimension) */
 #define OR_MAX_ROUNDS 128
 
 /* Cellular-matrix geometry: ViewGridQuadMD (basic_components/include/ViewGrid.h:251-265)
  * built on extents ceil(max'-min')+1 (emst_coalition/src/SolutionEMST.cu:420-430). */
 typedef struct {
-    int dim;            /* DimCM */
+    int dim;            /* DimCM: grid and cone dimension */
     int R;              /* levelRadius */
     int32_t ext[3], pc[3], PC[3], ext_base[3], PCD[3], ext_dual[3];
     float p1[3];        /* FEuclid(FDual(0)) */
     int32_t d_max;      /* max dual extent: EMSTOperators.h:97 */
     int64_t ncell;
+    int pdim;           /* DimP: point dimension (stride of the point arrays; = dim
+                           except in the 2.5D mode SolutionEMST<3,2>) */
 } or_grid;
 
 typedef struct {
@@ -43,6 +45,7 @@ typedef struct {
     int64_t searched[OR_MAX_ROUNDS];        /* points whose cone search ran in round r */
     int64_t examined[OR_MAX_ROUNDS];        /* candidate points examined in round r */
     int64_t overfull_cells;                 /* cells above the reference cap of 32 */
+    int64_t cycles_cut;                     /* hooks withheld by the cycle rule */
 } or_trace;
 
 /* A1 */
@@ -50,8 +53,9 @@ void or_bbox(const float* pts, int64_t n, int dim, float* pmin, float* pmax);
 /* A2: returns unitX; out = rescaled coordinates; newmin/newmax = their extremes */
 double or_rescale(const float* pts, int64_t n, int dim, const float* pmin,
                   const float* pmax, float* out, double* newmin, double* newmax);
-/* A3 */
+/* A3: grid of dimension dim (DimCM) over points of dimension pdim (DimP) */
 void or_grid_init(or_grid* g, int dim, int R, const double* newmin, const double* newmax);
+void or_grid_init_pd(or_grid* g, int dim, int pdim, int R, const double* newmin, const double* newmax);
 int or_find_cell(const or_grid* g, const float* p, int32_t* cell);
 int64_t or_cell_linear(const or_grid* g, const int32_t* cell);
 /* A4 */
@@ -65,16 +69,20 @@ int or_octants_ref_point(const or_grid* g, const float* pts, const int32_t* cell
                          const int64_t* cell_start, const int32_t* cell_pts,
                          const int32_t* label, int32_t p, int32_t* cid, float* cd2,
                          int64_t* examined);
-/* Exact cone-nearest (Yao) restatement of the same per-point contract. */
-int or_octants_exact_point(const float* pts, int64_t n, int dim, const int32_t* label,
+/* Exact cone-nearest (Yao) restatement of the same per-point contract.
+ * dim = point dimension (distances), cdim = cone dimension (2 with dim 3 is
+ * the 2.5D mode: wedges over the xy plane, 3D distances). */
+int or_octants_exact_point(const float* pts, int64_t n, int dim, int cdim, const int32_t* label,
                            int32_t p, int32_t* cid, float* cd2, int64_t* examined);
 /* Stage A7 over all points (mode 0 = exact, 1 = reference ring search). */
-void or_octants_all(int mode, const or_grid* g, const float* pts, int64_t n, int dim,
+void or_octants_all(int mode, const or_grid* g, const float* pts, int64_t n, int dim, int cdim,
                     const int32_t* label, int32_t* cid, float* cd2, int64_t* examined);
 
 /* A8-A14: Boruvka driver.  mode 0 = exact cone search, 1 = reference ring
- * search on grid g.  eu/ev/eround need n-1 entries.  Returns edge count. */
-int64_t or_boruvka(int mode, const or_grid* g, const float* pts, int64_t n, int dim,
+ * search on grid g (g->dim = cdim, g->pdim = dim).  eu/ev/eround need n-1
+ * entries.  Returns edge count; tr->cycles_cut counts hooks withheld by the
+ * cycle rule (2.5D only; see or_boruvka). */
+int64_t or_boruvka(int mode, const or_grid* g, const float* pts, int64_t n, int dim, int cdim,
                    int max_rounds, int32_t* eu, int32_t* ev, int32_t* eround,
                    int32_t* label_out, int32_t* cand_out, or_trace* tr);
 

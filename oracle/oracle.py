@@ -23,7 +23,7 @@ class Grid(ctypes.Structure):
         ("ext", ctypes.c_int32 * 3), ("pc", ctypes.c_int32 * 3), ("PC", ctypes.c_int32 * 3),
         ("ext_base", ctypes.c_int32 * 3), ("PCD", ctypes.c_int32 * 3),
         ("ext_dual", ctypes.c_int32 * 3), ("p1", ctypes.c_float * 3),
-        ("d_max", ctypes.c_int32), ("ncell", ctypes.c_int64),
+        ("d_max", ctypes.c_int32), ("ncell", ctypes.c_int64), ("pdim", ctypes.c_int),
     ]
 
 
@@ -34,6 +34,7 @@ class Trace(ctypes.Structure):
         ("searched", ctypes.c_int64 * MAX_ROUNDS),
         ("examined", ctypes.c_int64 * MAX_ROUNDS),
         ("overfull_cells", ctypes.c_int64),
+        ("cycles_cut", ctypes.c_int64),
     ]
 
 
@@ -54,13 +55,14 @@ def lib():
         L.or_rescale.argtypes = [P, i64, i32, P, P, P, P, P]
         L.or_rescale.restype = dbl
         L.or_grid_init.argtypes = [P, i32, i32, P, P]
+        L.or_grid_init_pd.argtypes = [P, i32, i32, i32, P, P]
         L.or_find_cell.argtypes = [P, P, P]
         L.or_d2.argtypes = [P, P, i32]
         L.or_d2.restype = ctypes.c_float
         L.or_cell_build.argtypes = [P, P, i64, P, P, P]
         L.or_cell_build.restype = i64
-        L.or_octants_all.argtypes = [i32, P, P, i64, i32, P, P, P, P]
-        L.or_boruvka.argtypes = [i32, P, P, i64, i32, i32, P, P, P, P, P, P]
+        L.or_octants_all.argtypes = [i32, P, P, i64, i32, i32, P, P, P, P]
+        L.or_boruvka.argtypes = [i32, P, P, i64, i32, i32, i32, P, P, P, P, P, P]
         L.or_boruvka.restype = i64
         L.or_prim_float.argtypes = [P, i64, i32, P, P]
         L.or_prim_float.restype = i64
@@ -101,10 +103,13 @@ def rescale(pts):
     return out, u, nmin, nmax
 
 
-def grid(dim, R, nmin, nmax):
+def grid(dim, R, nmin, nmax, pdim=None):
+    """Reference cell grid of dimension dim over points of dimension pdim
+    (default dim; dim 2 over 3D points is the 2.5D mode's grid)."""
     g = Grid()
-    lib().or_grid_init(ctypes.byref(g), dim, R, _p(np.ascontiguousarray(nmin, np.float64)),
-                       _p(np.ascontiguousarray(nmax, np.float64)))
+    lib().or_grid_init_pd(ctypes.byref(g), dim, pdim or dim, R,
+                          _p(np.ascontiguousarray(nmin, np.float64)),
+                          _p(np.ascontiguousarray(nmax, np.float64)))
     return g
 
 
@@ -124,13 +129,15 @@ def d2(a, b):
     return float(lib().or_d2(_p(a), _p(b), a.shape[0]))
 
 
-def octants(rescaled, labels=None, mode=0, R=1):
+def octants(rescaled, labels=None, mode=0, R=1, cdim=None):
     """One cone-search stage over all points from the initial state.
-    mode 0 = exact cone minimum, 1 = the reference ring search on an R grid.
-    Returns (cand_id [n,K], cand_d2 [n,K], examined)."""
+    mode 0 = exact cone minimum, 1 = the reference ring search on an R grid;
+    cdim = cone dimension (default the point dimension; 2 over 3D points is
+    the 2.5D mode).  Returns (cand_id [n,K], cand_d2 [n,K], examined)."""
     rescaled = np.ascontiguousarray(rescaled, dtype=np.float32)
     n, dim = rescaled.shape
-    K = ncones(dim)
+    cdim = cdim or dim
+    K = ncones(cdim)
     if labels is None:
         labels = np.arange(n, dtype=np.int32)
     labels = np.ascontiguousarray(labels, dtype=np.int32)
@@ -140,8 +147,8 @@ def octants(rescaled, labels=None, mode=0, R=1):
     g = None
     if mode == 1:
         _, _, nmin, nmax = _extremes(rescaled)
-        g = grid(dim, R, nmin, nmax)
-    lib().or_octants_all(mode, ctypes.byref(g) if g is not None else None, _p(rescaled), n, dim,
+        g = grid(cdim, R, nmin, nmax, pdim=dim)
+    lib().or_octants_all(mode, ctypes.byref(g) if g is not None else None, _p(rescaled), n, dim, cdim,
                          _p(labels), _p(cid), _p(cd2), ctypes.byref(ex))
     return cid, cd2, ex.value
 
@@ -156,12 +163,14 @@ def _extremes(rescaled):
     return None, None, nmin, nmax
 
 
-def boruvka(rescaled, mode=0, R=1, max_rounds=100, want_cands=False):
+def boruvka(rescaled, mode=0, R=1, max_rounds=100, want_cands=False, cdim=None):
     """A8-A14 on already rescaled points.  Returns a dict with edges (u<v,
-    in insertion order), per-edge round, final labels, trace."""
+    in insertion order), per-edge round, final labels, trace.  cdim: cone
+    dimension (default the point dimension; 2 over 3D points = 2.5D)."""
     rescaled = np.ascontiguousarray(rescaled, dtype=np.float32)
     n, dim = rescaled.shape
-    K = ncones(dim)
+    cdim = cdim or dim
+    K = ncones(cdim)
     eu = np.zeros(max(n - 1, 1), np.int32)
     ev = np.zeros(max(n - 1, 1), np.int32)
     er = np.zeros(max(n - 1, 1), np.int32)
@@ -171,8 +180,8 @@ def boruvka(rescaled, mode=0, R=1, max_rounds=100, want_cands=False):
     g = None
     if mode == 1:
         _, _, nmin, nmax = _extremes(rescaled)
-        g = grid(dim, R, nmin, nmax)
-    m = lib().or_boruvka(mode, ctypes.byref(g) if g is not None else None, _p(rescaled), n, dim,
+        g = grid(cdim, R, nmin, nmax, pdim=dim)
+    m = lib().or_boruvka(mode, ctypes.byref(g) if g is not None else None, _p(rescaled), n, dim, cdim,
                          max_rounds, _p(eu), _p(ev), _p(er), _p(lab),
                          _p(cand) if cand is not None else None, ctypes.byref(tr))
     rounds = tr.rounds
@@ -183,6 +192,7 @@ def boruvka(rescaled, mode=0, R=1, max_rounds=100, want_cands=False):
         "searched": list(tr.searched[:rounds]),
         "examined": list(tr.examined[:rounds]),
         "overfull_cells": tr.overfull_cells,
+        "cycles_cut": tr.cycles_cut,
         "cands": cand[:n] if cand is not None else None,
     }
 
@@ -223,11 +233,12 @@ def canon(u, v):
     return np.stack([a[key], b[key]], axis=1)
 
 
-def emst(pts, mode=0, R=1, max_rounds=100):
+def emst(pts, mode=0, R=1, max_rounds=100, cdim=None):
     """Whole path on input coordinates: rescale, Boruvka, FP64 weight on the
-    original coordinates."""
+    original coordinates.  cdim=2 on 3D points is the 2.5D mode (the rescale
+    is the 3D one, SolutionEMST.cu:305-336 with DimP = 3)."""
     rs, unit, _, _ = rescale(pts)
-    res = boruvka(rs, mode=mode, R=R, max_rounds=max_rounds)
+    res = boruvka(rs, mode=mode, R=R, max_rounds=max_rounds, cdim=cdim)
     res["rescaled"] = rs
     res["unitX"] = unit
     res["weight"] = weight(pts, res["u"], res["v"])
