@@ -26,7 +26,7 @@ def profiled_traffic(args):
     None when no summary covers it."""
     tags = {(1, 10_000_000, 3): "gauss64_1e7_3d", (0, 1_000_000, 3): "uniform_1e6_3d"}
     tag = tags.get((args.kind, args.n, args.dim))
-    if tag is None:
+    if tag is None or args.cone_dim not in (0, args.dim):
         return None
     path = os.path.join(ROOT, "profiles", f"k_cone_search_{tag}.md")
     if not os.path.exists(path):
@@ -47,6 +47,8 @@ def main():
     ap.add_argument("--kind", type=int, default=1, help="0 uniform, 1 Gaussian mixture, 2 power-law")
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--bucket", type=int, default=0)
+    ap.add_argument("--cone-dim", type=int, default=0,
+                    help="2 with --dim 3: the 2.5D mode (wedges over the xy plane, 3D distances)")
     ap.add_argument("--plain", action="store_true", help="plain seed only (skip the learned form)")
     ap.add_argument("--cpu-sample", type=int, default=40_000,
                     help="points of the same distribution timed on the CPU oracle (0 = skip)")
@@ -87,9 +89,10 @@ def main():
     dev = torch.from_numpy(pts).to("cuda:0")
     torch.cuda.synchronize()
 
-    K = 2 * args.dim * (1 << (args.dim - 1))
+    cdim = args.cone_dim or args.dim
+    K = 2 * cdim * (1 << (cdim - 1))
     model_path = os.path.join(ge.PKG_DIR, "models", f"radius_{args.dim}d.npy")
-    model = np.load(model_path) if (os.path.exists(model_path) and not args.plain) else None
+    model = np.load(model_path) if (os.path.exists(model_path) and not args.plain and cdim == args.dim) else None
     model_clouds = None
     if model is not None:
         meta = os.path.join(ge.PKG_DIR, "models", "train_radius.json")
@@ -97,7 +100,7 @@ def main():
             model_clouds = json.load(open(meta)).get("clouds")
 
     def timed(form_model):
-        ctx = pkg.Calculateur(dim=args.dim, bucket_size=args.bucket)
+        ctx = pkg.Calculateur(dim=args.dim, bucket_size=args.bucket, cone_dim=args.cone_dim)
         if form_model is not None:
             ctx.set_radius_model(form_model)
         search_ms, total_ms = [], []
@@ -170,7 +173,7 @@ def main():
         sample = pkg.synth(args.kind, args.cpu_sample, args.dim, args.seed)
         oracle.lib()
         t1 = time.perf_counter()
-        r = oracle.emst(sample, mode=1, R=1)
+        r = oracle.emst(sample, mode=1, R=1, cdim=cdim)
         ct = time.perf_counter() - t1
         assert len(r["u"]) == args.cpu_sample - 1
         cpu = {"value": round(args.cpu_sample / ct / 1e6, 5), "unit": "Mpoints/s", "cores": 1,
@@ -184,7 +187,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (project generator, seeded)",
-        "config": {"workload": f"EMST {args.n} pts {args.dim}D {names.get(args.kind)} seed {args.seed}",
+        "config": {"workload": f"EMST {args.n} pts {args.dim}D{' (2.5D: xy wedges)' if cdim < args.dim else ''} "
+                               f"{names.get(args.kind)} seed {args.seed}",
                    "n": args.n, "dim": args.dim, "form": "learned" if learned is not None else "plain",
                    "edges": int(m), "weight": w,
                    "gpu_total_ms_median": round(head["total_ms"], 3),
@@ -195,7 +199,8 @@ def main():
                    "learned_vs_plain": round(plain["dt"] / learned["dt"], 3) if learned else None,
                    "radius_model_training_clouds": model_clouds,
                    "learned_seed": seed_report,
-                   "same_tree": learned is not None},
+                   "same_tree": learned is not None,
+                   "cone_dim": cdim, "cycles_cut": int(head["stats"]["cycles_cut"])},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
