@@ -78,7 +78,9 @@ struct emst_ctx {
     DevBuf<double> partial;
     DevBuf<float> model, feat, pred, seed, ls;     // learned seed radius
     DevBuf<Cloud> dcl;
-    DevBuf<uint8_t> perm, cursor;      // cones ranked by key, per-point scan position (k_round_min)
+    DevBuf<uint64_t> rkey;             // cone candidates ranked by key, K per point (k_round_min)
+    DevBuf<int32_t> rpos;
+    DevBuf<uint8_t> cursor;            // per-point scan position in its ranked candidates
     DevBuf<Params> params;
     DevBuf<double> cloud_w;
     DevBuf<int32_t> cloud_e, cloud_r;
@@ -196,7 +198,8 @@ int alloc_all(emst_ctx* c)
     ENS(ed2, n);
     ENS(partial, kWeightBlocks);
     ENS(geo, 16);
-    ENS(perm, n * K);
+    ENS(rkey, n * K);
+    ENS(rpos, n * K);
     ENS(cursor, n);
     ENS(dup_rep, n);
     ENS(dup_list, n);
@@ -362,8 +365,8 @@ int build(emst_ctx* c)
     }
     const DupState dup = dup_state(c);
     launch_build_tree(c->pts.p, c->dcl.p, c->ncl, c->nleaves, c->Lmax, c->bucket, c->nodes.p, dup.rep, s);
-    launch_round_init(n, c->label.p, c->eu.p, c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->C,
-                      (uint32_t)c->target_edges, (uint32_t)c->opt.max_rounds, s);
+    launch_round_init(n, c->label.p, c->label2.p, c->fixed.p, c->eu.p, c->corr.p, c->corr_key.p, c->comp1.p,
+                      c->comp2.p, c->C, (uint32_t)c->target_edges, (uint32_t)c->opt.max_rounds, s);
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ev[1], s));
     const float* seed_r = nullptr;
@@ -381,7 +384,8 @@ int build(emst_ctx* c)
                        c->corr_key.p, c->C, true, dup, s);
     launch_dup_fill(c->cdim, n, c->label.p, c->pts.p, c->inv.p, c->cand_key.p, c->cand_pos.p, nullptr, c->corr.p,
                     c->corr_key.p, c->C, dup, s);
-    launch_rank_cones(c->cdim, n, c->cand_key.p, nullptr, n, c->perm.p, c->cursor.p, nullptr, s);
+    launch_rank_cones(c->cdim, n, c->cand_key.p, c->cand_pos.p, nullptr, n, c->rkey.p, c->rpos.p, c->cursor.p,
+                      nullptr, s);
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ev[2], s));
     // while grouping is on only group points can hold a zero-distance or no
@@ -450,11 +454,12 @@ void issue_round(emst_ctx* c, bool tr)
         const DupState d = dup_state(c);
         launch_dup_fill(c->cdim, c->n, c->label.p, c->pts.p, c->inv.p, c->cand_key.p, c->cand_pos.p, c->corr.p,
                         nullptr, nullptr, c->C, d, s);
-        launch_rank_cones(c->cdim, c->n, c->cand_key.p, d.list, d.npoints, c->perm.p, c->cursor.p, c->C, s);
+        launch_rank_cones(c->cdim, c->n, c->cand_key.p, c->cand_pos.p, d.list, d.npoints, c->rkey.p, c->rpos.p,
+                          c->cursor.p, c->C, s);
     }
     const CycleState cy = cycle_state(c);
     for (int ph = 0; ph < 4; ++ph) {
-        launch_round_phase(ph, c->K, c->n, c->pts.p, c->cand_key.p, c->cand_pos.p, c->perm.p, c->cursor.p,
+        launch_round_phase(ph, c->K, c->n, c->pts.p, c->rkey.p, c->rpos.p, c->cursor.p,
                            c->label.p, c->label2.p,
                            c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->fixed.p, c->C, c->eu.p,
                            c->ev_.p, c->ed2.p, c->eround.p, cy, s);
@@ -472,7 +477,7 @@ int resume_flatten(emst_ctx* c, Counters* h)
     for (int again = 0; h->unflat; ++again) {
         if (again == 64) return fail(c, EMST_E_HIP, "flatten did not converge (cycle in the hook forest)");
         HIPCHK(c, hipMemsetAsync(&c->C->unflat, 0, sizeof(uint32_t), s));
-        launch_round_phase(3, c->K, c->n, c->pts.p, c->cand_key.p, c->cand_pos.p, c->perm.p, c->cursor.p,
+        launch_round_phase(3, c->K, c->n, c->pts.p, c->rkey.p, c->rpos.p, c->cursor.p,
                            c->label.p, c->label2.p,
                            c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->fixed.p, c->C, c->eu.p,
                            c->ev_.p, c->ed2.p, c->eround.p, CycleState{}, s);
@@ -521,7 +526,8 @@ int round_once(emst_ctx* c, int32_t* finished)
             launch_cone_search(c->dim, c->cdim, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, n, c->list.p,
                                h.research, c->label.p, true, c->inv.p, nullptr, c->seed_scale, c->cand_key.p,
                                c->cand_pos.p, c->corr.p, nullptr, nullptr, c->C, false, DupState{}, s);
-            launch_rank_cones(c->cdim, n, c->cand_key.p, c->list.p, h.research, c->perm.p, c->cursor.p, nullptr, s);
+            launch_rank_cones(c->cdim, n, c->cand_key.p, c->cand_pos.p, c->list.p, h.research, c->rkey.p, c->rpos.p,
+                              c->cursor.p, nullptr, s);
         }
         LAUNCHCHK(c);
         searched = h.zero_or_empty;
@@ -570,7 +576,7 @@ std::vector<uintptr_t> graph_key(emst_ctx* c)
             (uintptr_t)c->cand_pos.p, (uintptr_t)c->label.p, (uintptr_t)c->label2.p, (uintptr_t)c->corr.p,
             (uintptr_t)c->corr_key.p, (uintptr_t)c->comp1.p, (uintptr_t)c->comp2.p, (uintptr_t)c->fixed.p,
             (uintptr_t)c->eu.p, (uintptr_t)c->ev_.p, (uintptr_t)c->ed2.p, (uintptr_t)c->eround.p,
-            (uintptr_t)c->inv.p, (uintptr_t)c->perm.p, (uintptr_t)c->cursor.p, (uintptr_t)d.rep, (uintptr_t)d.list, (uintptr_t)d.gslot, (uintptr_t)d.m2,
+            (uintptr_t)c->inv.p, (uintptr_t)c->rkey.p, (uintptr_t)c->rpos.p, (uintptr_t)c->cursor.p, (uintptr_t)d.rep, (uintptr_t)d.list, (uintptr_t)d.gslot, (uintptr_t)d.m2,
             (uintptr_t)d.bkey, (uintptr_t)d.bpos};
 }
 
@@ -864,8 +870,9 @@ int emst_get_edges(emst_ctx* c, int32_t* u, int32_t* v, float* d2, int64_t* m)
     // slots are indexed by absorbed root; empty slots sort last
     launch_edge_keys(c->eu.p, c->ev_.p, c->n, c->keys0.p, s);
     radix_sort(c->keys0.p, c->vals0.p, c->keys1.p, c->vals1.p, c->n, 64, true, c->hist.p, c->scan_tmp.p, s);
-    // reuse label2/list as staging (free between rounds)
-    int32_t* du = c->label2.p;
+    // staging: the sort's second key buffer (free once the sort has landed in
+    // keys0) and the research list (rebuilt in every round)
+    int32_t* du = reinterpret_cast<int32_t*>(c->keys1.p);
     int32_t* dv = c->list.p;
     float* dd = reinterpret_cast<float*>(c->vals1.p);
     launch_edges_out(c->keys0.p, c->vals0.p, c->ed2.p, cnt, du, dv, dd, s);

@@ -47,7 +47,8 @@ struct Counters {
     uint32_t target;        // sum over clouds of (points - 1)
     uint32_t max_rounds;
     uint32_t dup_settled;   // no coincident group spans two components any more
-    uint32_t pad3[2];
+    uint32_t dup_big;       // runs longer than one wave handles, queued for a grid-wide pass
+    uint32_t pad3;
 };
 
 // Coincident-point groups.  Points with identical rescaled coordinates form a
@@ -156,18 +157,20 @@ void launch_cone_search(int dim, int cdim, const float4* pts, const Node* nodes,
 void launch_dup_fill(int dim, int64_t n, const int32_t* label, const float4* pts, const int32_t* inv,
                      uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
                      uint64_t* first_key, Counters* C, DupState dup, hipStream_t s);
-void launch_round_init(int64_t n, int32_t* label, int32_t* eu, int32_t* corr, uint64_t* corr_key,
-                       uint64_t* comp1, uint32_t* comp2, Counters* C, uint32_t target, uint32_t max_rounds,
-                       hipStream_t s);
+void launch_round_init(int64_t n, int32_t* label, int32_t* label2, int32_t* fixed, int32_t* eu, int32_t* corr,
+                       uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, Counters* C, uint32_t target,
+                       uint32_t max_rounds, hipStream_t s);
 // Phase of a round: 0 per-point target + component min (first key), 1 second
 // key + winners, 2 hook, 3 flatten.
 // Ranks each point's cones by key for k_round_min (all points, or the qlist
 // points); C != nullptr: skipped once the rounds are done or groups settled.
 // dim: the cone dimension.
-void launch_rank_cones(int dim, int64_t n, const uint64_t* cand_key, const int32_t* qlist, int64_t nq,
-                       uint8_t* perm, uint8_t* cursor, const Counters* C, hipStream_t s);
-void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const uint64_t* cand_key,
-                        const int32_t* cand_pos, const uint8_t* perm, uint8_t* cursor, int32_t* label,
+void launch_rank_cones(int dim, int64_t n, const uint64_t* cand_key, const int32_t* cand_pos,
+                       const int32_t* qlist, int64_t nq, uint64_t* rkey, int32_t* rpos, uint8_t* cursor,
+                       const Counters* C, hipStream_t s);
+// rkey / rpos: the ranked candidates of k_rank_cones (point-major, K per point)
+void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const uint64_t* rkey,
+                        const int32_t* rpos, uint8_t* cursor, int32_t* label,
                         int32_t* label2, int32_t* corr,
                         uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, int32_t* fixed,
                         Counters* C, int32_t* eu, int32_t* ev, float* ed2, int32_t* eround,

@@ -425,6 +425,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_sorted(const float* __restric
 // equal keys; the sort is stable over input order, so within a run positions
 // ascend with input id and a group's first position holds its smallest id.
 constexpr int kMixedRunMax = 1024;   // longest run holding >= 2 locations that is split
+constexpr int kBigRun = 4096;        // longer runs are filled by the whole grid (k_dup_big)
 
 __device__ __forceinline__ bool same_xyz(const float4& a, const float4& b)
 {
@@ -449,6 +450,7 @@ __global__ __launch_bounds__(kBlock) void k_dup_runs(const uint64_t* __restrict_
                                                     Counters* __restrict__ C)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) C->dup_big = 0;
     if (i >= n) return;
     rep[i] = -1;
     const uint64_t k = keys[i];
@@ -460,10 +462,15 @@ __global__ __launch_bounds__(kBlock) void k_dup_runs(const uint64_t* __restrict_
 // is one group; a run over several locations (distinct points closer than a
 // Morton cell) of at most kMixedRunMax points is split by pairwise
 // comparison; a longer one sets dup_overflow, which turns grouping off.
+// A run of more than kBigRun points (a heavily repeated location: one wave
+// would walk it alone while the grid idles) gets its group slot and list
+// range here and is checked and filled by k_dup_big; its (start, end, list
+// base) go to the back of `runs`, entry k at runs[n - 3k - 3 ..]: the front
+// holds at most n / 2 run heads and the back at most 3n / (kBigRun + 1).
 __global__ __launch_bounds__(kBlock) void k_dup_group(const uint64_t* __restrict__ keys,
                                                      const float4* __restrict__ pts, int64_t n,
                                                      int32_t* __restrict__ rep,
-                                                     const int32_t* __restrict__ runs,
+                                                     int32_t* __restrict__ runs,
                                                      int32_t* __restrict__ list, int32_t* __restrict__ gslot,
                                                      Counters* __restrict__ C)
 {
@@ -481,6 +488,17 @@ __global__ __launch_bounds__(kBlock) void k_dup_group(const uint64_t* __restrict
             else hi = mid;
         }
         const int64_t e = lo;
+        if (e - s > kBigRun) {
+            if (lane == 0) {
+                gslot[s] = (int32_t)atomicAdd(&C->dup_groups, 1u);
+                const uint32_t base = atomicAdd(&C->dup_points, (uint32_t)(e - s));
+                const int64_t k = (int64_t)atomicAdd(&C->dup_big, 1u);
+                runs[n - 3 * k - 3] = (int32_t)s;
+                runs[n - 3 * k - 2] = (int32_t)e;
+                runs[n - 3 * k - 1] = (int32_t)base;
+            }
+            continue;
+        }
         const float4 head = pts[s];
         bool mixed = false;
         // four independent loads per lane per trip
@@ -523,6 +541,29 @@ __global__ __launch_bounds__(kBlock) void k_dup_group(const uint64_t* __restrict
         } else if (lane == 0) {
             atomicOr(&C->dup_overflow, 1u);
         }
+    }
+}
+
+// The long runs queued by k_dup_group, each spread over the whole grid: one
+// group when every point sits at the run head's location, else a mixed run
+// too long to split (grouping off).
+__global__ __launch_bounds__(kBlock) void k_dup_big(const float4* __restrict__ pts, int64_t n,
+                                                   int32_t* __restrict__ rep, const int32_t* __restrict__ runs,
+                                                   int32_t* __restrict__ list, Counters* __restrict__ C)
+{
+    const uint32_t nb = C->dup_big;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (uint32_t k = 0; k < nb; ++k) {
+        const int64_t s = runs[n - 3 * (int64_t)k - 3], e = runs[n - 3 * (int64_t)k - 2];
+        const uint32_t base = (uint32_t)runs[n - 3 * (int64_t)k - 1];
+        const float4 head = pts[s];
+        bool mixed = false;
+        for (int64_t i = s + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride) {
+            mixed |= !same_xyz(pts[i], head);
+            rep[i] = (int32_t)s;
+            list[base + (uint32_t)(i - s)] = (int32_t)i;
+        }
+        if (__ballot(mixed) != 0ull && __lane_id() == 0) atomicOr(&C->dup_overflow, 1u);
     }
 }
 
@@ -1149,6 +1190,8 @@ __global__ __launch_bounds__(kBlock) void k_dup_fill(int64_t n, const int32_t* _
 
 // ------------------------------------------------------- Boruvka rounds
 __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __restrict__ label,
+                                                      int32_t* __restrict__ label2,
+                                                      int32_t* __restrict__ fixed,
                                                       int32_t* __restrict__ eu,
                                                       int32_t* __restrict__ corr,
                                                       uint64_t* __restrict__ corr_key,
@@ -1165,6 +1208,8 @@ __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __res
     }
     if (i >= n) return;
     label[i] = (int32_t)i;          // K_EMST_initDisjointSet (EMSTOperators.h:330-343)
+    label2[i] = (int32_t)i;
+    fixed[i] = 0;                   // round stamps start at 1
     eu[i] = -1;                     // no edge absorbed this root yet
     corr[i] = -1;                   // correspondenceMap reset (SolutionEMST.cu:464-466)
     corr_key[i] = kEmptyKey;
@@ -1255,31 +1300,37 @@ __device__ __forceinline__ void atomic_min_grouped(uint64_t* arr, int32_t lab, u
     }
 }
 
-// Rank of each point's cones by key (bitonic network over K padded to a power
-// of two, all indices static so the keys stay in registers): perm[r][p] =
-// cone of rank r; cursor[p] = 0.  The order is (key, cone): padding entries
-// (key ~0, index >= K) rank after every real cone, empty ones included, so
-// ranks 0..K-1 are exactly the K cones.  Points: [0, n) or qlist[0..nq); with C the
-// kernel does nothing once the rounds are done or the groups have settled
-// (it follows a group refresh in a replayed round).
+// Each point's cone candidates in key order, for the rounds (bitonic network
+// over K padded to a power of two, all indices static so the keys stay in
+// registers): rkey[p*K + r], rpos[p*K + r] = key and sorted position of the
+// candidate of rank r; cursor[p] = 0.  Equal keys carry equal positions (a
+// key holds the candidate's id; empty cones and the padding are (~0, -1)),
+// so the position rides along as payload and ranks 0..K-1 are exactly the K
+// cones.  Point-major rows keep a point's next ranks in one cache line for
+// k_round_min; each row is written with 16-byte stores.  Points: [0, n) or
+// qlist[0..nq); with C the kernel does nothing once the rounds are done or
+// the groups have settled (it follows a group refresh in a replayed round).
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_rank_cones(int64_t n, const uint64_t* __restrict__ cand_key,
+                                                      const int32_t* __restrict__ cand_pos,
                                                       const int32_t* __restrict__ qlist, int64_t nq,
-                                                      uint8_t* __restrict__ perm, uint8_t* __restrict__ cursor,
+                                                      uint64_t* __restrict__ rkey, int32_t* __restrict__ rpos,
+                                                      uint8_t* __restrict__ cursor,
                                                       const Counters* __restrict__ C)
 {
     using CC = ConeCfg<D>;
     constexpr int P = CC::K <= 8 ? 8 : 32;
+    static_assert(CC::K % 4 == 0, "rows are written in 16-byte pieces");
     if (C && (C->done || C->dup_settled)) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nq) return;
     const int64_t p = qlist ? (int64_t)qlist[t] : t;
     uint64_t key[P];
-    uint32_t idx[P];
+    int32_t pos[P];
 #pragma unroll
     for (int c = 0; c < P; ++c) {
         key[c] = c < CC::K ? cand_key[(int64_t)c * n + p] : kEmptyKey;
-        idx[c] = (uint32_t)c;
+        pos[c] = c < CC::K ? cand_pos[(int64_t)c * n + p] : -1;
     }
 #pragma unroll
     for (int k = 2; k <= P; k <<= 1)
@@ -1290,15 +1341,21 @@ __global__ __launch_bounds__(kBlock) void k_rank_cones(int64_t n, const uint64_t
                 const int l = i ^ j;
                 if (l > i) {
                     const bool up = (i & k) == 0;
-                    const bool gt = key[i] > key[l] || (key[i] == key[l] && idx[i] > idx[l]);
-                    if (gt == up) {
+                    if ((key[i] > key[l]) == up && key[i] != key[l]) {
                         const uint64_t tk = key[i]; key[i] = key[l]; key[l] = tk;
-                        const uint32_t ti = idx[i]; idx[i] = idx[l]; idx[l] = ti;
+                        const int32_t tp = pos[i]; pos[i] = pos[l]; pos[l] = tp;
                     }
                 }
             }
+    uint4* __restrict__ ko = reinterpret_cast<uint4*>(rkey + p * CC::K);
+    uint4* __restrict__ po = reinterpret_cast<uint4*>(rpos + p * CC::K);
 #pragma unroll
-    for (int r = 0; r < CC::K; ++r) perm[(int64_t)r * n + p] = (uint8_t)idx[r];
+    for (int r = 0; r < CC::K; r += 2)
+        ko[r / 2] = make_uint4((uint32_t)key[r], (uint32_t)(key[r] >> 32), (uint32_t)key[r + 1],
+                               (uint32_t)(key[r + 1] >> 32));
+#pragma unroll
+    for (int r = 0; r < CC::K; r += 4)
+        po[r / 4] = make_uint4((uint32_t)pos[r], (uint32_t)pos[r + 1], (uint32_t)pos[r + 2], (uint32_t)pos[r + 3]);
     cursor[p] = 0;
 }
 
@@ -1312,84 +1369,83 @@ __global__ __launch_bounds__(kBlock) void k_rank_cones(int64_t n, const uint64_t
 // ones already found inside the component, which never leave it (components
 // only merge), so each candidate is looked at about once over all rounds.
 __global__ __launch_bounds__(kBlock) void k_round_min(int K, int64_t n, const float4* __restrict__ pts,
-                                                     const uint64_t* __restrict__ cand_key,
-                                                     const int32_t* __restrict__ cand_pos,
-                                                     const uint8_t* __restrict__ perm,
+                                                     const uint64_t* __restrict__ rkey,
+                                                     const int32_t* __restrict__ rpos,
                                                      uint8_t* __restrict__ cursor,
                                                      const int32_t* __restrict__ label,
-                                                     int32_t* __restrict__ label2,
                                                      int32_t* __restrict__ corr,
                                                      uint64_t* __restrict__ corr_key,
-                                                     uint64_t* __restrict__ comp1,
-                                                     int32_t* __restrict__ fixed, const Counters* C)
+                                                     uint64_t* __restrict__ comp1, const Counters* C)
 {
     if (C->done) return;
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool inb = p < n;
-    int32_t lp = 0;
-    uint64_t key1 = kEmptyKey;
-    bool valid = false;
-    if (inb) {
-        lp = label[p];
-        label2[p] = lp;
-        fixed[p] = 0;
-        int32_t c = corr[p];
-        uint64_t ck = corr_key[p];
-        // corr == -2: every cone candidate already inside the component; since
-        // components only merge, the point never has an outgoing candidate again
-        // (until a repeated cone search gives it new candidates)
-        if (c != -2 && !(c >= 0 && label[c] != lp)) {
-            c = -1;
-            ck = kEmptyKey;
-            // kStep ranks per step, their loads independent of each other, so
-            // a long run of candidates inside the component costs few round
-            // trips to memory
-            constexpr int kStep = 4;
-            int r = cursor[p];
-            bool more = r < K;
-            while (more) {
-                uint64_t key[kStep];
-                int32_t q[kStep], lq[kStep];
+    {
+        const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const bool inb = p < n;
+        int32_t lp = 0;
+        uint64_t key1 = kEmptyKey;
+        bool valid = false;
+        if (inb) {
+            lp = label[p];
+            int32_t c = corr[p];
+            uint64_t ck = corr_key[p];
+            // corr == -2: every cone candidate already inside the component;
+            // since components only merge, the point never has an outgoing
+            // candidate again (until a repeated cone search or a group
+            // refresh gives it new candidates)
+            if (c != -2 && !(c >= 0 && label[c] != lp)) {
+                c = -1;
+                ck = kEmptyKey;
+                // kStep ranks per step, their loads independent of each other,
+                // so a long run of candidates inside the component costs few
+                // round trips to memory
+                constexpr int kStep = 4;
+                int r = cursor[p];
+                bool more = r < K;
+                const uint64_t* __restrict__ kr = rkey + p * (int64_t)K;
+                const int32_t* __restrict__ pr = rpos + p * (int64_t)K;
+                while (more) {
+                    uint64_t key[kStep];
+                    int32_t q[kStep], lq[kStep];
 #pragma unroll
-                for (int i = 0; i < kStep; ++i) {
-                    const int cone = r + i < K ? (int)perm[(int64_t)(r + i) * n + p] : K;
-                    key[i] = kEmptyKey;
-                    q[i] = -1;
-                    if (cone < K) {             // anything else is not a cone index: the end
-                        key[i] = cand_key[(int64_t)cone * n + p];
-                        q[i] = cand_pos[(int64_t)cone * n + p];
+                    for (int i = 0; i < kStep; ++i) {
+                        key[i] = kEmptyKey;
+                        q[i] = -1;
+                        if (r + i < K) {
+                            key[i] = kr[r + i];
+                            q[i] = pr[r + i];
+                        }
                     }
-                }
 #pragma unroll
-                for (int i = 0; i < kStep; ++i) lq[i] = key[i] != kEmptyKey ? label[q[i]] : lp;
+                    for (int i = 0; i < kStep; ++i) lq[i] = key[i] != kEmptyKey ? label[q[i]] : lp;
 #pragma unroll
-                for (int i = 0; i < kStep; ++i) {
-                    if (!more) continue;
-                    if (key[i] == kEmptyKey) {      // ranked last: no outgoing candidate left
-                        r = K;
-                        more = false;
-                    } else if (lq[i] != lp) {
-                        ck = key[i];
-                        c = q[i];
-                        more = false;
-                    } else {
-                        ++r;
+                    for (int i = 0; i < kStep; ++i) {
+                        if (!more) continue;
+                        if (key[i] == kEmptyKey) {      // ranked last: no outgoing candidate left
+                            r = K;
+                            more = false;
+                        } else if (lq[i] != lp) {
+                            ck = key[i];
+                            c = q[i];
+                            more = false;
+                        } else {
+                            ++r;
+                        }
                     }
+                    more = more && r < K;
                 }
-                more = more && r < K;
+                cursor[p] = (uint8_t)r;
+                corr[p] = c == -1 ? -2 : c;
+                corr_key[p] = ck;
             }
-            cursor[p] = (uint8_t)r;
-            corr[p] = c == -1 ? -2 : c;
-            corr_key[p] = ck;
+            if (c >= 0) {
+                const uint32_t pid = __float_as_uint(pts[p].w);
+                const uint32_t qid = (uint32_t)ck;
+                key1 = (ck & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
+                valid = true;
+            }
         }
-        if (c >= 0) {
-            const uint32_t pid = __float_as_uint(pts[p].w);
-            const uint32_t qid = (uint32_t)ck;
-            key1 = (ck & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
-            valid = true;
-        }
+        atomic_min_grouped(comp1, lp, key1, valid);
     }
-    atomic_min_grouped(comp1, lp, key1, valid);
 }
 
 // second phase: among members matching (d2, min id), the smallest max id.
@@ -1410,7 +1466,8 @@ __global__ __launch_bounds__(kBlock) void k_round_comp2(int64_t n, const float4*
     if (key1 == comp1[lp]) atomicMin(&comp2[lp], max(pid, qid));
 }
 
-// winner flag (fixedMap, NodeEMST.h:947-949)
+// winner flag (fixedMap, NodeEMST.h:947-949), stamped with the round number:
+// fixed[p] == round marks this round's winners, and older stamps need no reset
 __global__ __launch_bounds__(kBlock) void k_round_winner(int64_t n, const float4* __restrict__ pts,
                                                         const int32_t* __restrict__ label,
                                                         const int32_t* __restrict__ corr,
@@ -1426,11 +1483,14 @@ __global__ __launch_bounds__(kBlock) void k_round_winner(int64_t n, const float4
     const uint64_t ck = corr_key[p];
     const uint32_t pid = __float_as_uint(pts[p].w), qid = (uint32_t)ck;
     const uint64_t key1 = (ck & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
-    if (key1 == comp1[lp] && max(pid, qid) == comp2[lp]) fixed[p] = 1;
+    if (key1 == comp1[lp] && max(pid, qid) == comp2[lp]) fixed[p] = (int32_t)C->round;
 }
 
 // hook/union (EMSTOperators.h:1267-1303): roots read from the flattened
-// labels, written into label2.  Non-mutual: root(p) -> root(q).  Mutual pair:
+// labels, written into label2.  label2 is a parent forest whose chase from
+// any point ends at its component root (k_round_init starts it as the
+// identity; only roots are written here and the flatten only halves paths),
+// so it needs no copy of the labels between rounds.  Non-mutual: root(p) -> root(q).  Mutual pair:
 // the endpoint with the smaller id keeps its root.  Each hook absorbs exactly
 // one root and a root is absorbed at most once over the run, so the edge is
 // stored at the absorbed root's index (eu = -1 marks an empty slot); the edge
@@ -1454,15 +1514,15 @@ __global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const floa
     __shared__ uint32_t cnt;
     if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
+    uint32_t added = 0;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool add = false;
-    if (p < n && fixed[p] && !(cut && cut[label[p]])) {
+    if (p < n && fixed[p] == round && !(cut && cut[label[p]])) {
         const int32_t q = corr[p];
         const uint64_t ck = corr_key[p];
         const uint32_t pid = __float_as_uint(pts[p].w), qid = (uint32_t)ck;
         const int32_t rp = label[p], rq = label[q];
         int32_t absorbed = -1;
-        if (!fixed[q] || corr[q] != (int32_t)p) {
+        if (fixed[q] != round || corr[q] != (int32_t)p) {
             label2[rp] = rq;
             absorbed = rp;
         } else if (pid < qid) {
@@ -1470,15 +1530,15 @@ __global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const floa
             absorbed = rq;
         }
         if (absorbed >= 0) {
-            add = true;
+            ++added;
             eu[absorbed] = (int32_t)min(pid, qid);
             ev[absorbed] = (int32_t)max(pid, qid);
             ed2[absorbed] = __uint_as_float((uint32_t)(ck >> 32));
             eround[absorbed] = round;
         }
     }
-    const uint64_t am = __ballot(add);
-    if (__lane_id() == 0 && am) atomicAdd(&cnt, (uint32_t)__popcll(am));
+    for (int off = 32; off > 0; off >>= 1) added += (uint32_t)__shfl_xor((int)added, off);
+    if (__lane_id() == 0 && added) atomicAdd(&cnt, added);
     __syncthreads();
     if (threadIdx.x == 0 && cnt) atomicAdd(&C->edges, cnt);
 }
@@ -1558,7 +1618,8 @@ __global__ __launch_bounds__(kBlock) void k_cycle_cut(int64_t n, const int32_t* 
 
 // flatten (EMSTOperators.h:412-433): chase label2 to the root, with path
 // halving on label2 (every write stores an ancestor, so concurrent chases stay
-// valid); reset the component minima for the next round.
+// valid and label2 stays a parent forest); reset the component minima for the
+// next round.
 // Each launch takes at most kFlattenSteps steps per point; a point still off
 // its root raises C->unflat and the host launches the kernel again, so every
 // wave ends even if the hook forest were malformed.
@@ -1583,6 +1644,9 @@ __global__ __launch_bounds__(kBlock) void k_round_flatten(int64_t n, int32_t* __
         ++steps;
     }
     if (nx != r) atomicOr(&C->unflat, 1u);
+    // p itself now points at the root (or the farthest ancestor reached), so
+    // the next round's chase from p starts there
+    if (steps > 0) label2[p] = r;
     label[p] = r;
     comp1[p] = kEmptyKey;
     comp2[p] = 0xffffffffu;
@@ -1839,6 +1903,8 @@ void launch_dup_groups(const uint64_t* keys, const float4* pts, int64_t n, int32
     hipLaunchKernelGGL(k_dup_runs, dim3(nblocks(n)), dim3(kBlock), 0, s, keys, n, rep, runs, C);
     const unsigned g = (unsigned)std::min<int64_t>(nblocks(n), 1024);
     hipLaunchKernelGGL(k_dup_group, dim3(g), dim3(kBlock), 0, s, keys, pts, n, rep, runs, list, gslot, C);
+    hipLaunchKernelGGL(k_dup_big, dim3((unsigned)std::min<int64_t>(nblocks(n), 512)), dim3(kBlock), 0, s, pts, n,
+                       rep, runs, list, C);
     hipLaunchKernelGGL(k_dup_finish, dim3(1), dim3(1), 0, s, C);
 }
 
@@ -1884,12 +1950,12 @@ void launch_dup_fill(int dim, int64_t n, const int32_t* label, const float4* pts
                            revive, first_corr, first_key, C, dup);
 }
 
-void launch_round_init(int64_t n, int32_t* label, int32_t* eu, int32_t* corr, uint64_t* corr_key,
-                       uint64_t* comp1, uint32_t* comp2, Counters* C, uint32_t target, uint32_t max_rounds,
-                       hipStream_t s)
+void launch_round_init(int64_t n, int32_t* label, int32_t* label2, int32_t* fixed, int32_t* eu, int32_t* corr,
+                       uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, Counters* C, uint32_t target,
+                       uint32_t max_rounds, hipStream_t s)
 {
     hipLaunchKernelGGL(k_round_init, dim3(nblocks(std::max<int64_t>(n, 1))), dim3(kBlock), 0, s, n, label,
-                       eu, corr, corr_key, comp1, comp2, C, target, max_rounds);
+                       label2, fixed, eu, corr, corr_key, comp1, comp2, C, target, max_rounds);
 }
 
 void launch_round_begin(Counters* C, hipStream_t s)
@@ -1909,20 +1975,17 @@ void launch_flag_research(int K, int64_t n, const uint64_t* cand_key, int32_t* l
     hipLaunchKernelGGL(k_flag_research, dim3(nblocks(n)), dim3(kBlock), 0, s, K, n, cand_key, list, C);
 }
 
-void launch_rank_cones(int dim, int64_t n, const uint64_t* cand_key, const int32_t* qlist, int64_t nq,
-                       uint8_t* perm, uint8_t* cursor, const Counters* C, hipStream_t s)
+void launch_rank_cones(int dim, int64_t n, const uint64_t* cand_key, const int32_t* cand_pos,
+                       const int32_t* qlist, int64_t nq, uint64_t* rkey, int32_t* rpos, uint8_t* cursor,
+                       const Counters* C, hipStream_t s)
 {
     if (nq <= 0) return;
-    if (dim == 3)
-        hipLaunchKernelGGL(k_rank_cones<3>, dim3(nblocks(nq)), dim3(kBlock), 0, s, n, cand_key, qlist, nq, perm,
-                           cursor, C);
-    else
-        hipLaunchKernelGGL(k_rank_cones<2>, dim3(nblocks(nq)), dim3(kBlock), 0, s, n, cand_key, qlist, nq, perm,
-                           cursor, C);
+    hipLaunchKernelGGL(dim == 3 ? k_rank_cones<3> : k_rank_cones<2>, dim3(nblocks(nq)), dim3(kBlock), 0, s, n,
+                       cand_key, cand_pos, qlist, nq, rkey, rpos, cursor, C);
 }
 
-void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const uint64_t* cand_key,
-                        const int32_t* cand_pos, const uint8_t* perm, uint8_t* cursor, int32_t* label,
+void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const uint64_t* rkey,
+                        const int32_t* rpos, uint8_t* cursor, int32_t* label,
                         int32_t* label2, int32_t* corr,
                         uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, int32_t* fixed,
                         Counters* C, int32_t* eu, int32_t* ev, float* ed2, int32_t* eround,
@@ -1932,8 +1995,8 @@ void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const ui
     const unsigned g = nblocks(n);
     switch (phase) {
     case 0:
-        hipLaunchKernelGGL(k_round_min, dim3(g), dim3(kBlock), 0, s, K, n, pts, cand_key, cand_pos, perm, cursor,
-                           label, label2, corr, corr_key, comp1, fixed, C);
+        hipLaunchKernelGGL(k_round_min, dim3(g), dim3(kBlock), 0, s, K, n, pts, rkey, rpos, cursor,
+                           label, corr, corr_key, comp1, C);
         break;
     case 1:
         hipLaunchKernelGGL(k_round_comp2, dim3(g), dim3(kBlock), 0, s, n, pts, label, corr, corr_key, comp1,

@@ -197,20 +197,25 @@ def _coincident_case(case, oracle_mod):
         pts = np.repeat(base, rng.integers(1, 5, len(base)), axis=0)
     elif case == "big_group":
         pts = np.concatenate([rng.random((2000, 2), dtype=np.float32), np.full((3000, 2), 0.5, np.float32)])
+    elif case == "huge_groups":
+        # two locations repeated beyond what one wave walks (k_dup_big)
+        pts = np.concatenate([rng.random((1500, 2), dtype=np.float32), np.full((5000, 2), 0.25, np.float32),
+                              np.full((4500, 2), 0.75, np.float32)])
     elif case == "near_twins":
         base = rng.random((800, 3), dtype=np.float32)
         step = np.nextafter(base, np.float32(2))
         pts = np.concatenate([base, step, base[:300], step[:200]])
-    elif case == "wide_run":
-        # 1400 points on two distinct locations of one Morton cell: too long
-        # to split, so grouping turns off
+    elif case in ("wide_run", "huge_mixed"):
+        # points on two distinct locations of one Morton cell: a run too long
+        # to split (1400 points; 5000, one the grid fills), so grouping turns off
+        copies = 700 if case == "wide_run" else 2500
         other = rng.random((500, 3), dtype=np.float32)
         for a in np.linspace(0.3, 0.7, 41, dtype=np.float32):
             b = np.float32(a + np.spacing(a))
-            pts = np.concatenate([other, np.full((700, 3), a, np.float32), np.full((700, 3), b, np.float32)])
+            pts = np.concatenate([other, np.full((copies, 3), a, np.float32), np.full((copies, 3), b, np.float32)])
             rs = oracle_mod.rescale(pts)[0]
             keys = _morton_keys(rs)
-            if not np.array_equal(rs[500], rs[1200]) and keys[500] == keys[1200]:
+            if not np.array_equal(rs[500], rs[500 + copies]) and keys[500] == keys[500 + copies]:
                 break
         else:
             pytest.skip("no two locations found in one Morton cell")
@@ -220,8 +225,8 @@ def _coincident_case(case, oracle_mod):
     return rng.permutation(pts)
 
 
-@pytest.mark.parametrize("case,grouped", [("groups2", 1), ("groups3", 1), ("big_group", 1),
-                                          ("near_twins", 1), ("wide_run", 0), ("underflow", 0)])
+@pytest.mark.parametrize("case,grouped", [("groups2", 1), ("groups3", 1), ("big_group", 1), ("huge_groups", 1),
+                                          ("near_twins", 1), ("wide_run", 0), ("huge_mixed", 0), ("underflow", 0)])
 def test_coincident_groups(pkg, oracle_mod, case, grouped):
     """Points sharing a location search once per location; their cones,
     the tree, the per-round sizes and the labels equal the per-point oracle.
