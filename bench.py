@@ -233,11 +233,10 @@ def run_batch(args, pkg):
     torch.cuda.synchronize()
     total = int(sizes.sum())
 
-    tuned = [(0, 0.0)] * len(groups)
+    tuned = [{} for _ in groups]
 
     def make(trace):
-        return [pkg.Calculateur(dim=g["dim"], trace=trace, bucket_size=b, seed_scale=sc)
-                for g, (b, sc) in zip(groups, tuned)]
+        return [pkg.Calculateur(dim=g["dim"], **dict(t, trace=trace)) for g, t in zip(groups, tuned)]
 
     def step(ctxs):
         errs = []
@@ -262,8 +261,8 @@ def run_batch(args, pkg):
         # untimed: leaf size and seed scale per dimension from the autotuner
         for i, (c, g) in enumerate(zip(ctxs, groups)):
             c.initialize_clouds_device(g["dev"].data_ptr(), g["sizes"])
-            b, sc, _ = c.autotune()
-            tuned[i] = (b, sc)
+            c.autotune()
+            tuned[i] = {k: v for k, v in c.options().items() if k in ("bucket_size", "seed_scale", "search_split")}
     for _ in range(args.warmup):
         step(ctxs)
     torch.cuda.synchronize()
@@ -299,7 +298,7 @@ def run_batch(args, pkg):
         "config": {"workload": f"{B} clouds, sizes log-uniform 1e2-1e5, half 2D half 3D, uniform or 8 Gaussians",
                    "clouds": B, "points": total, "edges": edges,
                    "clouds_2d": int((dims == 2).sum()), "points_2d": int(sizes[dims == 2].sum()),
-                   "autotuned_bucket_scale": {f"{g['dim']}d": list(t) for g, t in zip(groups, tuned)},
+                   "autotuned": {f"{g['dim']}d": t for g, t in zip(groups, tuned)},
                    "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
                    "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
                    "latency_ms_max": round(float(lat.max()), 3)},

@@ -26,7 +26,8 @@ EXPORTS = [
     "emst_get_rescaled", "emst_get_trace", "emst_get_candidates", "emst_get_cell_grid",
     "emst_get_times", "emst_get_search_stats", "emst_synth", "emst_version",
     "emst_set_radius_model", "emst_get_radius_samples", "emst_mlp_train", "emst_mlp_eval", "emst_mlp_grad",
-    "emst_set_clouds", "emst_get_cloud_results", "emst_autotune", "emst_get_shape", "emst_read_points",
+    "emst_set_clouds", "emst_get_cloud_results", "emst_autotune", "emst_get_shape", "emst_get_options",
+    "emst_read_points",
     "emst_write_solution",
 ]
 CLI_PATH = os.path.join(_HERE, "emst_cli")
@@ -45,7 +46,7 @@ class Options(ctypes.Structure):
                 ("max_rounds", ctypes.c_int32), ("bucket_size", ctypes.c_int32),
                 ("trace", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("radius_mode", ctypes.c_int32), ("seed_scale", ctypes.c_float),
-                ("cone_dim", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("cone_dim", ctypes.c_int32), ("search_split", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
 
 
 class RoundTrace(ctypes.Structure):
@@ -99,6 +100,7 @@ def lib():
     L.emst_get_cloud_results.argtypes = [P, P, P, P]
     L.emst_autotune.argtypes = [P, P, P, P]
     L.emst_get_shape.argtypes = [P, P, P]
+    L.emst_get_options.argtypes = [P, P]
     L.emst_read_points.argtypes = [ctypes.c_char_p, i32, P, P]
     L.emst_write_solution.argtypes = [P, ctypes.c_char_p]
     L.emst_version.restype = ctypes.c_char_p
@@ -139,7 +141,7 @@ class Calculateur:
     """One EMST context on one device (CalculateurEMST / SolutionEMST)."""
 
     def __init__(self, dim=3, level_radius=1, max_rounds=100, bucket_size=0, trace=False, device=0,
-                 seed_scale=0.0, cone_dim=0):
+                 seed_scale=0.0, cone_dim=0, search_split=0):
         """cone_dim=2 with dim=3: the 2.5D mode (SolutionEMST<3,2>, 3D points,
         wedges over the xy plane)."""
         L = lib()
@@ -149,6 +151,7 @@ class Calculateur:
         o.bucket_size, o.trace, o.device = bucket_size, int(trace), device
         o.seed_scale = seed_scale
         o.cone_dim = cone_dim
+        o.search_split = search_split
         self._L = L
         self._c = L.emst_create(ctypes.byref(o))
         if not self._c:
@@ -222,6 +225,15 @@ class Calculateur:
         ms = ctypes.c_float(0)
         self._chk(self._L.emst_autotune(self._c, ctypes.byref(b), ctypes.byref(sc), ctypes.byref(ms)), "autotune")
         return b.value, sc.value, ms.value
+
+    def options(self):
+        """The context's current options (after autotune: the chosen leaf
+        size, seed scale and search form) as a dict of Calculateur arguments."""
+        o = Options()
+        self._chk(self._L.emst_get_options(self._c, ctypes.byref(o)), "options")
+        return {"level_radius": o.level_radius, "max_rounds": o.max_rounds, "bucket_size": o.bucket_size,
+                "trace": bool(o.trace), "device": o.device, "seed_scale": o.seed_scale, "cone_dim": o.cone_dim,
+                "search_split": o.search_split}
 
     def cloud_results(self):
         """Per cloud: (FP64 weight, edge count, completion round)."""
@@ -344,12 +356,12 @@ def mlp_eval(model, X, device=0):
 
 
 def _stats_names():
-    return ["packets", "nodes", "leaves", "leaf_points", "pairs", "group_points", "grouped", "cycles_cut"]
+    return ["packets", "nodes", "leaves", "leaf_points", "pairs", "group_points", "grouped", "cycles_cut", "resumed"]
 
 
 def search_stats(calc):
-    t = (ctypes.c_int64 * 8)()
-    k = ctypes.c_int32(8)
+    t = (ctypes.c_int64 * 9)()
+    k = ctypes.c_int32(9)
     calc._chk(calc._L.emst_get_search_stats(calc._c, t, ctypes.byref(k)), "stats")
     return dict(zip(_stats_names(), list(t)[: k.value]))
 

@@ -143,14 +143,26 @@ void launch_build_tree(const float4* pts, const Cloud* cl, int ncl, int64_t nlea
 // Cone search over queries [0,n) (qlist == nullptr) or qlist[0..nq).
 // check_labels: skip zero-distance points of the query's own component.
 // collect_stats: add the traversal counters to C.
+// Scratch of a split cone search (pass 0 over all queries, the rest over the
+// queries left open, compacted): P = packets of 64 queries.
+struct SplitScratch {
+    uint64_t* open_mask;    // P
+    uint32_t* cnt;          // P + 1
+    uint32_t* off;          // P + 1
+    uint32_t* tmp;          // scan_tmp_words(P + 1)
+    int32_t* list;          // n
+    int after;              // the first launch runs passes 0..after
+    double compact_frac;    // resume from a compacted list when at most this share of the queries is open
+};
 // dim: point dimension (distances); cdim: cone dimension (cdim = 2 with dim =
-// 3 is the 2.5D mode: wedges over the xy plane).
+// 3 is the 2.5D mode: wedges over the xy plane).  split (nullptr: one launch
+// over every pass) applies to full searches (qlist == nullptr).
 void launch_cone_search(int dim, int cdim, const float4* pts, const Node* nodes, const Cloud* cl, int ncl, int bucket,
                         int64_t n, const int32_t* qlist, int64_t nq, const int32_t* label,
                         bool check_labels, const int32_t* inv, const float* seed_r, float seed_scale,
                         uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
                         uint64_t* first_key, Counters* C, bool collect_stats, DupState dup,
-                        hipStream_t s);
+                        const SplitScratch* split, hipStream_t s);
 // Cones of every group point for the current labels (dim: the cone dimension) (after the grouped
 // cone search, and at the start of each round while a group may still span
 // several components).  Sets C->dup_split when one does.

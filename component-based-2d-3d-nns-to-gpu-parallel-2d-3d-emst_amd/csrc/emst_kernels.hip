@@ -773,6 +773,18 @@ constexpr int kMaxCappedPasses = 6;
 //
 // Seeds: seed_r (per sorted point) when given, else seed_scale times the mean
 // spacing of the point's own leaf cell ((box volume / points)^(1/D)).
+//
+// Passes [pass_lo, pass_hi] run here.  With open_out, a query still holding
+// an open cone after pass_hi stores its cone bests so far (cand_key) and its
+// lane bit in open_out[packet]; a later launch (pass_lo = pass_hi + 1, nq_dev
+// = the number of such queries, counted on the device) reloads them and goes
+// on, in one of two forms chosen on the device: when at most compact_max
+// queries are open, packets of the compacted list qlist (after pass 0 most
+// queries are done, so the later passes then run on full waves instead of
+// dragging a few live lanes through whole traversals); otherwise each packet
+// resumes its own open lanes (resume_mask).  Resuming after pass k rebuilds
+// the state exactly: R_k by the same float products, and a cone is open after
+// pass k iff its best bound exceeds R_k^2.
 template <int D, int DC>
 __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ pts,
                                                    const Node* __restrict__ nodes,
@@ -790,17 +802,35 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
                                                    int32_t* __restrict__ first_corr,
                                                    uint64_t* __restrict__ first_key,
                                                    Counters* __restrict__ ctr, int collect_stats,
-                                                   DupState dup)
+                                                   DupState dup, int pass_lo, int pass_hi,
+                                                   uint64_t* __restrict__ open_out,
+                                                   const uint32_t* __restrict__ nq_dev,
+                                                   const uint64_t* __restrict__ resume_mask,
+                                                   int64_t compact_max)
 {
     using C = ConeCfg<DC>;
+    bool compacted = true;
+    if (nq_dev) {
+        nq = *nq_dev;
+        compacted = nq <= compact_max;
+        if (compacted ? (int64_t)blockIdx.x * 64 >= nq : resume_mask[blockIdx.x] == 0ull)
+            return;     // nothing to resume in this packet
+    }
     __shared__ unsigned long long best[C::K][64];
     __shared__ int32_t stk_id[64];
     __shared__ int32_t stk_ver[64];
     __shared__ unsigned long long stk_mask[64];
     const int lane = __lane_id();
     const int64_t qi = (int64_t)blockIdx.x * 64 + lane;
-    bool active = qi < nq;
-    const int64_t p = active ? (qlist ? (int64_t)qlist[qi] : qi) : 0;
+    bool active;
+    int64_t p;
+    if (compacted) {
+        active = qi < nq;
+        p = active ? (qlist ? (int64_t)qlist[qi] : qi) : 0;
+    } else {
+        active = (resume_mask[blockIdx.x] >> lane) & 1ull;     // set bits are points of this packet
+        p = active ? qi : 0;
+    }
     // grouping on: group members do not search (k_dup_fill gives them their
     // representative's cones) and no point takes a candidate at its own
     // location; otherwise only the point itself is skipped
@@ -820,12 +850,13 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
     const float px = me.x, py = me.y, pz = me.z;
     const int32_t mylab = (check_labels && active) ? label[p] : 0;
 #pragma unroll
-    for (int c = 0; c < C::K; ++c) best[c][lane] = kEmptyKey;
+    for (int c = 0; c < C::K; ++c) best[c][lane] = (pass_lo > 0 && active) ? cand_key[(int64_t)c * n + p] : kEmptyKey;
 
     uint32_t st_nodes = 0, st_leaves = 0, st_pts = 0, st_pairs = 0;
     // one traversal per cloud among the wave's queries (a packet of a single
     // cloud, the usual case, makes one)
     const int my_cl = active ? find_cloud(clouds, ncl, p) : -1;
+    uint32_t open_final = 0;
     uint64_t pending = __ballot(active);
     while (pending) {
         const int cl = __builtin_amdgcn_readfirstlane(__shfl(my_cl, (int)__builtin_ctzll(pending)));
@@ -859,6 +890,16 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
         float r2 = r * r;
         uint32_t open = mine ? (uint32_t)((1ull << C::K) - 1) : 0u;
         int32_t inner = -1;
+        for (int k = 0; k < pass_lo; ++k) {
+            inner = (int32_t)__float_as_uint(r2);
+            r2 *= 4.f;
+        }
+        if (pass_lo > 0) {
+            open = 0u;
+#pragma unroll
+            for (int c = 0; c < C::K; ++c)
+                if (mine && key_bound(best[c][lane]) > inner) open |= 1u << c;
+        }
         int32_t bound[C::K];
         int32_t maxb = -1;
 
@@ -879,7 +920,7 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
             return dxx * dxx + dyy * dyy + dzz * dzz;
         };
 
-        for (int pass = 0; pass <= kMaxCappedPasses; ++pass) {
+        for (int pass = pass_lo; pass <= pass_hi; ++pass) {
             if (__ballot(open != 0u) == 0ull) break;
             const bool last = pass == kMaxCappedPasses;
             const int32_t cap = last ? 0x7fffffff : (int32_t)__float_as_uint(r2);
@@ -1032,6 +1073,7 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
             inner = cap;
             r2 *= 4.f;
         }
+        if (mine) open_final = open;
     }
     if (collect_stats && lane == 0) {
         atomicAdd(&ctr->packets, 1ull);
@@ -1041,6 +1083,16 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
         atomicAdd(&ctr->pairs, (unsigned long long)st_pairs);
     }
     if (__ballot(underflow) != 0ull && lane == 0) atomicOr(&ctr->underflow, 1u);
+    if (open_out) {
+        const bool unfinished = active && open_final != 0u;
+        const uint64_t om = __ballot(unfinished);
+        if (lane == 0) open_out[blockIdx.x] = om;
+        if (unfinished) {
+            // the bests so far, for the launch that resumes this query
+            for (int c = 0; c < C::K; ++c) cand_key[(int64_t)c * n + p] = best[c][lane];
+            return;
+        }
+    }
     if (!active) return;
     if (revive && revive[p] == -2) revive[p] = -1;
     const int64_t gs = is_rep ? (int64_t)dup.gslot[p] : 0;
@@ -1067,6 +1119,25 @@ __global__ __launch_bounds__(64) void k_cone_search(const float4* __restrict__ p
         first_corr[p] = pmin >= 0 ? pmin : -2;
         first_key[p] = kmin;
     }
+}
+
+// Queries left open by a split cone search, in position order: per-packet
+// counts (entry P = 0, so that the exclusive scan's entry P is the total),
+// then each packet's open lanes at its scanned offset.
+__global__ __launch_bounds__(kBlock) void k_open_count(const uint64_t* __restrict__ open_mask, int64_t P,
+                                                      uint32_t* __restrict__ cnt)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b <= P) cnt[b] = b < P ? (uint32_t)__popcll(open_mask[b]) : 0u;
+}
+
+__global__ __launch_bounds__(64) void k_open_fill(const uint64_t* __restrict__ open_mask,
+                                                 const uint32_t* __restrict__ off, int32_t* __restrict__ list)
+{
+    const int64_t b = blockIdx.x;
+    const uint64_t m = open_mask[b];
+    const int lane = __lane_id();
+    if ((m >> lane) & 1ull) list[off[b] + (uint32_t)__popcll(m & lanemask_lt())] = (int32_t)(b * 64 + lane);
 }
 
 // Start of a group refresh: m2 = ~0 for every group, dup_split = 0.
@@ -1922,15 +1993,33 @@ void launch_cone_search(int dim, int cdim, const float4* pts, const Node* nodes,
                         bool check_labels, const int32_t* inv, const float* seed_r, float seed_scale,
                         uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
                         uint64_t* first_key, Counters* C, bool collect_stats, DupState dup,
-                        hipStream_t s)
+                        const SplitScratch* split, hipStream_t s)
 {
     if (nq <= 0) return;
     const unsigned g = nblocks(nq, 64);
     const int cs = collect_stats ? 1 : 0;
     auto kern = dim == 3 ? (cdim == 2 ? k_cone_search<3, 2> : k_cone_search<3, 3>) : k_cone_search<2, 2>;
+    if (!split || qlist || split->after < 0 || split->after >= kMaxCappedPasses) {
+        hipLaunchKernelGGL(kern, dim3(g), dim3(64), 0, s, pts, nodes, cl, ncl, bucket, n, qlist, nq, label,
+                           check_labels ? 1 : 0, inv, seed_r, seed_scale, cand_key, cand_pos, revive, first_corr,
+                           first_key, C, cs, dup, 0, kMaxCappedPasses, (uint64_t*)nullptr,
+                           (const uint32_t*)nullptr, (const uint64_t*)nullptr, (int64_t)0);
+        return;
+    }
+    // passes 0..after over every query, then the open ones compacted into full waves
+    const int64_t P = (int64_t)g;
     hipLaunchKernelGGL(kern, dim3(g), dim3(64), 0, s, pts, nodes, cl, ncl, bucket, n, qlist, nq, label,
                        check_labels ? 1 : 0, inv, seed_r, seed_scale, cand_key, cand_pos, revive, first_corr,
-                       first_key, C, cs, dup);
+                       first_key, C, cs, dup, 0, split->after, split->open_mask, (const uint32_t*)nullptr,
+                       (const uint64_t*)nullptr, (int64_t)0);
+    hipLaunchKernelGGL(k_open_count, dim3(nblocks(P + 1)), dim3(kBlock), 0, s, split->open_mask, P, split->cnt);
+    exclusive_scan_u32(split->cnt, split->off, P + 1, split->tmp, s);
+    hipLaunchKernelGGL(k_open_fill, dim3((unsigned)P), dim3(64), 0, s, split->open_mask, split->off, split->list);
+    hipLaunchKernelGGL(kern, dim3(g), dim3(64), 0, s, pts, nodes, cl, ncl, bucket, n,
+                       (const int32_t*)split->list, nq, label, check_labels ? 1 : 0, inv, seed_r, seed_scale,
+                       cand_key, cand_pos, revive, first_corr, first_key, C, cs, dup, split->after + 1, kMaxCappedPasses,
+                       (uint64_t*)nullptr, (const uint32_t*)(split->off + P), (const uint64_t*)split->open_mask,
+                       (int64_t)(split->compact_frac * (double)nq));
 }
 
 void launch_dup_fill(int dim, int64_t n, const int32_t* label, const float4* pts, const int32_t* inv,

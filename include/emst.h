@@ -64,7 +64,13 @@ typedef struct {
                                whose edge is the cycle's largest under (d2, min id, max id)
                                skips that round's hook (emst_get_search_stats entry 7 counts
                                these).  Not with the learned seed radius. */
-    int32_t reserved[3];
+    int32_t search_split;   /* round-1 cone search launches (changes work, never results):
+                               0 auto (3D cones and >= 2^21 points: 3 when at most 10% of the
+                               queries are left with open cones after the first pass, else 2;
+                               otherwise 1); 1 one launch for
+                               every pass; 2 first pass, then the open queries resumed in place;
+                               3 first pass, then the open queries compacted into full waves */
+    int32_t reserved[2];
 } emst_options;
 
 /* One row of the per-round trace: the columns of the reference's stats file
@@ -131,10 +137,11 @@ int emst_get_cloud_results(emst_ctx* ctx, double* weight, int32_t* edges, int32_
 /* Autotuner (no counterpart in the reference, whose cell size is the fixed
  * level radius): times the build and cone search of the context's current
  * points on the device for every leaf size (points per leaf cell: 16, 32,
- * 64) and plain seed scale (2, 3, 4 leaf spacings), keeps the fastest pair in
- * the context (used by every later set_points / set_clouds / run) and returns
- * it; best_ms = its build + search time.  The result of a run never depends
- * on either; any output may be NULL. */
+ * 64) and plain seed scale (2, 3, 4 leaf spacings), then every search_split
+ * form (1, 2, 3) for the fastest pair; keeps the fastest choice in the
+ * context (used by every later set_points / set_clouds / run) and returns
+ * the pair; best_ms = its build + search time.  The result of a run never
+ * depends on any of them; any output may be NULL. */
 int emst_autotune(emst_ctx* ctx, int32_t* best_bucket, float* best_scale, float* best_ms);
 
 /* Replaces SolutionEMST::run() (emst_coalition/src/SolutionEMSTOperators.cu:146-153):
@@ -201,7 +208,9 @@ int emst_get_times(emst_ctx* ctx, float* times, int32_t* n_times);
  * evaluated (leaf points x lanes that needed the leaf), [5] points sharing
  * their location with another point, [6] 1 when those were searched as
  * groups (one search per location), 0 when every point searched itself,
- * [7] hooks withheld by the 2.5D mode's cycle rule over the rounds run so far.
+ * [7] hooks withheld by the 2.5D mode's cycle rule over the rounds run so far,
+ * [8] queries still open after the first pass of a split search (see
+ * emst_options.search_split; 0 when the search ran as one launch).
  * No counterpart in the reference (its per-phase timers are in
  * SolutionEMSTOperators.cu:166-373).  n_stats: capacity in, count out. */
 int emst_get_search_stats(emst_ctx* ctx, int64_t* stats, int32_t* n_stats);
@@ -253,6 +262,11 @@ int emst_mlp_grad(int32_t device, const float* model, const float* feat, const f
 
 /* Points and dimension of the context's current point set. */
 int emst_get_shape(emst_ctx* ctx, int64_t* n, int32_t* dim);
+
+/* The context's current options, including what emst_autotune chose
+ * (bucket_size, seed_scale, search_split), e.g. to create further contexts
+ * that run the same way.  No counterpart in the reference. */
+int emst_get_options(emst_ctx* ctx, emst_options* opt);
 
 /* ---- on-disk formats (host only; no GPU needed for the reader) ------------
  * Replaces InputRW::readFile (emst_coalition/include/InputRW.h:44-623): the

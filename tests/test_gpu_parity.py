@@ -83,6 +83,24 @@ def test_cone_candidates_bit_exact(pkg, oracle_mod, dim, kind, n, seed):
     assert np.array_equal(cd2.view(np.uint32), ocd2.view(np.uint32))
 
 
+@pytest.mark.parametrize("dim,split", [(3, 1), (3, 2), (3, 3), (3, 0), (2, 2), (2, 3)])
+def test_split_search_forms(pkg, oracle_mod, dim, split):
+    """The round-1 cone search in one launch, or a first pass followed by the
+    queries left with open cones, resumed in place or compacted into full
+    waves: every form gives the exact cone minima."""
+    pts = pkg.synth(1, 12000, dim, 81)
+    print("case", dim, split, flush=True)
+    with pkg.Calculateur(dim=dim, search_split=split) as c:
+        c.initialize(pts)
+        cand, cd2 = c.candidates()
+        st = pkg.search_stats(c)
+        resc, _ = c.getAdaptiveMap()
+    assert (st["resumed"] > 0) == (split in (2, 3))      # auto: one launch below 2^21 points
+    ocid, ocd2, _ = oracle_mod.octants(resc, mode=0)
+    assert np.array_equal(cand, ocid)
+    assert np.array_equal(cd2.view(np.uint32), ocd2.view(np.uint32))
+
+
 @pytest.mark.parametrize("dim,kind,n,seed", [(2, 0, 6000, 21), (3, 0, 6000, 22), (2, 1, 6000, 23),
                                              (3, 1, 6000, 24), (2, 2, 5000, 25), (3, 2, 5000, 26)])
 def test_tree_equals_complete_graph_prim(pkg, oracle_mod, dim, kind, n, seed):
