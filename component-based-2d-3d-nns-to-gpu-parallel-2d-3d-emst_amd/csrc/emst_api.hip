@@ -377,7 +377,6 @@ int build(emst_ctx* c)
     launch_round_init(n, c->label.p, c->label2.p, c->fixed.p, c->eu.p, c->corr.p, c->corr_key.p, c->comp1.p,
                       c->comp2.p, c->C, (uint32_t)c->target_edges, (uint32_t)c->opt.max_rounds, s);
     LAUNCHCHK(c);
-    HIPCHK(c, hipEventRecord(c->ev[1], s));
     const float* seed_r = nullptr;
     if (c->learned && c->ncl == 1 && c->cdim == c->dim) {
         int rc2;
@@ -388,6 +387,8 @@ int build(emst_ctx* c)
         launch_radius_seed(n, c->model.p, c->feat.p, c->ls.p, c->pred.p, c->seed.p, s);
         seed_r = c->seed.p;
     }
+    LAUNCHCHK(c);
+    HIPCHK(c, hipEventRecord(c->ev[1], s));
     // 3D cones over 2^21 points or more: pass 0 over all queries, the open
     // ones then resumed from a compacted list when at most 10% remain, else
     // in place.  Measured at 10^7 points: 8% open on the clustered cloud
@@ -408,6 +409,8 @@ int build(emst_ctx* c)
     launch_cone_search(c->dim, c->cdim, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, n, nullptr, n,
                        c->label.p, false, c->inv.p, seed_r, c->seed_scale, c->cand_key.p, c->cand_pos.p, nullptr, c->corr.p,
                        c->corr_key.p, c->C, true, dup, use_split ? &split : nullptr, s);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipEventRecord(c->ev[2], s));
     // read with the counters below (into the context, which outlives any
     // early return while the copy is queued)
     c->resumed_h = 0;
@@ -419,12 +422,12 @@ int build(emst_ctx* c)
     launch_rank_cones(c->cdim, n, c->cand_key.p, c->cand_pos.p, nullptr, n, c->rkey.p, c->rpos.p, c->cursor.p,
                       nullptr, s);
     LAUNCHCHK(c);
-    HIPCHK(c, hipEventRecord(c->ev[2], s));
     // while grouping is on only group points can hold a zero-distance or no
     // candidate, and k_dup_fill refreshes those; the queue serves the
     // per-point form
     if (!c->grouped) launch_flag_research(c->K, n, c->cand_key.p, c->list.p, c->C, s);
     LAUNCHCHK(c);
+    HIPCHK(c, hipEventRecord(c->ev[3], s));
     Counters h{};
     int rc = read_counters(c, &h);
     if (rc) return rc;
@@ -445,7 +448,7 @@ int build(emst_ctx* c)
     c->stats[7] = 0;
     c->stats[8] = c->resumed_h;
     if (c->grouped) c->groups_settled = h.dup_split == 0u;
-    c->times[0] = elapsed(c->ev[0], c->ev[1]);
+    c->times[0] = elapsed(c->ev[0], c->ev[1]) + elapsed(c->ev[2], c->ev[3]);
     c->times[1] = elapsed(c->ev[1], c->ev[2]);
     c->built = true;
     c->rounds = 0;

@@ -198,8 +198,10 @@ int emst_get_cell_grid(emst_ctx* ctx, int32_t level_radius, int32_t* ext_dual,
                        int32_t* cell_of, int64_t* cell_start, int32_t* cell_pts);
 
 /* Kernel times of the last emst_run (ms, device events on the context's
- * stream): [0] build (bbox, rescale, sort, hierarchy), [1] cone search,
- * [2] Boruvka rounds, [3] total.  n_times: capacity in, count out. */
+ * stream): [0] build (bbox, rescale, sort, hierarchy, coincident groups, the
+ * learned seed's features and inference when set, and the candidates ranked
+ * for the rounds), [1] round-1 cone search, [2] Boruvka rounds, [3] total.
+ * n_times: capacity in, count out. */
 int emst_get_times(emst_ctx* ctx, float* times, int32_t* n_times);
 
 /* Counters of the last full cone search (the work behind its bytes):
