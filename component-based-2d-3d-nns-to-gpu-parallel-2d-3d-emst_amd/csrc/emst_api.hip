@@ -93,7 +93,7 @@ struct emst_ctx {
     DevBuf<uint32_t> dup_m2;
     DevBuf<uint64_t> dup_bkey;
     // cycle rule of the 2.5D mode (CycleState)
-    DevBuf<int32_t> cy_j0, cy_j;
+    DevBuf<int32_t> cy_j0, cy_j, cy_roots;
     DevBuf<uint64_t> cy_kh;
     DevBuf<uint32_t> cy_kl;
     DevBuf<uint8_t> cy_onc, cy_cut;
@@ -216,6 +216,7 @@ int alloc_all(emst_ctx* c)
     ENS(dup_gslot, n);
     if (c->cdim < c->dim) {
         ENS(cy_j0, n);
+        ENS(cy_roots, n);
         ENS(cy_j, 2 * n);
         ENS(cy_kh, 2 * n);
         ENS(cy_kl, 2 * n);
@@ -291,6 +292,7 @@ CycleState cycle_state(emst_ctx* c)
     if (c->cdim >= c->dim) return y;
     const int64_t n = c->n;
     y.j0 = c->cy_j0.p;
+    y.roots = c->cy_roots.p;
     y.j[0] = c->cy_j.p;
     y.j[1] = c->cy_j.p + n;
     y.kh[0] = c->cy_kh.p;
@@ -605,7 +607,7 @@ std::vector<uintptr_t> graph_key(emst_ctx* c)
 {
     const DupState d = dup_state(c);
     const CycleState y = cycle_state(c);
-    return {(uintptr_t)c->n, (uintptr_t)c->K, (uintptr_t)c->dim, (uintptr_t)c->cdim, (uintptr_t)y.j0,
+    return {(uintptr_t)c->n, (uintptr_t)c->K, (uintptr_t)c->dim, (uintptr_t)c->cdim, (uintptr_t)y.j0, (uintptr_t)y.roots,
             (uintptr_t)y.j[0], (uintptr_t)y.kh[0], (uintptr_t)y.kl[0], (uintptr_t)y.onc, (uintptr_t)y.cut,
             (uintptr_t)c->grouped, (uintptr_t)d.npoints,
             (uintptr_t)d.ngroups, (uintptr_t)c->C, (uintptr_t)c->pts.p, (uintptr_t)c->cand_key.p,

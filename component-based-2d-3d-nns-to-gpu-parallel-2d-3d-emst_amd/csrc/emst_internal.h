@@ -48,7 +48,7 @@ struct Counters {
     uint32_t max_rounds;
     uint32_t dup_settled;   // no coincident group spans two components any more
     uint32_t dup_big;       // runs longer than one wave handles, queued for a grid-wide pass
-    uint32_t pad3;
+    uint32_t nroots;        // roots listed by k_cycle_init this round (2.5D cycle rule)
 };
 
 // Coincident-point groups.  Points with identical rescaled coordinates form a
@@ -77,13 +77,16 @@ struct DupState {
 // with two different edges; hooking along it would leave a cyclic label
 // chain.  Per round, over the component roots: j0 = the root that the chosen
 // edge leads to (the root itself when it has no outgoing candidate or chose
-// the same edge as that root: a mutual pair, which the hook resolves);
-// pointer doubling (j, kh/kl ping-pong, `steps` steps with 2^steps >= points)
-// gives the largest chosen edge (kh = d2 bits << 32 | min id, kl = max id)
-// on the path of 2^steps hops, which for a root on a cycle is the cycle's
-// largest; onc marks the roots on cycles, cut the root whose own edge is its
-// cycle's largest: that root does not hook this round.  j0 == nullptr: off.
+// the same edge as that root: a mutual pair, which the hook resolves); the
+// roots with another target are listed (roots, Counters::nroots); pointer
+// doubling over the list (j, kh/kl ping-pong, S steps with 2^S > listed
+// roots, at most `steps`) gives the largest chosen edge (kh = d2 bits << 32 |
+// min id, kl = max id) on the path of 2^S hops, which for a root on a cycle
+// is the cycle's largest; onc marks the roots on cycles, cut the root whose
+// own edge is its cycle's largest: that root does not hook this round.
+// j0 == nullptr: off.
 struct CycleState {
+    int32_t* roots;         // listed roots (n)
     int32_t* j0;
     int32_t* j[2];
     uint64_t* kh[2];

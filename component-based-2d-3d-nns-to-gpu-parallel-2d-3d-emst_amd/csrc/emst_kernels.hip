@@ -1301,6 +1301,7 @@ __global__ void k_round_begin(Counters* C)
     }
     C->round += 1;
     C->edges_prev = C->edges;
+    C->nroots = 0;
 }
 
 // A round that added no edge leaves a forest with no outgoing candidate: done.
@@ -1587,7 +1588,7 @@ __global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const floa
     __syncthreads();
     uint32_t added = 0;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n && fixed[p] == round && !(cut && cut[label[p]])) {
+    if (p < n && fixed[p] == round && !(cut && round > 1 && cut[label[p]])) {
         const int32_t q = corr[p];
         const uint64_t ck = corr_key[p];
         const uint32_t pid = __float_as_uint(pts[p].w), qid = (uint32_t)ck;
@@ -1615,76 +1616,139 @@ __global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const floa
 }
 
 // ---- cycle rule of the 2.5D mode (CycleState).  Roots are the points whose
-// flattened label is their own position; every kernel visits roots only.
+// flattened label is their own position.  In round 1 every component is one
+// point and its chosen edge is its nearest neighbour, whose choices close
+// only mutual pairs (the edges along a cycle of choices could not increase),
+// so the rule starts at round 2.
+//
+// k_cycle_init: per root, its target j0 (itself when it has no outgoing
+// candidate or is in a mutual pair), both doubling buffers, onc = cut = 0;
+// the roots with another target are listed (cy.roots, C->nroots), one
+// block-wide reservation per chunk of kCyChunk points.
+constexpr int kCyChunk = 8192;
+constexpr int kCyPer = kCyChunk / kBlock;      // points per thread, one bit each
+
 __global__ __launch_bounds__(kBlock) void k_cycle_init(int64_t n, const int32_t* __restrict__ label,
                                                       const uint64_t* __restrict__ comp1,
                                                       const uint32_t* __restrict__ comp2, CycleState cy,
-                                                      const Counters* C)
+                                                      Counters* C)
 {
-    if (C->done) return;
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n || label[r] != (int32_t)r) return;
-    int32_t t = (int32_t)r;
-    const uint64_t k1 = comp1[r];
-    const uint32_t k2 = comp2[r];
-    if (k1 != kEmptyKey) {
-        // the chosen edge (d2, min id, max id): its end outside the component
-        const int32_t la = label[cy.inv[(uint32_t)k1]], lb = label[cy.inv[k2]];
-        const int32_t o = la == (int32_t)r ? lb : la;
-        if (!(comp1[o] == k1 && comp2[o] == k2)) t = o;     // else a mutual pair
+    if (C->done || C->round <= 1u) return;
+    __shared__ uint32_t wsum[kBlock / 64];
+    __shared__ uint32_t base;
+    const int64_t c0 = (int64_t)blockIdx.x * kCyChunk;
+    uint32_t bits = 0;
+    for (int i = 0; i < kCyPer; ++i) {
+        const int64_t r = c0 + (int64_t)i * kBlock + threadIdx.x;
+        if (r >= n || label[r] != (int32_t)r) continue;
+        int32_t t = (int32_t)r;
+        const uint64_t k1 = comp1[r];
+        const uint32_t k2 = comp2[r];
+        if (k1 != kEmptyKey) {
+            // the chosen edge (d2, min id, max id): its end outside the component
+            const int32_t la = label[cy.inv[(uint32_t)k1]], lb = label[cy.inv[k2]];
+            const int32_t o = la == (int32_t)r ? lb : la;
+            if (!(comp1[o] == k1 && comp2[o] == k2)) t = o;     // else a mutual pair
+        }
+        const uint64_t h = t == (int32_t)r ? 0ull : k1;
+        const uint32_t l = t == (int32_t)r ? 0u : k2;
+        cy.j0[r] = t;
+        cy.j[0][r] = t;
+        cy.j[1][r] = t;
+        cy.kh[0][r] = h;
+        cy.kh[1][r] = h;
+        cy.kl[0][r] = l;
+        cy.kl[1][r] = l;
+        cy.onc[r] = 0;
+        cy.cut[r] = 0;
+        if (t != (int32_t)r) bits |= 1u << i;
     }
-    cy.j0[r] = t;
-    cy.j[0][r] = t;
-    cy.kh[0][r] = t == (int32_t)r ? 0ull : k1;
-    cy.kl[0][r] = t == (int32_t)r ? 0u : k2;
-    cy.onc[r] = 0;
-}
-
-// one doubling step: hop 2^k further, keep the largest edge key on the way
-__global__ __launch_bounds__(kBlock) void k_cycle_step(int64_t n, const int32_t* __restrict__ label,
-                                                      CycleState cy, int src, const Counters* C)
-{
-    if (C->done) return;
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n || label[r] != (int32_t)r) return;
-    const int dst = src ^ 1;
-    const int32_t j = cy.j[src][r];
-    uint64_t h = cy.kh[src][r];
-    uint32_t l = cy.kl[src][r];
-    const uint64_t h2 = cy.kh[src][j];
-    const uint32_t l2 = cy.kl[src][j];
-    if (h2 > h || (h2 == h && l2 > l)) {
-        h = h2;
-        l = l2;
+    // block-exclusive offsets of the listed roots, in thread order
+    const uint32_t cnt = (uint32_t)__popc(bits);
+    uint32_t incl = cnt;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = (uint32_t)__shfl_up((int)incl, off);
+        if (__lane_id() >= off) incl += o;
     }
-    cy.j[dst][r] = cy.j[src][j];
-    cy.kh[dst][r] = h;
-    cy.kl[dst][r] = l;
+    const int w = threadIdx.x / 64;
+    if (__lane_id() == 63) wsum[w] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int k = 0; k < kBlock / 64; ++k) {
+            const uint32_t x = wsum[k];
+            wsum[k] = tot;
+            tot += x;
+        }
+        base = tot ? atomicAdd(&C->nroots, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t at = base + wsum[w] + incl - cnt;
+    for (int i = 0; i < kCyPer; ++i)
+        if ((bits >> i) & 1u) cy.roots[at++] = (int32_t)(c0 + (int64_t)i * kBlock + threadIdx.x);
 }
 
-// 2^steps hops from any root end on a cycle (or at a root that leads nowhere),
-// and every root of a cycle is such an end
-__global__ __launch_bounds__(kBlock) void k_cycle_mark(int64_t n, const int32_t* __restrict__ label,
-                                                      CycleState cy, int fin, const Counters* C)
+// number of doubling steps for R listed roots: 2^S > R covers any path of
+// listed roots, which ends on a cycle or at a root with no target
+__device__ __forceinline__ int cycle_steps(uint32_t R, int max_steps)
 {
-    if (C->done) return;
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n || label[r] != (int32_t)r) return;
-    cy.onc[cy.j[fin][r]] = 1;
+    int S = 0;
+    while (S < max_steps && (1ull << S) <= (uint64_t)R) ++S;
+    return S;
 }
 
-__global__ __launch_bounds__(kBlock) void k_cycle_cut(int64_t n, const int32_t* __restrict__ label,
-                                                     const uint64_t* __restrict__ comp1,
+// doubling step k (reads buffer k & 1): hop 2^k further, keep the largest
+// edge key on the way; steps past cycle_steps do nothing
+__global__ __launch_bounds__(kBlock) void k_cycle_step(CycleState cy, int k, const Counters* C)
+{
+    if (C->done || C->round <= 1u) return;
+    const uint32_t R = C->nroots;
+    if (k >= cycle_steps(R, cy.steps)) return;
+    const int src = k & 1, dst = src ^ 1;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (int64_t)R;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t r = cy.roots[t];
+        const int32_t j = cy.j[src][r];
+        uint64_t h = cy.kh[src][r];
+        uint32_t l = cy.kl[src][r];
+        const uint64_t h2 = cy.kh[src][j];
+        const uint32_t l2 = cy.kl[src][j];
+        if (h2 > h || (h2 == h && l2 > l)) {
+            h = h2;
+            l = l2;
+        }
+        cy.j[dst][r] = cy.j[src][j];
+        cy.kh[dst][r] = h;
+        cy.kl[dst][r] = l;
+    }
+}
+
+// after the steps, every listed root's hop target lies on a cycle (or is a
+// root with no target), and every root of a cycle is such a target
+__global__ __launch_bounds__(kBlock) void k_cycle_mark(CycleState cy, const Counters* C)
+{
+    if (C->done || C->round <= 1u) return;
+    const uint32_t R = C->nroots;
+    const int fin = cycle_steps(R, cy.steps) & 1;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (int64_t)R;
+         t += (int64_t)gridDim.x * blockDim.x)
+        cy.onc[cy.j[fin][cy.roots[t]]] = 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cycle_cut(const uint64_t* __restrict__ comp1,
                                                      const uint32_t* __restrict__ comp2, CycleState cy,
-                                                     int fin, Counters* C)
+                                                     Counters* C)
 {
-    if (C->done) return;
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n || label[r] != (int32_t)r) return;
-    const bool cut = cy.onc[r] && cy.j0[r] != (int32_t)r && comp1[r] == cy.kh[fin][r] &&
-                     comp2[r] == cy.kl[fin][r];
-    cy.cut[r] = cut ? 1 : 0;
-    if (cut) atomicAdd(&C->cycles_cut, 1u);
+    if (C->done || C->round <= 1u) return;
+    const uint32_t R = C->nroots;
+    const int fin = cycle_steps(R, cy.steps) & 1;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (int64_t)R;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t r = cy.roots[t];
+        const bool cut = cy.onc[r] && comp1[r] == cy.kh[fin][r] && comp2[r] == cy.kl[fin][r];
+        cy.cut[r] = cut ? 1 : 0;
+        if (cut) atomicAdd(&C->cycles_cut, 1u);
+    }
 }
 
 // flatten (EMSTOperators.h:412-433): chase label2 to the root, with path
@@ -2095,12 +2159,13 @@ void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const ui
         break;
     case 2:
         if (cy.j0) {
-            hipLaunchKernelGGL(k_cycle_init, dim3(g), dim3(kBlock), 0, s, n, label, comp1, comp2, cy, C);
+            const unsigned gc = (unsigned)std::min<int64_t>(g, 1024);
+            hipLaunchKernelGGL(k_cycle_init, dim3((unsigned)((n + kCyChunk - 1) / kCyChunk)), dim3(kBlock), 0, s, n,
+                               label, comp1, comp2, cy, C);
             for (int k = 0; k < cy.steps; ++k)
-                hipLaunchKernelGGL(k_cycle_step, dim3(g), dim3(kBlock), 0, s, n, label, cy, k & 1, C);
-            hipLaunchKernelGGL(k_cycle_mark, dim3(g), dim3(kBlock), 0, s, n, label, cy, cy.steps & 1, C);
-            hipLaunchKernelGGL(k_cycle_cut, dim3(g), dim3(kBlock), 0, s, n, label, comp1, comp2, cy,
-                               cy.steps & 1, C);
+                hipLaunchKernelGGL(k_cycle_step, dim3(gc), dim3(kBlock), 0, s, cy, k, C);
+            hipLaunchKernelGGL(k_cycle_mark, dim3(gc), dim3(kBlock), 0, s, cy, C);
+            hipLaunchKernelGGL(k_cycle_cut, dim3(gc), dim3(kBlock), 0, s, comp1, comp2, cy, C);
         }
         hipLaunchKernelGGL(k_round_hook, dim3(nblocks(n, kHookBlock)), dim3(kHookBlock), 0, s, n, pts, label,
                            label2, corr, corr_key, fixed, C, eu, ev, ed2, eround, cy.j0 ? cy.cut : nullptr);
