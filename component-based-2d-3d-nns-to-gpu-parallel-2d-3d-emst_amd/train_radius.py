@@ -45,7 +45,7 @@ VALID_CLOUDS = {
     "uniform": {2: [(0, 2_000_000, 101)], 3: [(0, 2_000_000, 102)]},
     "mixed": {2: [(0, 2_000_000, 101), (2, 2_000_000, 15)], 3: [(0, 2_000_000, 102), (1, 2_000_000, 16)]},
 }
-OFFSETS = (-0.25, 0.0, 0.25, 0.5, 0.75, 1.0)
+OFFSETS = (-0.25, 0.0, 0.25, 0.5, 0.75, 1.0, 1.25, 1.5)
 
 
 def samples(pkg, dim, clouds):
