@@ -50,7 +50,7 @@ def main():
     ap.add_argument("--cone-dim", type=int, default=0,
                     help="2 with --dim 3: the 2.5D mode (wedges over the xy plane, 3D distances)")
     ap.add_argument("--plain", action="store_true", help="plain seed only (skip the learned form)")
-    ap.add_argument("--cpu-sample", type=int, default=40_000,
+    ap.add_argument("--cpu-sample", type=int, default=200_000,
                     help="points of the same distribution timed on the CPU oracle (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (scales rocprof byte counters)")
@@ -59,6 +59,8 @@ def main():
                          "3D, each uniform or 8 Gaussians (generator seed --batch-seed)")
     ap.add_argument("--batch-seed", type=int, default=5)
     ap.add_argument("--no-autotune", action="store_true", help="batch: keep the default leaf size and seed")
+    ap.add_argument("--tsplib", action="store_true",
+                    help="time every TSPLIB instance of the reference's 2D tests (tests/golden/tsplib)")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -83,6 +85,9 @@ def main():
     pkg.lib()
     if args.batch:
         run_batch(args, pkg)
+        return
+    if args.tsplib:
+        run_tsplib(args, pkg)
         return
     names = {0: "uniform", 1: "gauss64+10%bg", 2: "powerlaw16"}
     pts = pkg.synth(args.kind, args.n, args.dim, args.seed)
@@ -172,14 +177,26 @@ def main():
         import oracle
         sample = pkg.synth(args.kind, args.cpu_sample, args.dim, args.seed)
         oracle.lib()
+        # all host threads (the per-point searches in parallel, the rounds
+        # sequential), then one thread on a fifth of the sample
+        cores = oracle.threads(0)
         t1 = time.perf_counter()
         r = oracle.emst(sample, mode=1, R=1, cdim=cdim)
         ct = time.perf_counter() - t1
         assert len(r["u"]) == args.cpu_sample - 1
-        cpu = {"value": round(args.cpu_sample / ct / 1e6, 5), "unit": "Mpoints/s", "cores": 1,
+        n1 = max(args.cpu_sample // 5, 2)
+        oracle.threads(1)
+        t1 = time.perf_counter()
+        r1 = oracle.emst(sample[:n1], mode=1, R=1, cdim=cdim)
+        c1 = time.perf_counter() - t1
+        oracle.threads(cores)
+        assert len(r1["u"]) == n1 - 1
+        cpu = {"value": round(args.cpu_sample / ct / 1e6, 5), "unit": "Mpoints/s", "cores": cores,
                "kind": "port",
                "sample": f"{args.cpu_sample} points of the same distribution and seed, oracle restatement "
-                         f"of the reference ring search (R=1), single thread, {ct:.1f} s"}
+                         f"of the reference ring search (R=1), {cores} threads (per-point searches in "
+                         f"parallel, rounds sequential), {ct:.1f} s; single thread on the first {n1} points: "
+                         f"{n1 / c1 / 1e6:.5f} Mpoints/s ({c1:.1f} s)"}
 
     out = {
         "metric": "EMST Mpoints/s at 10^7 points, learned-radius form (plain form in config)",
@@ -203,6 +220,52 @@ def main():
                    "cone_dim": cdim, "cycles_cut": int(head["stats"]["cycles_cut"])},
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def run_tsplib(args, pkg):
+    """The TSPLIB sweep: each instance read with the library's TSPLIB reader,
+    its points resident on the device, then W untimed and K timed EMSTs;
+    per instance the median device time (emst_get_times total).  value = the
+    sum of the medians over all instances."""
+    import gzip
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    tdir = os.path.join(ROOT, "tests", "golden", "tsplib")
+    rows = []
+    with tempfile.TemporaryDirectory() as tmp:
+        for name in sorted(os.listdir(tdir)):
+            path = os.path.join(tmp, name[:-3])
+            with open(path, "wb") as f:
+                f.write(gzip.open(os.path.join(tdir, name)).read())
+            pts = pkg.read_points(path, 2)
+            n = len(pts)
+            dev = torch.from_numpy(np.ascontiguousarray(pts)).to("cuda:0")
+            torch.cuda.synchronize()
+            with pkg.Calculateur(dim=2) as c:
+                ms = []
+                for k in range(args.warmup + args.steps):
+                    c.initialize_device(dev.data_ptr(), n, 2)
+                    c.run()
+                    if k >= args.warmup:
+                        ms.append(c.times()[3])
+                w, m = c.evaluate()
+            assert m == n - 1, f"{name}: {m} edges for {n} cities"
+            med = sorted(ms)[len(ms) // 2]
+            rows.append({"instance": name[:-7], "n": n, "ms": round(med, 3), "mpts": round(n / med / 1e3, 3)})
+    total = sum(r["ms"] for r in rows)
+    big = max(rows, key=lambda r: r["n"])
+    out = {
+        "metric": f"TSPLIB sweep: EMST device time summed over {len(rows)} instances", "value": round(total, 3),
+        "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": False,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f32",
+        "data": "TSPLIB instances of the reference's 2D tests (tests/golden/tsplib)",
+        "config": {"workload": "TSPLIB sweep, 2D", "instances": len(rows),
+                   "cities": int(sum(r["n"] for r in rows)), "largest": big, "per_instance": rows},
     }
     print(json.dumps(out))
 

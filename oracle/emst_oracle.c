@@ -15,6 +15,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* ------------------------------------------------------------------ A1 --
  * Bounding box as InputRW::readUniform / readTSP compute it
@@ -408,16 +411,34 @@ static int64_t octants_round(int mode, const or_grid* g, const or_cells* cs, con
                              float* cd2, int64_t* examined)
 {
     const int K = 2 * cdim * (1 << (cdim - 1));
-    int64_t searched = 0;
+    int64_t searched = 0, ex = 0;
+    /* every point's search reads shared state and writes only its own cones,
+     * so the points run in parallel (OpenMP; one thread without it) */
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : searched, ex)
     for (int64_t p = 0; p < n; ++p) {
+        int64_t e = 0;
         if (mode == 1)
             searched += or_octants_ref_point(g, pts, cs->cell_of, cs->cell_start, cs->cell_pts,
-                                             label, (int32_t)p, cid + p * K, cd2 + p * K, examined);
+                                             label, (int32_t)p, cid + p * K, cd2 + p * K, &e);
         else
             searched += or_octants_exact_point(pts, n, dim, cdim, label, (int32_t)p, cid + p * K,
-                                               cd2 + p * K, examined);
+                                               cd2 + p * K, &e);
+        ex += e;
     }
+    if (examined) *examined += ex;
     return searched;
+}
+
+/* Threads of the parallel searches (1 without OpenMP); n > 0 sets them. */
+int or_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }
 
 static void cells_alloc(const or_grid* g, const float* pts, int64_t n, or_cells* cs,

@@ -95,6 +95,10 @@ int64_t or_prim_fp64(const float* pts, int64_t n, int dim, int32_t* eu, int32_t*
 /* A15 */
 double or_weight(const float* orig, int dim, const int32_t* eu, const int32_t* ev, int64_t m);
 
+/* Threads of the per-point cone searches (OpenMP); n > 0 sets them.  Returns
+ * the count now in use. */
+int or_threads(int n);
+
 #ifdef __cplusplus
 }
 #endif

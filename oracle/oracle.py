@@ -70,6 +70,8 @@ def lib():
         L.or_prim_fp64.restype = i64
         L.or_weight.argtypes = [P, i32, P, P, i64]
         L.or_weight.restype = dbl
+        L.or_threads.argtypes = [i32]
+        L.or_threads.restype = i32
         _lib = L
     return _lib
 
@@ -214,6 +216,11 @@ def prim_fp64(pts):
     w = ctypes.c_double(0)
     m = lib().or_prim_fp64(_p(pts), n, dim, _p(eu), _p(ev), ctypes.byref(w))
     return eu[:m].copy(), ev[:m].copy(), w.value
+
+
+def threads(n=0):
+    """Threads of the oracle's per-point searches; n > 0 sets them."""
+    return lib().or_threads(n)
 
 
 def weight(orig, u, v):
