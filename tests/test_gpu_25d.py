@@ -40,12 +40,13 @@ def run25(pkg, pts, trace=False, step=False, **kw):
     return out
 
 
-@pytest.mark.parametrize("seed,n,zscale,ties", [(1, 3000, 1.0, False), (2, 2500, 0.05, True),
-                                                (3, 2000, 5.0, True), (4, 70, 1.0, False)])
-def test_wedge_candidates_bit_exact(pkg, oracle_mod, seed, n, zscale, ties):
+@pytest.mark.parametrize("seed,n,zscale,ties,split", [(1, 3000, 1.0, False, 0), (2, 2500, 0.05, True, 0),
+                                                      (3, 2000, 5.0, True, 0), (4, 70, 1.0, False, 0),
+                                                      (5, 3000, 1.0, True, 2), (6, 3000, 0.3, False, 3)])
+def test_wedge_candidates_bit_exact(pkg, oracle_mod, seed, n, zscale, ties, split):
     pts = cloud(seed, n, zscale, ties)
-    print("case", seed, n, zscale, ties, flush=True)
-    with pkg.Calculateur(dim=3, cone_dim=2) as c:
+    print("case", seed, n, zscale, ties, split, flush=True)
+    with pkg.Calculateur(dim=3, cone_dim=2, search_split=split) as c:
         c.initialize(pts)
         cand, cd2 = c.candidates()
         resc, _ = c.getAdaptiveMap()

@@ -273,6 +273,28 @@ def test_coincident_groups(pkg, oracle_mod, case, grouped):
     assert np.array_equal(lab, ref["labels"])
 
 
+@pytest.mark.parametrize("case,split", [("groups3", 2), ("groups3", 3), ("groups2", 3), ("near_twins", 3)])
+def test_coincident_groups_split_search(pkg, oracle_mod, case, split):
+    """Coincident groups with the split cone search (first pass, then the
+    open queries resumed in place or compacted): the representatives' saved
+    '+' cones and every point's cones stay exact."""
+    pts = _coincident_case(case, oracle_mod)
+    print("case", case, split, pts.shape, flush=True)
+    with pkg.Calculateur(dim=pts.shape[1], search_split=split) as c:
+        c.initialize(pts)
+        cand, cd2 = c.candidates()
+        st = pkg.search_stats(c)
+        resc, _ = c.getAdaptiveMap()
+        c.run()
+        u, v, _ = c.getLinks()
+    assert st["grouped"] == 1 and st["resumed"] > 0
+    ocid, ocd2, _ = oracle_mod.octants(resc, mode=0)
+    assert np.array_equal(cand, ocid)
+    assert np.array_equal(cd2.view(np.uint32), ocd2.view(np.uint32))
+    ref = oracle_mod.emst(pts, mode=0)
+    assert np.array_equal(np.stack([u, v], 1).astype(np.int64), oracle_mod.canon(ref["u"], ref["v"]))
+
+
 @pytest.mark.parametrize("n", [0, 1])
 def test_tiny(pkg, n):
     pts = np.zeros((n, 3), np.float32)
