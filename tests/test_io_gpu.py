@@ -83,3 +83,19 @@ def test_cli_3d_mode_from_config(pkg, tmp_path, oracle_mod):
     edges = sorted({(min(i, j), max(i, j)) for i, js in nodes.items() for j in js})
     ref = oracle_mod.emst(read_points(golden_path("1000randomCity.3d")), mode=0)
     assert np.array_equal(np.array(edges, np.int64), oracle_mod.canon(ref["u"], ref["v"]))
+
+
+def test_cli_2p5d_mode_from_config(pkg, tmp_path, oracle_mod):
+    """functionModeChoice = 6 (RUN_2DPOINT5): 3D points with cones over the
+    xy plane; the forest equals the oracle's 2.5D forest, per round."""
+    cfg = tmp_path / "config.cfg"
+    cfg.write_text("[coalition_global_param]\nfunctionModeChoice = 6\n[global_param]\ntraceActive = 1\n")
+    r = subprocess.run([pkg.CLI_PATH, golden_path("2000randomCity.3d"), str(tmp_path / "o25"),
+                        str(tmp_path / "o25.stats"), str(cfg)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, nodes, _ = _links(tmp_path / "o25.links")
+    edges = sorted({(min(i, j), max(i, j)) for i, js in nodes.items() for j in js})
+    ref = oracle_mod.emst(read_points(golden_path("2000randomCity.3d")), mode=0, cdim=2)
+    assert np.array_equal(np.array(edges, np.int64), oracle_mod.canon(ref["u"], ref["v"]))
+    rows = [l.split() for l in open(tmp_path / "o25.stats").read().split("\n")[1:] if l.strip()]
+    assert [int(r[6]) for r in rows[:-1]] == ref["cum_edges"]      # size column, one row per round
