@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--cone-dim", type=int, default=0,
                     help="2 with --dim 3: the 2.5D mode (wedges over the xy plane, 3D distances)")
     ap.add_argument("--plain", action="store_true", help="plain seed only (skip the learned form)")
+    ap.add_argument("--round-search", type=int, default=0, choices=(0, 1),
+                    help="1: per-round component search instead of the round-1 cone search (plain seed)")
     ap.add_argument("--cpu-sample", type=int, default=200_000,
                     help="points of the same distribution timed on the CPU oracle (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -97,7 +99,8 @@ def main():
     cdim = args.cone_dim or args.dim
     K = 2 * cdim * (1 << (cdim - 1))
     model_path = os.path.join(ge.PKG_DIR, "models", f"radius_{args.dim}d.npy")
-    model = np.load(model_path) if (os.path.exists(model_path) and not args.plain and cdim == args.dim) else None
+    model = np.load(model_path) if (os.path.exists(model_path) and not args.plain and cdim == args.dim
+                                    and not args.round_search) else None
     model_clouds = None
     if model is not None:
         meta = os.path.join(ge.PKG_DIR, "models", "train_radius.json")
@@ -105,7 +108,8 @@ def main():
             model_clouds = json.load(open(meta)).get("clouds")
 
     def timed(form_model):
-        ctx = pkg.Calculateur(dim=args.dim, bucket_size=args.bucket, cone_dim=args.cone_dim)
+        ctx = pkg.Calculateur(dim=args.dim, bucket_size=args.bucket, cone_dim=args.cone_dim,
+                              round_search=args.round_search)
         if form_model is not None:
             ctx.set_radius_model(form_model)
         search_ms, total_ms = [], []
@@ -166,10 +170,12 @@ def main():
     # time from HIP events on the context stream
     alg_bytes = args.n * (8 * args.dim + 4 + 8 * K)
     s_ms = head["search_ms"]
-    achieved = alg_bytes / (s_ms * 1e-3) / 1e9
+    achieved = alg_bytes / (s_ms * 1e-3) / 1e9 if s_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                 "frac": round(achieved / 8000.0, 4), "traffic": profiled_traffic(args),
                 "kernel": "k_cone_search", "kernel_ms": round(s_ms, 3)}
+    if args.round_search:
+        roofline = None     # no round-1 cone search: the searches are spread over the rounds
 
     cpu = None
     if args.cpu_sample > 0:
@@ -217,7 +223,8 @@ def main():
                    "radius_model_training_clouds": model_clouds,
                    "learned_seed": seed_report,
                    "same_tree": learned is not None,
-                   "cone_dim": cdim, "cycles_cut": int(head["stats"]["cycles_cut"])},
+                   "cone_dim": cdim, "cycles_cut": int(head["stats"]["cycles_cut"]),
+                   "round_search": args.round_search},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

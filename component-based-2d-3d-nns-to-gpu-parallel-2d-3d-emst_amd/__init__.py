@@ -46,7 +46,8 @@ class Options(ctypes.Structure):
                 ("max_rounds", ctypes.c_int32), ("bucket_size", ctypes.c_int32),
                 ("trace", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("radius_mode", ctypes.c_int32), ("seed_scale", ctypes.c_float),
-                ("cone_dim", ctypes.c_int32), ("search_split", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
+                ("cone_dim", ctypes.c_int32), ("search_split", ctypes.c_int32), ("round_search", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 1)]
 
 
 class RoundTrace(ctypes.Structure):
@@ -141,7 +142,7 @@ class Calculateur:
     """One EMST context on one device (CalculateurEMST / SolutionEMST)."""
 
     def __init__(self, dim=3, level_radius=1, max_rounds=100, bucket_size=0, trace=False, device=0,
-                 seed_scale=0.0, cone_dim=0, search_split=0):
+                 seed_scale=0.0, cone_dim=0, search_split=0, round_search=0):
         """cone_dim=2 with dim=3: the 2.5D mode (SolutionEMST<3,2>, 3D points,
         wedges over the xy plane)."""
         L = lib()
@@ -152,6 +153,7 @@ class Calculateur:
         o.seed_scale = seed_scale
         o.cone_dim = cone_dim
         o.search_split = search_split
+        o.round_search = round_search
         self._L = L
         self._c = L.emst_create(ctypes.byref(o))
         if not self._c:
@@ -233,7 +235,7 @@ class Calculateur:
         self._chk(self._L.emst_get_options(self._c, ctypes.byref(o)), "options")
         return {"level_radius": o.level_radius, "max_rounds": o.max_rounds, "bucket_size": o.bucket_size,
                 "trace": bool(o.trace), "device": o.device, "seed_scale": o.seed_scale, "cone_dim": o.cone_dim,
-                "search_split": o.search_split}
+                "search_split": o.search_split, "round_search": o.round_search}
 
     def cloud_results(self):
         """Per cloud: (FP64 weight, edge count, completion round)."""

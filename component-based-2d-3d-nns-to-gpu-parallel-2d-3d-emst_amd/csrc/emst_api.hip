@@ -97,6 +97,7 @@ struct emst_ctx {
     DevBuf<uint64_t> cy_kh;
     DevBuf<uint32_t> cy_kl;
     DevBuf<uint8_t> cy_onc, cy_cut;
+    DevBuf<int32_t> nlab;              // node component labels (per-round component search)
     bool no_grouping = false;         // set when distinct points met at float distance 0
     bool grouped = false;
     bool groups_settled = false;      // no group spans two components any more
@@ -171,7 +172,8 @@ int choose_bucket(const emst_options& o, int dim)
 int alloc_all(emst_ctx* c)
 {
     const int64_t n = c->n;
-    const int K = c->K;
+    const bool comp = c->opt.round_search == 1;
+    const int K = comp ? 0 : c->K;      // the component search keeps no cone candidates
     int rc;
 #define ENS(buf, cnt) \
     if ((rc = ensure(c, c->buf, (size_t)(cnt))) != EMST_OK) return rc;
@@ -214,6 +216,7 @@ int alloc_all(emst_ctx* c)
     ENS(dup_list, n);
     ENS(dup_runs, n);
     ENS(dup_gslot, n);
+    if (comp) ENS(nlab, 2 * c->nleaves);
     if (c->cdim < c->dim) {
         ENS(cy_j0, n);
         ENS(cy_roots, n);
@@ -308,6 +311,20 @@ CycleState cycle_state(emst_ctx* c)
     return y;
 }
 
+// the split-search buffers, also the per-round component search's query list
+SplitScratch split_scratch(emst_ctx* c)
+{
+    SplitScratch sc{};
+    sc.open_mask = c->open_mask.p;
+    sc.cnt = c->open_cnt.p;
+    sc.off = c->open_cnt.p + ((c->n + 63) / 64 + 1);
+    sc.tmp = c->open_tmp.p;
+    sc.list = c->list.p;
+    sc.after = 0;
+    sc.compact_frac = 0.10;
+    return sc;
+}
+
 DupState dup_state(emst_ctx* c)
 {
     DupState d{};
@@ -356,7 +373,8 @@ int build(emst_ctx* c)
     c->grouped = false;
     c->groups_settled = false;
     c->dup_points = c->dup_groups = 0;
-    if (!c->no_grouping) {
+    const bool comp = c->opt.round_search == 1;
+    if (!c->no_grouping && !comp) {
         launch_dup_groups(c->keys0.p, c->pts.p, n, c->dup_rep.p, c->dup_runs.p, c->dup_list.p,
                           c->dup_gslot.p, c->C, s);
         LAUNCHCHK(c);
@@ -379,6 +397,27 @@ int build(emst_ctx* c)
     launch_round_init(n, c->label.p, c->label2.p, c->fixed.p, c->eu.p, c->corr.p, c->corr_key.p, c->comp1.p,
                       c->comp2.p, c->C, (uint32_t)c->target_edges, (uint32_t)c->opt.max_rounds, s);
     LAUNCHCHK(c);
+    if (comp) {
+        // the per-round component search has no round-1 search: every round
+        // searches (launch_comp_round)
+        HIPCHK(c, hipEventRecord(c->ev[1], s));
+        HIPCHK(c, hipEventRecord(c->ev[2], s));
+        HIPCHK(c, hipEventRecord(c->ev[3], s));
+        Counters h{};
+        int rc = read_counters(c, &h);
+        if (rc) return rc;
+        c->zero_or_empty = 0;
+        for (auto& x : c->stats) x = 0;
+        c->resumed_h = 0;
+        c->times[0] = elapsed(c->ev[0], c->ev[1]);
+        c->times[1] = 0.f;
+        c->built = true;
+        c->rounds = 0;
+        c->edges = 0;
+        c->finished = c->target_edges == 0;
+        c->trace.clear();
+        return EMST_OK;
+    }
     const float* seed_r = nullptr;
     if (c->learned && c->ncl == 1 && c->cdim == c->dim) {
         int rc2;
@@ -497,6 +536,13 @@ void issue_round(emst_ctx* c, bool tr)
     }
     const CycleState cy = cycle_state(c);
     for (int ph = 0; ph < 4; ++ph) {
+        if (ph == 0 && c->opt.round_search == 1) {
+            launch_comp_round(c->dim, c->n, c->pts.p, c->nodes.p, c->nlab.p, c->dcl.p, c->ncl, c->nleaves, c->Lmax,
+                              c->bucket, c->label.p, c->corr.p, c->corr_key.p, c->comp1.p, c->inv.p, c->seed_scale,
+                              split_scratch(c), c->C, s);
+            if (tr) (void)hipEventRecord(c->ev[4], s);
+            continue;
+        }
         launch_round_phase(ph, c->K, c->n, c->pts.p, c->rkey.p, c->rpos.p, c->cursor.p,
                            c->label.p, c->label2.p,
                            c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->fixed.p, c->C, c->eu.p,
@@ -608,6 +654,8 @@ std::vector<uintptr_t> graph_key(emst_ctx* c)
     const DupState d = dup_state(c);
     const CycleState y = cycle_state(c);
     return {(uintptr_t)c->n, (uintptr_t)c->K, (uintptr_t)c->dim, (uintptr_t)c->cdim, (uintptr_t)y.j0, (uintptr_t)y.roots,
+            (uintptr_t)c->opt.round_search, (uintptr_t)c->nlab.p, (uintptr_t)c->open_mask.p, (uintptr_t)c->open_cnt.p,
+            (uintptr_t)c->open_tmp.p, (uintptr_t)c->list.p, (uintptr_t)c->nodes.p, (uintptr_t)c->dcl.p,
             (uintptr_t)y.j[0], (uintptr_t)y.kh[0], (uintptr_t)y.kl[0], (uintptr_t)y.onc, (uintptr_t)y.cut,
             (uintptr_t)c->grouped, (uintptr_t)d.npoints,
             (uintptr_t)d.ngroups, (uintptr_t)c->C, (uintptr_t)c->pts.p, (uintptr_t)c->cand_key.p,
@@ -690,6 +738,14 @@ emst_ctx* emst_create(const emst_options* opt)
     if (opt) o = *opt;
     else emst_default_options(&o);
     if (o.dim != 2 && o.dim != 3) { g_create_error = "dim must be 2 or 3"; return nullptr; }
+    if (o.round_search != 0 && o.round_search != 1) {
+        g_create_error = "round_search must be 0 or 1";
+        return nullptr;
+    }
+    if (o.round_search == 1 && o.cone_dim == 2 && o.dim == 3) {
+        g_create_error = "the 2.5D mode needs the cone search (round_search 0)";
+        return nullptr;
+    }
     if (o.search_split < 0 || o.search_split > 3) {
         g_create_error = "search_split must be 0..3";
         return nullptr;
@@ -995,6 +1051,7 @@ int emst_get_candidates(emst_ctx* c, int32_t* cand, float* cd2, int32_t* k)
 {
     int rc = ensure_built(c);
     if (rc) return rc;
+    if (c->opt.round_search == 1) return fail(c, EMST_E_STATE, "the per-round component search keeps no cone candidates");
     if (k) *k = c->K;
     if (!cand && !cd2) return EMST_OK;
     if (!cand || !cd2) return fail(c, EMST_E_ARG, "cand and cand_d2 are both required");
@@ -1121,6 +1178,8 @@ int emst_set_radius_model(emst_ctx* c, const float* model)
     }
     if (c->opt.cone_dim == 2 && c->opt.dim == 3)
         return fail(c, EMST_E_ARG, "the learned seed radius is not defined for the 2.5D mode");
+    if (c->opt.round_search == 1)
+        return fail(c, EMST_E_ARG, "the learned seed radius seeds the cone search (round_search 0)");
     int rc = ensure(c, c->model, EMST_MLP_PARAMS);
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(c->model.p, model, sizeof(float) * EMST_MLP_PARAMS, hipMemcpyHostToDevice, c->stream));
@@ -1137,6 +1196,7 @@ int emst_get_radius_samples(emst_ctx* c, float* feat, float* target, int64_t* m)
     if (rc) return rc;
     if (c->ncl != 1) return fail(c, EMST_E_STATE, "radius samples come from a single-cloud context");
     if (c->cdim != c->dim) return fail(c, EMST_E_STATE, "radius samples come from full-dimensional cones");
+    if (c->opt.round_search == 1) return fail(c, EMST_E_STATE, "radius samples come from the cone search");
     const int64_t n = c->n;
     if (!feat && !target) { *m = n; return EMST_OK; }
     if (*m < n) { *m = n; return fail(c, EMST_E_ARG, "sample arrays too small"); }

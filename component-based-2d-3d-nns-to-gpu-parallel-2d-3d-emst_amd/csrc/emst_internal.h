@@ -166,6 +166,15 @@ void launch_cone_search(int dim, int cdim, const float4* pts, const Node* nodes,
                         uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
                         uint64_t* first_key, Counters* C, bool collect_stats, DupState dup,
                         const SplitScratch* split, hipStream_t s);
+// Phase 0 of a round of the per-round component search (the alternative to
+// the round-1 cone search): node component labels (nlab: 2 * nleaves words),
+// kept targets, and a search for the points whose target joined their
+// component; writes corr / corr_key and the component minima (comp1).
+// sc: open_mask, cnt, off, tmp and list as for a split cone search.
+void launch_comp_round(int dim, int64_t n, const float4* pts, const Node* nodes, int32_t* nlab, const Cloud* cl,
+                       int ncl, int64_t nleaves, int64_t Lmax, int bucket, const int32_t* label, int32_t* corr,
+                       uint64_t* corr_key, uint64_t* comp1, const int32_t* inv, float seed_scale,
+                       const SplitScratch& sc, Counters* C, hipStream_t s);
 // Cones of every group point for the current labels (dim: the cone dimension) (after the grouped
 // cone search, and at the start of each round while a group may still span
 // several components).  Sets C->dup_split when one does.

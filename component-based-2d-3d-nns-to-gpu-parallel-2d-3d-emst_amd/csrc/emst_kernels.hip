@@ -1619,6 +1619,298 @@ __global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const floa
     if (threadIdx.x == 0 && cnt) atomicAdd(&C->edges, cnt);
 }
 
+// ---- per-round component search (NodeSpiralSearchMD::search,
+// emst_coalition/include/NodeEMST.h:67-509; the reference's alternative to
+// the round-1 cone search, SolutionEMST.h:55-56,187-194).  Every round, each
+// point whose previous target has joined its component looks for its nearest
+// point in another component, under (d2, id); a point whose target is still
+// outside keeps it (:160-169).  The search is exact per point, so a
+// component's smallest member result is its shortest outgoing edge, and the
+// rounds build the same forests as the cone search.  Subtrees that lie in
+// the point's own component are skipped (node component labels, rebuilt
+// every round), which plays the part of the reference's permanent slab
+// closures against same-component points (:293-318).
+
+constexpr int32_t kNlabEmpty = -3;      // a node without points
+constexpr int32_t kNlabMixed = -1;      // a node holding several components
+
+// component label of every leaf: the common label of its points
+__global__ __launch_bounds__(kBlock) void k_nlab_leaves(const int32_t* __restrict__ label,
+                                                       const Cloud* __restrict__ cl, int ncl, int64_t nleaves,
+                                                       int bucket, int32_t* __restrict__ nlab, const Counters* C)
+{
+    if (C->done) return;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nleaves) return;
+    const int c = find_cloud_leaf(cl, ncl, g);
+    const int64_t j = g - cl[c].lbase;
+    const int64_t s0 = cl[c].pos + j * bucket;
+    const int64_t s1 = min(cl[c].pos + cl[c].n, s0 + bucket);
+    int32_t lab = kNlabEmpty;
+    for (int64_t s = s0; s < s1; ++s) {
+        const int32_t l = label[s];
+        lab = (lab == kNlabEmpty || lab == l) ? l : kNlabMixed;
+    }
+    nlab[2 * cl[c].lbase + cl[c].L + j] = lab;
+}
+
+__global__ __launch_bounds__(kBlock) void k_nlab_level(int32_t* __restrict__ nlab, const Cloud* __restrict__ cl,
+                                                      int ncl, int64_t lo, const Counters* C)
+{
+    if (C->done) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)ncl * lo) return;
+    const int c = (int)(t / lo);
+    if (lo >= cl[c].L) return;
+    int32_t* base = nlab + 2 * cl[c].lbase;
+    const int64_t i = lo + t % lo;
+    const int32_t a = base[2 * i], b = base[2 * i + 1];
+    base[i] = a == kNlabEmpty ? b : (b == kNlabEmpty ? a : (a == b ? a : kNlabMixed));
+}
+
+// keep-previous (NodeEMST.h:160-169) and the component minimum of the kept
+// targets; the points that must search are flagged per 64-position packet
+__global__ __launch_bounds__(kBlock) void k_comp_prep(int64_t n, const float4* __restrict__ pts,
+                                                     const int32_t* __restrict__ label,
+                                                     const int32_t* __restrict__ corr,
+                                                     const uint64_t* __restrict__ corr_key,
+                                                     uint64_t* __restrict__ comp1,
+                                                     uint64_t* __restrict__ open_mask, const Counters* C)
+{
+    if (C->done) return;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool need = false, valid = false;
+    int32_t lp = 0;
+    uint64_t key1 = kEmptyKey;
+    if (p < n) {
+        lp = label[p];
+        const int32_t c = corr[p];
+        if (c >= 0 && label[c] != lp) {
+            const uint64_t ck = corr_key[p];
+            const uint32_t pid = __float_as_uint(pts[p].w), qid = (uint32_t)ck;
+            key1 = (ck & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
+            valid = true;
+        } else {
+            need = c != -2;         // -2: nothing outside the component (its cloud is one tree)
+        }
+    }
+    atomic_min_grouped(comp1, lp, key1, valid);
+    const uint64_t m = __ballot(need);
+    if (__lane_id() == 0 && p < n) open_mask[p / 64] = m;
+}
+
+// nearest point in another component, for the points of qlist (count in
+// nq_dev): packet traversal as in the cone search, one best per query, by
+// iterative deepening from the seed radius R_0 (capped passes R_k = 2^k R_0,
+// then one uncapped).  Points closer than the previous target's distance were
+// in the component at that search and still are, so they are skipped.
+template <int D>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_comp_search(
+    const float4* __restrict__ pts, const Node* __restrict__ nodes, const int32_t* __restrict__ nlab,
+    const Cloud* __restrict__ clouds, int ncl, int bucket, const int32_t* __restrict__ qlist,
+    const uint32_t* __restrict__ nq_dev, const int32_t* __restrict__ label, const int32_t* __restrict__ inv,
+    float seed_scale, int32_t* __restrict__ corr, uint64_t* __restrict__ corr_key,
+    uint64_t* __restrict__ comp1, const Counters* C)
+{
+    if (C->done) return;
+    const int64_t nq = *nq_dev;
+    if ((int64_t)blockIdx.x * 64 >= nq) return;
+    __shared__ unsigned long long best[64];
+    __shared__ int32_t stk_id[64];
+    __shared__ int32_t stk_ver[64];
+    __shared__ unsigned long long stk_mask[64];
+    const int lane = __lane_id();
+    const int64_t qi = (int64_t)blockIdx.x * 64 + lane;
+    const bool active = qi < nq;
+    const int64_t p = active ? (int64_t)qlist[qi] : 0;
+    const float4 me = pts[p];
+    const float px = me.x, py = me.y, pz = me.z;
+    const int32_t mylab = active ? label[p] : -1;
+    const uint64_t prev = active ? corr_key[p] : kEmptyKey;
+    best[lane] = kEmptyKey;
+    const int my_cl = active ? find_cloud(clouds, ncl, p) : -1;
+    uint64_t pending = __ballot(active);
+    while (pending) {
+        const int cl = __builtin_amdgcn_readfirstlane(__shfl(my_cl, (int)__builtin_ctzll(pending)));
+        const bool mine = my_cl == cl;
+        pending &= ~__ballot(mine);
+        const int64_t cpos = clouds[cl].pos, cend = clouds[cl].pos + clouds[cl].n;
+        const int64_t L = clouds[cl].L;
+        const Node* __restrict__ tree = nodes + 2 * clouds[cl].lbase;
+        const int32_t* __restrict__ tlab = nlab + 2 * clouds[cl].lbase;
+        // seed: twice the previous target's distance, at least seed_scale
+        // times the mean spacing of the point's leaf cell
+        float r = 1.f;
+        int32_t inner = -1;
+        if (mine) {
+            const int64_t lp = p - cpos;
+            const Node own = tree[L + lp / bucket];
+            const int64_t s0 = cpos + (lp / bucket) * bucket;
+            const float cnt = (float)(min(cend, s0 + bucket) - s0);
+            float vol = 1.f, ext = 0.f;
+            for (int k = 0; k < D; ++k) {
+                const float e = own.hi[k] - own.lo[k];
+                vol *= e;
+                ext = fmaxf(ext, e);
+            }
+            float sp = vol > 0.f ? powf(vol / cnt, 1.f / D) : ext / cnt;
+            if (!(sp > 0.f)) sp = 1e-6f;
+            r = seed_scale * sp;
+            if (prev != kEmptyKey) {
+                const uint32_t dbits = (uint32_t)(prev >> 32);
+                inner = (int32_t)dbits - 1;
+                r = fmaxf(r, 2.f * sqrtf(__uint_as_float(dbits)));
+            }
+        }
+        float r2 = r * r;
+        bool open = mine;
+        float cx = mine ? px : 0.f, cy = mine ? py : 0.f, cz = mine ? pz : 0.f;
+        float na = mine ? 1.f : 0.f;
+        for (int off = 32; off > 0; off >>= 1) {
+            cx += __shfl_xor(cx, off);
+            cy += __shfl_xor(cy, off);
+            cz += __shfl_xor(cz, off);
+            na += __shfl_xor(na, off);
+        }
+        cx /= na; cy /= na; cz /= na;
+        auto cdist = [&](const Node& x) {
+            const float dxx = fmaxf(fmaxf(x.lo[0] - cx, cx - x.hi[0]), 0.f);
+            const float dyy = fmaxf(fmaxf(x.lo[1] - cy, cy - x.hi[1]), 0.f);
+            const float dzz = fmaxf(fmaxf(x.lo[2] - cz, cz - x.hi[2]), 0.f);
+            return dxx * dxx + dyy * dyy + dzz * dzz;
+        };
+        // a box is needed when it holds another component and its nearest
+        // point may beat the bound (ties kept: they can still win on id)
+        // and its farthest point lies beyond `inner`
+        auto needed = [&](const Node& nd, int32_t nl, int32_t bnd) {
+            if (!open || nl == mylab || nl == kNlabEmpty) return false;
+            float m[3], u[3];
+            const float lo[3] = {nd.lo[0] - px, nd.lo[1] - py, nd.lo[2] - pz};
+            const float hi[3] = {nd.hi[0] - px, nd.hi[1] - py, nd.hi[2] - pz};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                m[k] = fmaxf(fmaxf(lo[k], -hi[k]), 0.f);
+                u[k] = fmaxf(-lo[k], hi[k]);
+            }
+            const float lb = D == 3 ? d2_of(m[0], m[1], m[2]) : d2_of(m[0], m[1]);
+            const float far = D == 3 ? d2_of(u[0], u[1], u[2]) : d2_of(u[0], u[1]);
+            return (int32_t)__float_as_uint(lb) <= bnd && (int32_t)__float_as_uint(far) > inner;
+        };
+        for (int pass = 0; pass <= kMaxCappedPasses; ++pass) {
+            if (__ballot(open) == 0ull) break;
+            const bool last = pass == kMaxCappedPasses;
+            const int32_t cap = last ? 0x7fffffff : (int32_t)__float_as_uint(r2);
+            int32_t bnd = min(key_bound(best[lane]), cap);
+            int sp = 0;
+            int ver = 0;
+            if (lane == 0) { stk_id[0] = 1; stk_ver[0] = -1; }
+            sp = 1;
+            __builtin_amdgcn_wave_barrier();
+            while (sp > 0) {
+                --sp;
+                const int node = __builtin_amdgcn_readfirstlane(stk_id[sp]);
+                const int pver = __builtin_amdgcn_readfirstlane(stk_ver[sp]);
+                bool need;
+                if (pver == ver) {
+                    need = (stk_mask[sp] >> lane) & 1ull;
+                } else {
+                    need = needed(tree[node], tlab[node], bnd);
+                }
+                const uint64_t needm = __ballot(need);
+                if (needm == 0ull) continue;
+                if (node >= L) {
+                    const int64_t s0 = cpos + (int64_t)(node - L) * bucket;
+                    const int cnt = (int)max<int64_t>(min(cend, s0 + bucket) - s0, 0);
+                    const int j = lane & (bucket - 1);
+                    const int qslot = lane / bucket;
+                    const int qpc = 64 / bucket;
+                    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                    int32_t vlab = -1;
+                    if (j < cnt) {
+                        v = pts[s0 + j];
+                        vlab = label[s0 + j];
+                    }
+                    uint64_t rem = needm;
+                    while (rem) {
+                        int myq = -1;
+                        for (int k = 0; k < qpc; ++k) {
+                            const int l = rem ? (int)__builtin_ctzll(rem) : -1;
+                            if (rem) rem &= rem - 1;
+                            if (k == qslot) myq = l;
+                        }
+                        const int src = myq >= 0 ? myq : 0;
+                        const float qx = __shfl(px, src), qy = __shfl(py, src), qz = __shfl(pz, src);
+                        const int32_t qb = __shfl(bnd, src), qin = __shfl(inner, src), ql = __shfl(mylab, src);
+                        const float dx = v.x - qx, dy = v.y - qy, dz = v.z - qz;
+                        const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
+                        const int32_t db = (int32_t)__float_as_uint(d2);
+                        if (myq >= 0 && j < cnt && vlab != ql && db <= qb && db > qin)
+                            atomicMin(&best[myq],
+                                      ((unsigned long long)__float_as_uint(d2) << 32) | __float_as_uint(v.w));
+                    }
+                    if (need) bnd = min(key_bound(best[lane]), cap);
+                    ++ver;
+                } else {
+                    const bool leafkids = 2 * (int64_t)node >= L;
+                    const int first = leafkids ? 2 * node : 4 * node;
+                    const int nk = leafkids ? 2 : 4;
+                    Node bx[4];
+                    int32_t bl[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (i < nk) {
+                            bx[i] = tree[first + i];
+                            bl[i] = tlab[first + i];
+                        }
+                    uint64_t mk[4];
+                    float dk[4];
+                    int id[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        mk[i] = i < nk ? __ballot(needed(bx[i], bl[i], bnd)) : 0ull;
+                        dk[i] = mk[i] ? cdist(bx[i]) : -1.f;
+                        id[i] = first + i;
+                    }
+                    auto ce = [&](int a, int b) {
+                        if (dk[a] < dk[b]) {
+                            const float td = dk[a]; dk[a] = dk[b]; dk[b] = td;
+                            const uint64_t tm = mk[a]; mk[a] = mk[b]; mk[b] = tm;
+                            const int ti = id[a]; id[a] = id[b]; id[b] = ti;
+                        }
+                    };
+                    ce(0, 1); ce(2, 3); ce(0, 2); ce(1, 3); ce(1, 2);
+                    if (lane == 0) {
+                        int t = sp;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            if (mk[i]) { stk_id[t] = id[i]; stk_ver[t] = ver; stk_mask[t] = mk[i]; ++t; }
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) sp += mk[i] != 0ull ? 1 : 0;
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            if (open && key_bound(best[lane]) <= cap) open = false;
+            inner = cap;
+            r2 *= 4.f;
+        }
+    }
+    // the point's result, and the first phase of its component's minimum
+    const uint64_t k = best[lane];
+    bool valid = false;
+    uint64_t key1 = kEmptyKey;
+    if (active) {
+        corr[p] = k == kEmptyKey ? -2 : inv[(uint32_t)k];
+        corr_key[p] = k;
+        if (k != kEmptyKey) {
+            const uint32_t pid = __float_as_uint(me.w), qid = (uint32_t)k;
+            key1 = (k & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
+            valid = true;
+        }
+    }
+    atomic_min_grouped(comp1, mylab, key1, valid);
+}
+
 // ---- cycle rule of the 2.5D mode (CycleState).  Roots are the points whose
 // flattened label is their own position.  In round 1 every component is one
 // point and its chosen edge is its nearest neighbour, whose choices close
@@ -2088,6 +2380,28 @@ void launch_cone_search(int dim, int cdim, const float4* pts, const Node* nodes,
                        cand_key, cand_pos, revive, first_corr, first_key, C, cs, dup, split->after + 1, kMaxCappedPasses,
                        (uint64_t*)nullptr, (const uint32_t*)(split->off + P), (const uint64_t*)split->open_mask,
                        (int64_t)(split->compact_frac * (double)nq));
+}
+
+void launch_comp_round(int dim, int64_t n, const float4* pts, const Node* nodes, int32_t* nlab, const Cloud* cl,
+                       int ncl, int64_t nleaves, int64_t Lmax, int bucket, const int32_t* label, int32_t* corr,
+                       uint64_t* corr_key, uint64_t* comp1, const int32_t* inv, float seed_scale,
+                       const SplitScratch& sc, Counters* C, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_nlab_leaves, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, label, cl, ncl, nleaves, bucket,
+                       nlab, (const Counters*)C);
+    for (int64_t lo = Lmax / 2; lo >= 1; lo /= 2)
+        hipLaunchKernelGGL(k_nlab_level, dim3(nblocks((int64_t)ncl * lo)), dim3(kBlock), 0, s, nlab, cl, ncl, lo,
+                           (const Counters*)C);
+    const int64_t P = (n + 63) / 64;
+    hipLaunchKernelGGL(k_comp_prep, dim3(nblocks(n)), dim3(kBlock), 0, s, n, pts, label, corr, corr_key, comp1,
+                       sc.open_mask, (const Counters*)C);
+    hipLaunchKernelGGL(k_open_count, dim3(nblocks(P + 1)), dim3(kBlock), 0, s, sc.open_mask, P, sc.cnt);
+    exclusive_scan_u32(sc.cnt, sc.off, P + 1, sc.tmp, s);
+    hipLaunchKernelGGL(k_open_fill, dim3((unsigned)P), dim3(64), 0, s, sc.open_mask, sc.off, sc.list);
+    hipLaunchKernelGGL(dim == 3 ? k_comp_search<3> : k_comp_search<2>, dim3((unsigned)P), dim3(64), 0, s, pts, nodes,
+                       (const int32_t*)nlab, cl, ncl, bucket, (const int32_t*)sc.list, (const uint32_t*)(sc.off + P),
+                       label, inv, seed_scale, corr, corr_key, comp1, (const Counters*)C);
 }
 
 void launch_dup_fill(int dim, int64_t n, const int32_t* label, const float4* pts, const int32_t* inv,

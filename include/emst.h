@@ -70,7 +70,15 @@ typedef struct {
                                otherwise 1); 1 one launch for
                                every pass; 2 first pass, then the open queries resumed in place;
                                3 first pass, then the open queries compacted into full waves */
-    int32_t reserved[2];
+    int32_t round_search;   /* 0: round-1 cone search, candidates reused by every round (the
+                               reference's default build, NodeEMST.h:960-1386); 1: per-round
+                               component search, each point searching for its nearest point in
+                               another component whenever its target has joined its own
+                               (NodeSpiralSearchMD, NodeEMST.h:67-509, selected in the reference by
+                               EMST_SPIRAL_SEARCH_KD_SLAB_2 0, SolutionEMST.h:55-56).  Both build
+                               the same forests.  1 has no cone candidates, no learned seed and no
+                               2.5D mode */
+    int32_t reserved[1];
 } emst_options;
 
 /* One row of the per-round trace: the columns of the reference's stats file

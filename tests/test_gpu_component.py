@@ -1,0 +1,101 @@
+"""The per-round component search (round_search = 1; NodeSpiralSearchMD,
+emst_coalition/include/NodeEMST.h:67-509, the reference's alternative to the
+round-1 cone search) against the oracle and the cone search: the same edge
+sets, per-round forest sizes and labels.  Each round's component minimum is
+the component's shortest outgoing edge under (d2, min id, max id) in both
+searches, so the forests must agree round by round."""
+import numpy as np
+import pytest
+
+from conftest import golden_path, read_points
+
+pytestmark = pytest.mark.gpu
+WTOL = 1e-12
+
+
+def run(pkg, pts, step=False, **kw):
+    with pkg.Calculateur(dim=pts.shape[1], trace=True, **kw) as c:
+        c.initialize(pts)
+        if step:
+            while c.activate():
+                pass
+        else:
+            c.run()
+        u, v, d = c.getLinks()
+        w, m = c.evaluate()
+        out = dict(e=np.stack([u, v], 1).astype(np.int64), d2=d, w=w, labels=c.labels(),
+                   sizes=[r["size"] for r in c.trace()])
+    return out
+
+
+@pytest.mark.parametrize("dim,kind,n,seed", [(2, 0, 5000, 31), (3, 0, 5000, 32), (3, 1, 5000, 33),
+                                             (2, 2, 5000, 34), (3, 2, 4000, 35), (2, 0, 9, 36)])
+def test_matches_oracle_round_by_round(pkg, oracle_mod, dim, kind, n, seed):
+    pts = pkg.synth(kind, n, dim, seed)
+    print("case", dim, kind, n, seed, flush=True)
+    g = run(pkg, pts, round_search=1)
+    ref = oracle_mod.emst(pts, mode=0)
+    assert np.array_equal(g["e"], oracle_mod.canon(ref["u"], ref["v"]))
+    assert g["sizes"] == ref["cum_edges"]
+    assert np.array_equal(g["labels"], ref["labels"])
+    assert abs(g["w"] - ref["weight"]) <= WTOL * ref["weight"]
+
+
+@pytest.mark.parametrize("name", ["3000randomCity.2d", "1000randomCity.2d", "2000randomCity.3d"])
+def test_reference_data(pkg, oracle_mod, name):
+    """The reference's own point files, stepped round by round."""
+    pts = read_points(golden_path(name))
+    g = run(pkg, pts, round_search=1, step=True)
+    ref = oracle_mod.emst(pts, mode=0)
+    assert np.array_equal(g["e"], oracle_mod.canon(ref["u"], ref["v"]))
+    assert g["sizes"] == ref["cum_edges"]
+
+
+@pytest.mark.parametrize("dim,kind,n,seed", [(3, 0, 200_000, 41), (2, 2, 200_000, 42), (3, 1, 150_000, 43)])
+def test_equals_cone_search(pkg, dim, kind, n, seed):
+    """Larger clouds: identical edges (and d2 bits) and per-round sizes as the cone search."""
+    pts = pkg.synth(kind, n, dim, seed)
+    print("case", dim, kind, n, seed, flush=True)
+    a = run(pkg, pts, round_search=1)
+    b = run(pkg, pts, round_search=0)
+    assert np.array_equal(a["e"], b["e"])
+    assert np.array_equal(a["d2"].view(np.uint32), b["d2"].view(np.uint32))
+    assert a["sizes"] == b["sizes"]
+    assert np.array_equal(a["labels"], b["labels"])
+
+
+def test_coincident_points(pkg, oracle_mod):
+    rng = np.random.default_rng(44)
+    base = rng.random((1200, 3), dtype=np.float32)
+    pts = rng.permutation(np.repeat(base, rng.integers(1, 6, len(base)), axis=0))
+    g = run(pkg, pts, round_search=1)
+    ref = oracle_mod.emst(pts, mode=0)
+    assert np.array_equal(g["e"], oracle_mod.canon(ref["u"], ref["v"]))
+    assert g["sizes"] == ref["cum_edges"]
+
+
+def test_batch_equals_single_clouds(pkg):
+    rng = np.random.default_rng(45)
+    clouds = [pkg.synth(int(rng.integers(0, 3)), int(rng.integers(2, 4000)), 3, 500 + i) for i in range(7)]
+    with pkg.Calculateur(dim=3, round_search=1) as c:
+        c.initialize_clouds(clouds)
+        c.run()
+        u, v, _ = c.getLinks()
+        off = c.offsets
+    for i, pts in enumerate(clouds):
+        s = run(pkg, pts, round_search=1)
+        sel = (u >= off[i]) & (u < off[i + 1])
+        eb = np.stack([u[sel] - off[i], v[sel] - off[i]], 1).astype(np.int64)
+        assert np.array_equal(eb, s["e"])
+
+
+def test_refusals(pkg):
+    pts = pkg.synth(0, 100, 3, 46)
+    with pkg.Calculateur(dim=3, round_search=1) as c:
+        c.initialize(pts)
+        with pytest.raises(pkg.EmstError):
+            c.candidates()
+        with pytest.raises(pkg.EmstError):
+            c.set_radius_model(np.zeros(pkg.MLP_PARAMS, np.float32))
+    with pytest.raises(pkg.EmstError):
+        pkg.Calculateur(dim=3, cone_dim=2, round_search=1)
