@@ -98,6 +98,8 @@ struct emst_ctx {
     DevBuf<uint32_t> cy_kl;
     DevBuf<uint8_t> cy_onc, cy_cut;
     DevBuf<int32_t> nlab;              // node component labels (per-round component search)
+    DevBuf<uint32_t> nmin;             // node minimum ids (per-round component search)
+    DevBuf<int32_t> list2;             // queries open after pass 0 (per-round component search)
     bool no_grouping = false;         // set when distinct points met at float distance 0
     bool grouped = false;
     bool groups_settled = false;      // no group spans two components any more
@@ -216,7 +218,11 @@ int alloc_all(emst_ctx* c)
     ENS(dup_list, n);
     ENS(dup_runs, n);
     ENS(dup_gslot, n);
-    if (comp) ENS(nlab, 2 * c->nleaves);
+    if (comp) {
+        ENS(nlab, 2 * c->nleaves);
+        ENS(nmin, 2 * c->nleaves);
+        ENS(list2, n);
+    }
     if (c->cdim < c->dim) {
         ENS(cy_j0, n);
         ENS(cy_roots, n);
@@ -320,6 +326,7 @@ SplitScratch split_scratch(emst_ctx* c)
     sc.off = c->open_cnt.p + ((c->n + 63) / 64 + 1);
     sc.tmp = c->open_tmp.p;
     sc.list = c->list.p;
+    sc.list2 = c->list2.p;
     sc.after = 0;
     sc.compact_frac = 0.10;
     return sc;
@@ -400,6 +407,8 @@ int build(emst_ctx* c)
     if (comp) {
         // the per-round component search has no round-1 search: every round
         // searches (launch_comp_round)
+        launch_node_min_ids(c->pts.p, c->dcl.p, c->ncl, c->nleaves, c->Lmax, c->bucket, c->nmin.p, s);
+        LAUNCHCHK(c);
         HIPCHK(c, hipEventRecord(c->ev[1], s));
         HIPCHK(c, hipEventRecord(c->ev[2], s));
         HIPCHK(c, hipEventRecord(c->ev[3], s));
@@ -537,8 +546,8 @@ void issue_round(emst_ctx* c, bool tr)
     const CycleState cy = cycle_state(c);
     for (int ph = 0; ph < 4; ++ph) {
         if (ph == 0 && c->opt.round_search == 1) {
-            launch_comp_round(c->dim, c->n, c->pts.p, c->nodes.p, c->nlab.p, c->dcl.p, c->ncl, c->nleaves, c->Lmax,
-                              c->bucket, c->label.p, c->corr.p, c->corr_key.p, c->comp1.p, c->inv.p, c->seed_scale,
+            launch_comp_round(c->dim, c->n, c->pts.p, c->nodes.p, c->nlab.p, c->nmin.p, c->dcl.p, c->ncl,
+                              c->nleaves, c->Lmax, c->bucket, c->label.p, c->corr.p, c->corr_key.p, c->comp1.p, c->inv.p, c->seed_scale,
                               split_scratch(c), c->C, s);
             if (tr) (void)hipEventRecord(c->ev[4], s);
             continue;
@@ -654,7 +663,7 @@ std::vector<uintptr_t> graph_key(emst_ctx* c)
     const DupState d = dup_state(c);
     const CycleState y = cycle_state(c);
     return {(uintptr_t)c->n, (uintptr_t)c->K, (uintptr_t)c->dim, (uintptr_t)c->cdim, (uintptr_t)y.j0, (uintptr_t)y.roots,
-            (uintptr_t)c->opt.round_search, (uintptr_t)c->nlab.p, (uintptr_t)c->open_mask.p, (uintptr_t)c->open_cnt.p,
+            (uintptr_t)c->opt.round_search, (uintptr_t)c->nlab.p, (uintptr_t)c->nmin.p, (uintptr_t)c->list2.p, (uintptr_t)c->open_mask.p, (uintptr_t)c->open_cnt.p,
             (uintptr_t)c->open_tmp.p, (uintptr_t)c->list.p, (uintptr_t)c->nodes.p, (uintptr_t)c->dcl.p,
             (uintptr_t)y.j[0], (uintptr_t)y.kh[0], (uintptr_t)y.kl[0], (uintptr_t)y.onc, (uintptr_t)y.cut,
             (uintptr_t)c->grouped, (uintptr_t)d.npoints,

@@ -154,6 +154,7 @@ struct SplitScratch {
     uint32_t* off;          // P + 1
     uint32_t* tmp;          // scan_tmp_words(P + 1)
     int32_t* list;          // n
+    int32_t* list2;         // n (component search: the queries left open after pass 0)
     int after;              // the first launch runs passes 0..after
     double compact_frac;    // resume from a compacted list when at most this share of the queries is open
 };
@@ -170,11 +171,18 @@ void launch_cone_search(int dim, int cdim, const float4* pts, const Node* nodes,
 // the round-1 cone search): node component labels (nlab: 2 * nleaves words),
 // kept targets, and a search for the points whose target joined their
 // component; writes corr / corr_key and the component minima (comp1).
-// sc: open_mask, cnt, off, tmp and list as for a split cone search.
-void launch_comp_round(int dim, int64_t n, const float4* pts, const Node* nodes, int32_t* nlab, const Cloud* cl,
-                       int ncl, int64_t nleaves, int64_t Lmax, int bucket, const int32_t* label, int32_t* corr,
-                       uint64_t* corr_key, uint64_t* comp1, const int32_t* inv, float seed_scale,
-                       const SplitScratch& sc, Counters* C, hipStream_t s);
+// nmin: per-node minimum point id (launch_node_min_ids).  sc: open_mask, cnt,
+// off, tmp, list and list2; pass 0 runs over every searching point, the
+// later passes over those it left open, so that they see the component
+// minima pass 0 found.
+void launch_comp_round(int dim, int64_t n, const float4* pts, const Node* nodes, int32_t* nlab,
+                       const uint32_t* nmin, const Cloud* cl, int ncl, int64_t nleaves, int64_t Lmax, int bucket,
+                       const int32_t* label, int32_t* corr, uint64_t* corr_key, uint64_t* comp1, const int32_t* inv,
+                       float seed_scale, const SplitScratch& sc, Counters* C, hipStream_t s);
+// Smallest point id under every node of the hierarchy (2 * nleaves words,
+// laid out as the nodes; 0xffffffff for a node without points).
+void launch_node_min_ids(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, int64_t Lmax, int bucket,
+                         uint32_t* nmin, hipStream_t s);
 // Cones of every group point for the current labels (dim: the cone dimension) (after the grouped
 // cone search, and at the start of each round while a group may still span
 // several components).  Sets C->dup_split when one does.

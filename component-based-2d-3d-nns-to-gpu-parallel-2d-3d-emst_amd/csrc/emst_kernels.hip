@@ -1144,6 +1144,18 @@ __global__ __launch_bounds__(64) void k_open_fill(const uint64_t* __restrict__ o
     if ((m >> lane) & 1ull) list[off[b] + (uint32_t)__popcll(m & lanemask_lt())] = (int32_t)(b * 64 + lane);
 }
 
+// Same, for packets of a query list: dst receives src's entries whose lanes
+// are flagged.
+__global__ __launch_bounds__(64) void k_open_fill_map(const uint64_t* __restrict__ open_mask,
+                                                     const uint32_t* __restrict__ off,
+                                                     const int32_t* __restrict__ src, int32_t* __restrict__ dst)
+{
+    const int64_t b = blockIdx.x;
+    const uint64_t m = open_mask[b];
+    const int lane = __lane_id();
+    if ((m >> lane) & 1ull) dst[off[b] + (uint32_t)__popcll(m & lanemask_lt())] = src[b * 64 + lane];
+}
+
 // Start of a group refresh: m2 = ~0 for every group, dup_split = 0.
 __global__ __launch_bounds__(kBlock) void k_dup_reset(Counters* __restrict__ C, DupState dup)
 {
@@ -1668,18 +1680,53 @@ __global__ __launch_bounds__(kBlock) void k_nlab_level(int32_t* __restrict__ nla
     base[i] = a == kNlabEmpty ? b : (b == kNlabEmpty ? a : (a == b ? a : kNlabMixed));
 }
 
-// keep-previous (NodeEMST.h:160-169) and the component minimum of the kept
-// targets; the points that must search are flagged per 64-position packet
-__global__ __launch_bounds__(kBlock) void k_comp_prep(int64_t n, const float4* __restrict__ pts,
+// smallest point id under every leaf, then every level (static per build):
+// with ties on d2 broken by id, a node at the bound can only matter when it
+// holds a smaller id than the best so far (runs of coincident points)
+__global__ __launch_bounds__(kBlock) void k_nmin_leaves(const float4* __restrict__ pts,
+                                                       const Cloud* __restrict__ cl, int ncl, int64_t nleaves,
+                                                       int bucket, uint32_t* __restrict__ nmin)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nleaves) return;
+    const int c = find_cloud_leaf(cl, ncl, g);
+    const int64_t j = g - cl[c].lbase;
+    const int64_t s0 = cl[c].pos + j * bucket;
+    const int64_t s1 = min(cl[c].pos + cl[c].n, s0 + bucket);
+    uint32_t m = 0xffffffffu;
+    for (int64_t s = s0; s < s1; ++s) m = min(m, __float_as_uint(pts[s].w));
+    nmin[2 * cl[c].lbase + cl[c].L + j] = m;
+}
+
+__global__ __launch_bounds__(kBlock) void k_nmin_level(uint32_t* __restrict__ nmin, const Cloud* __restrict__ cl,
+                                                      int ncl, int64_t lo)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)ncl * lo) return;
+    const int c = (int)(t / lo);
+    if (lo >= cl[c].L) return;
+    uint32_t* base = nmin + 2 * cl[c].lbase;
+    const int64_t i = lo + t % lo;
+    base[i] = min(base[2 * i], base[2 * i + 1]);
+}
+
+// Per point, corr / corr_key between rounds:
+//   corr >= 0   the exact nearest point outside the component at its last
+//               search, key (d2, id); kept while it stays outside
+//               (NodeEMST.h:160-169);
+//   corr == -1  no exact result; the high word of corr_key is a lower bound
+//               on the d2 of the nearest outside point (all ones: none);
+//   corr == -2  nothing outside the component in its cloud (final).
+// Keep-previous: the component minimum of the kept targets.
+__global__ __launch_bounds__(kBlock) void k_comp_keep(int64_t n, const float4* __restrict__ pts,
                                                      const int32_t* __restrict__ label,
                                                      const int32_t* __restrict__ corr,
                                                      const uint64_t* __restrict__ corr_key,
-                                                     uint64_t* __restrict__ comp1,
-                                                     uint64_t* __restrict__ open_mask, const Counters* C)
+                                                     uint64_t* __restrict__ comp1, const Counters* C)
 {
     if (C->done) return;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool need = false, valid = false;
+    bool valid = false;
     int32_t lp = 0;
     uint64_t key1 = kEmptyKey;
     if (p < n) {
@@ -1690,11 +1737,34 @@ __global__ __launch_bounds__(kBlock) void k_comp_prep(int64_t n, const float4* _
             const uint32_t pid = __float_as_uint(pts[p].w), qid = (uint32_t)ck;
             key1 = (ck & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
             valid = true;
-        } else {
-            need = c != -2;         // -2: nothing outside the component (its cloud is one tree)
         }
     }
     atomic_min_grouped(comp1, lp, key1, valid);
+}
+
+// The points that must search, flagged per 64-position packet: those without
+// a kept target whose lower bound does not already exceed the component's
+// kept minimum (such a point cannot hold the component's shortest edge this
+// round; it keeps its bound for the next).
+__global__ __launch_bounds__(kBlock) void k_comp_prep(int64_t n, const int32_t* __restrict__ label,
+                                                     int32_t* __restrict__ corr,
+                                                     const uint64_t* __restrict__ corr_key,
+                                                     const uint64_t* __restrict__ comp1,
+                                                     uint64_t* __restrict__ open_mask, const Counters* C)
+{
+    if (C->done) return;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool need = false;
+    if (p < n) {
+        const int32_t lp = label[p];
+        const int32_t c = corr[p];
+        if (c == -1 || (c >= 0 && label[c] == lp)) {
+            const uint32_t lb = (uint32_t)(corr_key[p] >> 32);
+            const int32_t lbb = lb == 0xffffffffu ? -1 : (int32_t)lb;
+            need = lbb <= key_bound(comp1[lp]);
+            if (!need) corr[p] = -1;    // the absorbed target's d2 stays as the bound
+        }
+    }
     const uint64_t m = __ballot(need);
     if (__lane_id() == 0 && p < n) open_mask[p / 64] = m;
 }
@@ -1702,19 +1772,27 @@ __global__ __launch_bounds__(kBlock) void k_comp_prep(int64_t n, const float4* _
 // nearest point in another component, for the points of qlist (count in
 // nq_dev): packet traversal as in the cone search, one best per query, by
 // iterative deepening from the seed radius R_0 (capped passes R_k = 2^k R_0,
-// then one uncapped).  Points closer than the previous target's distance were
-// in the component at that search and still are, so they are skipped.
+// then one uncapped).  Points below the point's lower bound (its previous
+// target's d2) were in the component at that search and still are, so they
+// are skipped.  The component's shortest edge found so far (comp1, read at
+// each pass) also bounds the search: a point whose nearest outside point lies
+// beyond it cannot hold the component's edge, and it stops with that d2 as
+// its lower bound (corr = -1).  Exact results enter comp1 as they close.
 template <int D>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_comp_search(
     const float4* __restrict__ pts, const Node* __restrict__ nodes, const int32_t* __restrict__ nlab,
     const Cloud* __restrict__ clouds, int ncl, int bucket, const int32_t* __restrict__ qlist,
     const uint32_t* __restrict__ nq_dev, const int32_t* __restrict__ label, const int32_t* __restrict__ inv,
     float seed_scale, int32_t* __restrict__ corr, uint64_t* __restrict__ corr_key,
-    uint64_t* __restrict__ comp1, const Counters* C)
+    uint64_t* __restrict__ comp1, const uint32_t* __restrict__ nmin, int pass_lo, int pass_hi,
+    uint64_t* __restrict__ open_out, const Counters* C)
 {
     if (C->done) return;
     const int64_t nq = *nq_dev;
-    if ((int64_t)blockIdx.x * 64 >= nq) return;
+    if ((int64_t)blockIdx.x * 64 >= nq) {
+        if (open_out != nullptr && threadIdx.x == 0) open_out[blockIdx.x] = 0ull;
+        return;
+    }
     __shared__ unsigned long long best[64];
     __shared__ int32_t stk_id[64];
     __shared__ int32_t stk_ver[64];
@@ -1729,6 +1807,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     const uint64_t prev = active ? corr_key[p] : kEmptyKey;
     best[lane] = kEmptyKey;
     const int my_cl = active ? find_cloud(clouds, ncl, p) : -1;
+    const uint32_t pid = __float_as_uint(me.w);
+    bool bounded = false, left_open = false;
+    int32_t lbnd = 0;
     uint64_t pending = __ballot(active);
     while (pending) {
         const int cl = __builtin_amdgcn_readfirstlane(__shfl(my_cl, (int)__builtin_ctzll(pending)));
@@ -1738,6 +1819,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         const int64_t L = clouds[cl].L;
         const Node* __restrict__ tree = nodes + 2 * clouds[cl].lbase;
         const int32_t* __restrict__ tlab = nlab + 2 * clouds[cl].lbase;
+        const uint32_t* __restrict__ tmin = nmin + 2 * clouds[cl].lbase;
         // seed: twice the previous target's distance, at least seed_scale
         // times the mean spacing of the point's leaf cell
         float r = 1.f;
@@ -1756,13 +1838,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             float sp = vol > 0.f ? powf(vol / cnt, 1.f / D) : ext / cnt;
             if (!(sp > 0.f)) sp = 1e-6f;
             r = seed_scale * sp;
-            if (prev != kEmptyKey) {
+            if ((uint32_t)(prev >> 32) != 0xffffffffu) {
                 const uint32_t dbits = (uint32_t)(prev >> 32);
                 inner = (int32_t)dbits - 1;
                 r = fmaxf(r, 2.f * sqrtf(__uint_as_float(dbits)));
             }
         }
         float r2 = r * r;
+        // a launch that resumes at pass_lo: the state the earlier passes left
+        for (int k = 0; k < pass_lo; ++k) {
+            inner = (int32_t)__float_as_uint(r2);
+            r2 *= 4.f;
+        }
         bool open = mine;
         float cx = mine ? px : 0.f, cy = mine ? py : 0.f, cz = mine ? pz : 0.f;
         float na = mine ? 1.f : 0.f;
@@ -1779,10 +1866,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             const float dzz = fmaxf(fmaxf(x.lo[2] - cz, cz - x.hi[2]), 0.f);
             return dxx * dxx + dyy * dyy + dzz * dzz;
         };
-        // a box is needed when it holds another component and its nearest
-        // point may beat the bound (ties kept: they can still win on id)
-        // and its farthest point lies beyond `inner`
-        auto needed = [&](const Node& nd, int32_t nl, int32_t bnd) {
+        // a box is needed when it holds another component, its nearest point
+        // is within the limit and may beat the best so far under (d2, id)
+        // (at equal d2 only with a smaller id), and its farthest point lies
+        // beyond `inner`
+        auto needed = [&](const Node& nd, int32_t nl, uint32_t nm, int32_t lim, uint64_t bk) {
             if (!open || nl == mylab || nl == kNlabEmpty) return false;
             float m[3], u[3];
             const float lo[3] = {nd.lo[0] - px, nd.lo[1] - py, nd.lo[2] - pz};
@@ -1794,13 +1882,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             }
             const float lb = D == 3 ? d2_of(m[0], m[1], m[2]) : d2_of(m[0], m[1]);
             const float far = D == 3 ? d2_of(u[0], u[1], u[2]) : d2_of(u[0], u[1]);
-            return (int32_t)__float_as_uint(lb) <= bnd && (int32_t)__float_as_uint(far) > inner;
+            const uint32_t lbb = __float_as_uint(lb);
+            return (int32_t)lbb <= lim && (((uint64_t)lbb << 32) | nm) < bk &&
+                   (int32_t)__float_as_uint(far) > inner;
         };
-        for (int pass = 0; pass <= kMaxCappedPasses; ++pass) {
-            if (__ballot(open) == 0ull) break;
+        for (int pass = pass_lo; pass <= pass_hi; ++pass) {
             const bool last = pass == kMaxCappedPasses;
             const int32_t cap = last ? 0x7fffffff : (int32_t)__float_as_uint(r2);
-            int32_t bnd = min(key_bound(best[lane]), cap);
+            const int32_t cbb = open ? key_bound(comp1[mylab]) : 0x7fffffff;
+            // nothing outside the component lies within `inner`: with the
+            // component's edge at most that long, the point cannot hold it
+            if (open && cbb <= inner) {
+                open = false;
+                bounded = true;
+                lbnd = inner;
+            }
+            if (__ballot(open) == 0ull) break;
+            const int32_t lim = min(cap, cbb);
+            uint64_t bk = best[lane];
+            int32_t bnd = min(key_bound(bk), lim);
             int sp = 0;
             int ver = 0;
             if (lane == 0) { stk_id[0] = 1; stk_ver[0] = -1; }
@@ -1814,7 +1914,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                 if (pver == ver) {
                     need = (stk_mask[sp] >> lane) & 1ull;
                 } else {
-                    need = needed(tree[node], tlab[node], bnd);
+                    need = needed(tree[node], tlab[node], tmin[node], lim, bk);
                 }
                 const uint64_t needm = __ballot(need);
                 if (needm == 0ull) continue;
@@ -1848,7 +1948,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                             atomicMin(&best[myq],
                                       ((unsigned long long)__float_as_uint(d2) << 32) | __float_as_uint(v.w));
                     }
-                    if (need) bnd = min(key_bound(best[lane]), cap);
+                    if (need) {
+                        bk = best[lane];
+                        bnd = min(key_bound(bk), lim);
+                    }
                     ++ver;
                 } else {
                     const bool leafkids = 2 * (int64_t)node >= L;
@@ -1856,23 +1959,28 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                     const int nk = leafkids ? 2 : 4;
                     Node bx[4];
                     int32_t bl[4];
+                    uint32_t bm[4];
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
                         if (i < nk) {
                             bx[i] = tree[first + i];
                             bl[i] = tlab[first + i];
+                            bm[i] = tmin[first + i];
                         }
                     uint64_t mk[4];
                     float dk[4];
                     int id[4];
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        mk[i] = i < nk ? __ballot(needed(bx[i], bl[i], bnd)) : 0ull;
+                        mk[i] = i < nk ? __ballot(needed(bx[i], bl[i], bm[i], lim, bk)) : 0ull;
                         dk[i] = mk[i] ? cdist(bx[i]) : -1.f;
                         id[i] = first + i;
                     }
+                    // pushed farthest first, so the nearest pops first; at
+                    // equal distance the lower node (smaller ids in a run of
+                    // coincident points) pops first
                     auto ce = [&](int a, int b) {
-                        if (dk[a] < dk[b]) {
+                        if (dk[a] < dk[b] || (dk[a] == dk[b] && id[a] < id[b])) {
                             const float td = dk[a]; dk[a] = dk[b]; dk[b] = td;
                             const uint64_t tm = mk[a]; mk[a] = mk[b]; mk[b] = tm;
                             const int ti = id[a]; id[a] = id[b]; id[b] = ti;
@@ -1890,25 +1998,45 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                     __builtin_amdgcn_wave_barrier();
                 }
             }
-            if (open && key_bound(best[lane]) <= cap) open = false;
+            // everything outside the component up to min(best, lim) has been
+            // seen: best is exact when within lim; else nothing lies within
+            // lim, and with lim = cbb the point stops on its lower bound
+            bool fresh = false;
+            uint64_t key1 = kEmptyKey;
+            if (open) {
+                const uint64_t kb = best[lane];
+                if (key_bound(kb) <= lim) {
+                    open = false;
+                    if (kb != kEmptyKey) {
+                        fresh = true;
+                        key1 = (kb & 0xffffffff00000000ull) | (uint64_t)min(pid, (uint32_t)kb);
+                    }
+                } else if (cbb <= cap) {
+                    open = false;
+                    bounded = true;
+                    lbnd = cbb;
+                }
+            }
+            atomic_min_grouped(comp1, mylab, key1, fresh);
             inner = cap;
             r2 *= 4.f;
         }
+        if (mine) left_open = open;
     }
-    // the point's result, and the first phase of its component's minimum
-    const uint64_t k = best[lane];
-    bool valid = false;
-    uint64_t key1 = kEmptyKey;
-    if (active) {
-        corr[p] = k == kEmptyKey ? -2 : inv[(uint32_t)k];
-        corr_key[p] = k;
-        if (k != kEmptyKey) {
-            const uint32_t pid = __float_as_uint(me.w), qid = (uint32_t)k;
-            key1 = (k & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
-            valid = true;
+    if (open_out != nullptr) {
+        const uint64_t m = __ballot(left_open);
+        if (lane == 0) open_out[blockIdx.x] = m;
+    }
+    if (active && !left_open) {
+        const uint64_t k = best[lane];
+        if (bounded) {
+            corr[p] = -1;
+            corr_key[p] = ((uint64_t)(uint32_t)lbnd << 32) | 0xffffffffull;
+        } else {
+            corr[p] = k == kEmptyKey ? -2 : inv[(uint32_t)k];
+            corr_key[p] = k;
         }
     }
-    atomic_min_grouped(comp1, mylab, key1, valid);
 }
 
 // ---- cycle rule of the 2.5D mode (CycleState).  Roots are the points whose
@@ -2382,10 +2510,19 @@ void launch_cone_search(int dim, int cdim, const float4* pts, const Node* nodes,
                        (int64_t)(split->compact_frac * (double)nq));
 }
 
-void launch_comp_round(int dim, int64_t n, const float4* pts, const Node* nodes, int32_t* nlab, const Cloud* cl,
-                       int ncl, int64_t nleaves, int64_t Lmax, int bucket, const int32_t* label, int32_t* corr,
-                       uint64_t* corr_key, uint64_t* comp1, const int32_t* inv, float seed_scale,
-                       const SplitScratch& sc, Counters* C, hipStream_t s)
+void launch_node_min_ids(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, int64_t Lmax, int bucket,
+                         uint32_t* nmin, hipStream_t s)
+{
+    if (nleaves <= 0) return;
+    hipLaunchKernelGGL(k_nmin_leaves, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, pts, cl, ncl, nleaves, bucket, nmin);
+    for (int64_t lo = Lmax / 2; lo >= 1; lo /= 2)
+        hipLaunchKernelGGL(k_nmin_level, dim3(nblocks((int64_t)ncl * lo)), dim3(kBlock), 0, s, nmin, cl, ncl, lo);
+}
+
+void launch_comp_round(int dim, int64_t n, const float4* pts, const Node* nodes, int32_t* nlab,
+                       const uint32_t* nmin, const Cloud* cl, int ncl, int64_t nleaves, int64_t Lmax, int bucket,
+                       const int32_t* label, int32_t* corr, uint64_t* corr_key, uint64_t* comp1, const int32_t* inv,
+                       float seed_scale, const SplitScratch& sc, Counters* C, hipStream_t s)
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_nlab_leaves, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, label, cl, ncl, nleaves, bucket,
@@ -2394,14 +2531,25 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, const Node* nodes,
         hipLaunchKernelGGL(k_nlab_level, dim3(nblocks((int64_t)ncl * lo)), dim3(kBlock), 0, s, nlab, cl, ncl, lo,
                            (const Counters*)C);
     const int64_t P = (n + 63) / 64;
-    hipLaunchKernelGGL(k_comp_prep, dim3(nblocks(n)), dim3(kBlock), 0, s, n, pts, label, corr, corr_key, comp1,
-                       sc.open_mask, (const Counters*)C);
+    hipLaunchKernelGGL(k_comp_keep, dim3(nblocks(n)), dim3(kBlock), 0, s, n, pts, label, (const int32_t*)corr,
+                       (const uint64_t*)corr_key, comp1, (const Counters*)C);
+    hipLaunchKernelGGL(k_comp_prep, dim3(nblocks(n)), dim3(kBlock), 0, s, n, label, corr, (const uint64_t*)corr_key,
+                       (const uint64_t*)comp1, sc.open_mask, (const Counters*)C);
     hipLaunchKernelGGL(k_open_count, dim3(nblocks(P + 1)), dim3(kBlock), 0, s, sc.open_mask, P, sc.cnt);
     exclusive_scan_u32(sc.cnt, sc.off, P + 1, sc.tmp, s);
     hipLaunchKernelGGL(k_open_fill, dim3((unsigned)P), dim3(64), 0, s, sc.open_mask, sc.off, sc.list);
-    hipLaunchKernelGGL(dim == 3 ? k_comp_search<3> : k_comp_search<2>, dim3((unsigned)P), dim3(64), 0, s, pts, nodes,
-                       (const int32_t*)nlab, cl, ncl, bucket, (const int32_t*)sc.list, (const uint32_t*)(sc.off + P),
-                       label, inv, seed_scale, corr, corr_key, comp1, (const Counters*)C);
+    auto kern = dim == 3 ? k_comp_search<3> : k_comp_search<2>;
+    // pass 0 over the searching points; their open packets' lanes into open_mask
+    hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, nodes, (const int32_t*)nlab, cl, ncl, bucket,
+                       (const int32_t*)sc.list, (const uint32_t*)(sc.off + P), label, inv, seed_scale, corr, corr_key,
+                       comp1, nmin, 0, 0, sc.open_mask, (const Counters*)C);
+    hipLaunchKernelGGL(k_open_count, dim3(nblocks(P + 1)), dim3(kBlock), 0, s, sc.open_mask, P, sc.cnt);
+    exclusive_scan_u32(sc.cnt, sc.off, P + 1, sc.tmp, s);
+    hipLaunchKernelGGL(k_open_fill_map, dim3((unsigned)P), dim3(64), 0, s, sc.open_mask, sc.off,
+                       (const int32_t*)sc.list, sc.list2);
+    hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, nodes, (const int32_t*)nlab, cl, ncl, bucket,
+                       (const int32_t*)sc.list2, (const uint32_t*)(sc.off + P), label, inv, seed_scale, corr, corr_key,
+                       comp1, nmin, 1, kMaxCappedPasses, (uint64_t*)nullptr, (const Counters*)C);
 }
 
 void launch_dup_fill(int dim, int64_t n, const int32_t* label, const float4* pts, const int32_t* inv,

@@ -99,3 +99,16 @@ def test_refusals(pkg):
             c.set_radius_model(np.zeros(pkg.MLP_PARAMS, np.float32))
     with pytest.raises(pkg.EmstError):
         pkg.Calculateur(dim=3, cone_dim=2, round_search=1)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_large_coincident_groups_equal_cone_search(pkg, dim):
+    """Runs of thousands of identical points (ties at d2 = 0 broken on ids)."""
+    rng = np.random.default_rng(47 + dim)
+    base = rng.random((3000, dim), dtype=np.float32)
+    big = np.repeat(base[:3], [6000, 2500, 40], axis=0)
+    pts = rng.permutation(np.concatenate([base, big, base[10:400]]))
+    a = run(pkg, pts, round_search=1)
+    b = run(pkg, pts, round_search=0)
+    assert np.array_equal(a["e"], b["e"])
+    assert a["sizes"] == b["sizes"]
