@@ -98,12 +98,12 @@ def main():
 
     cdim = args.cone_dim or args.dim
     K = 2 * cdim * (1 << (cdim - 1))
-    model_path = os.path.join(ge.PKG_DIR, "models", f"radius_{args.dim}d.npy")
-    model = np.load(model_path) if (os.path.exists(model_path) and not args.plain and cdim == args.dim
-                                    and not args.round_search) else None
+    tag = "comp_" if args.round_search else ""
+    model_path = os.path.join(ge.PKG_DIR, "models", f"radius_{tag}{args.dim}d.npy")
+    model = np.load(model_path) if (os.path.exists(model_path) and not args.plain and cdim == args.dim) else None
     model_clouds = None
     if model is not None:
-        meta = os.path.join(ge.PKG_DIR, "models", "train_radius.json")
+        meta = os.path.join(ge.PKG_DIR, "models", f"train_radius{'_comp' if tag else ''}.json")
         if os.path.exists(meta):
             model_clouds = json.load(open(meta)).get("clouds")
 
@@ -132,7 +132,7 @@ def main():
         dt = time.perf_counter() - t0
         u, v, _ = ctx.getLinks()
         st = pkg.search_stats(ctx)
-        samples = ctx.radius_samples() if form_model is not None else None
+        samples = ctx.radius_samples() if (form_model is not None and not args.round_search) else None
         ctx.close()
         assert m == args.n - 1, f"tree has {m} edges"
         med = lambda xs: sorted(xs)[len(xs) // 2]
@@ -146,7 +146,7 @@ def main():
         assert learned["w"] == plain["w"]
     head = learned if learned is not None else plain
     seed_report = None
-    if learned is not None:
+    if learned is not None and not args.round_search:
         # hit rate: points whose seed already covers every non-empty cone's
         # nearest point (pass 0 closes them all); savings: traversal work
         X, T = learned["samples"]

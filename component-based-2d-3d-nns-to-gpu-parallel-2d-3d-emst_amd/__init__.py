@@ -47,7 +47,7 @@ class Options(ctypes.Structure):
                 ("trace", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("radius_mode", ctypes.c_int32), ("seed_scale", ctypes.c_float),
                 ("cone_dim", ctypes.c_int32), ("search_split", ctypes.c_int32), ("round_search", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 1)]
+                ("log_samples", ctypes.c_int32)]
 
 
 class RoundTrace(ctypes.Structure):
@@ -142,7 +142,7 @@ class Calculateur:
     """One EMST context on one device (CalculateurEMST / SolutionEMST)."""
 
     def __init__(self, dim=3, level_radius=1, max_rounds=100, bucket_size=0, trace=False, device=0,
-                 seed_scale=0.0, cone_dim=0, search_split=0, round_search=0):
+                 seed_scale=0.0, cone_dim=0, search_split=0, round_search=0, log_samples=0):
         """cone_dim=2 with dim=3: the 2.5D mode (SolutionEMST<3,2>, 3D points,
         wedges over the xy plane)."""
         L = lib()
@@ -154,6 +154,7 @@ class Calculateur:
         o.cone_dim = cone_dim
         o.search_split = search_split
         o.round_search = round_search
+        o.log_samples = log_samples
         self._L = L
         self._c = L.emst_create(ctypes.byref(o))
         if not self._c:
@@ -235,7 +236,7 @@ class Calculateur:
         self._chk(self._L.emst_get_options(self._c, ctypes.byref(o)), "options")
         return {"level_radius": o.level_radius, "max_rounds": o.max_rounds, "bucket_size": o.bucket_size,
                 "trace": bool(o.trace), "device": o.device, "seed_scale": o.seed_scale, "cone_dim": o.cone_dim,
-                "search_split": o.search_split, "round_search": o.round_search}
+                "search_split": o.search_split, "round_search": o.round_search, "log_samples": o.log_samples}
 
     def cloud_results(self):
         """Per cloud: (FP64 weight, edge count, completion round)."""
@@ -320,7 +321,9 @@ class Calculateur:
         self._chk(self._L.emst_set_radius_model(self._c, _p(m)), "radius model")
 
     def radius_samples(self):
-        """(features n x 8, targets n) of the last cone search."""
+        """(features m x 8, targets m): every point's, from the last cone search
+        (round_search 0), or those logged by the last component-search run
+        (round_search 1 with log_samples=1)."""
         m = ctypes.c_int64(0)
         self._chk(self._L.emst_get_radius_samples(self._c, None, None, ctypes.byref(m)), "samples")
         X = np.empty((m.value, 8), np.float32)

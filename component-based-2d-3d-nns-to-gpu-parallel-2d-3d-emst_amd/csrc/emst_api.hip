@@ -99,7 +99,11 @@ struct emst_ctx {
     DevBuf<uint8_t> cy_onc, cy_cut;
     DevBuf<int32_t> nlab;              // node component labels (per-round component search)
     DevBuf<uint32_t> nmin;             // node minimum ids (per-round component search)
+    DevBuf<unsigned long long> visit;  // arrival state of the bottom-up node passes
     DevBuf<int32_t> list2;             // queries open after pass 0 (per-round component search)
+    DevBuf<int32_t> csize;             // component sizes at the roots (learned component seed)
+    DevBuf<uint32_t> cbox, nq_keep;    // component boxes at the roots; searching points' count
+    DevBuf<float> samples;             // logged (features, target) pairs of the component search
     bool no_grouping = false;         // set when distinct points met at float distance 0
     bool grouped = false;
     bool groups_settled = false;      // no group spans two components any more
@@ -221,6 +225,7 @@ int alloc_all(emst_ctx* c)
     if (comp) {
         ENS(nlab, 2 * c->nleaves);
         ENS(nmin, 2 * c->nleaves);
+        ENS(visit, 2 * c->nleaves);
         ENS(list2, n);
     }
     if (c->cdim < c->dim) {
@@ -317,6 +322,49 @@ CycleState cycle_state(emst_ctx* c)
     return y;
 }
 
+constexpr int64_t kSampleCap = 1 << 20;         // logged pairs per run
+constexpr int64_t kSamplesPerRound = 1 << 16;   // at most, per round
+
+// the learned seed of the component search: buffers when a model is set or
+// pairs are logged
+bool comp_features_on(const emst_ctx* c)
+{
+    return c->opt.round_search == 1 && (c->learned || c->opt.log_samples == 1);
+}
+
+int ensure_comp_seed(emst_ctx* c)
+{
+    if (!comp_features_on(c)) return EMST_OK;
+    const int64_t n = c->n;
+    int rc;
+    if ((rc = ensure(c, c->csize, (size_t)n)) || (rc = ensure(c, c->cbox, (size_t)n * 6)) ||
+        (rc = ensure(c, c->feat, (size_t)n * 8)) || (rc = ensure(c, c->ls, (size_t)n)) ||
+        (rc = ensure(c, c->pred, (size_t)n)) || (rc = ensure(c, c->seed, (size_t)n)) ||
+        (rc = ensure(c, c->nq_keep, 2)))
+        return rc;
+    if (c->opt.log_samples == 1 && (rc = ensure(c, c->samples, (size_t)kSampleCap * 9))) return rc;
+    return EMST_OK;
+}
+
+CompSeed comp_seed(emst_ctx* c)
+{
+    CompSeed cs{};
+    cs.features_on = comp_features_on(c);
+    if (!cs.features_on) return cs;
+    cs.csize = c->csize.p;
+    cs.cbox = c->cbox.p;
+    cs.X = c->feat.p;
+    cs.LS = c->ls.p;
+    cs.Y = c->pred.p;
+    cs.R = c->seed.p;
+    cs.model = c->learned ? c->model.p : nullptr;
+    cs.samples = c->opt.log_samples == 1 ? c->samples.p : nullptr;
+    cs.cap = kSampleCap;
+    cs.stride = std::max<int64_t>(1, (c->n + kSamplesPerRound - 1) / kSamplesPerRound);
+    cs.nq_keep = c->nq_keep.p;
+    return cs;
+}
+
 // the split-search buffers, also the per-round component search's query list
 SplitScratch split_scratch(emst_ctx* c)
 {
@@ -407,7 +455,11 @@ int build(emst_ctx* c)
     if (comp) {
         // the per-round component search has no round-1 search: every round
         // searches (launch_comp_round)
-        launch_node_min_ids(c->pts.p, c->dcl.p, c->ncl, c->nleaves, c->Lmax, c->bucket, c->nmin.p, s);
+        HIPCHK(c, hipMemsetAsync(c->visit.p, 0, sizeof(unsigned long long) * 2 * c->nleaves, s));
+        launch_node_min_ids(c->pts.p, c->dcl.p, c->ncl, c->nleaves, c->bucket, c->nmin.p, c->visit.p, s);
+        int rc1 = ensure_comp_seed(c);
+        if (rc1) return rc1;
+        if (comp_features_on(c)) launch_comp_seed_init(n, c->pts.p, c->csize.p, c->cbox.p, s);
         LAUNCHCHK(c);
         HIPCHK(c, hipEventRecord(c->ev[1], s));
         HIPCHK(c, hipEventRecord(c->ev[2], s));
@@ -546,9 +598,9 @@ void issue_round(emst_ctx* c, bool tr)
     const CycleState cy = cycle_state(c);
     for (int ph = 0; ph < 4; ++ph) {
         if (ph == 0 && c->opt.round_search == 1) {
-            launch_comp_round(c->dim, c->n, c->pts.p, c->nodes.p, c->nlab.p, c->nmin.p, c->dcl.p, c->ncl,
-                              c->nleaves, c->Lmax, c->bucket, c->label.p, c->corr.p, c->corr_key.p, c->comp1.p, c->inv.p, c->seed_scale,
-                              split_scratch(c), c->C, s);
+            launch_comp_round(c->dim, c->n, c->pts.p, c->nodes.p, c->nlab.p, c->nmin.p, c->visit.p, c->dcl.p, c->ncl,
+                              c->nleaves, c->bucket, c->label.p, c->corr.p, c->corr_key.p, c->comp1.p,
+                              c->inv.p, c->seed_scale, split_scratch(c), comp_seed(c), c->eu.p, c->eround.p, c->C, s);
             if (tr) (void)hipEventRecord(c->ev[4], s);
             continue;
         }
@@ -663,7 +715,10 @@ std::vector<uintptr_t> graph_key(emst_ctx* c)
     const DupState d = dup_state(c);
     const CycleState y = cycle_state(c);
     return {(uintptr_t)c->n, (uintptr_t)c->K, (uintptr_t)c->dim, (uintptr_t)c->cdim, (uintptr_t)y.j0, (uintptr_t)y.roots,
-            (uintptr_t)c->opt.round_search, (uintptr_t)c->nlab.p, (uintptr_t)c->nmin.p, (uintptr_t)c->list2.p, (uintptr_t)c->open_mask.p, (uintptr_t)c->open_cnt.p,
+            (uintptr_t)c->opt.round_search, (uintptr_t)comp_features_on(c), (uintptr_t)c->learned,
+            (uintptr_t)c->model.p, (uintptr_t)c->csize.p, (uintptr_t)c->cbox.p, (uintptr_t)c->feat.p, (uintptr_t)c->ls.p,
+            (uintptr_t)c->pred.p, (uintptr_t)c->seed.p, (uintptr_t)c->samples.p, (uintptr_t)c->nq_keep.p,
+            (uintptr_t)c->opt.log_samples, (uintptr_t)c->nlab.p, (uintptr_t)c->nmin.p, (uintptr_t)c->visit.p, (uintptr_t)c->list2.p, (uintptr_t)c->open_mask.p, (uintptr_t)c->open_cnt.p,
             (uintptr_t)c->open_tmp.p, (uintptr_t)c->list.p, (uintptr_t)c->nodes.p, (uintptr_t)c->dcl.p,
             (uintptr_t)y.j[0], (uintptr_t)y.kh[0], (uintptr_t)y.kl[0], (uintptr_t)y.onc, (uintptr_t)y.cut,
             (uintptr_t)c->grouped, (uintptr_t)d.npoints,
@@ -753,6 +808,10 @@ emst_ctx* emst_create(const emst_options* opt)
     }
     if (o.round_search == 1 && o.cone_dim == 2 && o.dim == 3) {
         g_create_error = "the 2.5D mode needs the cone search (round_search 0)";
+        return nullptr;
+    }
+    if (o.log_samples != 0 && o.log_samples != 1) {
+        g_create_error = "log_samples must be 0 or 1";
         return nullptr;
     }
     if (o.search_split < 0 || o.search_split > 3) {
@@ -1187,8 +1246,6 @@ int emst_set_radius_model(emst_ctx* c, const float* model)
     }
     if (c->opt.cone_dim == 2 && c->opt.dim == 3)
         return fail(c, EMST_E_ARG, "the learned seed radius is not defined for the 2.5D mode");
-    if (c->opt.round_search == 1)
-        return fail(c, EMST_E_ARG, "the learned seed radius seeds the cone search (round_search 0)");
     int rc = ensure(c, c->model, EMST_MLP_PARAMS);
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(c->model.p, model, sizeof(float) * EMST_MLP_PARAMS, hipMemcpyHostToDevice, c->stream));
@@ -1205,7 +1262,26 @@ int emst_get_radius_samples(emst_ctx* c, float* feat, float* target, int64_t* m)
     if (rc) return rc;
     if (c->ncl != 1) return fail(c, EMST_E_STATE, "radius samples come from a single-cloud context");
     if (c->cdim != c->dim) return fail(c, EMST_E_STATE, "radius samples come from full-dimensional cones");
-    if (c->opt.round_search == 1) return fail(c, EMST_E_STATE, "radius samples come from the cone search");
+    if (c->opt.round_search == 1) {
+        if (c->opt.log_samples != 1)
+            return fail(c, EMST_E_STATE, "the component search logs samples with options.log_samples = 1");
+        Counters h{};
+        if ((rc = read_counters(c, &h))) return rc;
+        const int64_t got = std::min<int64_t>((int64_t)h.nsamples, kSampleCap);
+        if (!feat && !target) { *m = got; return EMST_OK; }
+        if (*m < got) { *m = got; return fail(c, EMST_E_ARG, "sample arrays too small"); }
+        *m = got;
+        if (got == 0) return EMST_OK;
+        std::vector<float> h9((size_t)got * 9);
+        HIPCHK(c, hipMemcpyAsync(h9.data(), c->samples.p, sizeof(float) * h9.size(), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int64_t i = 0; i < got; ++i) {
+            if (feat) std::memcpy(feat + i * 8, &h9[(size_t)i * 9], sizeof(float) * 8);
+            if (target) target[i] = h9[(size_t)i * 9 + 8];
+        }
+        return EMST_OK;
+    }
     const int64_t n = c->n;
     if (!feat && !target) { *m = n; return EMST_OK; }
     if (*m < n) { *m = n; return fail(c, EMST_E_ARG, "sample arrays too small"); }

@@ -49,6 +49,7 @@ struct Counters {
     uint32_t dup_settled;   // no coincident group spans two components any more
     uint32_t dup_big;       // runs longer than one wave handles, queued for a grid-wide pass
     uint32_t nroots;        // roots listed by k_cycle_init this round (2.5D cycle rule)
+    uint32_t nsamples;      // seed samples logged by the component search over the run
 };
 
 // Coincident-point groups.  Points with identical rescaled coordinates form a
@@ -175,14 +176,41 @@ void launch_cone_search(int dim, int cdim, const float4* pts, const Node* nodes,
 // off, tmp, list and list2; pass 0 runs over every searching point, the
 // later passes over those it left open, so that they see the component
 // minima pass 0 found.
+// Learned seed of the component search: per-component size and box, kept at
+// the roots and merged after every round; features of the searching points
+// (list order), the model's output and the seed radius per sorted position.
+// features_on: sizes, boxes and features are maintained (a model, or logging).
+struct CompSeed {
+    bool features_on;
+    int32_t* csize;         // n
+    uint32_t* cbox;         // 6n: f2ord of lo x, y, z, hi x, y, z
+    float* X;               // 8n
+    float* LS;              // n: log of the leaf spacing
+    float* Y;               // n
+    float* R;               // n
+    const float* model;     // nullptr: plain seeds
+    float* samples;         // 9 floats per sample (8 features, target), or nullptr
+    int64_t cap;            // samples that fit
+    int64_t stride;         // every stride-th searching point is logged
+    uint32_t* nq_keep;      // 2 words: the counts the sample log and the model run on
+};
 void launch_comp_round(int dim, int64_t n, const float4* pts, const Node* nodes, int32_t* nlab,
-                       const uint32_t* nmin, const Cloud* cl, int ncl, int64_t nleaves, int64_t Lmax, int bucket,
+                       const uint32_t* nmin, unsigned long long* visit, const Cloud* cl, int ncl, int64_t nleaves,
+                       int bucket,
                        const int32_t* label, int32_t* corr, uint64_t* corr_key, uint64_t* comp1, const int32_t* inv,
-                       float seed_scale, const SplitScratch& sc, Counters* C, hipStream_t s);
+                       float seed_scale, const SplitScratch& sc, const CompSeed& cs, const int32_t* eu,
+                       const int32_t* eround, Counters* C, hipStream_t s);
+// Sizes and boxes of the single-point components (the start of a run).
+void launch_comp_seed_init(int64_t n, const float4* pts, int32_t* csize, uint32_t* cbox, hipStream_t s);
+// MLP inference over X[0..*m_dev) (the comp search's device-side count).
+void launch_mlp_infer_dev(const float* model, const float* X, const uint32_t* m_dev, int64_t m_max, float* Y,
+                          hipStream_t s);
 // Smallest point id under every node of the hierarchy (2 * nleaves words,
-// laid out as the nodes; 0xffffffff for a node without points).
-void launch_node_min_ids(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, int64_t Lmax, int bucket,
-                         uint32_t* nmin, hipStream_t s);
+// laid out as the nodes; 0xffffffff for a node without points).  visit:
+// 2 * nleaves words of arrival state, zero before the first bottom-up launch
+// over this hierarchy (k_nmin_tree, k_nlab_tree) and kept between them.
+void launch_node_min_ids(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, int bucket, uint32_t* nmin,
+                         unsigned long long* visit, hipStream_t s);
 // Cones of every group point for the current labels (dim: the cone dimension) (after the grouped
 // cone search, and at the start of each round while a group may still span
 // several components).  Sets C->dup_split when one does.

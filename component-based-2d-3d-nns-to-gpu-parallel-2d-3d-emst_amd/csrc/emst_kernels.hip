@@ -1144,16 +1144,51 @@ __global__ __launch_bounds__(64) void k_open_fill(const uint64_t* __restrict__ o
     if ((m >> lane) & 1ull) list[off[b] + (uint32_t)__popcll(m & lanemask_lt())] = (int32_t)(b * 64 + lane);
 }
 
-// Same, for packets of a query list: dst receives src's entries whose lanes
-// are flagged.
-__global__ __launch_bounds__(64) void k_open_fill_map(const uint64_t* __restrict__ open_mask,
-                                                     const uint32_t* __restrict__ off,
-                                                     const int32_t* __restrict__ src, int32_t* __restrict__ dst)
+// The same compaction in three launches: per-tile exclusive scan of the
+// packets' open counts (entry P = 0), the tile sums, then each packet's
+// lanes at its offset (src: entries of a query list, else positions); the
+// total lands in off[P].
+__global__ __launch_bounds__(kBlock) void k_open_scan_local(const uint64_t* __restrict__ open_mask, int64_t P,
+                                                           uint32_t* __restrict__ off, uint32_t* __restrict__ sums)
+{
+    __shared__ uint32_t sh[16];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t t = 0;
+    for (int j = 0; j < kScanItems; ++j) {
+        v[j] = base + j < P ? (uint32_t)__popcll(open_mask[base + j]) : 0u;
+        t += v[j];
+    }
+    uint32_t total;
+    uint32_t run = block_excl_scan(t, sh, total);
+    for (int j = 0; j < kScanItems; ++j) {
+        if (base + j <= P) off[base + j] = run;
+        run += v[j];
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(64) void k_open_fill_scan(const uint64_t* __restrict__ open_mask, int64_t P,
+                                                      uint32_t* __restrict__ off, const uint32_t* __restrict__ sums,
+                                                      const int32_t* __restrict__ src, int32_t* __restrict__ dst)
 {
     const int64_t b = blockIdx.x;
-    const uint64_t m = open_mask[b];
     const int lane = __lane_id();
-    if ((m >> lane) & 1ull) dst[off[b] + (uint32_t)__popcll(m & lanemask_lt())] = src[b * 64 + lane];
+    if (b == 0 && lane == 0) off[P] += sums[P / kScanTile];
+    const uint64_t m = open_mask[b];
+    if (!((m >> lane) & 1ull)) return;
+    const uint32_t o = off[b] + sums[b / kScanTile] + (uint32_t)__popcll(m & lanemask_lt());
+    dst[o] = src != nullptr ? src[b * 64 + lane] : (int32_t)(b * 64 + lane);
+}
+
+void open_compact(const uint64_t* open_mask, int64_t P, uint32_t* off, uint32_t* sums, const int32_t* src,
+                  int32_t* dst, hipStream_t s)
+{
+    const int64_t nb = (P + 1 + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_open_scan_local, dim3((unsigned)nb), dim3(kBlock), 0, s, open_mask, P, off, sums);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, sums, nb);
+    hipLaunchKernelGGL(k_open_fill_scan, dim3((unsigned)P), dim3(64), 0, s, open_mask, P, off, (const uint32_t*)sums,
+                       src, dst);
 }
 
 // Start of a group refresh: m2 = ~0 for every group, dup_split = 0.
@@ -1290,7 +1325,7 @@ __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __res
     if (i == 0) {
         C->edges = 0; C->research = 0; C->zero_or_empty = 0; C->unflat = 0;
         C->packets = C->nodes = C->leaves = C->leaf_points = C->pairs = 0;
-        C->round = 0; C->done = 0; C->edges_prev = 0; C->dup_settled = 0;
+        C->round = 0; C->done = 0; C->edges_prev = 0; C->dup_settled = 0; C->nsamples = 0;
         C->target = target; C->max_rounds = max_rounds;
     }
     if (i >= n) return;
@@ -1646,10 +1681,56 @@ __global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const floa
 constexpr int32_t kNlabEmpty = -3;      // a node without points
 constexpr int32_t kNlabMixed = -1;      // a node holding several components
 
-// component label of every leaf: the common label of its points
-__global__ __launch_bounds__(kBlock) void k_nlab_leaves(const int32_t* __restrict__ label,
-                                                       const Cloud* __restrict__ cl, int ncl, int64_t nleaves,
-                                                       int bucket, int32_t* __restrict__ nlab, const Counters* C)
+// Node values of the hierarchy, bottom-up in one launch: one thread per leaf
+// climbs from its leaf; at each parent the first child to arrive leaves its
+// value and stops, the second combines both and goes on.  The arrival count
+// and the first value share one 64-bit word per node (`visit`, updated only
+// by compare-and-swap, so no fence is needed between them); every launch
+// brings exactly two arrivals to every inner node, so the count's parity
+// tells first from second without a reset between launches.
+template <class Op>
+__device__ __forceinline__ void climb(typename Op::T v, typename Op::T* __restrict__ base,
+                                      unsigned long long* __restrict__ vbase, int64_t i)
+{
+    while (i > 1) {
+        const int64_t par = i >> 1;
+        unsigned long long old = vbase[par];
+        bool first;
+        for (;;) {
+            const uint32_t cnt = (uint32_t)(old >> 32);
+            first = (cnt & 1u) == 0u;
+            const typename Op::T nv = first ? v : Op::combine((typename Op::T)(uint32_t)old, v);
+            const unsigned long long nw = ((unsigned long long)(cnt + 1u) << 32) | (uint32_t)nv;
+            const unsigned long long got = atomicCAS(&vbase[par], old, nw);
+            if (got == old) {
+                v = nv;
+                break;
+            }
+            old = got;
+        }
+        if (first) return;
+        i = par;
+        base[i] = v;
+    }
+}
+
+struct LabelOp {
+    typedef int32_t T;
+    __device__ static int32_t combine(int32_t a, int32_t b)
+    {
+        return a == kNlabEmpty ? b : (b == kNlabEmpty ? a : (a == b ? a : kNlabMixed));
+    }
+};
+struct MinOp {
+    typedef uint32_t T;
+    __device__ static uint32_t combine(uint32_t a, uint32_t b) { return min(a, b); }
+};
+
+// component label of every node: the common label of its points
+__global__ __launch_bounds__(kBlock) void k_nlab_tree(const int32_t* __restrict__ label,
+                                                     const Cloud* __restrict__ cl, int ncl, int64_t nleaves,
+                                                     int bucket, int32_t* __restrict__ nlab,
+                                                     unsigned long long* __restrict__ visit, const Counters* C)
 {
     if (C->done) return;
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1659,33 +1740,19 @@ __global__ __launch_bounds__(kBlock) void k_nlab_leaves(const int32_t* __restric
     const int64_t s0 = cl[c].pos + j * bucket;
     const int64_t s1 = min(cl[c].pos + cl[c].n, s0 + bucket);
     int32_t lab = kNlabEmpty;
-    for (int64_t s = s0; s < s1; ++s) {
-        const int32_t l = label[s];
-        lab = (lab == kNlabEmpty || lab == l) ? l : kNlabMixed;
-    }
-    nlab[2 * cl[c].lbase + cl[c].L + j] = lab;
-}
-
-__global__ __launch_bounds__(kBlock) void k_nlab_level(int32_t* __restrict__ nlab, const Cloud* __restrict__ cl,
-                                                      int ncl, int64_t lo, const Counters* C)
-{
-    if (C->done) return;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)ncl * lo) return;
-    const int c = (int)(t / lo);
-    if (lo >= cl[c].L) return;
+    for (int64_t s = s0; s < s1; ++s) lab = LabelOp::combine(lab, label[s]);
     int32_t* base = nlab + 2 * cl[c].lbase;
-    const int64_t i = lo + t % lo;
-    const int32_t a = base[2 * i], b = base[2 * i + 1];
-    base[i] = a == kNlabEmpty ? b : (b == kNlabEmpty ? a : (a == b ? a : kNlabMixed));
+    base[cl[c].L + j] = lab;
+    climb<LabelOp>(lab, base, visit + 2 * cl[c].lbase, cl[c].L + j);
 }
 
-// smallest point id under every leaf, then every level (static per build):
-// with ties on d2 broken by id, a node at the bound can only matter when it
-// holds a smaller id than the best so far (runs of coincident points)
-__global__ __launch_bounds__(kBlock) void k_nmin_leaves(const float4* __restrict__ pts,
-                                                       const Cloud* __restrict__ cl, int ncl, int64_t nleaves,
-                                                       int bucket, uint32_t* __restrict__ nmin)
+// smallest point id under every node (static per build): with ties on d2
+// broken by id, a node at the bound can only matter when it holds a smaller
+// id than the best so far (runs of coincident points)
+__global__ __launch_bounds__(kBlock) void k_nmin_tree(const float4* __restrict__ pts,
+                                                     const Cloud* __restrict__ cl, int ncl, int64_t nleaves,
+                                                     int bucket, uint32_t* __restrict__ nmin,
+                                                     unsigned long long* __restrict__ visit)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nleaves) return;
@@ -1695,19 +1762,9 @@ __global__ __launch_bounds__(kBlock) void k_nmin_leaves(const float4* __restrict
     const int64_t s1 = min(cl[c].pos + cl[c].n, s0 + bucket);
     uint32_t m = 0xffffffffu;
     for (int64_t s = s0; s < s1; ++s) m = min(m, __float_as_uint(pts[s].w));
-    nmin[2 * cl[c].lbase + cl[c].L + j] = m;
-}
-
-__global__ __launch_bounds__(kBlock) void k_nmin_level(uint32_t* __restrict__ nmin, const Cloud* __restrict__ cl,
-                                                      int ncl, int64_t lo)
-{
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)ncl * lo) return;
-    const int c = (int)(t / lo);
-    if (lo >= cl[c].L) return;
     uint32_t* base = nmin + 2 * cl[c].lbase;
-    const int64_t i = lo + t % lo;
-    base[i] = min(base[2 * i], base[2 * i + 1]);
+    base[cl[c].L + j] = m;
+    climb<MinOp>(m, base, visit + 2 * cl[c].lbase, cl[c].L + j);
 }
 
 // Per point, corr / corr_key between rounds:
@@ -1769,6 +1826,170 @@ __global__ __launch_bounds__(kBlock) void k_comp_prep(int64_t n, const int32_t* 
     if (__lane_id() == 0 && p < n) open_mask[p / 64] = m;
 }
 
+// ---- learned seed of the component search.  A component's size and box
+// live at its root: single points at the start, and after every round each
+// root absorbed in it adds its own to its new root (one atomic per merged
+// component and box side).
+__global__ __launch_bounds__(kBlock) void k_comp_seed_init(int64_t n, const float4* __restrict__ pts,
+                                                          int32_t* __restrict__ csize, uint32_t* __restrict__ cbox)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float4 v = pts[p];
+    csize[p] = 1;
+    uint32_t* b = cbox + 6 * p;
+    b[0] = b[3] = f2ord(v.x);
+    b[1] = b[4] = f2ord(v.y);
+    b[2] = b[5] = f2ord(v.z);
+}
+
+__global__ __launch_bounds__(kBlock) void k_comp_merge(int64_t n, const int32_t* __restrict__ label,
+                                                      const int32_t* __restrict__ eu,
+                                                      const int32_t* __restrict__ eround,
+                                                      int32_t* __restrict__ csize, uint32_t* __restrict__ cbox,
+                                                      const Counters* C)
+{
+    if (C->done) return;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int prev = (int)C->round - 1;
+    if (prev < 1 || eu[p] < 0 || eround[p] != prev) return;
+    const int64_t r = label[p];
+    atomicAdd(&csize[r], csize[p]);
+    for (int k = 0; k < 3; ++k) {
+        atomicMin(&cbox[6 * r + k], cbox[6 * p + k]);
+        atomicMax(&cbox[6 * r + 3 + k], cbox[6 * p + 3 + k]);
+    }
+}
+
+// Features of every searching point (list order), free of the cloud's scale:
+// lengths are taken relative to s0, the mean spacing of the point's leaf cell
+// (LS = log s0).  The round, the component's size, the point's lower bound,
+// the component's box extent and mean spacing, the component's kept minimum
+// (an upper bound on its edge), and the density one and three levels up.
+// The model seeds rounds from kLearnedFrom on: the first rounds' searches are
+// nearest-neighbour searches of single points and tiny components, for which
+// the plain seed is as good and the model's cost would not pay.
+// nq_keep[0]: the count of points with features (every round when logging,
+// else from kLearnedFrom on), for the sample log; nq_keep[1]: the count the
+// model runs on (0 before kLearnedFrom or without a model).
+constexpr uint32_t kLearnedFrom = 3;
+
+__global__ __launch_bounds__(kBlock) void k_comp_features(int log_all, int has_model, int dim, const float4* __restrict__ pts,
+                                                         const Node* __restrict__ nodes,
+                                                         const Cloud* __restrict__ clouds, int ncl, int bucket,
+                                                         const int32_t* __restrict__ qlist,
+                                                         const uint32_t* __restrict__ nq_dev,
+                                                         const int32_t* __restrict__ label,
+                                                         const uint64_t* __restrict__ corr_key,
+                                                         const uint64_t* __restrict__ comp1,
+                                                         const int32_t* __restrict__ csize,
+                                                         const uint32_t* __restrict__ cbox, const Counters* C,
+                                                         float* __restrict__ X, float* __restrict__ LS,
+                                                         uint32_t* __restrict__ nq_keep)
+{
+    if (C->done) return;
+    const int64_t qi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool learned_round = has_model && C->round >= kLearnedFrom;
+    const uint32_t nq = (log_all || learned_round) ? *nq_dev : 0u;
+    if (qi == 0) {
+        nq_keep[0] = log_all ? nq : 0u;
+        nq_keep[1] = learned_round ? nq : 0u;
+    }
+    if (qi >= (int64_t)nq) return;
+    const int64_t p = qlist[qi];
+    const int c = find_cloud(clouds, ncl, p);
+    const int64_t L = clouds[c].L, cn = clouds[c].n;
+    const Node* __restrict__ tree = nodes + 2 * clouds[c].lbase;
+    int64_t node = L + (p - clouds[c].pos) / bucket;
+    float sp[4];
+    for (int k = 0; k < 4; ++k) {
+        const Node nd = tree[node];
+        int64_t first = node, last = node;
+        while (first < L) { first = 2 * first; last = 2 * last + 1; }
+        const int64_t a = (first - L) * bucket, b = min(cn, (last - L + 1) * bucket);
+        const float cnt = (float)max<int64_t>(b - a, 1);
+        float vol = 1.f, ext = 0.f;
+        for (int d = 0; d < dim; ++d) {
+            const float e = fmaxf(nd.hi[d] - nd.lo[d], 0.f);
+            vol *= e;
+            ext = fmaxf(ext, e);
+        }
+        const float sv = vol > 0.f ? powf(vol / cnt, 1.f / dim) : ext / cnt;
+        sp[k] = fmaxf(sv, 1e-6f);
+        if (node > 1) node >>= 1;
+    }
+    const float s0 = sp[0], lo = 1e-3f * s0;
+    const int32_t lab = label[p];
+    const uint32_t lbh = (uint32_t)(corr_key[p] >> 32);
+    const float lb = lbh == 0xffffffffu ? 0.f : sqrtf(__uint_as_float(lbh));
+    const uint64_t kept = comp1[lab];
+    const float kd = kept == kEmptyKey ? 1e3f * s0 : sqrtf(__uint_as_float((uint32_t)(kept >> 32)));
+    const uint32_t* b = cbox + 6 * (int64_t)lab;
+    float ext = 0.f, vol = 1.f;
+    for (int d = 0; d < dim; ++d) {
+        const float e = fmaxf(ord2f(b[3 + d]) - ord2f(b[d]), 0.f);
+        ext = fmaxf(ext, e);
+        vol *= e;
+    }
+    const float size = (float)max(csize[lab], 1);
+    const float spc = vol > 0.f ? powf(vol / size, 1.f / dim) : ext / size;
+    float* x = X + qi * 8;
+    x[0] = logf((float)max(C->round, 1u));
+    x[1] = logf(size);
+    x[2] = logf(fmaxf(lb, lo) / s0);
+    x[3] = logf(fmaxf(ext, lo) / s0);
+    x[4] = logf(fmaxf(spc, lo) / s0);
+    x[5] = logf(fmaxf(kd, lo) / s0);
+    x[6] = logf(sp[1] / sp[0]);
+    x[7] = logf(sp[3] / sp[0]);
+    LS[qi] = logf(s0);
+}
+
+// seed radius per sorted position: s0 * exp(model output + offset)
+__global__ __launch_bounds__(kBlock) void k_comp_seed(const int32_t* __restrict__ qlist,
+                                                     const uint32_t* __restrict__ nq_dev,
+                                                     const float* __restrict__ LS, const float* __restrict__ Y,
+                                                     float* __restrict__ R, const Counters* C)
+{
+    if (C->done) return;
+    const int64_t qi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= (int64_t)*nq_dev) return;
+    R[qlist[qi]] = expf(LS[qi] + fminf(Y[qi], 20.f));
+}
+
+// (features, target) of every stride-th searching point, logged from the
+// plain method's rounds to train the model.  Target: log over s0 of the
+// radius the point had to search, min(D, d): D = its component's edge length,
+// d = its nearest outside point's distance when the search found it exactly
+// (beyond D the component's edge stops the point).  Components without an
+// outgoing edge are skipped.
+__global__ __launch_bounds__(kBlock) void k_comp_samples(const int32_t* __restrict__ qlist,
+                                                        const uint32_t* __restrict__ nq_dev,
+                                                        const float* __restrict__ X, const float* __restrict__ LS,
+                                                        const int32_t* __restrict__ label,
+                                                        const int32_t* __restrict__ corr,
+                                                        const uint64_t* __restrict__ corr_key,
+                                                        const uint64_t* __restrict__ comp1, int64_t stride,
+                                                        int64_t cap, float* __restrict__ samples, Counters* C)
+{
+    if (C->done) return;
+    const int64_t qi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= (int64_t)*nq_dev || qi % stride != 0) return;
+    const int64_t p = qlist[qi];
+    const uint64_t k = comp1[label[p]];
+    if (k == kEmptyKey) return;
+    uint32_t db = (uint32_t)(k >> 32);
+    if (corr[p] >= 0) db = min(db, (uint32_t)(corr_key[p] >> 32));
+    const uint32_t slot = atomicAdd(&C->nsamples, 1u);
+    if ((int64_t)slot >= cap) return;
+    float* o = samples + 9 * (int64_t)slot;
+    for (int f = 0; f < 8; ++f) o[f] = X[qi * 8 + f];
+    const float s0 = expf(LS[qi]);
+    const float d = sqrtf(__uint_as_float(db));
+    o[8] = logf(fmaxf(d, 1e-3f * s0)) - LS[qi];
+}
+
 // nearest point in another component, for the points of qlist (count in
 // nq_dev): packet traversal as in the cone search, one best per query, by
 // iterative deepening from the seed radius R_0 (capped passes R_k = 2^k R_0,
@@ -1785,7 +2006,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     const uint32_t* __restrict__ nq_dev, const int32_t* __restrict__ label, const int32_t* __restrict__ inv,
     float seed_scale, int32_t* __restrict__ corr, uint64_t* __restrict__ corr_key,
     uint64_t* __restrict__ comp1, const uint32_t* __restrict__ nmin, int pass_lo, int pass_hi,
-    uint64_t* __restrict__ open_out, const Counters* C)
+    uint64_t* __restrict__ open_out, const float* __restrict__ seed_r, const Counters* C)
 {
     if (C->done) return;
     const int64_t nq = *nq_dev;
@@ -1837,17 +2058,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             }
             float sp = vol > 0.f ? powf(vol / cnt, 1.f / D) : ext / cnt;
             if (!(sp > 0.f)) sp = 1e-6f;
+            const bool has_lb = (uint32_t)(prev >> 32) != 0xffffffffu;
+            const uint32_t dbits = (uint32_t)(prev >> 32);
+            if (has_lb) inner = (int32_t)dbits - 1;
             r = seed_scale * sp;
-            if ((uint32_t)(prev >> 32) != 0xffffffffu) {
-                const uint32_t dbits = (uint32_t)(prev >> 32);
-                inner = (int32_t)dbits - 1;
-                r = fmaxf(r, 2.f * sqrtf(__uint_as_float(dbits)));
+            if (has_lb) r = fmaxf(r, 2.f * sqrtf(__uint_as_float(dbits)));
+            // learned: the predicted length of the component's edge, so
+            // that the first pass can already find it
+            if (seed_r != nullptr && C->round >= kLearnedFrom) {
+                r = seed_r[p];
+                if (has_lb) r = fmaxf(r, sqrtf(__uint_as_float(dbits)));
             }
+            if (!(r > 0.f)) r = 1e-6f;
         }
         float r2 = r * r;
         // a launch that resumes at pass_lo: the state the earlier passes left
         for (int k = 0; k < pass_lo; ++k) {
-            inner = (int32_t)__float_as_uint(r2);
+            inner = max(inner, (int32_t)__float_as_uint(r2));
             r2 *= 4.f;
         }
         bool open = mine;
@@ -2018,7 +2245,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                 }
             }
             atomic_min_grouped(comp1, mylab, key1, fresh);
-            inner = cap;
+            inner = max(inner, cap);
             r2 *= 4.f;
         }
         if (mine) left_open = open;
@@ -2510,46 +2737,71 @@ void launch_cone_search(int dim, int cdim, const float4* pts, const Node* nodes,
                        (int64_t)(split->compact_frac * (double)nq));
 }
 
-void launch_node_min_ids(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, int64_t Lmax, int bucket,
-                         uint32_t* nmin, hipStream_t s)
+void launch_node_min_ids(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, int bucket, uint32_t* nmin,
+                         unsigned long long* visit, hipStream_t s)
 {
     if (nleaves <= 0) return;
-    hipLaunchKernelGGL(k_nmin_leaves, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, pts, cl, ncl, nleaves, bucket, nmin);
-    for (int64_t lo = Lmax / 2; lo >= 1; lo /= 2)
-        hipLaunchKernelGGL(k_nmin_level, dim3(nblocks((int64_t)ncl * lo)), dim3(kBlock), 0, s, nmin, cl, ncl, lo);
+    hipLaunchKernelGGL(k_nmin_tree, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, pts, cl, ncl, nleaves, bucket, nmin,
+                       visit);
+}
+
+void launch_comp_seed_init(int64_t n, const float4* pts, int32_t* csize, uint32_t* cbox, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_comp_seed_init, dim3(nblocks(n)), dim3(kBlock), 0, s, n, pts, csize, cbox);
 }
 
 void launch_comp_round(int dim, int64_t n, const float4* pts, const Node* nodes, int32_t* nlab,
-                       const uint32_t* nmin, const Cloud* cl, int ncl, int64_t nleaves, int64_t Lmax, int bucket,
+                       const uint32_t* nmin, unsigned long long* visit, const Cloud* cl, int ncl, int64_t nleaves,
+                       int bucket,
                        const int32_t* label, int32_t* corr, uint64_t* corr_key, uint64_t* comp1, const int32_t* inv,
-                       float seed_scale, const SplitScratch& sc, Counters* C, hipStream_t s)
+                       float seed_scale, const SplitScratch& sc, const CompSeed& cs, const int32_t* eu,
+                       const int32_t* eround, Counters* C, hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_nlab_leaves, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, label, cl, ncl, nleaves, bucket,
-                       nlab, (const Counters*)C);
-    for (int64_t lo = Lmax / 2; lo >= 1; lo /= 2)
-        hipLaunchKernelGGL(k_nlab_level, dim3(nblocks((int64_t)ncl * lo)), dim3(kBlock), 0, s, nlab, cl, ncl, lo,
+    const unsigned g = nblocks(n);
+    if (cs.features_on)
+        hipLaunchKernelGGL(k_comp_merge, dim3(g), dim3(kBlock), 0, s, n, label, eu, eround, cs.csize, cs.cbox,
                            (const Counters*)C);
+    hipLaunchKernelGGL(k_nlab_tree, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, label, cl, ncl, nleaves, bucket,
+                       nlab, visit, (const Counters*)C);
     const int64_t P = (n + 63) / 64;
-    hipLaunchKernelGGL(k_comp_keep, dim3(nblocks(n)), dim3(kBlock), 0, s, n, pts, label, (const int32_t*)corr,
+    hipLaunchKernelGGL(k_comp_keep, dim3(g), dim3(kBlock), 0, s, n, pts, label, (const int32_t*)corr,
                        (const uint64_t*)corr_key, comp1, (const Counters*)C);
-    hipLaunchKernelGGL(k_comp_prep, dim3(nblocks(n)), dim3(kBlock), 0, s, n, label, corr, (const uint64_t*)corr_key,
+    hipLaunchKernelGGL(k_comp_prep, dim3(g), dim3(kBlock), 0, s, n, label, corr, (const uint64_t*)corr_key,
                        (const uint64_t*)comp1, sc.open_mask, (const Counters*)C);
-    hipLaunchKernelGGL(k_open_count, dim3(nblocks(P + 1)), dim3(kBlock), 0, s, sc.open_mask, P, sc.cnt);
-    exclusive_scan_u32(sc.cnt, sc.off, P + 1, sc.tmp, s);
-    hipLaunchKernelGGL(k_open_fill, dim3((unsigned)P), dim3(64), 0, s, sc.open_mask, sc.off, sc.list);
+    open_compact(sc.open_mask, P, sc.off, sc.tmp, nullptr, sc.list, s);
+    const uint32_t* nq = sc.off + P;
+    const float* seed_r = nullptr;
+    if (cs.features_on) {
+        hipLaunchKernelGGL(k_comp_features, dim3(g), dim3(kBlock), 0, s, cs.samples != nullptr ? 1 : 0,
+                           cs.model != nullptr ? 1 : 0, dim, pts, nodes, cl, ncl, bucket,
+                           (const int32_t*)sc.list, nq, label, (const uint64_t*)corr_key, (const uint64_t*)comp1,
+                           (const int32_t*)cs.csize, (const uint32_t*)cs.cbox, (const Counters*)C, cs.X, cs.LS,
+                           cs.nq_keep);
+        if (cs.model != nullptr) {
+            launch_mlp_infer_dev(cs.model, cs.X, cs.nq_keep + 1, n, cs.Y, s);
+            hipLaunchKernelGGL(k_comp_seed, dim3(g), dim3(kBlock), 0, s, (const int32_t*)sc.list,
+                               (const uint32_t*)(cs.nq_keep + 1), (const float*)cs.LS, (const float*)cs.Y, cs.R,
+                               (const Counters*)C);
+            seed_r = cs.R;
+        }
+    }
     auto kern = dim == 3 ? k_comp_search<3> : k_comp_search<2>;
     // pass 0 over the searching points; their open packets' lanes into open_mask
     hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, nodes, (const int32_t*)nlab, cl, ncl, bucket,
-                       (const int32_t*)sc.list, (const uint32_t*)(sc.off + P), label, inv, seed_scale, corr, corr_key,
-                       comp1, nmin, 0, 0, sc.open_mask, (const Counters*)C);
-    hipLaunchKernelGGL(k_open_count, dim3(nblocks(P + 1)), dim3(kBlock), 0, s, sc.open_mask, P, sc.cnt);
-    exclusive_scan_u32(sc.cnt, sc.off, P + 1, sc.tmp, s);
-    hipLaunchKernelGGL(k_open_fill_map, dim3((unsigned)P), dim3(64), 0, s, sc.open_mask, sc.off,
-                       (const int32_t*)sc.list, sc.list2);
+                       (const int32_t*)sc.list, nq, label, inv, seed_scale, corr, corr_key, comp1, nmin, 0, 0,
+                       sc.open_mask, seed_r, (const Counters*)C);
+    open_compact(sc.open_mask, P, sc.off, sc.tmp, (const int32_t*)sc.list, sc.list2, s);
     hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, nodes, (const int32_t*)nlab, cl, ncl, bucket,
-                       (const int32_t*)sc.list2, (const uint32_t*)(sc.off + P), label, inv, seed_scale, corr, corr_key,
-                       comp1, nmin, 1, kMaxCappedPasses, (uint64_t*)nullptr, (const Counters*)C);
+                       (const int32_t*)sc.list2, (const uint32_t*)(sc.off + P), label, inv, seed_scale, corr,
+                       corr_key, comp1, nmin, 1, kMaxCappedPasses, (uint64_t*)nullptr, seed_r, (const Counters*)C);
+    // comp1 now holds every component's edge: the sample log of this round
+    if (cs.features_on && cs.samples != nullptr)
+        hipLaunchKernelGGL(k_comp_samples, dim3(g), dim3(kBlock), 0, s, (const int32_t*)sc.list,
+                           (const uint32_t*)cs.nq_keep, (const float*)cs.X, (const float*)cs.LS, label,
+                           (const int32_t*)corr, (const uint64_t*)corr_key, (const uint64_t*)comp1, cs.stride,
+                           cs.cap, cs.samples, C);
 }
 
 void launch_dup_fill(int dim, int64_t n, const int32_t* label, const float4* pts, const int32_t* inv,

@@ -233,8 +233,9 @@ __global__ __launch_bounds__(64) void k_mlp(const float* __restrict__ X, const f
 // 64-sample tiles (same arithmetic as k_mlp's forward).
 __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ X, int64_t m,
                                                   const float* __restrict__ P, float* __restrict__ Y,
-                                                  int add_offset)
+                                                  int add_offset, const uint32_t* __restrict__ m_dev)
 {
+    if (m_dev != nullptr) m = *m_dev;
     __shared__ uint16_t sW1t[kH * kFP];
     __shared__ uint16_t sW2[kH * kH];
     __shared__ uint16_t sXt[4][kFP * 64];
@@ -449,8 +450,16 @@ void launch_radius_seed(int64_t n, const float* model, const float* X, const flo
 {
     if (n <= 0) return;
     const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
-    hipLaunchKernelGGL(k_mlp_infer, dim3(g), dim3(256), 0, s, X, n, model, Y, 1);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(g), dim3(256), 0, s, X, n, model, Y, 1, (const uint32_t*)nullptr);
     hipLaunchKernelGGL(k_radius_seed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, LS, Y, R);
+}
+
+void launch_mlp_infer_dev(const float* model, const float* X, const uint32_t* m_dev, int64_t m_max, float* Y,
+                          hipStream_t s)
+{
+    if (m_max <= 0) return;
+    const unsigned g = (unsigned)std::min<int64_t>((m_max + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(g), dim3(256), 0, s, X, m_max, model, Y, 1, m_dev);
 }
 
 }  // namespace emst

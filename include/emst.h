@@ -76,9 +76,10 @@ typedef struct {
                                another component whenever its target has joined its own
                                (NodeSpiralSearchMD, NodeEMST.h:67-509, selected in the reference by
                                EMST_SPIRAL_SEARCH_KD_SLAB_2 0, SolutionEMST.h:55-56).  Both build
-                               the same forests.  1 has no cone candidates, no learned seed and no
-                               2.5D mode */
-    int32_t reserved[1];
+                               the same forests.  1 has no cone candidates and no 2.5D mode; its
+                               learned seed predicts each component's edge length per round */
+    int32_t log_samples;    /* 1 (round_search 1): log the learned seed's training pairs during
+                               runs (emst_get_radius_samples) */
 } emst_options;
 
 /* One row of the per-round trace: the columns of the reference's stats file
@@ -226,7 +227,8 @@ int emst_get_times(emst_ctx* ctx, float* times, int32_t* n_times);
 int emst_get_search_stats(emst_ctx* ctx, int64_t* stats, int32_t* n_stats);
 
 /* ---- learned seed radius (no counterpart in the reference) -----------------
- * A multilayer perceptron 8 -> 64 -> 64 -> 1 (ReLU; BF16 weights and
+ * Cone search (round_search 0):
+ * a multilayer perceptron 8 -> 64 -> 64 -> 1 (ReLU; BF16 weights and
  * activations, FP32 accumulation on the matrix cores) predicts, per point,
  * log(R / s0): R = radius within which all the point's non-empty cones have
  * their nearest point, s0 = mean point spacing of its leaf cell.  The cone
@@ -242,14 +244,31 @@ int emst_get_search_stats(emst_ctx* ctx, int64_t* stats, int32_t* n_stats);
  * neighbour offsets / sum of their lengths.  Normalised features are clamped
  * to [-4, 4] before the first layer. */
 #define EMST_MLP_PARAMS 4818
+/* Component search (round_search 1; where the reference's ring search starts
+ * from its remembered d_min, NodeEMST.h:209-214): the same network predicts,
+ * every round for every searching point, log(min(D, d) / s0), D = the length
+ * of its component's shortest outgoing edge, d = the distance of the point's
+ * nearest point outside the component.  The point's first pass searches to
+ * max(s0 * exp(prediction + offset), its lower bound); the component's edge
+ * found by then bounds its other members' searches.  Features: log round;
+ * log component size; log(lower bound / s0) (the previous target's
+ * distance); log(extent of the component's box / s0); log(the component's
+ * mean spacing (box volume / size)^(1/D) / s0); log(the component's kept
+ * minimum / s0) (10^3 when none); log(s1/s0); log(s3/s0).  Lengths below
+ * 10^-3 s0 count as 10^-3 s0. */
 
-/* Use `model` for the seed radius of this context's cone searches (copied);
- * NULL returns to the plain seed (seed_scale x leaf spacing). */
+/* Use `model` for the seed radius of this context's searches (copied): a
+ * cone-search model with round_search 0, a component-search model with
+ * round_search 1.  NULL returns to the plain seed (seed_scale x leaf spacing;
+ * with round_search 1 at least twice the lower bound). */
 int emst_set_radius_model(emst_ctx* ctx, const float* model);
 
-/* Training pairs from the context's current points: features (m x 8) and
- * targets (m) of every point, from its last cone search.  Pass NULL arrays to
- * query *m; otherwise *m is the capacity in, count out. */
+/* Training pairs.  round_search 0: features (m x 8) and targets (m) of every
+ * point of the context's current points, from its last cone search.
+ * round_search 1: the pairs logged by the last run (options.log_samples = 1:
+ * every searching point of every round, subsampled to at most 2^16 per round
+ * and 2^20 in all).  Pass NULL arrays to query *m; otherwise *m is the
+ * capacity in, count out. */
 int emst_get_radius_samples(emst_ctx* ctx, float* feat, float* target, int64_t* m);
 
 /* Trains a model on the GPU with the hand-written forward/backward kernels:

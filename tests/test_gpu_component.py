@@ -96,7 +96,7 @@ def test_refusals(pkg):
         with pytest.raises(pkg.EmstError):
             c.candidates()
         with pytest.raises(pkg.EmstError):
-            c.set_radius_model(np.zeros(pkg.MLP_PARAMS, np.float32))
+            c.radius_samples()          # no log without log_samples=1
     with pytest.raises(pkg.EmstError):
         pkg.Calculateur(dim=3, cone_dim=2, round_search=1)
 
@@ -112,3 +112,72 @@ def test_large_coincident_groups_equal_cone_search(pkg, dim):
     b = run(pkg, pts, round_search=0)
     assert np.array_equal(a["e"], b["e"])
     assert a["sizes"] == b["sizes"]
+
+
+def _learned_run(pkg, pts, model):
+    with pkg.Calculateur(dim=pts.shape[1], trace=True, round_search=1) as c:
+        c.set_radius_model(model)
+        c.initialize(pts)
+        c.run()
+        u, v, d = c.getLinks()
+        return np.stack([u, v], 1).astype(np.int64), [r["size"] for r in c.trace()], c.labels()
+
+
+@pytest.mark.parametrize("dim,kind", [(3, 1), (2, 2)])
+def test_learned_seed_logs_trains_and_keeps_the_forest(pkg, dim, kind):
+    """Pairs logged from a plain run train a model on the GPU; any model and
+    offset (too small, too large) only changes the work: the forests of every
+    round equal the plain search's."""
+    pts = pkg.synth(kind, 60_000, dim, 49 + dim)
+    with pkg.Calculateur(dim=dim, round_search=1, log_samples=1) as c:
+        c.initialize(pts)
+        c.run()
+        X, T = c.radius_samples()
+    assert len(T) > 1000
+    assert np.isfinite(X).all() and np.isfinite(T).all()
+    # round and size features: round 1 (log 1 = 0) with single points (log 1 = 0)
+    assert X[:, 0].min() == 0.0 and X[:, 0].max() > 0.0
+    assert X[X[:, 0] == 0.0][:, 1].max() == 0.0
+    model, losses = pkg.train_radius_model(X, T, epochs=3, batch=1024, seed=dim)
+    assert losses[-1] < float(np.var(T))
+    plain = run(pkg, pts, round_search=1)
+    for off in (-3.0, 0.0, 3.0):
+        m = model.copy()
+        m[-1] = off
+        e, sizes, labels = _learned_run(pkg, pts, m)
+        assert np.array_equal(e, plain["e"])
+        assert sizes == plain["sizes"]
+        assert np.array_equal(labels, plain["labels"])
+
+
+def test_learned_seed_batch_and_oracle(pkg, oracle_mod):
+    """The learned seed on a batch of clouds, and against the oracle."""
+    rng = np.random.default_rng(51)
+    model = np.zeros(pkg.MLP_PARAMS, np.float32)
+    model[-1] = 0.5                      # a constant prediction: e^0.5 leaf spacings
+    clouds = [pkg.synth(int(rng.integers(0, 3)), int(rng.integers(2, 3000)), 3, 600 + i) for i in range(5)]
+    with pkg.Calculateur(dim=3, round_search=1) as c:
+        c.set_radius_model(model)
+        c.initialize_clouds(clouds)
+        c.run()
+        u, v, _ = c.getLinks()
+        off = c.offsets
+    for i, pts in enumerate(clouds):
+        ref = oracle_mod.emst(pts, mode=0)
+        sel = (u >= off[i]) & (u < off[i + 1])
+        eb = np.stack([u[sel] - off[i], v[sel] - off[i]], 1).astype(np.int64)
+        assert np.array_equal(eb, oracle_mod.canon(ref["u"], ref["v"]))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_shipped_component_models_keep_the_forest(pkg, dim):
+    import os
+    path = os.path.join(os.path.dirname(pkg.__file__), "models", f"radius_comp_{dim}d.npy")
+    if not os.path.exists(path):
+        pytest.skip(f"no trained {dim}D component-search model shipped")
+    if True:
+        model = np.load(path)
+        pts = pkg.synth(1, 100_000, dim, 52)
+        e, sizes, _ = _learned_run(pkg, pts, model)
+        plain = run(pkg, pts, round_search=1)
+        assert np.array_equal(e, plain["e"]) and sizes == plain["sizes"]
