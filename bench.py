@@ -107,9 +107,9 @@ def main():
         if os.path.exists(meta):
             model_clouds = json.load(open(meta)).get("clouds")
 
-    def timed(form_model):
+    def timed(form_model, round_search=args.round_search):
         ctx = pkg.Calculateur(dim=args.dim, bucket_size=args.bucket, cone_dim=args.cone_dim,
-                              round_search=args.round_search)
+                              round_search=round_search)
         if form_model is not None:
             ctx.set_radius_model(form_model)
         search_ms, total_ms = [], []
@@ -141,6 +141,11 @@ def main():
 
     plain = timed(None)
     learned = timed(model) if model is not None else None
+    # the other search form on the same points (same forest, checked)
+    other = None
+    if cdim == args.dim:
+        other = timed(None, round_search=1 - args.round_search)
+        assert other["edges_digest"] == plain["edges_digest"], "the two search forms built different trees"
     if learned is not None:
         assert learned["edges_digest"] == plain["edges_digest"], "learned seeds changed the tree"
         assert learned["w"] == plain["w"]
@@ -224,7 +229,9 @@ def main():
                    "learned_seed": seed_report,
                    "same_tree": learned is not None,
                    "cone_dim": cdim, "cycles_cut": int(head["stats"]["cycles_cut"]),
-                   "round_search": args.round_search},
+                   "round_search": args.round_search,
+                   "other_search_form_plain_mpts": (round(args.n * args.steps / other["dt"] / 1e6, 3)
+                                                    if other else None)},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

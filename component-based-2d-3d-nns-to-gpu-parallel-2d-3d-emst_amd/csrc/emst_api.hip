@@ -360,7 +360,7 @@ CompSeed comp_seed(emst_ctx* c)
     cs.model = c->learned ? c->model.p : nullptr;
     cs.samples = c->opt.log_samples == 1 ? c->samples.p : nullptr;
     cs.cap = kSampleCap;
-    cs.stride = std::max<int64_t>(1, (c->n + kSamplesPerRound - 1) / kSamplesPerRound);
+    cs.per_round = kSamplesPerRound;
     cs.nq_keep = c->nq_keep.p;
     return cs;
 }

@@ -191,7 +191,7 @@ struct CompSeed {
     const float* model;     // nullptr: plain seeds
     float* samples;         // 9 floats per sample (8 features, target), or nullptr
     int64_t cap;            // samples that fit
-    int64_t stride;         // every stride-th searching point is logged
+    int64_t per_round;      // pairs logged per round, at most (a stride over the searching points)
     uint32_t* nq_keep;      // 2 words: the counts the sample log and the model run on
 };
 void launch_comp_round(int dim, int64_t n, const float4* pts, const Node* nodes, int32_t* nlab,

@@ -1958,7 +1958,9 @@ __global__ __launch_bounds__(kBlock) void k_comp_seed(const int32_t* __restrict_
     R[qlist[qi]] = expf(LS[qi] + fminf(Y[qi], 20.f));
 }
 
-// (features, target) of every stride-th searching point, logged from the
+// (features, target) of every stride-th searching point, stride chosen per
+// round so that every round logs about `per_round` pairs (the late rounds,
+// with few searching points, weigh as much as the first), logged from the
 // plain method's rounds to train the model.  Target: log over s0 of the
 // radius the point had to search, min(D, d): D = its component's edge length,
 // d = its nearest outside point's distance when the search found it exactly
@@ -1970,12 +1972,14 @@ __global__ __launch_bounds__(kBlock) void k_comp_samples(const int32_t* __restri
                                                         const int32_t* __restrict__ label,
                                                         const int32_t* __restrict__ corr,
                                                         const uint64_t* __restrict__ corr_key,
-                                                        const uint64_t* __restrict__ comp1, int64_t stride,
+                                                        const uint64_t* __restrict__ comp1, int64_t per_round,
                                                         int64_t cap, float* __restrict__ samples, Counters* C)
 {
     if (C->done) return;
     const int64_t qi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (qi >= (int64_t)*nq_dev || qi % stride != 0) return;
+    const int64_t nq = *nq_dev;
+    const int64_t stride = max<int64_t>(1, (nq + per_round - 1) / per_round);
+    if (qi >= nq || qi % stride != 0) return;
     const int64_t p = qlist[qi];
     const uint64_t k = comp1[label[p]];
     if (k == kEmptyKey) return;
@@ -2800,7 +2804,7 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, const Node* nodes,
     if (cs.features_on && cs.samples != nullptr)
         hipLaunchKernelGGL(k_comp_samples, dim3(g), dim3(kBlock), 0, s, (const int32_t*)sc.list,
                            (const uint32_t*)cs.nq_keep, (const float*)cs.X, (const float*)cs.LS, label,
-                           (const int32_t*)corr, (const uint64_t*)corr_key, (const uint64_t*)comp1, cs.stride,
+                           (const int32_t*)corr, (const uint64_t*)corr_key, (const uint64_t*)comp1, cs.per_round,
                            cs.cap, cs.samples, C);
 }
 
