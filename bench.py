@@ -339,7 +339,8 @@ def run_batch(args, pkg):
         for i, (c, g) in enumerate(zip(ctxs, groups)):
             c.initialize_clouds_device(g["dev"].data_ptr(), g["sizes"])
             c.autotune()
-            tuned[i] = {k: v for k, v in c.options().items() if k in ("bucket_size", "seed_scale", "search_split")}
+            tuned[i] = {k: v for k, v in c.options().items()
+                        if k in ("bucket_size", "seed_scale", "search_split", "round_search")}
     for _ in range(args.warmup):
         step(ctxs)
     torch.cuda.synchronize()

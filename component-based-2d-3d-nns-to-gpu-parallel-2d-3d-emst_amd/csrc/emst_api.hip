@@ -778,6 +778,28 @@ int ensure_built(emst_ctx* c)
     return EMST_OK;
 }
 
+// The rounds of a built context, to the end; times[2] = their device time.
+int run_rounds(emst_ctx* c)
+{
+    int rc;
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    if (!c->opt.trace && (c->grouped || c->zero_or_empty == 0)) {
+        rc = run_graph(c);
+        if (rc) return rc;
+    } else {
+        int32_t fin = 0;
+        while (!fin) {
+            rc = round_once(c, &fin);
+            if (rc) return rc;
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev[3]));
+    c->times[2] = elapsed(c->ev[2], c->ev[3]);
+    c->times[3] = c->times[0] + c->times[1] + c->times[2];
+    return EMST_OK;
+}
+
 }  // namespace
 
 // ================================================================ C ABI
@@ -931,6 +953,9 @@ int emst_autotune(emst_ctx* c, int32_t* best_bucket, float* best_scale, float* b
     HIPCHK(c, hipSetDevice(c->opt.device));
     const bool learned = c->learned;
     c->learned = false;                 // tunes the plain seed
+    // the cone search's leaf size, seed and form are tuned on the cone form
+    const int rs0 = c->opt.round_search;
+    c->opt.round_search = 0;
     float best = 0.f;
     int bb = 64;
     float bs = c->seed_scale;
@@ -966,6 +991,25 @@ int emst_autotune(emst_ctx* c, int32_t* best_bucket, float* best_scale, float* b
         }
     }
     c->opt.search_split = rc ? split0 : bsplit;
+    // then the search itself: the round-1 cone search or the per-round
+    // component search, each timed over a whole run (their forests are the
+    // same).  Kept as the caller set it with a learned model (a model
+    // belongs to one search) or in the 2.5D mode (cone search only).
+    int brs = rs0;
+    if (!rc && !learned && c->cdim == c->dim) {
+        float tbest = 0.f;
+        for (int rs : {0, 1}) {
+            c->opt.round_search = rs;
+            c->no_grouping = false;
+            if ((rc = layout_clouds(c, bb)) || (rc = build(c)) || (rc = run_rounds(c))) break;
+            const float t = c->times[0] + c->times[1] + c->times[2];
+            if (tbest == 0.f || t < tbest) {
+                tbest = t;
+                brs = rs;
+            }
+        }
+    }
+    c->opt.round_search = brs;
     c->learned = learned;
     if (rc) return rc;
     c->opt.bucket_size = bb;
@@ -1011,22 +1055,7 @@ int emst_run(emst_ctx* c)
     HIPCHK(c, hipSetDevice(c->opt.device));
     rc = ensure_built(c);
     if (rc) return rc;
-    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    if (!c->opt.trace && (c->grouped || c->zero_or_empty == 0)) {
-        rc = run_graph(c);
-        if (rc) return rc;
-    } else {
-        int32_t fin = 0;
-        while (!fin) {
-            rc = round_once(c, &fin);
-            if (rc) return rc;
-        }
-    }
-    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-    HIPCHK(c, hipEventSynchronize(c->ev[3]));
-    c->times[2] = elapsed(c->ev[2], c->ev[3]);
-    c->times[3] = c->times[0] + c->times[1] + c->times[2];
-    return EMST_OK;
+    return run_rounds(c);
 }
 
 int emst_step(emst_ctx* c, int32_t* finished)

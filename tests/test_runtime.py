@@ -62,8 +62,10 @@ def test_autotune_keeps_the_tree(pkg, dim, kind):
         assert np.array_equal(e, ref)
         opt = c.options()
         assert opt["bucket_size"] == b and opt["seed_scale"] == sc and opt["search_split"] in (0, 1, 2, 3)
+        assert opt["round_search"] in (0, 1)
     # a new context built from the tuned options runs the same way
-    with pkg.Calculateur(dim=dim, **{k: opt[k] for k in ("bucket_size", "seed_scale", "search_split")}) as c2:
+    keys = ("bucket_size", "seed_scale", "search_split", "round_search")
+    with pkg.Calculateur(dim=dim, **{k: opt[k] for k in keys}) as c2:
         c2.initialize(pts)
         c2.run()
         e, _ = _edges(c2)
