@@ -2161,17 +2161,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                         v = pts[s0 + j];
                         vlab = label[s0 + j];
                     }
-                    // one (query, leaf point) pair per lane
-                    auto pair = [&](int myq, float qx, float qy, float qz, int32_t qb, int32_t qin, int32_t ql) {
-                        const float dx = v.x - qx, dy = v.y - qy, dz = v.z - qz;
-                        const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
-                        const int32_t db = (int32_t)__float_as_uint(d2);
-                        if (myq >= 0 && j < cnt && vlab != ql && db <= qb && db > qin)
-                            atomicMin(&best[myq],
-                                      ((unsigned long long)__float_as_uint(d2) << 32) | __float_as_uint(v.w));
-                    };
-                    // (a v_readlane broadcast when 64 / bucket = 1, as in the
-                    // cone search, measured 7% slower here)
+                    // one (query, leaf point) pair per lane; the query's state by
+                    // LDS permute (a v_readlane broadcast when 64 / bucket = 1, as
+                    // in the cone search, measured 7% slower here, and four
+                    // packets per block 6% slower)
                     uint64_t rem = needm;
                     while (rem) {
                         int myq = -1;
@@ -2181,8 +2174,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                             if (k == qslot) myq = l;
                         }
                         const int src = myq >= 0 ? myq : 0;
-                        pair(myq, __shfl(px, src), __shfl(py, src), __shfl(pz, src), __shfl(bnd, src),
-                             __shfl(inner, src), __shfl(mylab, src));
+                        const float qx = __shfl(px, src), qy = __shfl(py, src), qz = __shfl(pz, src);
+                        const int32_t qb = __shfl(bnd, src), qin = __shfl(inner, src), ql = __shfl(mylab, src);
+                        const float dx = v.x - qx, dy = v.y - qy, dz = v.z - qz;
+                        const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
+                        const int32_t db = (int32_t)__float_as_uint(d2);
+                        if (myq >= 0 && j < cnt && vlab != ql && db <= qb && db > qin)
+                            atomicMin(&best[myq],
+                                      ((unsigned long long)__float_as_uint(d2) << 32) | __float_as_uint(v.w));
                     }
                     if (need) {
                         bk = best[lane];
