@@ -169,15 +169,42 @@ def test_learned_seed_batch_and_oracle(pkg, oracle_mod):
         assert np.array_equal(eb, oracle_mod.canon(ref["u"], ref["v"]))
 
 
+@pytest.mark.parametrize("n", [0, 1, 2, 3])
+def test_tiny_inputs(pkg, oracle_mod, n):
+    pts = pkg.synth(0, max(n, 1), 3, 53)[:n]
+    with pkg.Calculateur(dim=3, round_search=1) as c:
+        c.initialize(pts)
+        c.run()
+        u, v, _ = c.getLinks()
+        w, m = c.evaluate()
+    assert m == max(n - 1, 0) and len(u) == m
+    if n >= 2:
+        ref = oracle_mod.emst(pts, mode=0)
+        assert np.array_equal(np.stack([u, v], 1).astype(np.int64), oracle_mod.canon(ref["u"], ref["v"]))
+
+
 @pytest.mark.parametrize("dim", [2, 3])
-def test_shipped_component_models_keep_the_forest(pkg, dim):
-    import os
-    path = os.path.join(os.path.dirname(pkg.__file__), "models", f"radius_comp_{dim}d.npy")
-    if not os.path.exists(path):
-        pytest.skip(f"no trained {dim}D component-search model shipped")
-    if True:
-        model = np.load(path)
-        pts = pkg.synth(1, 100_000, dim, 52)
-        e, sizes, _ = _learned_run(pkg, pts, model)
-        plain = run(pkg, pts, round_search=1)
-        assert np.array_equal(e, plain["e"]) and sizes == plain["sizes"]
+def test_all_points_coincident(pkg, dim):
+    """One location repeated 5000 times: every point's nearest is the smallest
+    other id, so round 1 builds the star around id 0."""
+    pts = np.repeat(np.full((1, dim), 0.25, np.float32), 5000, axis=0)
+    g = run(pkg, pts, round_search=1)
+    assert g["sizes"] == [4999]
+    e = g["e"]
+    assert np.array_equal(e[:, 0], np.zeros(4999)) and np.array_equal(e[:, 1], np.arange(1, 5000))
+
+
+def test_batch_with_tiny_clouds_and_round_cap(pkg):
+    rng = np.random.default_rng(54)
+    sizes = [1, 2, 0, 3, 500, 1, 2000]
+    clouds = [pkg.synth(int(rng.integers(0, 3)), max(k, 1), 2, 700 + i)[:k] for i, k in enumerate(sizes)]
+    out = {}
+    for rs in (0, 1):
+        with pkg.Calculateur(dim=2, round_search=rs, max_rounds=3) as c:
+            c.initialize_clouds(clouds)
+            c.run()
+            u, v, d = c.getLinks()
+            out[rs] = (np.stack([u, v], 1), d.view(np.uint32), c.cloud_results())
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    for a, b in zip(out[0][2], out[1][2]):
+        assert np.array_equal(a, b)
