@@ -2120,10 +2120,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         for (int pass = pass_lo; pass <= pass_hi; ++pass) {
             const bool last = pass == kMaxCappedPasses;
             const int32_t cbb = open ? key_bound(comp1[mylab]) : 0x7fffffff;
-            // a resumed point whose component's edge is known searches straight
-            // out to it in one pass instead of deepening by 4x per pass
-            const int32_t cap = last || (pass_lo > 0 && cbb != 0x7fffffff) ? 0x7fffffff
-                                                                             : (int32_t)__float_as_uint(r2);
+            // a point whose component already has an edge (a kept target, or
+            // a member's result) searches straight out to it in one pass
+            // instead of deepening by 4x per pass
+            const int32_t cap = last || cbb != 0x7fffffff ? 0x7fffffff : (int32_t)__float_as_uint(r2);
             // nothing outside the component lies within `inner`: with the
             // component's edge at most that long, the point cannot hold it
             if (open && cbb <= inner) {
