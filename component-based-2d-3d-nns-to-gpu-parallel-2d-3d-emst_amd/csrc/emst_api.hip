@@ -97,8 +97,6 @@ struct emst_ctx {
     DevBuf<uint64_t> cy_kh;
     DevBuf<uint32_t> cy_kl;
     DevBuf<uint8_t> cy_onc, cy_cut;
-    DevBuf<int32_t> nlab;              // node component labels (per-round component search)
-    DevBuf<uint32_t> nmin;             // node minimum ids (per-round component search)
     DevBuf<unsigned long long> visit;  // arrival state of the bottom-up node passes
     DevBuf<int32_t> list2;             // queries open after pass 0 (per-round component search)
     DevBuf<int32_t> csize;             // component sizes at the roots (learned component seed)
@@ -223,8 +221,6 @@ int alloc_all(emst_ctx* c)
     ENS(dup_runs, n);
     ENS(dup_gslot, n);
     if (comp) {
-        ENS(nlab, 2 * c->nleaves);
-        ENS(nmin, 2 * c->nleaves);
         ENS(visit, 2 * c->nleaves);
         ENS(list2, n);
     }
@@ -460,7 +456,7 @@ int build(emst_ctx* c)
         // the per-round component search has no round-1 search: every round
         // searches (launch_comp_round)
         HIPCHK(c, hipMemsetAsync(c->visit.p, 0, sizeof(unsigned long long) * 2 * c->nleaves, s));
-        launch_node_min_ids(c->pts.p, c->dcl.p, c->ncl, c->nleaves, c->bucket, c->nmin.p, c->visit.p, s);
+        launch_node_min_ids(c->pts.p, c->dcl.p, c->ncl, c->nleaves, c->bucket, c->nodes.p, c->visit.p, s);
         int rc1 = ensure_comp_seed(c);
         if (rc1) return rc1;
         if (comp_features_on(c)) launch_comp_seed_init(n, c->pts.p, c->csize.p, c->cbox.p, s);
@@ -602,7 +598,7 @@ void issue_round(emst_ctx* c, bool tr)
     const CycleState cy = cycle_state(c);
     for (int ph = 0; ph < 4; ++ph) {
         if (ph == 0 && c->opt.round_search == 1) {
-            launch_comp_round(c->dim, c->n, c->pts.p, c->nodes.p, c->nlab.p, c->nmin.p, c->visit.p, c->dcl.p, c->ncl,
+            launch_comp_round(c->dim, c->n, c->pts.p, c->nodes.p, c->visit.p, c->dcl.p, c->ncl,
                               c->nleaves, c->bucket, c->label.p, c->corr.p, c->corr_key.p, c->comp1.p,
                               c->inv.p, c->seed_scale, split_scratch(c), comp_seed(c), c->eu.p, c->eround.p, c->C, s);
             if (tr) (void)hipEventRecord(c->ev[4], s);
@@ -722,7 +718,7 @@ std::vector<uintptr_t> graph_key(emst_ctx* c)
             (uintptr_t)c->opt.round_search, (uintptr_t)comp_features_on(c), (uintptr_t)c->learned,
             (uintptr_t)c->model.p, (uintptr_t)c->csize.p, (uintptr_t)c->cbox.p, (uintptr_t)c->feat.p, (uintptr_t)c->ls.p,
             (uintptr_t)c->pred.p, (uintptr_t)c->seed.p, (uintptr_t)c->samples.p, (uintptr_t)c->nq_keep.p,
-            (uintptr_t)c->opt.log_samples, (uintptr_t)c->nlab.p, (uintptr_t)c->nmin.p, (uintptr_t)c->visit.p, (uintptr_t)c->list2.p, (uintptr_t)c->open_mask.p, (uintptr_t)c->open_cnt.p,
+            (uintptr_t)c->opt.log_samples, (uintptr_t)c->visit.p, (uintptr_t)c->list2.p, (uintptr_t)c->open_mask.p, (uintptr_t)c->open_cnt.p,
             (uintptr_t)c->open_tmp.p, (uintptr_t)c->list.p, (uintptr_t)c->nodes.p, (uintptr_t)c->dcl.p,
             (uintptr_t)y.j[0], (uintptr_t)y.kh[0], (uintptr_t)y.kl[0], (uintptr_t)y.onc, (uintptr_t)y.cut,
             (uintptr_t)c->grouped, (uintptr_t)d.npoints,

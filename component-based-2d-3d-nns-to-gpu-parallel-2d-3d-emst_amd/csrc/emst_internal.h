@@ -116,6 +116,10 @@ struct Cloud {
 struct alignas(32) Node {
     float lo[3];
     float hi[3];
+    // unused by the cone search; the component search keeps here the bits of
+    // the node's smallest point id (pad[0], per build) and of its component
+    // label (pad[1], per round: the common label of its points, -1 mixed,
+    // -3 empty)
     float pad[2];
 };
 
@@ -169,13 +173,12 @@ void launch_cone_search(int dim, int cdim, const float4* pts, const Node* nodes,
                         uint64_t* first_key, Counters* C, bool collect_stats, DupState dup,
                         const SplitScratch* split, hipStream_t s);
 // Phase 0 of a round of the per-round component search (the alternative to
-// the round-1 cone search): node component labels (nlab: 2 * nleaves words),
-// kept targets, and a search for the points whose target joined their
-// component; writes corr / corr_key and the component minima (comp1).
-// nmin: per-node minimum point id (launch_node_min_ids).  sc: open_mask, cnt,
-// off, tmp, list and list2; pass 0 runs over every searching point, the
-// later passes over those it left open, so that they see the component
-// minima pass 0 found.
+// the round-1 cone search): node component labels (into Node::pad[1]; visit:
+// the arrival words of launch_node_min_ids), kept targets, and a search for
+// the points whose target joined their component; writes corr / corr_key and
+// the component minima (comp1).  sc: open_mask, cnt, off, tmp, list and
+// list2; pass 0 runs over every searching point, the later passes over those
+// it left open, so that they see the component minima pass 0 found.
 // Learned seed of the component search: per-component size and box, kept at
 // the roots and merged after every round; features of the searching points
 // (list order), the model's output and the seed radius per sorted position.
@@ -194,10 +197,8 @@ struct CompSeed {
     int64_t per_round;      // pairs logged per round, at most (a stride over the searching points)
     uint32_t* nq_keep;      // 2 words: the counts the sample log and the model run on
 };
-void launch_comp_round(int dim, int64_t n, const float4* pts, const Node* nodes, int32_t* nlab,
-                       const uint32_t* nmin, unsigned long long* visit, const Cloud* cl, int ncl, int64_t nleaves,
-                       int bucket,
-                       const int32_t* label, int32_t* corr, uint64_t* corr_key, uint64_t* comp1, const int32_t* inv,
+void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsigned long long* visit,
+                       const Cloud* cl, int ncl, int64_t nleaves, int bucket, const int32_t* label, int32_t* corr, uint64_t* corr_key, uint64_t* comp1, const int32_t* inv,
                        float seed_scale, const SplitScratch& sc, const CompSeed& cs, const int32_t* eu,
                        const int32_t* eround, Counters* C, hipStream_t s);
 // Sizes and boxes of the single-point components (the start of a run).
@@ -205,11 +206,11 @@ void launch_comp_seed_init(int64_t n, const float4* pts, int32_t* csize, uint32_
 // MLP inference over X[0..*m_dev) (the comp search's device-side count).
 void launch_mlp_infer_dev(const float* model, const float* X, const uint32_t* m_dev, int64_t m_max, float* Y,
                           hipStream_t s);
-// Smallest point id under every node of the hierarchy (2 * nleaves words,
-// laid out as the nodes; 0xffffffff for a node without points).  visit:
+// Smallest point id under every node of the hierarchy, into Node::pad[0]
+// (0xffffffff for a node without points).  visit:
 // 2 * nleaves words of arrival state, zero before the first bottom-up launch
 // over this hierarchy (k_nmin_tree, k_nlab_tree) and kept between them.
-void launch_node_min_ids(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, int bucket, uint32_t* nmin,
+void launch_node_min_ids(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, int bucket, Node* nodes,
                          unsigned long long* visit, hipStream_t s);
 // Cones of every group point for the current labels (dim: the cone dimension) (after the grouped
 // cone search, and at the start of each round while a group may still span

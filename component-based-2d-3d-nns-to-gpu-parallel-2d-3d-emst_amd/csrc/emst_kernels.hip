@@ -1688,9 +1688,9 @@ constexpr int32_t kNlabMixed = -1;      // a node holding several components
 // by compare-and-swap, so no fence is needed between them); every launch
 // brings exactly two arrivals to every inner node, so the count's parity
 // tells first from second without a reset between launches.
-template <class Op>
-__device__ __forceinline__ void climb(typename Op::T v, typename Op::T* __restrict__ base,
-                                      unsigned long long* __restrict__ vbase, int64_t i)
+template <class Op, class Store>
+__device__ __forceinline__ void climb(typename Op::T v, Store store, unsigned long long* __restrict__ vbase,
+                                      int64_t i)
 {
     while (i > 1) {
         const int64_t par = i >> 1;
@@ -1710,7 +1710,7 @@ __device__ __forceinline__ void climb(typename Op::T v, typename Op::T* __restri
         }
         if (first) return;
         i = par;
-        base[i] = v;
+        store(i, v);
     }
 }
 
@@ -1729,7 +1729,7 @@ struct MinOp {
 // component label of every node: the common label of its points
 __global__ __launch_bounds__(kBlock) void k_nlab_tree(const int32_t* __restrict__ label,
                                                      const Cloud* __restrict__ cl, int ncl, int64_t nleaves,
-                                                     int bucket, int32_t* __restrict__ nlab,
+                                                     int bucket, Node* __restrict__ nodes,
                                                      unsigned long long* __restrict__ visit, const Counters* C)
 {
     if (C->done) return;
@@ -1741,9 +1741,10 @@ __global__ __launch_bounds__(kBlock) void k_nlab_tree(const int32_t* __restrict_
     const int64_t s1 = min(cl[c].pos + cl[c].n, s0 + bucket);
     int32_t lab = kNlabEmpty;
     for (int64_t s = s0; s < s1; ++s) lab = LabelOp::combine(lab, label[s]);
-    int32_t* base = nlab + 2 * cl[c].lbase;
-    base[cl[c].L + j] = lab;
-    climb<LabelOp>(lab, base, visit + 2 * cl[c].lbase, cl[c].L + j);
+    Node* base = nodes + 2 * cl[c].lbase;
+    auto store = [&](int64_t i, int32_t v) { base[i].pad[1] = __int_as_float(v); };
+    store(cl[c].L + j, lab);
+    climb<LabelOp>(lab, store, visit + 2 * cl[c].lbase, cl[c].L + j);
 }
 
 // smallest point id under every node (static per build): with ties on d2
@@ -1751,7 +1752,7 @@ __global__ __launch_bounds__(kBlock) void k_nlab_tree(const int32_t* __restrict_
 // id than the best so far (runs of coincident points)
 __global__ __launch_bounds__(kBlock) void k_nmin_tree(const float4* __restrict__ pts,
                                                      const Cloud* __restrict__ cl, int ncl, int64_t nleaves,
-                                                     int bucket, uint32_t* __restrict__ nmin,
+                                                     int bucket, Node* __restrict__ nodes,
                                                      unsigned long long* __restrict__ visit)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1762,9 +1763,10 @@ __global__ __launch_bounds__(kBlock) void k_nmin_tree(const float4* __restrict__
     const int64_t s1 = min(cl[c].pos + cl[c].n, s0 + bucket);
     uint32_t m = 0xffffffffu;
     for (int64_t s = s0; s < s1; ++s) m = min(m, __float_as_uint(pts[s].w));
-    uint32_t* base = nmin + 2 * cl[c].lbase;
-    base[cl[c].L + j] = m;
-    climb<MinOp>(m, base, visit + 2 * cl[c].lbase, cl[c].L + j);
+    Node* base = nodes + 2 * cl[c].lbase;
+    auto store = [&](int64_t i, uint32_t v) { base[i].pad[0] = __uint_as_float(v); };
+    store(cl[c].L + j, m);
+    climb<MinOp>(m, store, visit + 2 * cl[c].lbase, cl[c].L + j);
 }
 
 // Per point, corr / corr_key between rounds:
@@ -2005,11 +2007,11 @@ __global__ __launch_bounds__(kBlock) void k_comp_samples(const int32_t* __restri
 // its lower bound (corr = -1).  Exact results enter comp1 as they close.
 template <int D>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_comp_search(
-    const float4* __restrict__ pts, const Node* __restrict__ nodes, const int32_t* __restrict__ nlab,
+    const float4* __restrict__ pts, const Node* __restrict__ nodes,
     const Cloud* __restrict__ clouds, int ncl, int bucket, const int32_t* __restrict__ qlist,
     const uint32_t* __restrict__ nq_dev, const int32_t* __restrict__ label, const int32_t* __restrict__ inv,
     float seed_scale, int32_t* __restrict__ corr, uint64_t* __restrict__ corr_key,
-    uint64_t* __restrict__ comp1, const uint32_t* __restrict__ nmin, int pass_lo, int pass_hi,
+    uint64_t* __restrict__ comp1, int pass_lo, int pass_hi,
     uint64_t* __restrict__ open_out, const float* __restrict__ seed_r, const Counters* C)
 {
     if (C->done) return;
@@ -2043,8 +2045,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         const int64_t cpos = clouds[cl].pos, cend = clouds[cl].pos + clouds[cl].n;
         const int64_t L = clouds[cl].L;
         const Node* __restrict__ tree = nodes + 2 * clouds[cl].lbase;
-        const int32_t* __restrict__ tlab = nlab + 2 * clouds[cl].lbase;
-        const uint32_t* __restrict__ tmin = nmin + 2 * clouds[cl].lbase;
         // seed: twice the previous target's distance, at least seed_scale
         // times the mean spacing of the point's leaf cell
         float r = 1.f;
@@ -2148,7 +2148,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                 if (pver == ver) {
                     need = (stk_mask[sp] >> lane) & 1ull;
                 } else {
-                    need = needed(tree[node], tlab[node], tmin[node], lim, bk);
+                    {
+                        const Node nd = tree[node];
+                        need = needed(nd, __float_as_int(nd.pad[1]), __float_as_uint(nd.pad[0]), lim, bk);
+                    }
                 }
                 const uint64_t needm = __ballot(need);
                 if (needm == 0ull) continue;
@@ -2202,8 +2205,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                     for (int i = 0; i < 4; ++i)
                         if (i < nk) {
                             bx[i] = tree[first + i];
-                            bl[i] = tlab[first + i];
-                            bm[i] = tmin[first + i];
+                            bl[i] = __float_as_int(bx[i].pad[1]);
+                            bm[i] = __float_as_uint(bx[i].pad[0]);
                         }
                     uint64_t mk[4];
                     float dk[4];
@@ -2748,11 +2751,11 @@ void launch_cone_search(int dim, int cdim, const float4* pts, const Node* nodes,
                        (int64_t)(split->compact_frac * (double)nq));
 }
 
-void launch_node_min_ids(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, int bucket, uint32_t* nmin,
+void launch_node_min_ids(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, int bucket, Node* nodes,
                          unsigned long long* visit, hipStream_t s)
 {
     if (nleaves <= 0) return;
-    hipLaunchKernelGGL(k_nmin_tree, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, pts, cl, ncl, nleaves, bucket, nmin,
+    hipLaunchKernelGGL(k_nmin_tree, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, pts, cl, ncl, nleaves, bucket, nodes,
                        visit);
 }
 
@@ -2762,8 +2765,8 @@ void launch_comp_seed_init(int64_t n, const float4* pts, int32_t* csize, uint32_
     hipLaunchKernelGGL(k_comp_seed_init, dim3(nblocks(n)), dim3(kBlock), 0, s, n, pts, csize, cbox);
 }
 
-void launch_comp_round(int dim, int64_t n, const float4* pts, const Node* nodes, int32_t* nlab,
-                       const uint32_t* nmin, unsigned long long* visit, const Cloud* cl, int ncl, int64_t nleaves,
+void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsigned long long* visit,
+                       const Cloud* cl, int ncl, int64_t nleaves,
                        int bucket,
                        const int32_t* label, int32_t* corr, uint64_t* corr_key, uint64_t* comp1, const int32_t* inv,
                        float seed_scale, const SplitScratch& sc, const CompSeed& cs, const int32_t* eu,
@@ -2775,7 +2778,7 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, const Node* nodes,
         hipLaunchKernelGGL(k_comp_merge, dim3(g), dim3(kBlock), 0, s, n, label, eu, eround, cs.csize, cs.cbox,
                            (const Counters*)C);
     hipLaunchKernelGGL(k_nlab_tree, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, label, cl, ncl, nleaves, bucket,
-                       nlab, visit, (const Counters*)C);
+                       nodes, visit, (const Counters*)C);
     const int64_t P = (n + 63) / 64;
     hipLaunchKernelGGL(k_comp_keep, dim3(g), dim3(kBlock), 0, s, n, pts, label, (const int32_t*)corr,
                        (const uint64_t*)corr_key, comp1, (const Counters*)C);
@@ -2800,13 +2803,13 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, const Node* nodes,
     }
     auto kern = dim == 3 ? k_comp_search<3> : k_comp_search<2>;
     // pass 0 over the searching points; their open packets' lanes into open_mask
-    hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, nodes, (const int32_t*)nlab, cl, ncl, bucket,
-                       (const int32_t*)sc.list, nq, label, inv, seed_scale, corr, corr_key, comp1, nmin, 0, 0,
+    hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, (const Node*)nodes, cl, ncl, bucket,
+                       (const int32_t*)sc.list, nq, label, inv, seed_scale, corr, corr_key, comp1, 0, 0,
                        sc.open_mask, seed_r, (const Counters*)C);
     open_compact(sc.open_mask, P, sc.off, sc.tmp, (const int32_t*)sc.list, sc.list2, s);
-    hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, nodes, (const int32_t*)nlab, cl, ncl, bucket,
+    hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, (const Node*)nodes, cl, ncl, bucket,
                        (const int32_t*)sc.list2, (const uint32_t*)(sc.off + P), label, inv, seed_scale, corr,
-                       corr_key, comp1, nmin, 1, kMaxCappedPasses, (uint64_t*)nullptr, seed_r, (const Counters*)C);
+                       corr_key, comp1, 1, kMaxCappedPasses, (uint64_t*)nullptr, seed_r, (const Counters*)C);
     // comp1 now holds every component's edge: the sample log of this round
     if (cs.features_on && cs.samples != nullptr)
         hipLaunchKernelGGL(k_comp_samples, dim3(g), dim3(kBlock), 0, s, (const int32_t*)sc.list,
