@@ -4,8 +4,11 @@ Workload (default): 10^7 points in 3D, mixture of 64 isotropic Gaussians
 (centres uniform in [0,1)^3, sigma log-uniform 0.002-0.02) plus 10% uniform
 background, float32, clipped to the unit cube, generator seed 3.  One step =
 one whole EMST from device-resident points to the device-resident edge list
-and its FP64 weight: bbox, rescale, Morton sort, hierarchy, cone search,
-Boruvka rounds, weight.
+and its FP64 weight: bbox, rescale, Morton sort, hierarchy, the search (the
+round-1 cone search, or the per-round component search), Boruvka rounds,
+weight.  Every form (cone search plain and learned-seed, component search) is
+timed on the same points and must build the same tree; `value` is the
+headline form's: the faster one for the dimension unless --round-search picks.
 
 Prints one JSON line (rank 0).  --gpus above 1 is refused: the engine is one
 GPU per process and this build measures one GPU.
@@ -20,15 +23,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 
-def profiled_traffic(args):
-    """HBM bytes per cone-search launch from the committed rocprofv3 summary of
-    this workload (FETCH_SIZE + WRITE_SIZE, each scaled by a plain copy), or
-    None when no summary covers it."""
+def profiled_traffic(args, kernel="k_cone_search"):
+    """HBM bytes per launch of `kernel` (for k_comp_search: all its dispatches
+    of one EMST) from the committed rocprofv3 summary of this workload
+    (FETCH_SIZE + WRITE_SIZE, each scaled by a plain copy), or None when no
+    summary covers it."""
     tags = {(1, 10_000_000, 3): "gauss64_1e7_3d", (0, 1_000_000, 3): "uniform_1e6_3d"}
     tag = tags.get((args.kind, args.n, args.dim))
     if tag is None or args.cone_dim not in (0, args.dim):
         return None
-    path = os.path.join(ROOT, "profiles", f"k_cone_search_{tag}.md")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{tag}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -50,8 +54,9 @@ def main():
     ap.add_argument("--cone-dim", type=int, default=0,
                     help="2 with --dim 3: the 2.5D mode (wedges over the xy plane, 3D distances)")
     ap.add_argument("--plain", action="store_true", help="plain seed only (skip the learned form)")
-    ap.add_argument("--round-search", type=int, default=0, choices=(0, 1),
-                    help="1: per-round component search instead of the round-1 cone search (plain seed)")
+    ap.add_argument("--round-search", type=int, default=-1, choices=(-1, 0, 1),
+                    help="headline form: 0 the round-1 cone search, 1 the per-round component search, "
+                         "-1 the faster one for the dimension (component search in 3D, cone search in 2D)")
     ap.add_argument("--cpu-sample", type=int, default=200_000,
                     help="points of the same distribution timed on the CPU oracle (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -98,16 +103,21 @@ def main():
 
     cdim = args.cone_dim or args.dim
     K = 2 * cdim * (1 << (cdim - 1))
-    tag = "comp_" if args.round_search else ""
-    model_path = os.path.join(ge.PKG_DIR, "models", f"radius_{tag}{args.dim}d.npy")
-    model = np.load(model_path) if (os.path.exists(model_path) and not args.plain and cdim == args.dim) else None
-    model_clouds = None
-    if model is not None:
-        meta = os.path.join(ge.PKG_DIR, "models", f"train_radius{'_comp' if tag else ''}.json")
-        if os.path.exists(meta):
-            model_clouds = json.load(open(meta)).get("clouds")
+    # the headline form: the faster exact search for the dimension (measured:
+    # the per-round component search in 3D, the cone search in 2D and 2.5D)
+    rs = args.round_search if args.round_search >= 0 else (1 if (args.dim == 3 and cdim == 3) else 0)
 
-    def timed(form_model, round_search=args.round_search):
+    def load_model(tag):
+        path = os.path.join(ge.PKG_DIR, "models", f"radius_{tag}{args.dim}d.npy")
+        if args.plain or cdim != args.dim or not os.path.exists(path):
+            return None, None
+        meta = os.path.join(ge.PKG_DIR, "models", f"train_radius{'_comp' if tag else ''}.json")
+        return np.load(path), (json.load(open(meta)).get("clouds") if os.path.exists(meta) else None)
+
+    cone_model, model_clouds = load_model("")
+    comp_model, _ = load_model("comp_")
+
+    def timed(form_model, round_search):
         ctx = pkg.Calculateur(dim=args.dim, bucket_size=args.bucket, cone_dim=args.cone_dim,
                               round_search=round_search)
         if form_model is not None:
@@ -132,31 +142,36 @@ def main():
         dt = time.perf_counter() - t0
         u, v, _ = ctx.getLinks()
         st = pkg.search_stats(ctx)
-        samples = ctx.radius_samples() if (form_model is not None and not args.round_search) else None
+        samples = ctx.radius_samples() if (form_model is not None and not round_search) else None
         ctx.close()
         assert m == args.n - 1, f"tree has {m} edges"
         med = lambda xs: sorted(xs)[len(xs) // 2]
         return {"dt": dt, "w": w, "m": m, "search_ms": med(search_ms), "total_ms": med(total_ms),
-                "edges_digest": hash((u.tobytes(), v.tobytes())), "stats": st, "samples": samples}
+                "edges_digest": hash((u.tobytes(), v.tobytes())), "stats": st, "samples": samples,
+                "mpts": args.n * args.steps / dt / 1e6}
 
-    plain = timed(None)
-    learned = timed(model) if model is not None else None
-    # the other search form on the same points (same forest, checked)
-    other = None
+    # every form on the same points; each must build the same tree
+    forms = {"cone_plain": timed(None, 0)}
+    if cone_model is not None:
+        forms["cone_learned"] = timed(cone_model, 0)
     if cdim == args.dim:
-        other = timed(None, round_search=1 - args.round_search)
-        assert other["edges_digest"] == plain["edges_digest"], "the two search forms built different trees"
-    if learned is not None:
-        assert learned["edges_digest"] == plain["edges_digest"], "learned seeds changed the tree"
-        assert learned["w"] == plain["w"]
-    head = learned if learned is not None else plain
+        forms["comp_plain"] = timed(None, 1)
+        if comp_model is not None:
+            forms["comp_learned"] = timed(comp_model, 1)
+    ref = forms["cone_plain"]
+    for name, f in forms.items():
+        assert f["edges_digest"] == ref["edges_digest"] and f["w"] == ref["w"], f"{name} built a different tree"
+    fam = "comp" if rs == 1 else "cone"
+    head_name = f"{fam}_learned" if f"{fam}_learned" in forms else f"{fam}_plain"
+    head = forms[head_name]
+    plain, learned = forms["cone_plain"], forms.get("cone_learned")
     seed_report = None
-    if learned is not None and not args.round_search:
+    if learned is not None:
         # hit rate: points whose seed already covers every non-empty cone's
         # nearest point (pass 0 closes them all); savings: traversal work
         X, T = learned["samples"]
         has = T > -19.0
-        y = pkg.mlp_eval(model, X) + model[-1]
+        y = pkg.mlp_eval(cone_model, X) + cone_model[-1]
         ps, ls = plain["stats"], learned["stats"]
         seed_report = {
             "hit_rate_learned": round(float(np.mean((y >= T)[has])), 4),
@@ -167,20 +182,35 @@ def main():
             "pair_savings": round(1 - ls["pairs"] / ps["pairs"], 4),
         }
     ms_per_step = head["dt"] * 1e3 / args.steps
-    value = args.n * args.steps / head["dt"] / 1e6
+    value = head["mpts"]
     w, m = head["w"], head["m"]
 
-    # dominant kernel: the round-1 cone search; algorithmic bytes per point
-    # = 8*D (own coordinates) + 4 + 8*K (one (id, d2) per cone out);
-    # time from HIP events on the context stream
-    alg_bytes = args.n * (8 * args.dim + 4 + 8 * K)
-    s_ms = head["search_ms"]
-    achieved = alg_bytes / (s_ms * 1e-3) / 1e9 if s_ms > 0 else 0.0
+    if fam == "cone":
+        # dominant kernel: the round-1 cone search; algorithmic bytes per
+        # point = 8*D (own coordinates) + 4 + 8*K (one (id, d2) per cone out);
+        # time from HIP events on the context stream
+        alg_bytes = args.n * (8 * args.dim + 4 + 8 * K)
+        k_ms = head["search_ms"]
+        kernel = "k_cone_search"
+    else:
+        # dominant kernel: the component search's dispatches of one EMST (two
+        # per round); algorithmic bytes per query and dispatch = 44: its point
+        # (16), label (4), previous key (8) and list entry (4) in, its result
+        # (corr 4, key 8) out.  Time from HIP events around each dispatch, in
+        # one traced (host-issued) run on the context stream
+        with pkg.Calculateur(dim=args.dim, bucket_size=args.bucket, round_search=1, trace=True) as c:
+            if head_name == "comp_learned":
+                c.set_radius_model(comp_model)
+            c.initialize_device(dev.data_ptr(), args.n, args.dim)
+            c.run()
+            k_ms = c.times()[4]
+            queries = pkg.search_stats(c)["comp_queries"]
+        alg_bytes = 44 * queries
+        kernel = "k_comp_search"
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-                "frac": round(achieved / 8000.0, 4), "traffic": profiled_traffic(args),
-                "kernel": "k_cone_search", "kernel_ms": round(s_ms, 3)}
-    if args.round_search:
-        roofline = None     # no round-1 cone search: the searches are spread over the rounds
+                "frac": round(achieved / 8000.0, 4), "traffic": profiled_traffic(args, kernel),
+                "kernel": kernel, "kernel_ms": round(k_ms, 3)}
 
     cpu = None
     if args.cpu_sample > 0:
@@ -209,29 +239,34 @@ def main():
                          f"parallel, rounds sequential), {ct:.1f} s; single thread on the first {n1} points: "
                          f"{n1 / c1 / 1e6:.5f} Mpoints/s ({c1:.1f} s)"}
 
+    desc = {"cone_plain": "cone search, plain seed", "cone_learned": "cone search, learned seed",
+            "comp_plain": "per-round component search, plain seed",
+            "comp_learned": "per-round component search, learned seed"}
+    mp = lambda k: round(forms[k]["mpts"], 3) if k in forms else None
     out = {
-        "metric": "EMST Mpoints/s at 10^7 points, learned-radius form (plain form in config)",
+        "metric": "EMST Mpoints/s at 10^7 points (fastest exact form; learned-radius vs plain in config)",
         "value": round(value, 3), "unit": "Mpoints/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (project generator, seeded)",
         "config": {"workload": f"EMST {args.n} pts {args.dim}D{' (2.5D: xy wedges)' if cdim < args.dim else ''} "
                                f"{names.get(args.kind)} seed {args.seed}",
-                   "n": args.n, "dim": args.dim, "form": "learned" if learned is not None else "plain",
+                   "n": args.n, "dim": args.dim, "form": desc[head_name],
                    "edges": int(m), "weight": w,
                    "gpu_total_ms_median": round(head["total_ms"], 3),
-                   "plain_mpts": round(args.n * args.steps / plain["dt"] / 1e6, 3),
+                   "mpts_by_form": {k: mp(k) for k in desc},
+                   "same_tree_all_forms": True,
+                   # learned vs plain: the cone search's seed model (the
+                   # component search ships none: its model measured slower)
+                   "plain_mpts": mp("cone_plain"),
                    "plain_search_ms": round(plain["search_ms"], 3),
-                   "learned_mpts": round(args.n * args.steps / learned["dt"] / 1e6, 3) if learned else None,
+                   "learned_mpts": mp("cone_learned"),
                    "learned_search_ms": round(learned["search_ms"], 3) if learned else None,
                    "learned_vs_plain": round(plain["dt"] / learned["dt"], 3) if learned else None,
                    "radius_model_training_clouds": model_clouds,
                    "learned_seed": seed_report,
-                   "same_tree": learned is not None,
-                   "cone_dim": cdim, "cycles_cut": int(head["stats"]["cycles_cut"]),
-                   "round_search": args.round_search,
-                   "other_search_form_plain_mpts": (round(args.n * args.steps / other["dt"] / 1e6, 3)
-                                                    if other else None)},
+                   "cone_dim": cdim, "cycles_cut": int(forms["cone_plain"]["stats"]["cycles_cut"]),
+                   "round_search": rs},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

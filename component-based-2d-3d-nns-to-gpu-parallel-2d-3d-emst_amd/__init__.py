@@ -332,8 +332,8 @@ class Calculateur:
         return X, T
 
     def times(self):
-        t = (ctypes.c_float * 4)()
-        k = ctypes.c_int32(4)
+        t = (ctypes.c_float * 5)()
+        k = ctypes.c_int32(5)
         self._chk(self._L.emst_get_times(self._c, t, ctypes.byref(k)), "times")
         return list(t)[: k.value]
 
@@ -361,12 +361,13 @@ def mlp_eval(model, X, device=0):
 
 
 def _stats_names():
-    return ["packets", "nodes", "leaves", "leaf_points", "pairs", "group_points", "grouped", "cycles_cut", "resumed"]
+    return ["packets", "nodes", "leaves", "leaf_points", "pairs", "group_points", "grouped", "cycles_cut", "resumed",
+            "comp_queries"]
 
 
 def search_stats(calc):
-    t = (ctypes.c_int64 * 9)()
-    k = ctypes.c_int32(9)
+    t = (ctypes.c_int64 * 10)()
+    k = ctypes.c_int32(10)
     calc._chk(calc._L.emst_get_search_stats(calc._c, t, ctypes.byref(k)), "stats")
     return dict(zip(_stats_names(), list(t)[: k.value]))
 

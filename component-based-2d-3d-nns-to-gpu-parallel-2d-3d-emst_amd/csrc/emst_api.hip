@@ -64,10 +64,11 @@ struct emst_ctx {
     uint32_t zero_or_empty = 0;
     std::vector<emst_round_trace> trace;
     float times[4] = {0, 0, 0, 0};
-    int64_t stats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t stats[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t resumed_h = 0;           // queries resumed after the first search pass
     float seed_scale = 3.0f;          // plain seed: multiple of the leaf cell's mean spacing
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[12] = {};             // 8..11: around the component search's dispatches (traced runs)
+    float comp_search_ms = 0.f;         // those dispatches' time, summed over the traced run's rounds
     // device buffers
     DevBuf<float> xyz_own, resc;
     DevBuf<uint64_t> keys0, keys1, cand_key, corr_key, comp1;
@@ -468,6 +469,7 @@ int build(emst_ctx* c)
         int rc = read_counters(c, &h);
         if (rc) return rc;
         c->zero_or_empty = 0;
+        c->comp_search_ms = 0.f;
         for (auto& x : c->stats) x = 0;
         c->resumed_h = 0;
         c->times[0] = elapsed(c->ev[0], c->ev[1]);
@@ -600,7 +602,8 @@ void issue_round(emst_ctx* c, bool tr)
         if (ph == 0 && c->opt.round_search == 1) {
             launch_comp_round(c->dim, c->n, c->pts.p, c->nodes.p, c->visit.p, c->dcl.p, c->ncl,
                               c->nleaves, c->bucket, c->label.p, c->corr.p, c->corr_key.p, c->comp1.p,
-                              c->inv.p, c->seed_scale, split_scratch(c), comp_seed(c), c->eu.p, c->eround.p, c->C, s);
+                              c->inv.p, c->seed_scale, split_scratch(c), comp_seed(c), c->eu.p, c->eround.p, c->C, s,
+                              tr ? c->ev + 8 : nullptr);
             if (tr) (void)hipEventRecord(c->ev[4], s);
             continue;
         }
@@ -640,6 +643,7 @@ void absorb_counters(emst_ctx* c, const Counters& h)
     c->edges = h.edges;
     c->rounds = (int)h.round;
     c->stats[7] = (int64_t)h.cycles_cut;
+    c->stats[9] = (int64_t)h.comp_queries;
     if (h.done == 1u) c->finished = true;
 }
 
@@ -682,6 +686,8 @@ int round_once(emst_ctx* c, int32_t* finished)
     Counters h{};
     int rc = read_counters(c, &h);
     if (rc) return rc;
+    if (tr && c->opt.round_search == 1 && n > 0)
+        c->comp_search_ms += elapsed(c->ev[8], c->ev[9]) + elapsed(c->ev[10], c->ev[11]);
     if (h.done == 2u && (rc = resume_flatten(c, &h))) return rc;
     absorb_counters(c, h);
     c->groups_settled = h.dup_settled != 0u;
@@ -1265,8 +1271,8 @@ int emst_get_options(emst_ctx* c, emst_options* o)
 int emst_get_times(emst_ctx* c, float* times, int32_t* n_times)
 {
     if (!c || !times || !n_times) return EMST_E_ARG;
-    const int32_t k = std::min<int32_t>(*n_times, 4);
-    for (int i = 0; i < k; ++i) times[i] = c->times[i];
+    const int32_t k = std::min<int32_t>(*n_times, 5);
+    for (int i = 0; i < k; ++i) times[i] = i < 4 ? c->times[i] : c->comp_search_ms;
     *n_times = k;
     return EMST_OK;
 }
@@ -1339,7 +1345,7 @@ int emst_get_radius_samples(emst_ctx* c, float* feat, float* target, int64_t* m)
 int emst_get_search_stats(emst_ctx* c, int64_t* stats, int32_t* n_stats)
 {
     if (!c || !stats || !n_stats) return EMST_E_ARG;
-    const int32_t k = std::min<int32_t>(*n_stats, 9);
+    const int32_t k = std::min<int32_t>(*n_stats, 10);
     for (int i = 0; i < k; ++i) stats[i] = c->stats[i];
     *n_stats = k;
     return EMST_OK;

@@ -50,6 +50,7 @@ struct Counters {
     uint32_t dup_big;       // runs longer than one wave handles, queued for a grid-wide pass
     uint32_t nroots;        // roots listed by k_cycle_init this round (2.5D cycle rule)
     uint32_t nsamples;      // seed samples logged by the component search over the run
+    unsigned long long comp_queries;    // queries over the component search's dispatches, over the run
 };
 
 // Coincident-point groups.  Points with identical rescaled coordinates form a
@@ -200,7 +201,8 @@ struct CompSeed {
 void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsigned long long* visit,
                        const Cloud* cl, int ncl, int64_t nleaves, int bucket, const int32_t* label, int32_t* corr, uint64_t* corr_key, uint64_t* comp1, const int32_t* inv,
                        float seed_scale, const SplitScratch& sc, const CompSeed& cs, const int32_t* eu,
-                       const int32_t* eround, Counters* C, hipStream_t s);
+                       const int32_t* eround, Counters* C, hipStream_t s, hipEvent_t* ev = nullptr);
+// (ev: 4 events recorded around the two k_comp_search dispatches, or nullptr)
 // Sizes and boxes of the single-point components (the start of a run).
 void launch_comp_seed_init(int64_t n, const float4* pts, int32_t* csize, uint32_t* cbox, hipStream_t s);
 // MLP inference over X[0..*m_dev) (the comp search's device-side count).

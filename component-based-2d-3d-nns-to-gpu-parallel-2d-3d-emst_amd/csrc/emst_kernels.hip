@@ -1326,6 +1326,7 @@ __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __res
         C->edges = 0; C->research = 0; C->zero_or_empty = 0; C->unflat = 0;
         C->packets = C->nodes = C->leaves = C->leaf_points = C->pairs = 0;
         C->round = 0; C->done = 0; C->edges_prev = 0; C->dup_settled = 0; C->nsamples = 0;
+        C->comp_queries = 0;
         C->target = target; C->max_rounds = max_rounds;
     }
     if (i >= n) return;
@@ -2012,10 +2013,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     const uint32_t* __restrict__ nq_dev, const int32_t* __restrict__ label, const int32_t* __restrict__ inv,
     float seed_scale, int32_t* __restrict__ corr, uint64_t* __restrict__ corr_key,
     uint64_t* __restrict__ comp1, int pass_lo, int pass_hi,
-    uint64_t* __restrict__ open_out, const float* __restrict__ seed_r, const Counters* C)
+    uint64_t* __restrict__ open_out, const float* __restrict__ seed_r, Counters* C)
 {
     if (C->done) return;
     const int64_t nq = *nq_dev;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&C->comp_queries, (unsigned long long)nq);
     if ((int64_t)blockIdx.x * 64 >= nq) {
         if (open_out != nullptr && threadIdx.x == 0) open_out[blockIdx.x] = 0ull;
         return;
@@ -2770,7 +2772,7 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsig
                        int bucket,
                        const int32_t* label, int32_t* corr, uint64_t* corr_key, uint64_t* comp1, const int32_t* inv,
                        float seed_scale, const SplitScratch& sc, const CompSeed& cs, const int32_t* eu,
-                       const int32_t* eround, Counters* C, hipStream_t s)
+                       const int32_t* eround, Counters* C, hipStream_t s, hipEvent_t* ev)
 {
     if (n <= 0) return;
     const unsigned g = nblocks(n);
@@ -2803,13 +2805,17 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsig
     }
     auto kern = dim == 3 ? k_comp_search<3> : k_comp_search<2>;
     // pass 0 over the searching points; their open packets' lanes into open_mask
+    if (ev) (void)hipEventRecord(ev[0], s);
     hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, (const Node*)nodes, cl, ncl, bucket,
                        (const int32_t*)sc.list, nq, label, inv, seed_scale, corr, corr_key, comp1, 0, 0,
-                       sc.open_mask, seed_r, (const Counters*)C);
+                       sc.open_mask, seed_r, C);
+    if (ev) (void)hipEventRecord(ev[1], s);
     open_compact(sc.open_mask, P, sc.off, sc.tmp, (const int32_t*)sc.list, sc.list2, s);
+    if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, (const Node*)nodes, cl, ncl, bucket,
                        (const int32_t*)sc.list2, (const uint32_t*)(sc.off + P), label, inv, seed_scale, corr,
-                       corr_key, comp1, 1, kMaxCappedPasses, (uint64_t*)nullptr, seed_r, (const Counters*)C);
+                       corr_key, comp1, 1, kMaxCappedPasses, (uint64_t*)nullptr, seed_r, C);
+    if (ev) (void)hipEventRecord(ev[3], s);
     // comp1 now holds every component's edge: the sample log of this round
     if (cs.features_on && cs.samples != nullptr)
         hipLaunchKernelGGL(k_comp_samples, dim3(g), dim3(kBlock), 0, s, (const int32_t*)sc.list,

@@ -212,7 +212,9 @@ int emst_get_cell_grid(emst_ctx* ctx, int32_t level_radius, int32_t* ext_dual,
 /* Kernel times of the last emst_run (ms, device events on the context's
  * stream): [0] build (bbox, rescale, sort, hierarchy, coincident groups, the
  * learned seed's features and inference when set, and the candidates ranked
- * for the rounds), [1] round-1 cone search, [2] Boruvka rounds, [3] total.
+ * for the rounds), [1] round-1 cone search, [2] Boruvka rounds, [3] total,
+ * [4] the per-round component search's search dispatches (k_comp_search)
+ * summed over the rounds, in traced runs (options.trace; 0 otherwise).
  * n_times: capacity in, count out. */
 int emst_get_times(emst_ctx* ctx, float* times, int32_t* n_times);
 
@@ -224,7 +226,10 @@ int emst_get_times(emst_ctx* ctx, float* times, int32_t* n_times);
  * groups (one search per location), 0 when every point searched itself,
  * [7] hooks withheld by the 2.5D mode's cycle rule over the rounds run so far,
  * [8] queries still open after the first pass of a split search (see
- * emst_options.search_split; 0 when the search ran as one launch).
+ * emst_options.search_split; 0 when the search ran as one launch),
+ * [9] queries over all the per-round component search's dispatches of the
+ * run so far (round_search 1; a query resumed after its first pass counts
+ * twice).
  * No counterpart in the reference (its per-phase timers are in
  * SolutionEMSTOperators.cu:166-373).  n_stats: capacity in, count out. */
 int emst_get_search_stats(emst_ctx* ctx, int64_t* stats, int32_t* n_stats);

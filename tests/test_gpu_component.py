@@ -208,3 +208,21 @@ def test_batch_with_tiny_clouds_and_round_cap(pkg):
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     for a, b in zip(out[0][2], out[1][2]):
         assert np.array_equal(a, b)
+
+
+def test_context_reuse_across_point_sets(pkg):
+    """One context, several point sets of different sizes and dimensions'
+    distributions in a row (buffers regrown, the captured rounds re-captured):
+    each run equals a fresh context's."""
+    sets = [pkg.synth(1, 30_000, 3, 55), pkg.synth(0, 5_000, 3, 56), pkg.synth(2, 80_000, 3, 57),
+            pkg.synth(0, 5_000, 3, 56)]
+    with pkg.Calculateur(dim=3, round_search=1) as c:
+        for pts in sets:
+            c.initialize(pts)
+            c.run()
+            u, v, _ = c.getLinks()
+            fresh = run(pkg, pts, round_search=1)
+            assert np.array_equal(np.stack([u, v], 1).astype(np.int64), fresh["e"])
+            c.run()                     # a second run on the same points
+            u2, v2, _ = c.getLinks()
+            assert np.array_equal(u2, u) and np.array_equal(v2, v)
