@@ -8,7 +8,7 @@ and its FP64 weight: bbox, rescale, Morton sort, hierarchy, the search (the
 round-1 cone search, or the per-round component search), Boruvka rounds,
 weight.  Every form (cone search plain and learned-seed, component search) is
 timed on the same points and must build the same tree; `value` is the
-headline form's: the faster one for the dimension unless --round-search picks.
+headline form's: the fastest measured, unless --round-search picks a search.
 
 Prints one JSON line (rank 0).  --gpus above 1 is refused: the engine is one
 GPU per process and this build measures one GPU.
@@ -55,8 +55,8 @@ def main():
                     help="2 with --dim 3: the 2.5D mode (wedges over the xy plane, 3D distances)")
     ap.add_argument("--plain", action="store_true", help="plain seed only (skip the learned form)")
     ap.add_argument("--round-search", type=int, default=-1, choices=(-1, 0, 1),
-                    help="headline form: 0 the round-1 cone search, 1 the per-round component search, "
-                         "-1 the faster one for the dimension (component search in 3D, cone search in 2D)")
+                    help="headline form: the fastest of the cone search's forms (0), of the per-round "
+                         "component search's (1), or of all (-1)")
     ap.add_argument("--cpu-sample", type=int, default=200_000,
                     help="points of the same distribution timed on the CPU oracle (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -103,9 +103,6 @@ def main():
 
     cdim = args.cone_dim or args.dim
     K = 2 * cdim * (1 << (cdim - 1))
-    # the headline form: the faster exact search for the dimension (measured:
-    # the per-round component search in 3D, the cone search in 2D and 2.5D)
-    rs = args.round_search if args.round_search >= 0 else (1 if (args.dim == 3 and cdim == 3) else 0)
 
     def load_model(tag):
         path = os.path.join(ge.PKG_DIR, "models", f"radius_{tag}{args.dim}d.npy")
@@ -161,8 +158,16 @@ def main():
     ref = forms["cone_plain"]
     for name, f in forms.items():
         assert f["edges_digest"] == ref["edges_digest"] and f["w"] == ref["w"], f"{name} built a different tree"
-    fam = "comp" if rs == 1 else "cone"
-    head_name = f"{fam}_learned" if f"{fam}_learned" in forms else f"{fam}_plain"
+    # the headline form: the fastest measured (--round-search: the fastest of
+    # that search's forms)
+    if args.round_search < 0:
+        head_name = max(forms, key=lambda k: forms[k]["mpts"])
+    else:
+        fam = "comp" if args.round_search == 1 else "cone"
+        head_name = max((k for k in forms if k.startswith(fam)), key=lambda k: forms[k]["mpts"],
+                        default="cone_plain")
+    fam = "comp" if head_name.startswith("comp") else "cone"
+    rs = 1 if fam == "comp" else 0
     head = forms[head_name]
     plain, learned = forms["cone_plain"], forms.get("cone_learned")
     seed_report = None
