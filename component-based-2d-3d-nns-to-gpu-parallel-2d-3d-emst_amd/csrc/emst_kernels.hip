@@ -2006,15 +2006,17 @@ __global__ __launch_bounds__(kBlock) void k_comp_samples(const int32_t* __restri
 // each pass) also bounds the search: a point whose nearest outside point lies
 // beyond it cannot hold the component's edge, and it stops with that d2 as
 // its lower bound (corr = -1).  Exact results enter comp1 as they close.
-template <int D>
+template <int D, int BK>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_comp_search(
     const float4* __restrict__ pts, const Node* __restrict__ nodes,
-    const Cloud* __restrict__ clouds, int ncl, int bucket, const int32_t* __restrict__ qlist,
+    const Cloud* __restrict__ clouds, int ncl, int bucket_rt, const int32_t* __restrict__ qlist,
     const uint32_t* __restrict__ nq_dev, const int32_t* __restrict__ label, const int32_t* __restrict__ inv,
     float seed_scale, int32_t* __restrict__ corr, uint64_t* __restrict__ corr_key,
     uint64_t* __restrict__ comp1, int pass_lo, int pass_hi,
     uint64_t* __restrict__ open_out, const float* __restrict__ seed_r, Counters* C)
 {
+    // the leaf size: a compile-time constant for the default sizes
+    const int bucket = BK ? BK : bucket_rt;
     if (C->done) return;
     const int64_t nq = *nq_dev;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&C->comp_queries, (unsigned long long)nq);
@@ -2803,7 +2805,8 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsig
             seed_r = cs.R;
         }
     }
-    auto kern = dim == 3 ? k_comp_search<3> : k_comp_search<2>;
+    auto kern = dim == 3 ? (bucket == 64 ? k_comp_search<3, 64> : k_comp_search<3, 0>)
+                         : (bucket == 16 ? k_comp_search<2, 16> : k_comp_search<2, 0>);
     // pass 0 over the searching points; their open packets' lanes into open_mask
     if (ev) (void)hipEventRecord(ev[0], s);
     hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, (const Node*)nodes, cl, ncl, bucket,
