@@ -226,3 +226,19 @@ def test_context_reuse_across_point_sets(pkg):
             c.run()                     # a second run on the same points
             u2, v2, _ = c.getLinks()
             assert np.array_equal(u2, u) and np.array_equal(v2, v)
+
+
+def test_many_random_clouds_against_oracle(pkg, oracle_mod):
+    """60 seeded clouds of every generator kind, 2D and 3D, 2-2500 points:
+    edges, per-round sizes and labels equal the oracle's."""
+    rng = np.random.default_rng(58)
+    for i in range(60):
+        dim = 2 + (i % 2)
+        kind = int(rng.integers(0, 3))
+        n = int(rng.integers(2, 2500))
+        pts = pkg.synth(kind, n, dim, 900 + i)
+        g = run(pkg, pts, round_search=1)
+        ref = oracle_mod.emst(pts, mode=0)
+        assert np.array_equal(g["e"], oracle_mod.canon(ref["u"], ref["v"])), (i, dim, kind, n)
+        assert g["sizes"] == ref["cum_edges"], (i, dim, kind, n)
+        assert np.array_equal(g["labels"], ref["labels"]), (i, dim, kind, n)
