@@ -940,32 +940,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             refresh();
             int sp = 0;
             int ver = 0;        // bumps whenever a leaf tightened the bounds
-            // first pass (3D): the leaf of the first open lane goes first, so
-            // that the packet's lanes enter the tree with bounds from their
-            // own neighbourhood; the traversal skips it when it comes again
-            // (clustered 3D -0.7%; in 2D +1%, so not there)
-            int pre = -1;
-            bool pre_done = false;
-            if (D == 3 && pass == 0) {
-                const int fl = (int)__builtin_ctzll(__ballot(open != 0u));
-                const int myleaf = (int)(L + (mine ? (p - cpos) / bucket : 0));
-                pre = __builtin_amdgcn_readfirstlane(__shfl(myleaf, fl));
-            }
-            if (lane == 0) {
-                stk_id[0] = 1;
-                stk_ver[0] = -1;
-                if (pre >= 0) { stk_id[1] = pre; stk_ver[1] = -1; }
-            }
-            sp = pre >= 0 ? 2 : 1;
+            if (lane == 0) { stk_id[0] = 1; stk_ver[0] = -1; }
+            sp = 1;
             __builtin_amdgcn_wave_barrier();
             while (sp > 0) {
                 --sp;
                 const int node = __builtin_amdgcn_readfirstlane(stk_id[sp]);
                 const int pver = __builtin_amdgcn_readfirstlane(stk_ver[sp]);
-                if (node == pre) {
-                    if (pre_done) continue;
-                    pre_done = true;
-                }
                 // a node tested against the current bounds when pushed keeps its
                 // lane mask; after a leaf tightened the bounds it is tested again
                 bool need;
