@@ -1999,7 +1999,7 @@ __global__ __launch_bounds__(kBlock) void k_comp_samples(const int32_t* __restri
 
 // nearest point in another component, for the points of qlist (count in
 // nq_dev): packet traversal as in the cone search, one best per query, by
-// iterative deepening from the seed radius R_0 (capped passes R_k = 2^k R_0,
+// iterative deepening from the seed radius R_0 (capped passes R_k = 8^(k/2) R_0,
 // then one uncapped).  Points below the point's lower bound (its previous
 // target's d2) were in the component at that search and still are, so they
 // are skipped.  The component's shortest edge found so far (comp1, read at
@@ -2083,7 +2083,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         // a launch that resumes at pass_lo: the state the earlier passes left
         for (int k = 0; k < pass_lo; ++k) {
             inner = max(inner, (int32_t)__float_as_uint(r2));
-            r2 *= 4.f;
+            r2 *= 8.f;
         }
         bool open = mine;
         float cx = mine ? px : 0.f, cy = mine ? py : 0.f, cz = mine ? pz : 0.f;
@@ -2283,7 +2283,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             }
             atomic_min_grouped(comp1, mylab, key1, fresh);
             inner = max(inner, cap);
-            r2 *= 4.f;
+            r2 *= 8.f;
         }
         if (mine) left_open = open;
     }
