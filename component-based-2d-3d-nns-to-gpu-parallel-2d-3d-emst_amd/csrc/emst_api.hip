@@ -321,9 +321,10 @@ CycleState cycle_state(emst_ctx* c)
 
 constexpr int64_t kSampleCap = 1 << 20;         // logged pairs per run
 // Default plain seed of the 3D component search, in leaf spacings: its first
-// pass looks for one nearest point, not for every cone's.  At 10^7 points
-// 1.0-1.5 beat 3.0 by 3% (clustered) to 9% (uniform); in 2D 3.0 stays best.
-constexpr float kCompSeedScale3D = 1.25f;
+// pass looks for one nearest point, not for every cone's.  At 10^7 points 1.0
+// beats 3.0 by 10% (clustered) to 15% (uniform) and 1.25-2.0 by 2-6%; in 2D
+// 3.0 stays best.
+constexpr float kCompSeedScale3D = 1.0f;
 constexpr int64_t kSamplesPerRound = 1 << 16;   // at most, per round
 
 // the learned seed of the component search: buffers when a model is set or

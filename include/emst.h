@@ -54,7 +54,7 @@ typedef struct {
     int32_t device;         /* HIP device ordinal */
     int32_t radius_mode;    /* 0 plain; 1 learned seed radius (needs emst_set_radius_model) */
     float seed_scale;       /* plain-form first search radius, in mean spacings of the point's
-                               leaf cell (0 = default: 3.0, 1.25 for the 3D component search);
+                               leaf cell (0 = default: 3.0, 1.0 for the 3D component search);
                                changes work, never results */
     int32_t cone_dim;       /* dimension of the cones (DimCM): 0 = the point dimension; 2 with
                                3D points is the 2.5D mode, SolutionEMST<3,2>

@@ -52,8 +52,8 @@ def test_autotune_keeps_the_tree(pkg, dim, kind):
         c.run()
         ref, _ = _edges(c)
         b, sc, ms = c.autotune()
-        # 1.25: the 3D component search's seed, when that search wins
-        assert b in (16, 32, 64) and sc in (1.25, 2.0, 3.0, 4.0) and ms > 0
+        # 1.0: the 3D component search's seed, when that search wins
+        assert b in (16, 32, 64) and sc in (1.0, 2.0, 3.0, 4.0) and ms > 0
         c.run()
         e, _ = _edges(c)
         assert np.array_equal(e, ref)
