@@ -1168,17 +1168,23 @@ __global__ __launch_bounds__(kBlock) void k_open_scan_local(const uint64_t* __re
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(64) void k_open_fill_scan(const uint64_t* __restrict__ open_mask, int64_t P,
-                                                      uint32_t* __restrict__ off, const uint32_t* __restrict__ sums,
-                                                      const int32_t* __restrict__ src, int32_t* __restrict__ dst)
+// (one wave per packet, grid-stride: the lists are built every round, and a
+// launch of one block per packet costs more than its work when few are open)
+__global__ __launch_bounds__(kBlock) void k_open_fill_scan(const uint64_t* __restrict__ open_mask, int64_t P,
+                                                          uint32_t* __restrict__ off,
+                                                          const uint32_t* __restrict__ sums,
+                                                          const int32_t* __restrict__ src, int32_t* __restrict__ dst)
 {
-    const int64_t b = blockIdx.x;
     const int lane = __lane_id();
-    if (b == 0 && lane == 0) off[P] += sums[P / kScanTile];
-    const uint64_t m = open_mask[b];
-    if (!((m >> lane) & 1ull)) return;
-    const uint32_t o = off[b] + sums[b / kScanTile] + (uint32_t)__popcll(m & lanemask_lt());
-    dst[o] = src != nullptr ? src[b * 64 + lane] : (int32_t)(b * 64 + lane);
+    const int64_t waves = (int64_t)gridDim.x * (kBlock / 64);
+    const int64_t w0 = (int64_t)blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+    if (w0 == 0 && lane == 0) off[P] += sums[P / kScanTile];
+    for (int64_t b = w0; b < P; b += waves) {
+        const uint64_t m = open_mask[b];
+        if (!((m >> lane) & 1ull)) continue;
+        const uint32_t o = off[b] + sums[b / kScanTile] + (uint32_t)__popcll(m & lanemask_lt());
+        dst[o] = src != nullptr ? src[b * 64 + lane] : (int32_t)(b * 64 + lane);
+    }
 }
 
 void open_compact(const uint64_t* open_mask, int64_t P, uint32_t* off, uint32_t* sums, const int32_t* src,
@@ -1187,7 +1193,8 @@ void open_compact(const uint64_t* open_mask, int64_t P, uint32_t* off, uint32_t*
     const int64_t nb = (P + 1 + kScanTile - 1) / kScanTile;
     hipLaunchKernelGGL(k_open_scan_local, dim3((unsigned)nb), dim3(kBlock), 0, s, open_mask, P, off, sums);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, sums, nb);
-    hipLaunchKernelGGL(k_open_fill_scan, dim3((unsigned)P), dim3(64), 0, s, open_mask, P, off, (const uint32_t*)sums,
+    const unsigned gf = (unsigned)std::min<int64_t>((P + kBlock / 64 - 1) / (kBlock / 64), 2048);
+    hipLaunchKernelGGL(k_open_fill_scan, dim3(gf), dim3(kBlock), 0, s, open_mask, P, off, (const uint32_t*)sums,
                        src, dst);
 }
 
