@@ -100,6 +100,10 @@ struct emst_ctx {
     DevBuf<uint8_t> cy_onc, cy_cut;
     DevBuf<unsigned long long> visit;  // arrival state of the bottom-up node passes
     DevBuf<int32_t> list2;             // queries open after pass 0 (per-round component search)
+    DevBuf<int32_t> knn_pos;           // neighbour lists of the component search (NbrList)
+    DevBuf<uint64_t> knn_key;
+    DevBuf<uint8_t> knn_cur;
+    bool knn_built = false;            // the build wrote the lists (not with the learned component seed)
     DevBuf<int32_t> csize;             // component sizes at the roots (learned component seed)
     DevBuf<uint32_t> cbox, nq_keep;    // component boxes at the roots; searching points' count
     DevBuf<float> samples;             // logged (features, target) pairs of the component search
@@ -224,6 +228,9 @@ int alloc_all(emst_ctx* c)
     if (comp) {
         ENS(visit, 2 * c->nleaves);
         ENS(list2, n);
+        ENS(knn_pos, (size_t)kKnn * n);
+        ENS(knn_key, (size_t)kKnn * n);
+        ENS(knn_cur, n);
     }
     if (c->cdim < c->dim) {
         ENS(cy_j0, n);
@@ -398,6 +405,12 @@ DupState dup_state(emst_ctx* c)
 }
 
 // bbox, rescale, Morton sort, hierarchy, round-1 cone search
+NbrList nbr_list(emst_ctx* c)
+{
+    if (!c->knn_built) return NbrList{nullptr, nullptr, nullptr};
+    return NbrList{c->knn_pos.p, c->knn_key.p, c->knn_cur.p};
+}
+
 int build(emst_ctx* c)
 {
     const int64_t n = c->n;
@@ -429,6 +442,7 @@ int build(emst_ctx* c)
     HIPCHK(c, hipMemsetAsync(&c->C->dup_runs, 0, 8 * sizeof(uint32_t), s));
     c->grouped = false;
     c->groups_settled = false;
+    c->knn_built = false;
     c->dup_points = c->dup_groups = 0;
     const bool comp = c->opt.round_search == 1;
     if (!c->no_grouping && !comp) {
@@ -459,6 +473,12 @@ int build(emst_ctx* c)
         // searches (launch_comp_round)
         HIPCHK(c, hipMemsetAsync(c->visit.p, 0, sizeof(unsigned long long) * 2 * c->nleaves, s));
         launch_node_min_ids(c->pts.p, c->dcl.p, c->ncl, c->nleaves, c->bucket, c->nodes.p, c->visit.p, s);
+        // round 1's targets and the neighbour lists; the learned component
+        // seed keeps round 1's searches (its sample log starts there)
+        c->knn_built = !comp_features_on(c);
+        if (c->knn_built)
+            launch_knn_first(c->dim, n, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, c->corr.p,
+                             c->corr_key.p, nbr_list(c), s);
         int rc1 = ensure_comp_seed(c);
         if (rc1) return rc1;
         if (comp_features_on(c)) launch_comp_seed_init(n, c->pts.p, c->csize.p, c->cbox.p, s);
@@ -603,8 +623,8 @@ void issue_round(emst_ctx* c, bool tr)
         if (ph == 0 && c->opt.round_search == 1) {
             launch_comp_round(c->dim, c->n, c->pts.p, c->nodes.p, c->visit.p, c->dcl.p, c->ncl,
                               c->nleaves, c->bucket, c->label.p, c->corr.p, c->corr_key.p, c->comp1.p,
-                              c->inv.p, c->seed_scale, split_scratch(c), comp_seed(c), c->eu.p, c->eround.p, c->C, s,
-                              tr ? c->ev + 8 : nullptr);
+                              c->inv.p, c->seed_scale, split_scratch(c), comp_seed(c), nbr_list(c), c->eu.p,
+                              c->eround.p, c->C, s, tr ? c->ev + 8 : nullptr);
             if (tr) (void)hipEventRecord(c->ev[4], s);
             continue;
         }
@@ -725,7 +745,8 @@ std::vector<uintptr_t> graph_key(emst_ctx* c)
             (uintptr_t)c->opt.round_search, (uintptr_t)comp_features_on(c), (uintptr_t)c->learned,
             (uintptr_t)c->model.p, (uintptr_t)c->csize.p, (uintptr_t)c->cbox.p, (uintptr_t)c->feat.p, (uintptr_t)c->ls.p,
             (uintptr_t)c->pred.p, (uintptr_t)c->seed.p, (uintptr_t)c->samples.p, (uintptr_t)c->nq_keep.p,
-            (uintptr_t)c->opt.log_samples, (uintptr_t)c->visit.p, (uintptr_t)c->list2.p, (uintptr_t)c->open_mask.p, (uintptr_t)c->open_cnt.p,
+            (uintptr_t)c->opt.log_samples, (uintptr_t)c->visit.p, (uintptr_t)c->list2.p, (uintptr_t)c->knn_pos.p,
+            (uintptr_t)c->knn_key.p, (uintptr_t)c->knn_cur.p, (uintptr_t)c->knn_built, (uintptr_t)c->open_mask.p, (uintptr_t)c->open_cnt.p,
             (uintptr_t)c->open_tmp.p, (uintptr_t)c->list.p, (uintptr_t)c->nodes.p, (uintptr_t)c->dcl.p,
             (uintptr_t)y.j[0], (uintptr_t)y.kh[0], (uintptr_t)y.kl[0], (uintptr_t)y.onc, (uintptr_t)y.cut,
             (uintptr_t)c->grouped, (uintptr_t)d.npoints,

@@ -198,10 +198,28 @@ struct CompSeed {
     int64_t per_round;      // pairs logged per round, at most (a stride over the searching points)
     uint32_t* nq_keep;      // 2 words: the counts the sample log and the model run on
 };
+// Neighbour lists of the component search: every point's kKnn nearest other
+// points of its cloud under (d2, id), as positions and keys, list entry k of
+// point p at k * n + p (position -1, key all ones: the cloud has no more
+// points), and per point the entry its current target came from (kKnn: the
+// list is used up).
+constexpr int kKnn = 6;
+struct NbrList {
+    int32_t* pos;           // kKnn * n
+    uint64_t* key;          // kKnn * n
+    uint8_t* cur;           // n
+};
 void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsigned long long* visit,
                        const Cloud* cl, int ncl, int64_t nleaves, int bucket, const int32_t* label, int32_t* corr, uint64_t* corr_key, uint64_t* comp1, const int32_t* inv,
-                       float seed_scale, const SplitScratch& sc, const CompSeed& cs, const int32_t* eu,
-                       const int32_t* eround, Counters* C, hipStream_t s, hipEvent_t* ev = nullptr);
+                       float seed_scale, const SplitScratch& sc, const CompSeed& cs, const NbrList& nb,
+                       const int32_t* eu, const int32_t* eround, Counters* C, hipStream_t s,
+                       hipEvent_t* ev = nullptr);
+// The round-1 targets of the component search and the neighbour lists: one
+// traversal per 64-point packet (after launch_round_init and
+// launch_node_min_ids).  Writes corr / corr_key (every component is one
+// point, so the nearest neighbour is the round's exact target) and nb.
+void launch_knn_first(int dim, int64_t n, const float4* pts, const Node* nodes, const Cloud* cl, int ncl,
+                      int bucket, int32_t* corr, uint64_t* corr_key, const NbrList& nb, hipStream_t s);
 // (ev: 4 events recorded around the two k_comp_search dispatches, or nullptr)
 // Sizes and boxes of the single-point components (the start of a run).
 void launch_comp_seed_init(int64_t n, const float4* pts, int32_t* csize, uint32_t* cbox, hipStream_t s);

@@ -1787,9 +1787,9 @@ __global__ __launch_bounds__(kBlock) void k_nmin_tree(const float4* __restrict__
 // Keep-previous: the component minimum of the kept targets.
 __global__ __launch_bounds__(kBlock) void k_comp_keep(int64_t n, const float4* __restrict__ pts,
                                                      const int32_t* __restrict__ label,
-                                                     const int32_t* __restrict__ corr,
-                                                     const uint64_t* __restrict__ corr_key,
-                                                     uint64_t* __restrict__ comp1, const Counters* C)
+                                                     int32_t* __restrict__ corr,
+                                                     uint64_t* __restrict__ corr_key,
+                                                     uint64_t* __restrict__ comp1, NbrList nb, const Counters* C)
 {
     if (C->done) return;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1798,9 +1798,38 @@ __global__ __launch_bounds__(kBlock) void k_comp_keep(int64_t n, const float4* _
     uint64_t key1 = kEmptyKey;
     if (p < n) {
         lp = label[p];
-        const int32_t c = corr[p];
+        int32_t c = corr[p];
+        uint64_t ck = 0;
+        bool have_ck = false;
+        // a target that joined the component: the next listed point outside
+        // it (components only grow, so the listed points before it are inside
+        // for good); the list used up, its last key bounds the search
+        if (c >= 0 && label[c] == lp && nb.pos != nullptr) {
+            int j = nb.cur[p];
+            if (j < kKnn) {
+                int32_t e = -1;
+                for (++j; j < kKnn; ++j) {
+                    e = nb.pos[(int64_t)j * n + p];
+                    if (e < 0 || label[e] != lp) break;
+                }
+                if (j == kKnn) {
+                    c = -1;
+                    ck = (nb.key[(int64_t)(kKnn - 1) * n + p] & 0xffffffff00000000ull) | 0xffffffffull;
+                } else if (e < 0) {
+                    c = -2;             // the cloud's other points are all listed, all inside
+                    ck = kEmptyKey;
+                } else {
+                    c = e;
+                    ck = nb.key[(int64_t)j * n + p];
+                    have_ck = true;
+                }
+                nb.cur[p] = (uint8_t)j;
+                corr[p] = c;
+                corr_key[p] = ck;
+            }
+        }
         if (c >= 0 && label[c] != lp) {
-            const uint64_t ck = corr_key[p];
+            if (!have_ck) ck = corr_key[p];
             const uint32_t pid = __float_as_uint(pts[p].w), qid = (uint32_t)ck;
             key1 = (ck & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
             valid = true;
@@ -2310,6 +2339,171 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     }
 }
 
+// Round 1 of the component search and its neighbour lists.  Every component
+// is one point, so a point's nearest outside point is its nearest neighbour.
+// One wave per 64 consecutive positions keeps each lane's kKnn smallest keys
+// (d2, id) in registers: the leaves of the packet's own points first, then
+// the hierarchy from the root, nearest child first, a box needed while its
+// (lower bound, smallest id) may beat a lane's kKnn-th key.  Query per lane:
+// the leaf's points are read in turn at one address for the whole wave.
+template <int D>
+__global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __restrict__ pts,
+                                                 const Node* __restrict__ nodes, const Cloud* __restrict__ clouds,
+                                                 int ncl, int bucket, int32_t* __restrict__ corr,
+                                                 uint64_t* __restrict__ corr_key, NbrList nb)
+{
+    __shared__ int32_t stk_id[64];
+    __shared__ int32_t stk_ver[64];
+    __shared__ unsigned long long stk_mask[64];
+    const int lane = __lane_id();
+    const int64_t p = (int64_t)blockIdx.x * 64 + lane;
+    const bool active = p < n;
+    const float4 me = pts[active ? p : 0];
+    const float px = me.x, py = me.y, pz = me.z;
+    const int my_cl = active ? find_cloud(clouds, ncl, p) : -1;
+    uint64_t kk[kKnn];
+    int32_t kp[kKnn];
+#pragma unroll
+    for (int i = 0; i < kKnn; ++i) {
+        kk[i] = kEmptyKey;
+        kp[i] = -1;
+    }
+    // sorted insertion (keys are distinct: the id is in the low word)
+    auto insert = [&](uint64_t key, int32_t pos) {
+#pragma unroll
+        for (int i = kKnn - 1; i > 0; --i) {
+            const bool sh = key < kk[i - 1];
+            const bool here = key < kk[i];
+            kp[i] = sh ? kp[i - 1] : (here ? pos : kp[i]);
+            kk[i] = sh ? kk[i - 1] : (here ? key : kk[i]);
+        }
+        if (key < kk[0]) {
+            kk[0] = key;
+            kp[0] = pos;
+        }
+    };
+    uint64_t pending = __ballot(active);
+    while (pending) {
+        const int cl = __builtin_amdgcn_readfirstlane(__shfl(my_cl, (int)__builtin_ctzll(pending)));
+        const uint64_t minem = __ballot(my_cl == cl);
+        const bool mine = my_cl == cl;
+        pending &= ~minem;
+        const int64_t cpos = clouds[cl].pos, cend = clouds[cl].pos + clouds[cl].n;
+        const int64_t L = clouds[cl].L;
+        const Node* __restrict__ tree = nodes + 2 * clouds[cl].lbase;
+        float cx = mine ? px : 0.f, cy = mine ? py : 0.f, cz = mine ? pz : 0.f;
+        float na = mine ? 1.f : 0.f;
+        for (int off = 32; off > 0; off >>= 1) {
+            cx += __shfl_xor(cx, off);
+            cy += __shfl_xor(cy, off);
+            cz += __shfl_xor(cz, off);
+            na += __shfl_xor(na, off);
+        }
+        cx /= na; cy /= na; cz /= na;
+        auto cdist = [&](const Node& x) {
+            const float dxx = fmaxf(fmaxf(x.lo[0] - cx, cx - x.hi[0]), 0.f);
+            const float dyy = fmaxf(fmaxf(x.lo[1] - cy, cy - x.hi[1]), 0.f);
+            const float dzz = fmaxf(fmaxf(x.lo[2] - cz, cz - x.hi[2]), 0.f);
+            return dxx * dxx + dyy * dyy + dzz * dzz;
+        };
+        auto needed = [&](const Node& nd, uint32_t nm) {
+            if (!mine) return false;
+            const float q[3] = {px, py, pz};
+            float m[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) m[k] = fmaxf(fmaxf(nd.lo[k] - q[k], q[k] - nd.hi[k]), 0.f);
+            const float lb = D == 3 ? d2_of(m[0], m[1], m[2]) : d2_of(m[0], m[1]);
+            return (((uint64_t)__float_as_uint(lb) << 32) | nm) < kk[kKnn - 1];
+        };
+        auto scan_leaf = [&](int node) {
+            const int64_t s0 = cpos + (int64_t)(node - L) * bucket;
+            const int cnt = (int)max<int64_t>(min(cend, s0 + bucket) - s0, 0);
+            for (int j = 0; j < cnt; ++j) {
+                const float4 v = pts[s0 + j];
+                const float dx = v.x - px, dy = v.y - py, dz = v.z - pz;
+                const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
+                const uint64_t key = ((uint64_t)__float_as_uint(d2) << 32) | __float_as_uint(v.w);
+                if (mine && s0 + j != p && key < kk[kKnn - 1]) insert(key, (int32_t)(s0 + j));
+            }
+        };
+        // the leaves holding the packet's points of this cloud
+        const int first = (int)__builtin_ctzll(minem), last = 63 - (int)__builtin_clzll(minem);
+        const int64_t pf = (int64_t)blockIdx.x * 64 + first, pl = (int64_t)blockIdx.x * 64 + last;
+        const int leaf_lo = (int)(L + (pf - cpos) / bucket), leaf_hi = (int)(L + (pl - cpos) / bucket);
+        for (int lf = leaf_lo; lf <= leaf_hi; ++lf) scan_leaf(lf);
+        int sp = 1;
+        int ver = 0;
+        if (lane == 0) {
+            stk_id[0] = 1;
+            stk_ver[0] = -1;
+        }
+        __builtin_amdgcn_wave_barrier();
+        while (sp > 0) {
+            --sp;
+            const int node = __builtin_amdgcn_readfirstlane(stk_id[sp]);
+            const int pver = __builtin_amdgcn_readfirstlane(stk_ver[sp]);
+            bool need;
+            if (pver == ver) {
+                need = (stk_mask[sp] >> lane) & 1ull;
+            } else {
+                const Node nd = tree[node];
+                need = needed(nd, __float_as_uint(nd.pad[0]));
+            }
+            if (__ballot(need) == 0ull) continue;
+            if (node >= L) {
+                if (node >= leaf_lo && node <= leaf_hi) continue;
+                scan_leaf(node);
+                ++ver;
+            } else {
+                const bool leafkids = 2 * (int64_t)node >= L;
+                const int fc = leafkids ? 2 * node : 4 * node;
+                const int nk = leafkids ? 2 : 4;
+                uint64_t mk[4];
+                float dk[4];
+                int id[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    mk[i] = 0ull;
+                    dk[i] = -1.f;
+                    id[i] = fc + i;
+                    if (i < nk) {
+                        const Node bx = tree[fc + i];
+                        mk[i] = __ballot(needed(bx, __float_as_uint(bx.pad[0])));
+                        if (mk[i]) dk[i] = cdist(bx);
+                    }
+                }
+                // pushed farthest first, so the nearest pops first
+                auto ce = [&](int a, int b) {
+                    if (dk[a] < dk[b] || (dk[a] == dk[b] && id[a] < id[b])) {
+                        const float td = dk[a]; dk[a] = dk[b]; dk[b] = td;
+                        const uint64_t tm = mk[a]; mk[a] = mk[b]; mk[b] = tm;
+                        const int ti = id[a]; id[a] = id[b]; id[b] = ti;
+                    }
+                };
+                ce(0, 1); ce(2, 3); ce(0, 2); ce(1, 3); ce(1, 2);
+                if (lane == 0) {
+                    int t = sp;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (mk[i]) { stk_id[t] = id[i]; stk_ver[t] = ver; stk_mask[t] = mk[i]; ++t; }
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) sp += mk[i] != 0ull ? 1 : 0;
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int i = 0; i < kKnn; ++i) {
+        nb.pos[(int64_t)i * n + p] = kp[i];
+        nb.key[(int64_t)i * n + p] = kk[i];
+    }
+    corr[p] = kp[0] >= 0 ? kp[0] : -2;
+    corr_key[p] = kk[0];
+    nb.cur[p] = 0;
+}
+
 // ---- cycle rule of the 2.5D mode (CycleState).  Roots are the points whose
 // flattened label is their own position.  In round 1 every component is one
 // point and its chosen edge is its nearest neighbour, whose choices close
@@ -2799,8 +2993,8 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsig
                        const Cloud* cl, int ncl, int64_t nleaves,
                        int bucket,
                        const int32_t* label, int32_t* corr, uint64_t* corr_key, uint64_t* comp1, const int32_t* inv,
-                       float seed_scale, const SplitScratch& sc, const CompSeed& cs, const int32_t* eu,
-                       const int32_t* eround, Counters* C, hipStream_t s, hipEvent_t* ev)
+                       float seed_scale, const SplitScratch& sc, const CompSeed& cs, const NbrList& nb,
+                       const int32_t* eu, const int32_t* eround, Counters* C, hipStream_t s, hipEvent_t* ev)
 {
     if (n <= 0) return;
     const unsigned g = nblocks(n);
@@ -2810,8 +3004,8 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsig
     hipLaunchKernelGGL(k_nlab_tree, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, label, cl, ncl, nleaves, bucket,
                        nodes, visit, (const Counters*)C);
     const int64_t P = (n + 63) / 64;
-    hipLaunchKernelGGL(k_comp_keep, dim3(g), dim3(kBlock), 0, s, n, pts, label, (const int32_t*)corr,
-                       (const uint64_t*)corr_key, comp1, (const Counters*)C);
+    hipLaunchKernelGGL(k_comp_keep, dim3(g), dim3(kBlock), 0, s, n, pts, label, corr, corr_key, comp1, nb,
+                       (const Counters*)C);
     hipLaunchKernelGGL(k_comp_prep, dim3(g), dim3(kBlock), 0, s, n, label, corr, (const uint64_t*)corr_key,
                        (const uint64_t*)comp1, sc.open_mask, (const Counters*)C);
     open_compact(sc.open_mask, P, sc.off, sc.tmp, nullptr, sc.list, s);
@@ -2851,6 +3045,19 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsig
                            (const uint32_t*)cs.nq_keep, (const float*)cs.X, (const float*)cs.LS, label,
                            (const int32_t*)corr, (const uint64_t*)corr_key, (const uint64_t*)comp1, cs.per_round,
                            cs.cap, cs.samples, C);
+}
+
+void launch_knn_first(int dim, int64_t n, const float4* pts, const Node* nodes, const Cloud* cl, int ncl,
+                      int bucket, int32_t* corr, uint64_t* corr_key, const NbrList& nb, hipStream_t s)
+{
+    if (n <= 0) return;
+    const unsigned P = (unsigned)((n + 63) / 64);
+    if (dim == 3)
+        hipLaunchKernelGGL(k_knn_first<3>, dim3(P), dim3(64), 0, s, n, pts, nodes, cl, ncl, bucket, corr, corr_key,
+                           nb);
+    else
+        hipLaunchKernelGGL(k_knn_first<2>, dim3(P), dim3(64), 0, s, n, pts, nodes, cl, ncl, bucket, corr, corr_key,
+                           nb);
 }
 
 void launch_dup_fill(int dim, int64_t n, const int32_t* label, const float4* pts, const int32_t* inv,
