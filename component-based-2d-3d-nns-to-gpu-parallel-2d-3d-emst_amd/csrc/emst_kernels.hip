@@ -2400,6 +2400,16 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
             na += __shfl_xor(na, off);
         }
         cx /= na; cy /= na; cz /= na;
+        // the box of the group's points
+        float qlo[3] = {mine ? px : INFINITY, mine ? py : INFINITY, mine ? pz : INFINITY};
+        float qhi[3] = {mine ? px : -INFINITY, mine ? py : -INFINITY, mine ? pz : -INFINITY};
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                qlo[k] = fminf(qlo[k], __shfl_xor(qlo[k], off));
+                qhi[k] = fmaxf(qhi[k], __shfl_xor(qhi[k], off));
+            }
+        }
         auto cdist = [&](const Node& x) {
             const float dxx = fmaxf(fmaxf(x.lo[0] - cx, cx - x.hi[0]), 0.f);
             const float dyy = fmaxf(fmaxf(x.lo[1] - cy, cy - x.hi[1]), 0.f);
@@ -2418,7 +2428,25 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
         auto scan_leaf = [&](int node) {
             const int64_t s0 = cpos + (int64_t)(node - L) * bucket;
             const int cnt = (int)max<int64_t>(min(cend, s0 + bucket) - s0, 0);
-            for (int j = 0; j < cnt; ++j) {
+            // a leaf point farther from the packet's box than every lane's
+            // kKnn-th d2 cannot enter a list (its d2 to each of the packet's
+            // points is at least its box distance): only the others are paired
+            uint32_t t = mine ? (uint32_t)(kk[kKnn - 1] >> 32) : 0u;
+            for (int off = 32; off > 0; off >>= 1) t = max(t, (uint32_t)__shfl_xor((int)t, off));
+            bool keep = false;
+            if (lane < cnt) {
+                const float4 w = pts[s0 + lane];
+                const float wq[3] = {w.x, w.y, w.z};
+                float m[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) m[k] = fmaxf(fmaxf(qlo[k] - wq[k], wq[k] - qhi[k]), 0.f);
+                const float lb = D == 3 ? d2_of(m[0], m[1], m[2]) : d2_of(m[0], m[1]);
+                keep = __float_as_uint(lb) <= t;
+            }
+            uint64_t cand = __ballot(keep);
+            while (cand) {
+                const int j = (int)__builtin_ctzll(cand);
+                cand &= cand - 1;
                 const float4 v = pts[s0 + j];
                 const float dx = v.x - px, dy = v.y - py, dz = v.z - pz;
                 const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
