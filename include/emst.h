@@ -77,7 +77,9 @@ typedef struct {
                                another component whenever its target has joined its own
                                (NodeSpiralSearchMD, NodeEMST.h:67-509, selected in the reference by
                                EMST_SPIRAL_SEARCH_KD_SLAB_2 0, SolutionEMST.h:55-56).  Both build
-                               the same forests.  1 has no cone candidates and no 2.5D mode; its
+                               the same forests.  1 has no cone candidates and no 2.5D mode; it
+                               keeps each point's 6 nearest neighbours from the build (73 bytes
+                               per point), which answer the early rounds without a search; its
                                learned seed predicts each component's edge length per round */
     int32_t log_samples;    /* 1 (round_search 1): log the learned seed's training pairs during
                                runs (emst_get_radius_samples) */
