@@ -167,12 +167,14 @@ void release(DevBuf<T>& b)
 
 // leaf cells hold a power of two of points in [8, 64] (the cone search maps
 // one leaf point per lane, 64 / bucket queries per pass of the wave).  The
-// default is the fastest measured at 10^7 points: 16 in 2D (8 cones, cheap
-// pairs: smaller leaves cut pairs more than they add node tests), 64 in 3D.
+// default is the fastest measured at 10^7 points: for the cone search 16 in
+// 2D (8 cones, cheap pairs: smaller leaves cut pairs more than they add node
+// tests) and 64 in 3D; for the component search 64 in both (2D power-law:
+// 21.3 ms per EMST against 22.1 with 16).
 int choose_bucket(const emst_options& o, int dim)
 {
     int b = o.bucket_size;
-    if (b <= 0) b = dim == 2 ? 16 : 64;
+    if (b <= 0) b = dim == 2 && o.round_search != 1 ? 16 : 64;
     int r = 8;
     while (r < b && r < 64) r <<= 1;
     return r;

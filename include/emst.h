@@ -49,7 +49,7 @@ typedef struct {
     int32_t level_radius;   /* R of the reference cell grid (levelRadius); cells are 2R wide */
     int32_t max_rounds;     /* Boruvka round cap; the reference stops at 100 */
     int32_t bucket_size;    /* points per leaf cell of the search hierarchy: a power of two in
-                               [8, 64], rounded up (0 = 16 in 2D, 64 in 3D) */
+                               [8, 64], rounded up (0 = 16 for the 2D cone search, else 64) */
     int32_t trace;          /* 1: record per-round forest size, length and phase times */
     int32_t device;         /* HIP device ordinal */
     int32_t radius_mode;    /* 0 plain; 1 learned seed radius (needs emst_set_radius_model) */
