@@ -1032,18 +1032,23 @@ int emst_autotune(emst_ctx* c, int32_t* best_bucket, float* best_scale, float* b
     float bscale = bs;
     if (!rc && !learned && c->cdim == c->dim) {
         float tbest = 0.f;
+        int bbest = bb;
         for (int rs : {0, 1}) {
             c->opt.round_search = rs;
             c->seed_scale = rs == 1 && c->dim == 3 ? kCompSeedScale3D : bs;
+            // the component search at its own leaf size, 64 points
+            const int bk = rs == 1 ? 64 : bb;
             c->no_grouping = false;
-            if ((rc = layout_clouds(c, bb)) || (rc = build(c)) || (rc = run_rounds(c))) break;
+            if ((rc = layout_clouds(c, bk)) || (rc = build(c)) || (rc = run_rounds(c))) break;
             const float t = c->times[0] + c->times[1] + c->times[2];
             if (tbest == 0.f || t < tbest) {
                 tbest = t;
                 brs = rs;
                 bscale = c->seed_scale;
+                bbest = bk;
             }
         }
+        bb = bbest;
     }
     c->opt.round_search = brs;
     bs = bscale;

@@ -151,7 +151,8 @@ int emst_get_cloud_results(emst_ctx* ctx, double* weight, int32_t* edges, int32_
  * points on the device for every leaf size (points per leaf cell: 16, 32,
  * 64) and plain seed scale (2, 3, 4 leaf spacings), then every search_split
  * form (1, 2, 3) for the fastest pair, then whole runs with round_search 0
- * and 1 (not with a learned model set, nor in the 2.5D mode, where the
+ * and with 1 at 64-point leaves (not with a learned model set, nor in the
+ * 2.5D mode, where the
  * caller's choice stays); keeps the fastest choice in the context (used by
  * every later set_points / set_clouds / run) and returns the pair; best_ms =
  * its build + cone search time.  The result of a run never depends on any of
