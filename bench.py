@@ -217,32 +217,7 @@ def main():
                 "frac": round(achieved / 8000.0, 4), "traffic": profiled_traffic(args, kernel),
                 "kernel": kernel, "kernel_ms": round(k_ms, 3)}
 
-    cpu = None
-    if args.cpu_sample > 0:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle
-        sample = pkg.synth(args.kind, args.cpu_sample, args.dim, args.seed)
-        oracle.lib()
-        # all host threads (the per-point searches in parallel, the rounds
-        # sequential), then one thread on a fifth of the sample
-        cores = oracle.threads(0)
-        t1 = time.perf_counter()
-        r = oracle.emst(sample, mode=1, R=1, cdim=cdim)
-        ct = time.perf_counter() - t1
-        assert len(r["u"]) == args.cpu_sample - 1
-        n1 = max(args.cpu_sample // 5, 2)
-        oracle.threads(1)
-        t1 = time.perf_counter()
-        r1 = oracle.emst(sample[:n1], mode=1, R=1, cdim=cdim)
-        c1 = time.perf_counter() - t1
-        oracle.threads(cores)
-        assert len(r1["u"]) == n1 - 1
-        cpu = {"value": round(args.cpu_sample / ct / 1e6, 5), "unit": "Mpoints/s", "cores": cores,
-               "kind": "port",
-               "sample": f"{args.cpu_sample} points of the same distribution and seed, oracle restatement "
-                         f"of the reference ring search (R=1), {cores} threads (per-point searches in "
-                         f"parallel, rounds sequential), {ct:.1f} s; single thread on the first {n1} points: "
-                         f"{n1 / c1 / 1e6:.5f} Mpoints/s ({c1:.1f} s)"}
+    cpu = cpu_baseline(args, pkg, cdim) if args.cpu_sample > 0 else None
 
     desc = {"cone_plain": "cone search, plain seed", "cone_learned": "cone search, learned seed",
             "comp_plain": "per-round component search, plain seed",
@@ -276,6 +251,73 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def cpu_baseline(args, pkg, cdim):
+    """The oracle (the project's C restatement of the reference path) timed on
+    this host's cores, built here with -O3 -march=native.  value: a sample of
+    the benchmarked workload (--cpu-sample points of the same distribution and
+    seed; per-point searches in parallel, rounds sequential).  Also the
+    reference path's own CPU configuration: 10^4 uniform 2D points, seed 0,
+    with 1 thread and with all cores, median of 5 runs after 1 warm-up, over
+    the span of the reference's cuda_duree (after the cell build to the
+    finished tree, SolutionEMSTRW.cu:85-97,133-165)."""
+    import tempfile
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    med = lambda xs: sorted(xs)[len(xs) // 2]
+    with tempfile.TemporaryDirectory() as tmp:
+        try:
+            native = oracle.build_native(tmp)
+            build = "gcc -O3 -march=native -ffp-contract=off -fopenmp"
+        except Exception as e:          # no compiler on this host: the prebuilt -O2 library
+            native = oracle.lib()
+            build = f"prebuilt gcc -O2 (native build failed: {type(e).__name__})"
+        prev = oracle.select(native)
+    cores = oracle.threads(0)
+    try:
+        c1 = pkg.synth(0, 10_000, 2, 0)
+        c1_res = {}
+        for th in (1, cores):
+            oracle.threads(th)
+            spans, walls = [], []
+            for rep in range(6):
+                t1 = time.perf_counter()
+                r = oracle.emst(c1, mode=1, R=1)
+                wall = time.perf_counter() - t1
+                assert len(r["u"]) == len(c1) - 1
+                if rep:
+                    spans.append(r["ms_after_cells"])
+                    walls.append(wall * 1e3)
+            c1_res[th] = {"threads": th, "ms_median": round(med(spans), 3),
+                          "mpts": round(len(c1) / med(spans) / 1e3, 4),
+                          "ms_whole_call_median": round(med(walls), 3)}
+        oracle.threads(cores)
+        sample = pkg.synth(args.kind, args.cpu_sample, args.dim, args.seed)
+        t1 = time.perf_counter()
+        r = oracle.emst(sample, mode=1, R=1, cdim=cdim)
+        ct = time.perf_counter() - t1
+        assert len(r["u"]) == args.cpu_sample - 1
+        n1 = max(args.cpu_sample // 5, 2)
+        oracle.threads(1)
+        t1 = time.perf_counter()
+        r1 = oracle.emst(sample[:n1], mode=1, R=1, cdim=cdim)
+        c1t = time.perf_counter() - t1
+        assert len(r1["u"]) == n1 - 1
+    finally:
+        oracle.threads(cores)
+        oracle.select(prev)
+    return {"value": round(args.cpu_sample / ct / 1e6, 5), "unit": "Mpoints/s", "cores": cores, "kind": "port",
+            "sample": f"{args.cpu_sample} points of the benchmarked distribution and seed, the oracle's "
+                      f"restatement of the reference ring search (R=1), {cores} threads (per-point searches in "
+                      f"parallel, rounds sequential), {ct:.1f} s whole call; one thread on the first {n1} points: "
+                      f"{n1 / c1t / 1e6:.5f} Mpoints/s ({c1t:.1f} s)",
+            "build": build,
+            "reference_cpu_config": {"workload": "10^4 uniform 2D points, seed 0, R=1; median of 5 after 1 warm-up; "
+                                                 "span after the cell build to the finished tree",
+                                     "one_thread": c1_res[1], "all_cores": c1_res[cores]}}
 
 
 def run_tsplib(args, pkg):
@@ -335,16 +377,12 @@ def run_batch(args, pkg):
     import numpy as np
     import torch
 
-    rng = np.random.default_rng(args.batch_seed)
     B = args.batch
-    sizes = np.rint(np.exp(rng.uniform(np.log(1e2), np.log(1e5), B))).astype(np.int64)
-    dims = np.array([2] * (B // 2) + [3] * (B - B // 2))
-    rng.shuffle(dims)
-    kinds = np.where(rng.random(B) < 0.5, 0, 3)
+    sizes, dims, kinds, make = pkg.batch_workload(B, args.batch_seed)
     groups = []
     for d in (2, 3):
         idx = np.nonzero(dims == d)[0]
-        clouds = [pkg.synth(int(kinds[i]), int(sizes[i]), d, args.batch_seed * 1_000_003 + int(i)) for i in idx]
+        clouds = [make(i) for i in idx]
         packed = torch.from_numpy(np.concatenate(clouds)).to("cuda:0")
         groups.append({"dim": d, "idx": idx, "sizes": sizes[idx], "dev": packed})
     torch.cuda.synchronize()

@@ -122,6 +122,24 @@ def synth(kind, n, dim, seed):
     return out
 
 
+def batch_workload(clouds=4096, seed=5):
+    """The batched workload: `clouds` clouds with sizes round(exp(U[ln 1e2,
+    ln 1e5])), half 2D and half 3D (shuffled), each uniform (kind 0) or eight
+    Gaussians (kind 3) by a fair coin.  Returns (sizes, dims, kinds, make):
+    make(i) is cloud i's points, synth(kinds[i], sizes[i], dims[i],
+    seed * 1000003 + i)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.rint(np.exp(rng.uniform(np.log(1e2), np.log(1e5), clouds))).astype(np.int64)
+    dims = np.array([2] * (clouds // 2) + [3] * (clouds - clouds // 2))
+    rng.shuffle(dims)
+    kinds = np.where(rng.random(clouds) < 0.5, 0, 3)
+
+    def make(i):
+        return synth(int(kinds[i]), int(sizes[i]), int(dims[i]), seed * 1_000_003 + int(i))
+
+    return sizes, dims, kinds, make
+
+
 def read_points(path, dim):
     """InputRW::readFile: points of a .2d / .3d / .tsp / .points file as an
     n x dim float32 array (host only)."""

@@ -15,6 +15,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -177,6 +178,39 @@ int or_edge_less(float d1, int32_t a1, int32_t b1, float d2, int32_t a2, int32_t
     double x = d1, y = d2;
     if (x == y) return (a1 == a2) ? (b1 < b2) : (a1 < a2);
     return x < y;
+}
+
+void or_d2_pairs(const float* pts, int dim, const int32_t* eu, const int32_t* ev, int64_t m, float* out)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < m; ++i)
+        out[i] = or_d2(pts + (int64_t)eu[i] * dim, pts + (int64_t)ev[i] * dim, dim);
+}
+
+static int32_t uf_find(int32_t* parent, int32_t x)
+{
+    while (parent[x] != x) {
+        parent[x] = parent[parent[x]];
+        x = parent[x];
+    }
+    return x;
+}
+
+int64_t or_forest_components(int64_t n, const int32_t* eu, const int32_t* ev, int64_t m, int64_t* cycles)
+{
+    int64_t comps = n, cyc = 0;
+    int32_t* parent = (int32_t*)malloc(sizeof(int32_t) * (n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; ++i) parent[i] = (int32_t)i;
+    for (int64_t i = 0; i < m; ++i) {
+        if (eu[i] < 0 || ev[i] < 0 || eu[i] >= n || ev[i] >= n) { cyc++; continue; }
+        const int32_t a = uf_find(parent, eu[i]), b = uf_find(parent, ev[i]);
+        if (a == b) { cyc++; continue; }
+        parent[a < b ? b : a] = a < b ? a : b;
+        comps--;
+    }
+    free(parent);
+    if (cycles) *cycles = cyc;
+    return comps;
 }
 
 static int isinf_pair(double v1, int32_t p, int32_t q1, double v2, int32_t q2)
@@ -560,6 +594,8 @@ int64_t or_boruvka(int mode, const or_grid* g, const float* pts, int64_t n, int 
     int32_t* path = (int32_t*)malloc(sizeof(int32_t) * n);
     or_cells cs = {0, 0, 0};
     if (mode == 1) cells_alloc(g, pts, n, &cs, &tr->overfull_cells);
+    struct timespec t_start, t_end;
+    clock_gettime(CLOCK_MONOTONIC, &t_start);
     for (int64_t i = 0; i < n; ++i) {
         label[i] = (int32_t)i;              /* K_EMST_initDisjointSet, EMSTOperators.h:330-343 */
         corr[i] = -1;                        /* SolutionEMST.cu:464-466 */
@@ -635,6 +671,8 @@ int64_t or_boruvka(int mode, const or_grid* g, const float* pts, int64_t n, int 
         tr->cum_edges[iteration - 1] = (int32_t)m;
     }
     tr->rounds = iteration;
+    clock_gettime(CLOCK_MONOTONIC, &t_end);
+    tr->ms_after_cells = (double)(t_end.tv_sec - t_start.tv_sec) * 1e3 + (double)(t_end.tv_nsec - t_start.tv_nsec) * 1e-6;
     if (label_out) memcpy(label_out, label, sizeof(int32_t) * n);
     if (cand_out) memcpy(cand_out, cid, sizeof(int32_t) * n * K);
     for (int64_t i = 0; i < m; ++i)
@@ -679,6 +717,78 @@ int64_t or_prim_float(const float* pts, int64_t n, int dim, int32_t* eu, int32_t
     }
     free(in); free(kd); free(kfrom);
     return m;
+}
+
+/* Kruskal: candidate keys (d2 bits << 32 | min id) sorted by a stable LSD
+ * radix sort on 16-bit digits, runs of equal keys ordered by max id, then
+ * union-find in key order.  d2 >= 0, so its float bits order as the value. */
+int64_t or_kruskal(const float* pts, int64_t n, int dim, const int32_t* cu, const int32_t* cv, int64_t m,
+                   int32_t* eu, int32_t* ev)
+{
+    if (n <= 1 || m <= 0) return 0;
+    uint64_t* key = (uint64_t*)malloc(sizeof(uint64_t) * m);
+    uint64_t* key2 = (uint64_t*)malloc(sizeof(uint64_t) * m);
+    uint32_t* idx = (uint32_t*)malloc(sizeof(uint32_t) * m);
+    uint32_t* idx2 = (uint32_t*)malloc(sizeof(uint32_t) * m);
+    int64_t* cnt = (int64_t*)malloc(sizeof(int64_t) * 65537);
+    int64_t mm = 0;
+    for (int64_t i = 0; i < m; ++i) {
+        const int32_t a = cu[i] < cv[i] ? cu[i] : cv[i], b = cu[i] < cv[i] ? cv[i] : cu[i];
+        if (a == b || a < 0 || b >= n) continue;
+        float d = or_d2(pts + (int64_t)a * dim, pts + (int64_t)b * dim, dim);
+        uint32_t bits;
+        memcpy(&bits, &d, 4);
+        key[mm] = ((uint64_t)bits << 32) | (uint32_t)a;
+        idx[mm] = (uint32_t)i;
+        mm++;
+    }
+    for (int pass = 0; pass < 4; ++pass) {
+        const int sh = 16 * pass;
+        memset(cnt, 0, sizeof(int64_t) * 65537);
+        for (int64_t i = 0; i < mm; ++i) cnt[((key[i] >> sh) & 0xffff) + 1]++;
+        for (int d = 0; d < 65536; ++d) cnt[d + 1] += cnt[d];
+        for (int64_t i = 0; i < mm; ++i) {
+            const int64_t o = cnt[(key[i] >> sh) & 0xffff]++;
+            key2[o] = key[i];
+            idx2[o] = idx[i];
+        }
+        uint64_t* tk = key; key = key2; key2 = tk;
+        uint32_t* ti = idx; idx = idx2; idx2 = ti;
+    }
+    /* equal (d2, min id): order by max id (insertion sort within the run) */
+    for (int64_t s0 = 0; s0 < mm;) {
+        int64_t s1 = s0 + 1;
+        while (s1 < mm && key[s1] == key[s0]) s1++;
+        for (int64_t i = s0 + 1; i < s1; ++i) {
+            const uint32_t x = idx[i];
+            const int32_t bx = cu[x] < cv[x] ? cv[x] : cu[x];
+            int64_t j = i - 1;
+            while (j >= s0) {
+                const uint32_t y = idx[j];
+                const int32_t by = cu[y] < cv[y] ? cv[y] : cu[y];
+                if (by <= bx) break;
+                idx[j + 1] = y;
+                j--;
+            }
+            idx[j + 1] = x;
+        }
+        s0 = s1;
+    }
+    int32_t* parent = (int32_t*)malloc(sizeof(int32_t) * n);
+    for (int64_t i = 0; i < n; ++i) parent[i] = (int32_t)i;
+    int64_t e = 0;
+    for (int64_t i = 0; i < mm && e < n - 1; ++i) {
+        const uint32_t x = idx[i];
+        const int32_t a = cu[x] < cv[x] ? cu[x] : cv[x], b = cu[x] < cv[x] ? cv[x] : cu[x];
+        const int32_t ra = uf_find(parent, a), rb = uf_find(parent, b);
+        if (ra == rb) continue;
+        parent[ra < rb ? rb : ra] = ra < rb ? ra : rb;
+        eu[e] = a;
+        ev[e] = b;
+        e++;
+    }
+    free(parent); free(key); free(key2); free(idx); free(idx2); free(cnt);
+    return e;
 }
 
 static double d2_fp64(const float* a, const float* b, int dim)

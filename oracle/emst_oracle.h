@@ -46,6 +46,9 @@ typedef struct {
     int64_t examined[OR_MAX_ROUNDS];        /* candidate points examined in round r */
     int64_t overfull_cells;                 /* cells above the reference cap of 32 */
     int64_t cycles_cut;                     /* hooks withheld by the cycle rule */
+    double ms_after_cells;                  /* wall time from after the cell build to the
+                                               finished forest (the span of the reference's
+                                               cuda_duree, SolutionEMSTRW.cu:85-97,133-165) */
 } or_trace;
 
 /* A1 */
@@ -61,6 +64,11 @@ int64_t or_cell_linear(const or_grid* g, const int32_t* cell);
 /* A4 */
 float or_d2(const float* a, const float* b, int dim);
 int or_edge_less(float d1, int32_t a1, int32_t b1, float d2, int32_t a2, int32_t b2);
+/* A4 over an edge list: out[i] = or_d2(pts + eu[i]*dim, pts + ev[i]*dim, dim). */
+void or_d2_pairs(const float* pts, int dim, const int32_t* eu, const int32_t* ev, int64_t m, float* out);
+/* Union-find over the edge list: returns the number of components of the
+ * graph on n vertices; *cycles = edges whose ends were already connected. */
+int64_t or_forest_components(int64_t n, const int32_t* eu, const int32_t* ev, int64_t m, int64_t* cycles);
 /* A5: returns number of points outside the grid (should be 0) */
 int64_t or_cell_build(const or_grid* g, const float* pts, int64_t n,
                       int32_t* cell_of, int64_t* cell_start, int32_t* cell_pts);
@@ -91,6 +99,16 @@ int64_t or_boruvka(int mode, const or_grid* g, const float* pts, int64_t n, int 
 int64_t or_prim_float(const float* pts, int64_t n, int dim, int32_t* eu, int32_t* ev);
 int64_t or_prim_fp64(const float* pts, int64_t n, int dim, int32_t* eu, int32_t* ev,
                      double* weight);
+
+/* Kruskal under the same total order over a candidate edge list (either
+ * orientation, duplicates and self pairs allowed): writes the chosen edges
+ * (u < v) in key order and returns their count.  With every pair as a
+ * candidate it is the complete-graph MST (the tree of or_prim_float); with a
+ * candidate set that holds a spanning tree T it returns T exactly when no
+ * candidate edge could replace an edge of T (the cycle property over the
+ * candidates).  Ids must be below 2^31. */
+int64_t or_kruskal(const float* pts, int64_t n, int dim, const int32_t* cu, const int32_t* cv, int64_t m,
+                   int32_t* eu, int32_t* ev);
 
 /* A15 */
 double or_weight(const float* orig, int dim, const int32_t* eu, const int32_t* ev, int64_t m);

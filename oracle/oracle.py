@@ -35,6 +35,7 @@ class Trace(ctypes.Structure):
         ("examined", ctypes.c_int64 * MAX_ROUNDS),
         ("overfull_cells", ctypes.c_int64),
         ("cycles_cut", ctypes.c_int64),
+        ("ms_after_cells", ctypes.c_double),
     ]
 
 
@@ -43,37 +44,63 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def _bind(L):
+    P = ctypes.c_void_p
+    i64, i32, dbl = ctypes.c_int64, ctypes.c_int, ctypes.c_double
+    L.or_bbox.argtypes = [P, i64, i32, P, P]
+    L.or_rescale.argtypes = [P, i64, i32, P, P, P, P, P]
+    L.or_rescale.restype = dbl
+    L.or_grid_init.argtypes = [P, i32, i32, P, P]
+    L.or_grid_init_pd.argtypes = [P, i32, i32, i32, P, P]
+    L.or_find_cell.argtypes = [P, P, P]
+    L.or_d2.argtypes = [P, P, i32]
+    L.or_d2.restype = ctypes.c_float
+    L.or_d2_pairs.argtypes = [P, i32, P, P, i64, P]
+    L.or_forest_components.argtypes = [i64, P, P, i64, P]
+    L.or_forest_components.restype = i64
+    L.or_cell_build.argtypes = [P, P, i64, P, P, P]
+    L.or_cell_build.restype = i64
+    L.or_octants_all.argtypes = [i32, P, P, i64, i32, i32, P, P, P, P]
+    L.or_boruvka.argtypes = [i32, P, P, i64, i32, i32, i32, P, P, P, P, P, P]
+    L.or_boruvka.restype = i64
+    L.or_prim_float.argtypes = [P, i64, i32, P, P]
+    L.or_prim_float.restype = i64
+    L.or_prim_fp64.argtypes = [P, i64, i32, P, P, P]
+    L.or_prim_fp64.restype = i64
+    L.or_weight.argtypes = [P, i32, P, P, i64]
+    L.or_weight.restype = dbl
+    L.or_kruskal.argtypes = [P, i64, i32, P, P, i64, P, P]
+    L.or_kruskal.restype = i64
+    L.or_threads.argtypes = [i32]
+    L.or_threads.restype = i32
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        P = ctypes.c_void_p
-        i64, i32, dbl = ctypes.c_int64, ctypes.c_int, ctypes.c_double
-        L.or_bbox.argtypes = [P, i64, i32, P, P]
-        L.or_rescale.argtypes = [P, i64, i32, P, P, P, P, P]
-        L.or_rescale.restype = dbl
-        L.or_grid_init.argtypes = [P, i32, i32, P, P]
-        L.or_grid_init_pd.argtypes = [P, i32, i32, i32, P, P]
-        L.or_find_cell.argtypes = [P, P, P]
-        L.or_d2.argtypes = [P, P, i32]
-        L.or_d2.restype = ctypes.c_float
-        L.or_cell_build.argtypes = [P, P, i64, P, P, P]
-        L.or_cell_build.restype = i64
-        L.or_octants_all.argtypes = [i32, P, P, i64, i32, i32, P, P, P, P]
-        L.or_boruvka.argtypes = [i32, P, P, i64, i32, i32, i32, P, P, P, P, P, P]
-        L.or_boruvka.restype = i64
-        L.or_prim_float.argtypes = [P, i64, i32, P, P]
-        L.or_prim_float.restype = i64
-        L.or_prim_fp64.argtypes = [P, i64, i32, P, P, P]
-        L.or_prim_fp64.restype = i64
-        L.or_weight.argtypes = [P, i32, P, P, i64]
-        L.or_weight.restype = dbl
-        L.or_threads.argtypes = [i32]
-        L.or_threads.restype = i32
-        _lib = L
+        _lib = _bind(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+def build_native(out_dir):
+    """The timed CPU baseline's build: the same source compiled with -O3
+    -march=native for the host that runs it (into out_dir).  Returns the
+    loaded library, for select()."""
+    out = os.path.join(out_dir, "liboracle_emst_native.so")
+    subprocess.run(["make", "-s", "-C", _HERE, "native", f"NATIVE_OUT={out}"], check=True)
+    return _bind(ctypes.CDLL(out))
+
+
+def select(L):
+    """Make L (a library from lib() or build_native()) the one every function
+    here calls; returns the previous one."""
+    global _lib
+    old = lib()
+    _lib = L
+    return old
 
 
 def _p(a):
@@ -195,6 +222,7 @@ def boruvka(rescaled, mode=0, R=1, max_rounds=100, want_cands=False, cdim=None):
         "examined": list(tr.examined[:rounds]),
         "overfull_cells": tr.overfull_cells,
         "cycles_cut": tr.cycles_cut,
+        "ms_after_cells": tr.ms_after_cells,
         "cands": cand[:n] if cand is not None else None,
     }
 
@@ -228,6 +256,39 @@ def weight(orig, u, v):
     u = np.ascontiguousarray(u, dtype=np.int32)
     v = np.ascontiguousarray(v, dtype=np.int32)
     return lib().or_weight(_p(orig), orig.shape[1], _p(u), _p(v), len(u))
+
+
+def d2_pairs(rescaled, u, v):
+    """A4 for every edge (u[i], v[i]): float32 d2 with the reference's fused
+    operation order."""
+    rescaled = np.ascontiguousarray(rescaled, dtype=np.float32)
+    u = np.ascontiguousarray(u, dtype=np.int32)
+    v = np.ascontiguousarray(v, dtype=np.int32)
+    out = np.empty(len(u), np.float32)
+    lib().or_d2_pairs(_p(rescaled), rescaled.shape[1], _p(u), _p(v), len(u), _p(out))
+    return out
+
+
+def forest_components(n, u, v):
+    """(components of the graph on n vertices, edges that closed a cycle)."""
+    u = np.ascontiguousarray(u, dtype=np.int32)
+    v = np.ascontiguousarray(v, dtype=np.int32)
+    cyc = ctypes.c_int64(0)
+    comps = lib().or_forest_components(n, _p(u), _p(v), len(u), ctypes.byref(cyc))
+    return comps, cyc.value
+
+
+def kruskal(rescaled, cu, cv):
+    """Kruskal under the float total order over the candidate pairs (cu[i],
+    cv[i]); returns the chosen edges (u < v) in key order."""
+    rescaled = np.ascontiguousarray(rescaled, dtype=np.float32)
+    n, dim = rescaled.shape
+    cu = np.ascontiguousarray(cu, dtype=np.int32)
+    cv = np.ascontiguousarray(cv, dtype=np.int32)
+    eu = np.zeros(max(n - 1, 1), np.int32)
+    ev = np.zeros(max(n - 1, 1), np.int32)
+    m = lib().or_kruskal(_p(rescaled), n, dim, _p(cu), _p(cv), len(cu), _p(eu), _p(ev))
+    return eu[:m].copy(), ev[:m].copy()
 
 
 def canon(u, v):

@@ -358,10 +358,10 @@ def test_large_ring_search_parity(pkg, oracle_mod, dim, kind):
 
 
 @pytest.mark.parametrize("dim,kind", [(3, 1), (2, 0)])
-def test_full_size_properties(pkg, dim, kind):
-    """2e6 points: spanning tree, canonical order, d2 ordering key
-    consistent with the rescaled coordinates, weight equals the sum of edge
-    lengths, two runs identical."""
+def test_full_size_properties(pkg, oracle_mod, dim, kind):
+    """2e6 points: spanning tree, canonical order, every d2 equal bit for bit
+    to the oracle's fused d2 of the rescaled coordinates, weight equals the
+    sum of edge lengths, two runs identical."""
     n = 2_000_000
     pts = pkg.synth(kind, n, dim, 77)
     print("case", dim, kind, n, flush=True)
@@ -380,14 +380,10 @@ def test_full_size_properties(pkg, dim, kind):
     key = u.astype(np.int64) * n + v
     assert np.all(np.diff(key) > 0)
     assert _is_spanning_tree(n, u, v)
-    a, b = resc[u], resc[v]
-    dd = (a - b).astype(np.float32)
-    if dim == 2:
-        rd2 = np.float32(np.float64(dd[:, 1]) ** 2 + np.float64(np.float32(dd[:, 0] * dd[:, 0])))
-    else:
-        t = np.float32(np.float64(dd[:, 1]) ** 2 + np.float64(np.float32(dd[:, 0] * dd[:, 0])))
-        rd2 = np.float32(np.float64(dd[:, 2]) ** 2 + np.float64(t))
-    assert np.mean(rd2 == d) > 0.999
+    rescaled, _, _, _ = oracle_mod.rescale(pts)
+    assert np.array_equal(resc.view(np.uint32), rescaled.view(np.uint32))
+    rd2 = oracle_mod.d2_pairs(rescaled, u, v)
+    assert np.array_equal(rd2.view(np.uint32), d.view(np.uint32))
     a, b = pts[u].astype(np.float64), pts[v].astype(np.float64)
     w64 = np.sqrt(((a - b) ** 2).sum(1)).sum()
     assert abs(w - w64) <= 1e-6 * w64
