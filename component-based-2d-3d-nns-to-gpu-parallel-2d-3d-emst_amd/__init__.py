@@ -28,7 +28,7 @@ EXPORTS = [
     "emst_set_radius_model", "emst_get_radius_samples", "emst_mlp_train", "emst_mlp_eval", "emst_mlp_grad",
     "emst_set_clouds", "emst_get_cloud_results", "emst_autotune", "emst_get_shape", "emst_get_options",
     "emst_read_points",
-    "emst_write_solution",
+    "emst_write_solution", "emst_weight_noalloc",
 ]
 CLI_PATH = os.path.join(_HERE, "emst_cli")
 
@@ -104,6 +104,7 @@ def lib():
     L.emst_get_options.argtypes = [P, P]
     L.emst_read_points.argtypes = [ctypes.c_char_p, i32, P, P]
     L.emst_write_solution.argtypes = [P, ctypes.c_char_p]
+    L.emst_weight_noalloc.argtypes = [P, i32, P, P, i64, i32, P, P]
     L.emst_version.restype = ctypes.c_char_p
     _lib = L
     return L

@@ -9,6 +9,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -36,6 +37,36 @@ struct DevBuf {
         if (p) (void)hipFree(p);
     }
 };
+
+}  // namespace
+
+// Partial sums of emst_weight_noalloc: one k_weight block each.  A static
+// device array, so that entry point allocates nothing; each GPU the library
+// runs on holds its own copy, at its own address, until the process ends.
+// Host code never passes `scratch` itself to a kernel: it uses the address
+// that hipGetSymbolAddress gives for the current device (scratch_on).
+static __device__ __align__(256) double scratch[kWeightBlocks];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (which must be current),
+// looked up once per device.
+hipError_t scratch_on(int device, double** out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[device] = static_cast<double*>(p);
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
 
 }  // namespace
 
@@ -1149,6 +1180,49 @@ int emst_get_weight(emst_ctx* c, double* w, int64_t* size)
     const int64_t cnt = c->built ? (int64_t)c->edges : 0;
     if (size) *size = cnt;
     return weight_of(c, cnt > 0 ? c->n : 0, w);
+}
+
+int emst_weight_noalloc(const float* xyz, int32_t dim, const int32_t* eu, const int32_t* ev, int64_t m,
+                        int32_t device, void* stream, double* weight)
+{
+    if (!weight) { g_create_error = "weight must not be NULL"; return EMST_E_ARG; }
+    *weight = 0;
+    if (dim != 2 && dim != 3) { g_create_error = "dim must be 2 or 3"; return EMST_E_ARG; }
+    if (m < 0) { g_create_error = "m < 0"; return EMST_E_ARG; }
+    if (m >= ((int64_t)1 << 31)) { g_create_error = "m must be below 2^31"; return EMST_E_LIMIT; }
+    if (m > 0 && (!xyz || !eu || !ev)) { g_create_error = "NULL device array"; return EMST_E_ARG; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        g_create_error = "no HIP device";
+        return EMST_E_NODEVICE;
+    }
+    if (device < 0 || device >= ndev || device >= kMaxDevices) {
+        g_create_error = "device ordinal out of range";
+        return EMST_E_ARG;
+    }
+    hipError_t e = hipSetDevice(device);
+    double* part = nullptr;
+    if (e == hipSuccess) e = scratch_on(device, &part);
+    if (e != hipSuccess) {
+        g_create_error = std::string("scratch set-up: ") + hipGetErrorString(e);
+        return EMST_E_HIP;
+    }
+    if (m == 0) return EMST_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    launch_weight(xyz, dim, eu, ev, m, part, kWeightBlocks, s);
+    double h[kWeightBlocks];
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        g_create_error = std::string("weight: ") + hipGetErrorString(e);
+        return EMST_E_HIP;
+    }
+    double sum = 0;
+    for (double x : h) sum += x;
+    *weight = sum;
+    return EMST_OK;
 }
 
 int emst_get_labels(emst_ctx* c, int32_t* labels)

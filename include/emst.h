@@ -183,6 +183,23 @@ int emst_get_edges(emst_ctx* ctx, int32_t* u, int32_t* v, float* d2, int64_t* m)
  * of sqrt(float d2) over the forest edges, on the input coordinates. */
 int emst_get_weight(emst_ctx* ctx, double* weight, int64_t* size);
 
+/* No counterpart in the reference: the entry point for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The FP64 weight that emst_get_weight
+ * computes, for an edge list the caller holds on the device: the sum over
+ * i < m with eu[i] >= 0 of sqrt(double(d2)), d2 the float32 squared distance
+ * of xyz[eu[i]] and xyz[ev[i]] (xyz: n x dim float32, AoS, input
+ * coordinates).  xyz, eu and ev are device pointers on `device`; the work
+ * runs on `stream` (a hipStream_t; NULL = the null stream) and the call
+ * returns once *weight (host memory) is written.  It calls no hipMalloc or
+ * hipFree: its partial sums (512 doubles, 4 KiB) live in a static device
+ * array of the library, one per GPU, held until the process ends; every call
+ * on a GPU overwrites all of them, and calls that overlap on one GPU race.
+ * m must be below 2^31 and dim 2 or 3; on failure emst_last_error(NULL)
+ * gives the reason. */
+int emst_weight_noalloc(const float* xyz, int32_t dim, const int32_t* eu, const int32_t* ev, int64_t m,
+                        int32_t device, void* stream, double* weight);
+
 /* The disjoint-set labels (disjointSetMap, written by NeuralNetEMST::writeLinks
  * emst_coalition/include/NeuralNetEMST.h:310-318): root point index per point,
  * after the last round's flatten. */
