@@ -351,8 +351,8 @@ class Calculateur:
         return X, T
 
     def times(self):
-        t = (ctypes.c_float * 5)()
-        k = ctypes.c_int32(5)
+        t = (ctypes.c_float * 6)()
+        k = ctypes.c_int32(6)
         self._chk(self._L.emst_get_times(self._c, t, ctypes.byref(k)), "times")
         return list(t)[: k.value]
 
@@ -381,12 +381,12 @@ def mlp_eval(model, X, device=0):
 
 def _stats_names():
     return ["packets", "nodes", "leaves", "leaf_points", "pairs", "group_points", "grouped", "cycles_cut", "resumed",
-            "comp_queries"]
+            "comp_queries", "comp_nodes", "comp_leaves", "comp_pairs", "knn_nodes", "knn_leaves", "knn_pairs"]
 
 
 def search_stats(calc):
-    t = (ctypes.c_int64 * 10)()
-    k = ctypes.c_int32(10)
+    t = (ctypes.c_int64 * 16)()
+    k = ctypes.c_int32(16)
     calc._chk(calc._L.emst_get_search_stats(calc._c, t, ctypes.byref(k)), "stats")
     return dict(zip(_stats_names(), list(t)[: k.value]))
 

@@ -95,11 +95,13 @@ struct emst_ctx {
     uint32_t zero_or_empty = 0;
     std::vector<emst_round_trace> trace;
     float times[4] = {0, 0, 0, 0};
-    int64_t stats[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t stats[16] = {};
     uint32_t resumed_h = 0;           // queries resumed after the first search pass
     float seed_scale = 3.0f;          // plain seed: multiple of the leaf cell's mean spacing
-    hipEvent_t ev[12] = {};             // 8..11: around the component search's dispatches (traced runs)
+    hipEvent_t ev[14] = {};             // 8..11: around the component search's dispatches, 12..13:
+                                        // around the neighbour lists' dispatch (traced runs)
     float comp_search_ms = 0.f;         // those dispatches' time, summed over the traced run's rounds
+    float knn_ms = 0.f;                 // the neighbour lists' dispatch (traced runs)
     // device buffers
     DevBuf<float> xyz_own, resc;
     DevBuf<uint64_t> keys0, keys1, cand_key, corr_key, comp1;
@@ -107,6 +109,7 @@ struct emst_ctx {
     DevBuf<float4> pts;
     DevBuf<int32_t> inv, cand_pos, label, label2, corr, fixed, list, eu, ev_, eround, geo;
     DevBuf<Node> nodes;
+    DevBuf<int32_t> lstart, leaf_of, cloud_lf;     // LeafTable and each cloud's first leaf
     DevBuf<float> ed2;
     DevBuf<double> partial;
     DevBuf<float> model, feat, pred, seed, ls;     // learned seed radius
@@ -228,7 +231,6 @@ int alloc_all(emst_ctx* c)
     ENS(scan_tmp, scan_tmp_words((int64_t)radix_hist_words(n)) + scan_tmp_words(n) + 16);
     ENS(pts, n);
     ENS(inv, n);
-    ENS(nodes, 2 * c->nleaves);
     ENS(cand_key, n * K);
     ENS(cand_pos, n * K);
     ENS(label, n);
@@ -259,7 +261,6 @@ int alloc_all(emst_ctx* c)
     ENS(dup_runs, n);
     ENS(dup_gslot, n);
     if (comp) {
-        ENS(visit, 2 * c->nleaves);
         ENS(list2, n);
         ENS(knn_pos, (size_t)kKnn * n);
         ENS(knn_key, (size_t)kKnn * n);
@@ -301,27 +302,22 @@ static_assert(offsetof(Counters, cycles_cut) + sizeof(uint32_t) - offsetof(Count
 static_assert(offsetof(Counters, zero_or_empty) == offsetof(Counters, research) + sizeof(uint32_t),
               "research and zero_or_empty are cleared together");
 
-// Cloud positions, leaf counts and hierarchy offsets for leaves of `bucket`
-// points (hcl[k].n already set); sizes the buffers.
+// Cloud positions (hcl[k].n already set) and leaves of at most `bucket`
+// points; sizes the buffers.  The leaves themselves, and with them each
+// cloud's hierarchy, come from the sorted keys in the build (make_leaves).
 int layout_clouds(emst_ctx* c, int bucket)
 {
     c->bucket = bucket;
-    int64_t pos = 0, lbase = 0, Lmax = 1, target = 0;
+    int64_t pos = 0, target = 0;
     for (Cloud& cl : c->hcl) {
-        const int64_t nb = std::max<int64_t>(1, (cl.n + bucket - 1) / bucket);
-        int64_t L = 1;
-        while (L < nb) L <<= 1;
         cl.pos = pos;
-        cl.lbase = lbase;
-        cl.L = L;
+        cl.lbase = 0;
+        cl.L = 1;
+        cl.lfirst = 0;
+        cl.nl = 0;
         pos += cl.n;
-        lbase += L;
-        Lmax = std::max(Lmax, L);
         target += std::max<int64_t>(cl.n - 1, 0);
     }
-    c->nleaves = lbase;
-    c->Lmax = Lmax;
-    c->L = c->hcl[0].L;
     c->target_edges = target;
     // Morton bits per axis: the cloud index takes the top key bits of a batch
     int cb = 0;
@@ -334,6 +330,47 @@ int layout_clouds(emst_ctx* c, int bucket)
         return rc;
     c->P = c->params.p;
     HIPCHK(c, hipMemcpyAsync(c->dcl.p, c->hcl.data(), sizeof(Cloud) * c->ncl, hipMemcpyHostToDevice, c->stream));
+    return EMST_OK;
+}
+
+LeafTable leaf_table(const emst_ctx* c) { return LeafTable{c->lstart.p, c->leaf_of.p}; }
+
+// The leaves of the sorted points (LeafTable), then each cloud's hierarchy:
+// its leaf count sets its slot count L (a power of two) and its node offset.
+// One host synchronisation, for the leaf counts.
+int make_leaves(emst_ctx* c)
+{
+    const int64_t n = c->n;
+    hipStream_t s = c->stream;
+    int rc;
+    if ((rc = ensure(c, c->lstart, (size_t)n + 1)) || (rc = ensure(c, c->leaf_of, (size_t)n)) ||
+        (rc = ensure(c, c->cloud_lf, (size_t)c->ncl + 1)))
+        return rc;
+    // the sort's second key and value buffers are free here
+    launch_leaves(c->keys0.p, n, c->dcl.p, c->ncl, c->bucket, c->vals1.p, reinterpret_cast<uint32_t*>(c->keys1.p),
+                  c->scan_tmp.p, c->lstart.p, c->leaf_of.p, c->cloud_lf.p, s);
+    LAUNCHCHK(c);
+    std::vector<int32_t> lf(c->ncl + 1);
+    HIPCHK(c, hipMemcpyAsync(lf.data(), c->cloud_lf.p, sizeof(int32_t) * (c->ncl + 1), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    int64_t lbase = 0, Lmax = 1;
+    for (int k = 0; k < c->ncl; ++k) {
+        Cloud& cl = c->hcl[k];
+        cl.lfirst = lf[k];
+        cl.nl = lf[k + 1] - lf[k];
+        int64_t L = 1;
+        while (L < cl.nl) L <<= 1;
+        cl.L = L;
+        cl.lbase = lbase;
+        lbase += L;
+        Lmax = std::max(Lmax, L);
+    }
+    c->nleaves = lbase;
+    c->Lmax = Lmax;
+    c->L = c->hcl[0].L;
+    if ((rc = ensure(c, c->nodes, 2 * (size_t)lbase))) return rc;
+    if (c->opt.round_search == 1 && (rc = ensure(c, c->visit, 2 * (size_t)lbase))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->dcl.p, c->hcl.data(), sizeof(Cloud) * c->ncl, hipMemcpyHostToDevice, s));
     return EMST_OK;
 }
 
@@ -471,6 +508,8 @@ int build(emst_ctx* c)
     }
     radix_sort(c->keys0.p, c->vals0.p, c->keys1.p, c->vals1.p, n, key_bits, true, c->hist.p, c->scan_tmp.p, s);
     launch_gather_sorted(c->resc.p, c->vals0.p, n, c->dim, c->pts.p, c->inv.p, s);
+    int rcl = make_leaves(c);
+    if (rcl) return rcl;
     // coincident-point groups: dup_runs .. cycles_cut start at zero
     HIPCHK(c, hipMemsetAsync(&c->C->dup_runs, 0, 8 * sizeof(uint32_t), s));
     c->grouped = false;
@@ -497,7 +536,7 @@ int build(emst_ctx* c)
         }
     }
     const DupState dup = dup_state(c);
-    launch_build_tree(c->pts.p, c->dcl.p, c->ncl, c->nleaves, c->Lmax, c->bucket, c->nodes.p, dup.rep, s);
+    launch_build_tree(c->pts.p, c->dcl.p, c->ncl, c->nleaves, c->Lmax, leaf_table(c), c->nodes.p, dup.rep, s);
     launch_round_init(n, c->label.p, c->label2.p, c->fixed.p, c->eu.p, c->corr.p, c->corr_key.p, c->comp1.p,
                       c->comp2.p, c->C, (uint32_t)c->target_edges, (uint32_t)c->opt.max_rounds, s);
     LAUNCHCHK(c);
@@ -505,13 +544,16 @@ int build(emst_ctx* c)
         // the per-round component search has no round-1 search: every round
         // searches (launch_comp_round)
         HIPCHK(c, hipMemsetAsync(c->visit.p, 0, sizeof(unsigned long long) * 2 * c->nleaves, s));
-        launch_node_min_ids(c->pts.p, c->dcl.p, c->ncl, c->nleaves, c->bucket, c->nodes.p, c->visit.p, s);
+        launch_node_min_ids(c->pts.p, c->dcl.p, c->ncl, c->nleaves, leaf_table(c), c->nodes.p, c->visit.p, s);
         // round 1's targets and the neighbour lists; the learned component
         // seed keeps round 1's searches (its sample log starts there)
         c->knn_built = !comp_features_on(c);
+        const bool tr = c->opt.trace != 0;
+        if (tr) HIPCHK(c, hipEventRecord(c->ev[12], s));
         if (c->knn_built)
-            launch_knn_first(c->dim, n, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, c->corr.p,
-                             c->corr_key.p, nbr_list(c), s);
+            launch_knn_first(c->dim, n, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, leaf_table(c), c->corr.p,
+                             c->corr_key.p, nbr_list(c), c->C, tr, s);
+        if (tr) HIPCHK(c, hipEventRecord(c->ev[13], s));
         int rc1 = ensure_comp_seed(c);
         if (rc1) return rc1;
         if (comp_features_on(c)) launch_comp_seed_init(n, c->pts.p, c->csize.p, c->cbox.p, s);
@@ -528,6 +570,7 @@ int build(emst_ctx* c)
         c->resumed_h = 0;
         c->times[0] = elapsed(c->ev[0], c->ev[1]);
         c->times[1] = 0.f;
+        c->knn_ms = tr ? elapsed(c->ev[12], c->ev[13]) : 0.f;
         c->built = true;
         c->rounds = 0;
         c->edges = 0;
@@ -541,7 +584,7 @@ int build(emst_ctx* c)
         if ((rc2 = ensure(c, c->feat, (size_t)n * 8)) || (rc2 = ensure(c, c->pred, (size_t)n)) ||
             (rc2 = ensure(c, c->seed, (size_t)n)) || (rc2 = ensure(c, c->ls, (size_t)n)))
             return rc2;
-        launch_radius_features(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, c->feat.p, c->ls.p, s);
+        launch_radius_features(c->dim, c->pts.p, c->nodes.p, c->L, leaf_table(c), n, c->feat.p, c->ls.p, s);
         launch_radius_seed(n, c->model.p, c->feat.p, c->ls.p, c->pred.p, c->seed.p, s);
         seed_r = c->seed.p;
     }
@@ -564,7 +607,7 @@ int build(emst_ctx* c)
     const int mode = c->opt.search_split;
     split.compact_frac = mode == 2 ? -1.0 : (mode == 3 ? 1.0 : 0.10);
     const bool use_split = mode == 2 || mode == 3 || (mode == 0 && c->cdim == 3 && n >= ((int64_t)1 << 21));
-    launch_cone_search(c->dim, c->cdim, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, n, nullptr, n,
+    launch_cone_search(c->dim, c->cdim, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, leaf_table(c), n, nullptr, n,
                        c->label.p, false, c->inv.p, seed_r, c->seed_scale, c->cand_key.p, c->cand_pos.p, nullptr, c->corr.p,
                        c->corr_key.p, c->C, true, dup, use_split ? &split : nullptr, s);
     LAUNCHCHK(c);
@@ -655,7 +698,7 @@ void issue_round(emst_ctx* c, bool tr)
     for (int ph = 0; ph < 4; ++ph) {
         if (ph == 0 && c->opt.round_search == 1) {
             launch_comp_round(c->dim, c->n, c->pts.p, c->nodes.p, c->visit.p, c->dcl.p, c->ncl,
-                              c->nleaves, c->bucket, c->label.p, c->corr.p, c->corr_key.p, c->comp1.p,
+                              c->nleaves, c->bucket, leaf_table(c), c->label.p, c->corr.p, c->corr_key.p, c->comp1.p,
                               c->inv.p, c->seed_scale, split_scratch(c), comp_seed(c), nbr_list(c), c->eu.p,
                               c->eround.p, c->C, s, tr ? c->ev + 8 : nullptr);
             if (tr) (void)hipEventRecord(c->ev[4], s);
@@ -698,6 +741,12 @@ void absorb_counters(emst_ctx* c, const Counters& h)
     c->rounds = (int)h.round;
     c->stats[7] = (int64_t)h.cycles_cut;
     c->stats[9] = (int64_t)h.comp_queries;
+    c->stats[10] = (int64_t)h.comp_nodes;
+    c->stats[11] = (int64_t)h.comp_leaves;
+    c->stats[12] = (int64_t)h.comp_pairs;
+    c->stats[13] = (int64_t)h.knn_nodes;
+    c->stats[14] = (int64_t)h.knn_leaves;
+    c->stats[15] = (int64_t)h.knn_pairs;
     if (h.done == 1u) c->finished = true;
 }
 
@@ -726,7 +775,7 @@ int round_once(emst_ctx* c, int32_t* finished)
         int rc = read_counters(c, &h);
         if (rc) return rc;
         if (h.research > 0) {
-            launch_cone_search(c->dim, c->cdim, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, n, c->list.p,
+            launch_cone_search(c->dim, c->cdim, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, leaf_table(c), n, c->list.p,
                                h.research, c->label.p, true, c->inv.p, nullptr, c->seed_scale, c->cand_key.p,
                                c->cand_pos.p, c->corr.p, nullptr, nullptr, c->C, false, DupState{}, nullptr, s);
             launch_rank_cones(c->cdim, n, c->cand_key.p, c->cand_pos.p, c->list.p, h.research, c->rkey.p, c->rpos.p,
@@ -781,6 +830,7 @@ std::vector<uintptr_t> graph_key(emst_ctx* c)
             (uintptr_t)c->opt.log_samples, (uintptr_t)c->visit.p, (uintptr_t)c->list2.p, (uintptr_t)c->knn_pos.p,
             (uintptr_t)c->knn_key.p, (uintptr_t)c->knn_cur.p, (uintptr_t)c->knn_built, (uintptr_t)c->open_mask.p, (uintptr_t)c->open_cnt.p,
             (uintptr_t)c->open_tmp.p, (uintptr_t)c->list.p, (uintptr_t)c->nodes.p, (uintptr_t)c->dcl.p,
+            (uintptr_t)c->nleaves, (uintptr_t)c->lstart.p, (uintptr_t)c->leaf_of.p, (uintptr_t)c->bucket,
             (uintptr_t)y.j[0], (uintptr_t)y.kh[0], (uintptr_t)y.kl[0], (uintptr_t)y.onc, (uintptr_t)y.cut,
             (uintptr_t)c->grouped, (uintptr_t)d.npoints,
             (uintptr_t)d.ngroups, (uintptr_t)c->C, (uintptr_t)c->pts.p, (uintptr_t)c->cand_key.p,
@@ -1374,8 +1424,8 @@ int emst_get_options(emst_ctx* c, emst_options* o)
 int emst_get_times(emst_ctx* c, float* times, int32_t* n_times)
 {
     if (!c || !times || !n_times) return EMST_E_ARG;
-    const int32_t k = std::min<int32_t>(*n_times, 5);
-    for (int i = 0; i < k; ++i) times[i] = i < 4 ? c->times[i] : c->comp_search_ms;
+    const int32_t k = std::min<int32_t>(*n_times, 6);
+    for (int i = 0; i < k; ++i) times[i] = i < 4 ? c->times[i] : (i == 4 ? c->comp_search_ms : c->knn_ms);
     *n_times = k;
     return EMST_OK;
 }
@@ -1436,7 +1486,7 @@ int emst_get_radius_samples(emst_ctx* c, float* feat, float* target, int64_t* m)
         (rc = ensure(c, c->ls, (size_t)n)))
         return rc;
     hipStream_t s = c->stream;
-    launch_radius_features(c->dim, c->pts.p, c->nodes.p, c->L, c->bucket, n, c->feat.p, c->ls.p, s);
+    launch_radius_features(c->dim, c->pts.p, c->nodes.p, c->L, leaf_table(c), n, c->feat.p, c->ls.p, s);
     launch_radius_targets(c->K, n, c->cand_key.p, c->ls.p, c->pred.p, s);
     LAUNCHCHK(c);
     if (feat) HIPCHK(c, hipMemcpyAsync(feat, c->feat.p, sizeof(float) * n * 8, hipMemcpyDeviceToHost, s));
@@ -1448,7 +1498,7 @@ int emst_get_radius_samples(emst_ctx* c, float* feat, float* target, int64_t* m)
 int emst_get_search_stats(emst_ctx* c, int64_t* stats, int32_t* n_stats)
 {
     if (!c || !stats || !n_stats) return EMST_E_ARG;
-    const int32_t k = std::min<int32_t>(*n_stats, 10);
+    const int32_t k = std::min<int32_t>(*n_stats, 16);
     for (int i = 0; i < k; ++i) stats[i] = c->stats[i];
     *n_stats = k;
     return EMST_OK;

@@ -51,6 +51,11 @@ struct Counters {
     uint32_t nroots;        // roots listed by k_cycle_init this round (2.5D cycle rule)
     uint32_t nsamples;      // seed samples logged by the component search over the run
     unsigned long long comp_queries;    // queries over the component search's dispatches, over the run
+    // work of the component search's dispatches over the run: node tests
+    // (one per node a packet examines), leaf visits, point pairs evaluated
+    unsigned long long comp_nodes, comp_leaves, comp_pairs;
+    // the same for the neighbour lists at build (k_knn_first)
+    unsigned long long knn_nodes, knn_leaves, knn_pairs;
 };
 
 // Coincident-point groups.  Points with identical rescaled coordinates form a
@@ -104,13 +109,29 @@ struct CycleState {
 // one input array).  Cloud c owns the global ids and sorted positions
 // [pos, pos + n) (ids keep input order inside the cloud, so every tie breaks
 // as in a run of the cloud alone; its input coordinates start at float
-// pos * dim) and its own implicit binary hierarchy of L leaves (a power of
-// two) at nodes[2 * lbase + i], 1 <= i < 2L.
+// pos * dim) and its own implicit binary hierarchy of L leaf slots (a power
+// of two) at nodes[2 * lbase + i], 1 <= i < 2L.  Its nl leaves are the global
+// leaves [lfirst, lfirst + nl) of the LeafTable, in slots 0 .. nl - 1; the
+// slots past them are empty.
 struct Cloud {
     int64_t pos;
     int64_t n;
     int64_t lbase;
     int64_t L;
+    int64_t lfirst;
+    int64_t nl;
+};
+
+// Leaves of the search hierarchy: the maximal cells of the binary prefix tree
+// of the sorted keys (Morton prefixes) that hold at most `bucket` points; a
+// run of more than `bucket` equal keys, which no key bit splits, is cut every
+// `bucket` points from its start.  Numbered in sorted order, cloud by cloud:
+// leaf g holds the sorted positions [start[g], start[g + 1]); of[p] = the
+// leaf of position p.  Aligned cells keep the leaf boxes tight (a fixed
+// number of consecutive positions would straddle the jumps of the curve).
+struct LeafTable {
+    const int32_t* start;   // leaves + 1
+    const int32_t* of;      // n
 };
 
 // Search-hierarchy node: tight box of a contiguous range of Morton-sorted points.
@@ -147,8 +168,14 @@ void launch_dup_groups(const uint64_t* keys, const float4* pts, int64_t n, int32
                        int32_t* runs, int32_t* list, int32_t* gslot, Counters* C, hipStream_t s);
 // Leaf boxes leave out group members (rep == nullptr: grouping off).
 // nleaves = sum of the clouds' L, Lmax = largest L.
-void launch_build_tree(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, int64_t Lmax, int bucket,
+void launch_build_tree(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, int64_t Lmax, LeafTable lt,
                        Node* nodes, const int32_t* rep, hipStream_t s);
+// The leaves of the sorted keys (see LeafTable): mark and idx are n words of
+// scratch, tmp scan_tmp_words(n); start (n + 1 words at most) and of (n) are
+// written, and cloud_lf[c] = the first leaf of cloud c, cloud_lf[ncl] = the
+// number of leaves.
+void launch_leaves(const uint64_t* keys, int64_t n, const Cloud* cl, int ncl, int bucket, uint32_t* mark,
+                   uint32_t* idx, uint32_t* tmp, int32_t* start, int32_t* of, int32_t* cloud_lf, hipStream_t s);
 // Cone search over queries [0,n) (qlist == nullptr) or qlist[0..nq).
 // check_labels: skip zero-distance points of the query's own component.
 // collect_stats: add the traversal counters to C.
@@ -168,7 +195,7 @@ struct SplitScratch {
 // 3 is the 2.5D mode: wedges over the xy plane).  split (nullptr: one launch
 // over every pass) applies to full searches (qlist == nullptr).
 void launch_cone_search(int dim, int cdim, const float4* pts, const Node* nodes, const Cloud* cl, int ncl, int bucket,
-                        int64_t n, const int32_t* qlist, int64_t nq, const int32_t* label,
+                        LeafTable lt, int64_t n, const int32_t* qlist, int64_t nq, const int32_t* label,
                         bool check_labels, const int32_t* inv, const float* seed_r, float seed_scale,
                         uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
                         uint64_t* first_key, Counters* C, bool collect_stats, DupState dup,
@@ -210,7 +237,7 @@ struct NbrList {
     uint8_t* cur;           // n
 };
 void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsigned long long* visit,
-                       const Cloud* cl, int ncl, int64_t nleaves, int bucket, const int32_t* label, int32_t* corr, uint64_t* corr_key, uint64_t* comp1, const int32_t* inv,
+                       const Cloud* cl, int ncl, int64_t nleaves, int bucket, LeafTable lt, const int32_t* label, int32_t* corr, uint64_t* corr_key, uint64_t* comp1, const int32_t* inv,
                        float seed_scale, const SplitScratch& sc, const CompSeed& cs, const NbrList& nb,
                        const int32_t* eu, const int32_t* eround, Counters* C, hipStream_t s,
                        hipEvent_t* ev = nullptr);
@@ -219,7 +246,8 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsig
 // launch_node_min_ids).  Writes corr / corr_key (every component is one
 // point, so the nearest neighbour is the round's exact target) and nb.
 void launch_knn_first(int dim, int64_t n, const float4* pts, const Node* nodes, const Cloud* cl, int ncl,
-                      int bucket, int32_t* corr, uint64_t* corr_key, const NbrList& nb, hipStream_t s);
+                      int bucket, LeafTable lt, int32_t* corr, uint64_t* corr_key, const NbrList& nb, Counters* C, bool collect,
+                      hipStream_t s);
 // (ev: 4 events recorded around the two k_comp_search dispatches, or nullptr)
 // Sizes and boxes of the single-point components (the start of a run).
 void launch_comp_seed_init(int64_t n, const float4* pts, int32_t* csize, uint32_t* cbox, hipStream_t s);
@@ -230,7 +258,7 @@ void launch_mlp_infer_dev(const float* model, const float* X, const uint32_t* m_
 // (0xffffffff for a node without points).  visit:
 // 2 * nleaves words of arrival state, zero before the first bottom-up launch
 // over this hierarchy (k_nmin_tree, k_nlab_tree) and kept between them.
-void launch_node_min_ids(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, int bucket, Node* nodes,
+void launch_node_min_ids(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, LeafTable lt, Node* nodes,
                          unsigned long long* visit, hipStream_t s);
 // Cones of every group point for the current labels (dim: the cone dimension) (after the grouped
 // cone search, and at the start of each round while a group may still span
@@ -285,7 +313,7 @@ void launch_cell_grid(const float* resc, int64_t n, int dim, int pstride, const 
                       uint64_t* keys, uint32_t* cell_count, hipStream_t s);
 // learned seed radius (emst_mlp.hip)
 // X: n x 8 model features, LS: n x log(leaf spacing s0)
-void launch_radius_features(int dim, const float4* pts, const Node* nodes, int64_t L, int bucket, int64_t n,
+void launch_radius_features(int dim, const float4* pts, const Node* nodes, int64_t L, LeafTable lt, int64_t n,
                             float* X, float* LS, hipStream_t s);
 void launch_radius_targets(int K, int64_t n, const uint64_t* cand_key, const float* LS, float* T, hipStream_t s);
 void launch_radius_seed(int64_t n, const float* model, const float* X, const float* LS, float* Y, float* R,

@@ -74,6 +74,18 @@ __device__ __forceinline__ int find_cloud_leaf(const Cloud* cl, int ncl, int64_t
     return lo;
 }
 
+// sorted positions [s0, s1) of leaf slot j of a cloud (empty past its leaves)
+__device__ __forceinline__ void leaf_range(const LeafTable& lt, int64_t lfirst, int64_t nl, int64_t cend, int64_t j,
+                                           int64_t& s0, int64_t& s1)
+{
+    if (j < nl) {
+        s0 = lt.start[lfirst + j];
+        s1 = lt.start[lfirst + j + 1];
+    } else {
+        s0 = s1 = cend;
+    }
+}
+
 // ------------------------------------------------------------ bbox (A1)
 // Params.pmin/pmax hold order-mapped uints during the reduction.
 __global__ void k_bbox_init(Params* P, int ncl)
@@ -444,6 +456,86 @@ __device__ __forceinline__ void wave_append(bool take, int32_t v, int32_t* list,
     if (take) list[base + (uint32_t)__popcll(m & lanemask_lt())] = v;
 }
 
+
+// ------------------------------------------------------------ leaves
+// Position i starts a leaf when it starts its cloud, or when the smallest
+// prefix cell of the sorted keys that holds both i - 1 and i (the cell above
+// their highest differing bit) has more than `bucket` points: its two halves
+// then lie in different leaves.  The cell is contiguous around i, so two
+// binary searches within `bucket` positions of i decide it.  Inside a run of
+// equal keys longer than `bucket` a leaf starts every `bucket` points from
+// the run's start.
+__global__ __launch_bounds__(kBlock) void k_leaf_marks(const uint64_t* __restrict__ keys, int64_t n,
+                                                      const Cloud* __restrict__ cl, int ncl, int bucket,
+                                                      uint32_t* __restrict__ mark)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = find_cloud(cl, ncl, i);
+    const int64_t c0 = cl[c].pos, c1 = cl[c].pos + cl[c].n;
+    uint32_t m = 1u;
+    if (i > c0) {
+        const uint64_t a = keys[i - 1], b = keys[i];
+        if (a != b) {
+            const int h = 63 - __clzll(a ^ b);
+            const uint64_t mask = h >= 63 ? 0ull : ~((2ull << h) - 1ull);
+            const uint64_t pre = b & mask;
+            int64_t lo = max(c0, i - bucket), hi = i - 1;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if ((keys[mid] & mask) == pre) hi = mid;
+                else lo = mid + 1;
+            }
+            int64_t lo2 = i, hi2 = min(c1, i + bucket) - 1;
+            while (lo2 < hi2) {
+                const int64_t mid = (lo2 + hi2 + 1) >> 1;
+                if ((keys[mid] & mask) == pre) lo2 = mid;
+                else hi2 = mid - 1;
+            }
+            m = lo2 - lo + 1 > bucket ? 1u : 0u;
+        } else {
+            int64_t lo = i - 1, step = 1;
+            while (lo - step >= c0 && keys[lo - step] == b) {
+                lo -= step;
+                step <<= 1;
+            }
+            int64_t a0 = max(c0, lo - step), b0 = lo;
+            while (a0 < b0) {
+                const int64_t mid = (a0 + b0) >> 1;
+                if (keys[mid] == b) b0 = mid;
+                else a0 = mid + 1;
+            }
+            const bool long_run = a0 + bucket < c1 && keys[a0 + bucket] == b;
+            m = long_run && (i - a0) % bucket == 0 ? 1u : 0u;
+        }
+    }
+    mark[i] = m;
+}
+
+// idx: exclusive scan of mark.  start[leaf] = its first position, of[p] =
+// the leaf of p, start[leaves] = n.
+__global__ __launch_bounds__(kBlock) void k_leaf_fill(const uint32_t* __restrict__ mark,
+                                                     const uint32_t* __restrict__ idx, int64_t n,
+                                                     int32_t* __restrict__ start, int32_t* __restrict__ of)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t g = idx[i] + mark[i] - 1u;
+    of[i] = (int32_t)g;
+    if (mark[i]) start[idx[i]] = (int32_t)i;
+    if (i == n - 1) start[g + 1] = (int32_t)n;
+}
+
+// first leaf of every cloud (every non-empty cloud starts a leaf), and the count
+__global__ void k_cloud_leaves(const Cloud* __restrict__ cl, int ncl, const uint32_t* __restrict__ mark,
+                               const uint32_t* __restrict__ idx, int64_t n, int32_t* __restrict__ out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > ncl) return;
+    const int32_t total = n > 0 ? (int32_t)(idx[n - 1] + mark[n - 1]) : 0;
+    out[c] = (c < ncl && cl[c].pos < n) ? (int32_t)idx[cl[c].pos] : total;
+}
+
 // rep[] = -1 everywhere; the first position of every run of >= 2 equal keys is queued
 __global__ __launch_bounds__(kBlock) void k_dup_runs(const uint64_t* __restrict__ keys, int64_t n,
                                                     int32_t* __restrict__ rep, int32_t* __restrict__ runs,
@@ -580,19 +672,19 @@ __global__ void k_dup_finish(Counters* C)
 // test accepts.
 __global__ __launch_bounds__(kBlock) void k_leaves(const float4* __restrict__ pts,
                                                   const Cloud* __restrict__ cl, int ncl, int64_t nleaves,
-                                                  int bucket, Node* __restrict__ nodes,
+                                                  LeafTable lt, Node* __restrict__ nodes,
                                                   const int32_t* __restrict__ rep)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nleaves) return;
     const int c = find_cloud_leaf(cl, ncl, g);
-    const int64_t j = g - cl[c].lbase;          // the cloud's leaf j covers its points [j*bucket, ...)
+    const int64_t j = g - cl[c].lbase;          // the cloud's leaf slot j
     const bool grouped = rep != nullptr;
     Node nd;
     for (int k = 0; k < 3; ++k) { nd.lo[k] = INFINITY; nd.hi[k] = -INFINITY; }
     nd.pad[0] = nd.pad[1] = 0.f;
-    const int64_t s0 = cl[c].pos + j * bucket;
-    const int64_t s1 = min(cl[c].pos + cl[c].n, s0 + bucket);
+    int64_t s0, s1;
+    leaf_range(lt, cl[c].lfirst, cl[c].nl, cl[c].pos + cl[c].n, j, s0, s1);
     for (int64_t s = s0; s < s1; ++s) {
         if (grouped) {
             const int32_t r = rep[s];
@@ -793,7 +885,7 @@ template <int D, int DC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_cone_search(const float4* __restrict__ pts,
                                                    const Node* __restrict__ nodes,
                                                    const Cloud* __restrict__ clouds, int ncl,
-                                                   int bucket, int64_t n,
+                                                   int bucket, LeafTable lt, int64_t n,
                                                    const int32_t* __restrict__ qlist, int64_t nq,
                                                    const int32_t* __restrict__ label,
                                                    int check_labels,
@@ -866,8 +958,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         const int cl = __builtin_amdgcn_readfirstlane(__shfl(my_cl, (int)__builtin_ctzll(pending)));
         const bool mine = my_cl == cl;
         pending &= ~__ballot(mine);
-        const int64_t cpos = clouds[cl].pos, cend = clouds[cl].pos + clouds[cl].n;
-        const int64_t L = clouds[cl].L;
+        const int64_t cend = clouds[cl].pos + clouds[cl].n;
+        const int64_t L = clouds[cl].L, lfirst = clouds[cl].lfirst, nl = clouds[cl].nl;
         const Node* __restrict__ tree = nodes + 2 * clouds[cl].lbase;
 
         // seed radius
@@ -877,10 +969,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         } else if (seed_r) {
             r = seed_r[p];
         } else {
-            const int64_t lp = p - cpos;
-            const Node own = tree[L + lp / bucket];
-            const int64_t s0 = cpos + (lp / bucket) * bucket;
-            const float cnt = (float)(min(cend, s0 + bucket) - s0);
+            const int64_t lf = lt.of[p];
+            const Node own = tree[L + (lf - lfirst)];
+            const float cnt = (float)(lt.start[lf + 1] - lt.start[lf]);
             float vol = 1.f, ext = 0.f;
             for (int k = 0; k < D; ++k) {
                 const float e = own.hi[k] - own.lo[k];
@@ -960,8 +1051,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                 const uint64_t needm = __ballot(need);
                 if (needm == 0ull) continue;
                 if (node >= L) {
-                    const int64_t s0 = cpos + (int64_t)(node - L) * bucket;
-                    const int cnt = (int)max<int64_t>(min(cend, s0 + bucket) - s0, 0);
+                    int64_t s0, s1;
+                    leaf_range(lt, lfirst, nl, cend, node - L, s0, s1);
+                    const int cnt = (int)(s1 - s0);
                     ++st_leaves;
                     st_pts += (uint32_t)cnt;
                     st_pairs += (uint32_t)cnt * (uint32_t)__popcll(needm);
@@ -1334,6 +1426,8 @@ __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __res
         C->packets = C->nodes = C->leaves = C->leaf_points = C->pairs = 0;
         C->round = 0; C->done = 0; C->edges_prev = 0; C->dup_settled = 0; C->nsamples = 0;
         C->comp_queries = 0;
+        C->comp_nodes = C->comp_leaves = C->comp_pairs = 0;
+        C->knn_nodes = C->knn_leaves = C->knn_pairs = 0;
         C->target = target; C->max_rounds = max_rounds;
     }
     if (i >= n) return;
@@ -1737,7 +1831,7 @@ struct MinOp {
 // component label of every node: the common label of its points
 __global__ __launch_bounds__(kBlock) void k_nlab_tree(const int32_t* __restrict__ label,
                                                      const Cloud* __restrict__ cl, int ncl, int64_t nleaves,
-                                                     int bucket, Node* __restrict__ nodes,
+                                                     LeafTable lt, Node* __restrict__ nodes,
                                                      unsigned long long* __restrict__ visit, const Counters* C)
 {
     if (C->done) return;
@@ -1745,8 +1839,8 @@ __global__ __launch_bounds__(kBlock) void k_nlab_tree(const int32_t* __restrict_
     if (g >= nleaves) return;
     const int c = find_cloud_leaf(cl, ncl, g);
     const int64_t j = g - cl[c].lbase;
-    const int64_t s0 = cl[c].pos + j * bucket;
-    const int64_t s1 = min(cl[c].pos + cl[c].n, s0 + bucket);
+    int64_t s0, s1;
+    leaf_range(lt, cl[c].lfirst, cl[c].nl, cl[c].pos + cl[c].n, j, s0, s1);
     int32_t lab = kNlabEmpty;
     for (int64_t s = s0; s < s1; ++s) lab = LabelOp::combine(lab, label[s]);
     Node* base = nodes + 2 * cl[c].lbase;
@@ -1760,15 +1854,15 @@ __global__ __launch_bounds__(kBlock) void k_nlab_tree(const int32_t* __restrict_
 // id than the best so far (runs of coincident points)
 __global__ __launch_bounds__(kBlock) void k_nmin_tree(const float4* __restrict__ pts,
                                                      const Cloud* __restrict__ cl, int ncl, int64_t nleaves,
-                                                     int bucket, Node* __restrict__ nodes,
+                                                     LeafTable lt, Node* __restrict__ nodes,
                                                      unsigned long long* __restrict__ visit)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nleaves) return;
     const int c = find_cloud_leaf(cl, ncl, g);
     const int64_t j = g - cl[c].lbase;
-    const int64_t s0 = cl[c].pos + j * bucket;
-    const int64_t s1 = min(cl[c].pos + cl[c].n, s0 + bucket);
+    int64_t s0, s1;
+    leaf_range(lt, cl[c].lfirst, cl[c].nl, cl[c].pos + cl[c].n, j, s0, s1);
     uint32_t m = 0xffffffffu;
     for (int64_t s = s0; s < s1; ++s) m = min(m, __float_as_uint(pts[s].w));
     Node* base = nodes + 2 * cl[c].lbase;
@@ -1916,7 +2010,7 @@ constexpr uint32_t kLearnedFrom = 3;
 
 __global__ __launch_bounds__(kBlock) void k_comp_features(int log_all, int has_model, int dim, const float4* __restrict__ pts,
                                                          const Node* __restrict__ nodes,
-                                                         const Cloud* __restrict__ clouds, int ncl, int bucket,
+                                                         const Cloud* __restrict__ clouds, int ncl, LeafTable lt,
                                                          const int32_t* __restrict__ qlist,
                                                          const uint32_t* __restrict__ nq_dev,
                                                          const int32_t* __restrict__ label,
@@ -1938,16 +2032,19 @@ __global__ __launch_bounds__(kBlock) void k_comp_features(int log_all, int has_m
     if (qi >= (int64_t)nq) return;
     const int64_t p = qlist[qi];
     const int c = find_cloud(clouds, ncl, p);
-    const int64_t L = clouds[c].L, cn = clouds[c].n;
+    const int64_t L = clouds[c].L, lfirst = clouds[c].lfirst, nl = clouds[c].nl;
+    const int64_t cend = clouds[c].pos + clouds[c].n;
     const Node* __restrict__ tree = nodes + 2 * clouds[c].lbase;
-    int64_t node = L + (p - clouds[c].pos) / bucket;
+    int64_t node = L + (lt.of[p] - lfirst);
     float sp[4];
     for (int k = 0; k < 4; ++k) {
         const Node nd = tree[node];
         int64_t first = node, last = node;
         while (first < L) { first = 2 * first; last = 2 * last + 1; }
-        const int64_t a = (first - L) * bucket, b = min(cn, (last - L + 1) * bucket);
-        const float cnt = (float)max<int64_t>(b - a, 1);
+        int64_t a0, a1, b0, b1;
+        leaf_range(lt, lfirst, nl, cend, first - L, a0, a1);
+        leaf_range(lt, lfirst, nl, cend, last - L, b0, b1);
+        const float cnt = (float)max<int64_t>(b1 - a0, 1);
         float vol = 1.f, ext = 0.f;
         for (int d = 0; d < dim; ++d) {
             const float e = fmaxf(nd.hi[d] - nd.lo[d], 0.f);
@@ -2033,6 +2130,34 @@ __global__ __launch_bounds__(kBlock) void k_comp_samples(const int32_t* __restri
     o[8] = logf(fmaxf(d, 1e-3f * s0)) - LS[qi];
 }
 
+// LDS lists of the leaf gatherings (k_comp_search, k_knn_first): node ids of
+// the work list, and gathered leaves as (box d2 bits << 32) | leaf
+constexpr int kGatherWork = 512;
+constexpr int kGatherLeaves = 128;
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m)
+{
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// ascending bitonic sort of one 64-bit key per lane across the wave
+__device__ __forceinline__ uint64_t wave_sort_u64(uint64_t e)
+{
+    const int lane = __lane_id();
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint64_t o = shfl_xor_u64(e, j);
+            const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+            e = keep_min ? (o < e ? o : e) : (o > e ? o : e);
+        }
+    }
+    return e;
+}
+
 // nearest point in another component, for the points of qlist (count in
 // nq_dev): packet traversal as in the cone search, one best per query, by
 // iterative deepening from the seed radius R_0 (capped passes R_k = 8^(k/2) R_0,
@@ -2045,11 +2170,11 @@ __global__ __launch_bounds__(kBlock) void k_comp_samples(const int32_t* __restri
 template <int D, int BK>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_comp_search(
     const float4* __restrict__ pts, const Node* __restrict__ nodes,
-    const Cloud* __restrict__ clouds, int ncl, int bucket_rt, const int32_t* __restrict__ qlist,
+    const Cloud* __restrict__ clouds, int ncl, int bucket_rt, LeafTable lt, const int32_t* __restrict__ qlist,
     const uint32_t* __restrict__ nq_dev, const int32_t* __restrict__ label, const int32_t* __restrict__ inv,
     float seed_scale, int32_t* __restrict__ corr, uint64_t* __restrict__ corr_key,
     uint64_t* __restrict__ comp1, int pass_lo, int pass_hi,
-    uint64_t* __restrict__ open_out, const float* __restrict__ seed_r, Counters* C)
+    uint64_t* __restrict__ open_out, const float* __restrict__ seed_r, Counters* C, int collect)
 {
     // the leaf size: a compile-time constant for the default sizes
     const int bucket = BK ? BK : bucket_rt;
@@ -2064,6 +2189,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     __shared__ int32_t stk_id[64];
     __shared__ int32_t stk_ver[64];
     __shared__ unsigned long long stk_mask[64];
+    __shared__ int32_t work[kGatherWork];
+    __shared__ unsigned long long gathered[kGatherLeaves];
     const int lane = __lane_id();
     const int64_t qi = (int64_t)blockIdx.x * 64 + lane;
     const bool active = qi < nq;
@@ -2077,23 +2204,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     const uint32_t pid = __float_as_uint(me.w);
     bool bounded = false, left_open = false;
     int32_t lbnd = 0;
+    uint32_t st_nodes = 0, st_leaves = 0;       // work counters of this packet
+    uint64_t st_pairs = 0;
     uint64_t pending = __ballot(active);
     while (pending) {
         const int cl = __builtin_amdgcn_readfirstlane(__shfl(my_cl, (int)__builtin_ctzll(pending)));
         const bool mine = my_cl == cl;
         pending &= ~__ballot(mine);
-        const int64_t cpos = clouds[cl].pos, cend = clouds[cl].pos + clouds[cl].n;
-        const int64_t L = clouds[cl].L;
+        const int64_t cend = clouds[cl].pos + clouds[cl].n;
+        const int64_t L = clouds[cl].L, lfirst = clouds[cl].lfirst, nl = clouds[cl].nl;
         const Node* __restrict__ tree = nodes + 2 * clouds[cl].lbase;
         // seed: twice the previous target's distance, at least seed_scale
         // times the mean spacing of the point's leaf cell
         float r = 1.f;
         int32_t inner = -1;
         if (mine) {
-            const int64_t lp = p - cpos;
-            const Node own = tree[L + lp / bucket];
-            const int64_t s0 = cpos + (lp / bucket) * bucket;
-            const float cnt = (float)(min(cend, s0 + bucket) - s0);
+            const int64_t lf = lt.of[p];
+            const Node own = tree[L + (lf - lfirst)];
+            const float cnt = (float)(lt.start[lf + 1] - lt.start[lf]);
             float vol = 1.f, ext = 0.f;
             for (int k = 0; k < D; ++k) {
                 const float e = own.hi[k] - own.lo[k];
@@ -2185,117 +2313,211 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             bool pre_done = false;
             if (D == 3 && pass == 0) {
                 const int fl = (int)__builtin_ctzll(__ballot(open));
-                const int myleaf = (int)(L + (mine ? (p - cpos) / bucket : 0));
+                const int myleaf = (int)(L + (mine ? lt.of[p] - lfirst : 0));
                 pre = __builtin_amdgcn_readfirstlane(__shfl(myleaf, fl));
             }
-            if (lane == 0) {
-                stk_id[0] = 1;
-                stk_ver[0] = -1;
-                if (pre >= 0) { stk_id[1] = pre; stk_ver[1] = -1; }
-            }
-            sp = pre >= 0 ? 2 : 1;
-            __builtin_amdgcn_wave_barrier();
-            while (sp > 0) {
-                --sp;
-                const int node = __builtin_amdgcn_readfirstlane(stk_id[sp]);
-                const int pver = __builtin_amdgcn_readfirstlane(stk_ver[sp]);
-                if (node == pre) {
-                    if (pre_done) continue;
-                    pre_done = true;
+            // a leaf: one (query, leaf point) pair per lane; the query's state by
+            // LDS permute (a v_readlane broadcast when 64 / bucket = 1, as in
+            // the cone search, measured 7% slower here, and four packets per
+            // block 6% slower)
+            auto visit_leaf = [&](int node, bool need, uint64_t needm) {
+                int64_t s0, s1;
+                leaf_range(lt, lfirst, nl, cend, node - L, s0, s1);
+                const int cnt = (int)(s1 - s0);
+                ++st_leaves;
+                st_pairs += (uint64_t)cnt * (uint64_t)__popcll(needm);
+                const int j = lane & (bucket - 1);
+                const int qslot = lane / bucket;
+                const int qpc = 64 / bucket;
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                int32_t vlab = -1;
+                if (j < cnt) {
+                    v = pts[s0 + j];
+                    vlab = label[s0 + j];
                 }
-                bool need;
-                if (pver == ver) {
-                    need = (stk_mask[sp] >> lane) & 1ull;
-                } else {
-                    {
-                        const Node nd = tree[node];
-                        need = needed(nd, __float_as_int(nd.pad[1]), __float_as_uint(nd.pad[0]), lim, bk);
+                uint64_t rem = needm;
+                while (rem) {
+                    int myq = -1;
+                    for (int k = 0; k < qpc; ++k) {
+                        const int l = rem ? (int)__builtin_ctzll(rem) : -1;
+                        if (rem) rem &= rem - 1;
+                        if (k == qslot) myq = l;
                     }
+                    const int src = myq >= 0 ? myq : 0;
+                    const float qx = __shfl(px, src), qy = __shfl(py, src), qz = __shfl(pz, src);
+                    const int32_t qb = __shfl(bnd, src), qin = __shfl(inner, src), ql = __shfl(mylab, src);
+                    const float dx = v.x - qx, dy = v.y - qy, dz = v.z - qz;
+                    const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
+                    const int32_t db = (int32_t)__float_as_uint(d2);
+                    if (myq >= 0 && j < cnt && vlab != ql && db <= qb && db > qin)
+                        atomicMin(&best[myq],
+                                  ((unsigned long long)__float_as_uint(d2) << 32) | __float_as_uint(v.w));
                 }
-                const uint64_t needm = __ballot(need);
-                if (needm == 0ull) continue;
-                if (node >= L) {
-                    const int64_t s0 = cpos + (int64_t)(node - L) * bucket;
-                    const int cnt = (int)max<int64_t>(min(cend, s0 + bucket) - s0, 0);
-                    const int j = lane & (bucket - 1);
-                    const int qslot = lane / bucket;
-                    const int qpc = 64 / bucket;
-                    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-                    int32_t vlab = -1;
-                    if (j < cnt) {
-                        v = pts[s0 + j];
-                        vlab = label[s0 + j];
+                if (need) {
+                    bk = best[lane];
+                    bnd = min(key_bound(bk), lim);
+                }
+            };
+            // The gathering (a finite bound): every leaf whose box lies within
+            // the largest bound of the open queries from their box, of another
+            // component than all of them, and not inside all of their lower
+            // bounds, by box-to-box tests of 64 nodes a step; then those
+            // leaves nearest first, each tested per query.  A gathering that
+            // outgrows the LDS lists leaves the pass to the per-query walk.
+            bool done_by_gather = false;
+            uint32_t B = open ? (uint32_t)bnd : 0u;
+            for (int off = 32; off > 0; off >>= 1) B = max(B, (uint32_t)__shfl_xor((int)B, off));
+            if (B < 0x7f800000u) {
+                float qlo[3] = {open ? px : INFINITY, open ? py : INFINITY, open ? pz : INFINITY};
+                float qhi[3] = {open ? px : -INFINITY, open ? py : -INFINITY, open ? pz : -INFINITY};
+                int32_t imin = open ? inner : 0x7fffffff;
+                for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        qlo[k] = fminf(qlo[k], __shfl_xor(qlo[k], off));
+                        qhi[k] = fmaxf(qhi[k], __shfl_xor(qhi[k], off));
                     }
-                    // one (query, leaf point) pair per lane; the query's state by
-                    // LDS permute (a v_readlane broadcast when 64 / bucket = 1, as
-                    // in the cone search, measured 7% slower here, and four
-                    // packets per block 6% slower)
-                    uint64_t rem = needm;
-                    while (rem) {
-                        int myq = -1;
-                        for (int k = 0; k < qpc; ++k) {
-                            const int l = rem ? (int)__builtin_ctzll(rem) : -1;
-                            if (rem) rem &= rem - 1;
-                            if (k == qslot) myq = l;
+                    imin = min(imin, __shfl_xor(imin, off));
+                }
+                const int fo = (int)__builtin_ctzll(__ballot(open));
+                const int32_t lab0 = __builtin_amdgcn_readfirstlane(__shfl(mylab, fo));
+                const int32_t skip_lab = __ballot(open && mylab != lab0) == 0ull ? lab0 : (int32_t)0x80000000;
+                int head = 0, tail = 1, ngl = 0;
+                bool over = false;
+                if (lane == 0) work[0] = 1;
+                __builtin_amdgcn_wave_barrier();
+                while (head < tail) {
+                    const int take = min(64, tail - head);
+                    const int nd = lane < take ? work[head + lane] : 0;
+                    head += take;
+                    st_nodes += (uint32_t)take;
+                    bool pass_ = false, leaf = false;
+                    uint32_t bd = 0;
+                    if (lane < take) {
+                        const Node x = tree[nd];
+                        const int32_t xl = __float_as_int(x.pad[1]);
+                        float m[3], u[3];
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            m[k] = fmaxf(fmaxf(x.lo[k] - qhi[k], qlo[k] - x.hi[k]), 0.f);
+                            u[k] = fmaxf(x.hi[k] - qlo[k], qhi[k] - x.lo[k]);
                         }
-                        const int src = myq >= 0 ? myq : 0;
-                        const float qx = __shfl(px, src), qy = __shfl(py, src), qz = __shfl(pz, src);
-                        const int32_t qb = __shfl(bnd, src), qin = __shfl(inner, src), ql = __shfl(mylab, src);
-                        const float dx = v.x - qx, dy = v.y - qy, dz = v.z - qz;
-                        const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
-                        const int32_t db = (int32_t)__float_as_uint(d2);
-                        if (myq >= 0 && j < cnt && vlab != ql && db <= qb && db > qin)
-                            atomicMin(&best[myq],
-                                      ((unsigned long long)__float_as_uint(d2) << 32) | __float_as_uint(v.w));
+                        bd = __float_as_uint(D == 3 ? d2_of(m[0], m[1], m[2]) : d2_of(m[0], m[1]));
+                        const float far = D == 3 ? d2_of(u[0], u[1], u[2]) : d2_of(u[0], u[1]);
+                        leaf = nd >= L;
+                        pass_ = xl != kNlabEmpty && xl != skip_lab && bd <= B &&
+                                (int32_t)__float_as_uint(far) > imin && !(leaf && nd == pre);
                     }
-                    if (need) {
-                        bk = best[lane];
-                        bnd = min(key_bound(bk), lim);
+                    const uint64_t mi = __ballot(pass_ && !leaf), ml = __ballot(pass_ && leaf);
+                    const int ni = __popcll(mi), nlf = __popcll(ml);
+                    if (tail + 2 * ni > kGatherWork || ngl + nlf > kGatherLeaves) {
+                        over = true;
+                        break;
                     }
-                    ++ver;
-                } else {
-                    const bool leafkids = 2 * (int64_t)node >= L;
-                    const int first = leafkids ? 2 * node : 4 * node;
-                    const int nk = leafkids ? 2 : 4;
-                    Node bx[4];
-                    int32_t bl[4];
-                    uint32_t bm[4];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        if (i < nk) {
-                            bx[i] = tree[first + i];
-                            bl[i] = __float_as_int(bx[i].pad[1]);
-                            bm[i] = __float_as_uint(bx[i].pad[0]);
-                        }
-                    uint64_t mk[4];
-                    float dk[4];
-                    int id[4];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        mk[i] = i < nk ? __ballot(needed(bx[i], bl[i], bm[i], lim, bk)) : 0ull;
-                        dk[i] = mk[i] ? cdist(bx[i]) : -1.f;
-                        id[i] = first + i;
+                    const uint64_t lt_ = lanemask_lt();
+                    if (pass_ && !leaf) {
+                        const int r = tail + 2 * __popcll(mi & lt_);
+                        work[r] = 2 * nd;
+                        work[r + 1] = 2 * nd + 1;
                     }
-                    // pushed farthest first, so the nearest pops first; at
-                    // equal distance the lower node (smaller ids in a run of
-                    // coincident points) pops first
-                    auto ce = [&](int a, int b) {
-                        if (dk[a] < dk[b] || (dk[a] == dk[b] && id[a] < id[b])) {
-                            const float td = dk[a]; dk[a] = dk[b]; dk[b] = td;
-                            const uint64_t tm = mk[a]; mk[a] = mk[b]; mk[b] = tm;
-                            const int ti = id[a]; id[a] = id[b]; id[b] = ti;
-                        }
-                    };
-                    ce(0, 1); ce(2, 3); ce(0, 2); ce(1, 3); ce(1, 2);
-                    if (lane == 0) {
-                        int t = sp;
-#pragma unroll
-                        for (int i = 0; i < 4; ++i)
-                            if (mk[i]) { stk_id[t] = id[i]; stk_ver[t] = ver; stk_mask[t] = mk[i]; ++t; }
-                    }
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) sp += mk[i] != 0ull ? 1 : 0;
+                    if (pass_ && leaf) gathered[ngl + __popcll(ml & lt_)] = ((uint64_t)bd << 32) | (uint32_t)nd;
+                    tail += 2 * ni;
+                    ngl += nlf;
                     __builtin_amdgcn_wave_barrier();
+                }
+                if (!over) {
+                    done_by_gather = true;
+                    auto try_leaf = [&](int node) {
+                        const Node x = tree[node];
+                        ++st_nodes;
+                        const bool need = needed(x, __float_as_int(x.pad[1]), __float_as_uint(x.pad[0]), lim, bk);
+                        const uint64_t needm = __ballot(need);
+                        if (needm != 0ull) visit_leaf(node, need, needm);
+                    };
+                    if (pre >= 0) try_leaf(pre);
+                    for (int c0 = 0; c0 < ngl; c0 += 64) {
+                        const int cn = min(64, ngl - c0);
+                        const uint64_t e = wave_sort_u64(lane < cn ? gathered[c0 + lane] : kEmptyKey);
+                        for (int i = 0; i < cn; ++i) try_leaf(__builtin_amdgcn_readlane((int)(uint32_t)e, i));
+                    }
+                }
+            }
+            if (!done_by_gather) {
+                if (lane == 0) {
+                    stk_id[0] = 1;
+                    stk_ver[0] = -1;
+                    if (pre >= 0) { stk_id[1] = pre; stk_ver[1] = -1; }
+                }
+                sp = pre >= 0 ? 2 : 1;
+                __builtin_amdgcn_wave_barrier();
+                while (sp > 0) {
+                    --sp;
+                    const int node = __builtin_amdgcn_readfirstlane(stk_id[sp]);
+                    const int pver = __builtin_amdgcn_readfirstlane(stk_ver[sp]);
+                    if (node == pre) {
+                        if (pre_done) continue;
+                        pre_done = true;
+                    }
+                    bool need;
+                    if (pver == ver) {
+                        need = (stk_mask[sp] >> lane) & 1ull;
+                    } else {
+                        {
+                            const Node nd = tree[node];
+                            need = needed(nd, __float_as_int(nd.pad[1]), __float_as_uint(nd.pad[0]), lim, bk);
+                        }
+                        ++st_nodes;
+                    }
+                    const uint64_t needm = __ballot(need);
+                    if (needm == 0ull) continue;
+                    if (node >= L) {
+                        visit_leaf(node, need, needm);
+                        ++ver;
+                    } else {
+                        const bool leafkids = 2 * (int64_t)node >= L;
+                        const int first = leafkids ? 2 * node : 4 * node;
+                        const int nk = leafkids ? 2 : 4;
+                        st_nodes += (uint32_t)nk;
+                        Node bx[4];
+                        int32_t bl[4];
+                        uint32_t bm[4];
+    #pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            if (i < nk) {
+                                bx[i] = tree[first + i];
+                                bl[i] = __float_as_int(bx[i].pad[1]);
+                                bm[i] = __float_as_uint(bx[i].pad[0]);
+                            }
+                        uint64_t mk[4];
+                        float dk[4];
+                        int id[4];
+    #pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            mk[i] = i < nk ? __ballot(needed(bx[i], bl[i], bm[i], lim, bk)) : 0ull;
+                            dk[i] = mk[i] ? cdist(bx[i]) : -1.f;
+                            id[i] = first + i;
+                        }
+                        // pushed farthest first, so the nearest pops first; at
+                        // equal distance the lower node (smaller ids in a run of
+                        // coincident points) pops first
+                        auto ce = [&](int a, int b) {
+                            if (dk[a] < dk[b] || (dk[a] == dk[b] && id[a] < id[b])) {
+                                const float td = dk[a]; dk[a] = dk[b]; dk[b] = td;
+                                const uint64_t tm = mk[a]; mk[a] = mk[b]; mk[b] = tm;
+                                const int ti = id[a]; id[a] = id[b]; id[b] = ti;
+                            }
+                        };
+                        ce(0, 1); ce(2, 3); ce(0, 2); ce(1, 3); ce(1, 2);
+                        if (lane == 0) {
+                            int t = sp;
+    #pragma unroll
+                            for (int i = 0; i < 4; ++i)
+                                if (mk[i]) { stk_id[t] = id[i]; stk_ver[t] = ver; stk_mask[t] = mk[i]; ++t; }
+                        }
+    #pragma unroll
+                        for (int i = 0; i < 4; ++i) sp += mk[i] != 0ull ? 1 : 0;
+                        __builtin_amdgcn_wave_barrier();
+                    }
                 }
             }
             // everything outside the component up to min(best, lim) has been
@@ -2327,6 +2549,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         const uint64_t m = __ballot(left_open);
         if (lane == 0) open_out[blockIdx.x] = m;
     }
+    if (collect && lane == 0) {
+        atomicAdd(&C->comp_nodes, (unsigned long long)st_nodes);
+        atomicAdd(&C->comp_leaves, (unsigned long long)st_leaves);
+        atomicAdd(&C->comp_pairs, (unsigned long long)st_pairs);
+    }
     if (active && !left_open) {
         const uint64_t k = best[lane];
         if (bounded) {
@@ -2342,19 +2569,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
 // Round 1 of the component search and its neighbour lists.  Every component
 // is one point, so a point's nearest outside point is its nearest neighbour.
 // One wave per 64 consecutive positions keeps each lane's kKnn smallest keys
-// (d2, id) in registers: the leaves of the packet's own points first, then
-// the hierarchy from the root, nearest child first, a box needed while its
-// (lower bound, smallest id) may beat a lane's kKnn-th key.  Query per lane:
-// the leaf's points are read in turn at one address for the whole wave.
+// (d2, id) in registers.  The leaves holding the packet's own points come
+// first; then every lane's kKnn-th key bounds its search.  The packet
+// gathers, 64 nodes per step (one box-to-box test per lane), every other leaf
+// whose box lies within the largest of those bounds of the packet's box, and
+// visits them nearest first.  At a leaf, a lane takes part only while the
+// leaf box may still beat its kKnn-th key, and a leaf point enters the pair
+// loop only when it lies within the bound of an 8-lane group of the packet
+// that takes part (the group's box and its largest bound).  A packet whose
+// gathering outgrows the LDS lists walks the hierarchy per lane instead,
+// nearest child first.  In the pair loop each lane is one query and the
+// leaf's points are read in turn at one address for the whole wave.
 template <int D>
 __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __restrict__ pts,
                                                  const Node* __restrict__ nodes, const Cloud* __restrict__ clouds,
-                                                 int ncl, int bucket, int32_t* __restrict__ corr,
-                                                 uint64_t* __restrict__ corr_key, NbrList nb)
+                                                 int ncl, int bucket, LeafTable lt, int32_t* __restrict__ corr,
+                                                 uint64_t* __restrict__ corr_key, NbrList nb, Counters* C,
+                                                 int collect)
 {
     __shared__ int32_t stk_id[64];
     __shared__ int32_t stk_ver[64];
     __shared__ unsigned long long stk_mask[64];
+    __shared__ int32_t work[kGatherWork];
+    __shared__ unsigned long long gathered[kGatherLeaves];
+    __shared__ float4 gbox[16];             // per 8-lane group: lo (xyz), hi (xyz)
     const int lane = __lane_id();
     const int64_t p = (int64_t)blockIdx.x * 64 + lane;
     const bool active = p < n;
@@ -2382,40 +2620,31 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
             kp[0] = pos;
         }
     };
+    uint32_t st_nodes = 0, st_leaves = 0, st_steps = 0;     // work counters of this packet
     uint64_t pending = __ballot(active);
     while (pending) {
         const int cl = __builtin_amdgcn_readfirstlane(__shfl(my_cl, (int)__builtin_ctzll(pending)));
         const uint64_t minem = __ballot(my_cl == cl);
         const bool mine = my_cl == cl;
         pending &= ~minem;
-        const int64_t cpos = clouds[cl].pos, cend = clouds[cl].pos + clouds[cl].n;
-        const int64_t L = clouds[cl].L;
+        const int64_t cend = clouds[cl].pos + clouds[cl].n;
+        const int64_t L = clouds[cl].L, lfirst = clouds[cl].lfirst, nl = clouds[cl].nl;
         const Node* __restrict__ tree = nodes + 2 * clouds[cl].lbase;
-        float cx = mine ? px : 0.f, cy = mine ? py : 0.f, cz = mine ? pz : 0.f;
-        float na = mine ? 1.f : 0.f;
-        for (int off = 32; off > 0; off >>= 1) {
-            cx += __shfl_xor(cx, off);
-            cy += __shfl_xor(cy, off);
-            cz += __shfl_xor(cz, off);
-            na += __shfl_xor(na, off);
-        }
-        cx /= na; cy /= na; cz /= na;
-        // the box of the group's points
+        // the box of the group's points, and of each 8-lane group's
         float qlo[3] = {mine ? px : INFINITY, mine ? py : INFINITY, mine ? pz : INFINITY};
         float qhi[3] = {mine ? px : -INFINITY, mine ? py : -INFINITY, mine ? pz : -INFINITY};
-        for (int off = 32; off > 0; off >>= 1) {
+        for (int off = 1; off < 64; off <<= 1) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 qlo[k] = fminf(qlo[k], __shfl_xor(qlo[k], off));
                 qhi[k] = fmaxf(qhi[k], __shfl_xor(qhi[k], off));
             }
+            if (off == 4 && (lane & 7) == 0) {
+                gbox[2 * (lane >> 3)] = make_float4(qlo[0], qlo[1], qlo[2], 0.f);
+                gbox[2 * (lane >> 3) + 1] = make_float4(qhi[0], qhi[1], qhi[2], 0.f);
+            }
         }
-        auto cdist = [&](const Node& x) {
-            const float dxx = fmaxf(fmaxf(x.lo[0] - cx, cx - x.hi[0]), 0.f);
-            const float dyy = fmaxf(fmaxf(x.lo[1] - cy, cy - x.hi[1]), 0.f);
-            const float dzz = fmaxf(fmaxf(x.lo[2] - cz, cz - x.hi[2]), 0.f);
-            return dxx * dxx + dyy * dyy + dzz * dzz;
-        };
+        __builtin_amdgcn_wave_barrier();
         auto needed = [&](const Node& nd, uint32_t nm) {
             if (!mine) return false;
             const float q[3] = {px, py, pz};
@@ -2425,12 +2654,29 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
             const float lb = D == 3 ? d2_of(m[0], m[1], m[2]) : d2_of(m[0], m[1]);
             return (((uint64_t)__float_as_uint(lb) << 32) | nm) < kk[kKnn - 1];
         };
-        auto scan_leaf = [&](int node) {
-            const int64_t s0 = cpos + (int64_t)(node - L) * bucket;
-            const int cnt = (int)max<int64_t>(min(cend, s0 + bucket) - s0, 0);
-            // a leaf point farther from the packet's box than every lane's
-            // kKnn-th d2 cannot enter a list (its d2 to each of the packet's
-            // points is at least its box distance): only the others are paired
+        // one leaf point per candidate step, at one address for the wave
+        auto pairs = [&](int64_t s0, uint64_t cand, bool own) {
+            ++st_leaves;
+            st_steps += (uint32_t)__popcll(cand);
+            while (cand) {
+                const int j = (int)__builtin_ctzll(cand);
+                cand &= cand - 1;
+                const float4 v = pts[s0 + j];
+                const float dx = v.x - px, dy = v.y - py, dz = v.z - pz;
+                const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
+                const uint32_t db = __float_as_uint(d2);
+                if (mine && db <= (uint32_t)(kk[kKnn - 1] >> 32) && (!own || s0 + j != p)) {
+                    const uint64_t key = ((uint64_t)db << 32) | __float_as_uint(v.w);
+                    if (key < kk[kKnn - 1]) insert(key, (int32_t)(s0 + j));
+                }
+            }
+        };
+        // the packet's own leaves: a leaf point farther from the packet's box
+        // than every lane's kKnn-th d2 cannot enter a list
+        auto scan_own = [&](int node) {
+            int64_t s0, s1;
+            leaf_range(lt, lfirst, nl, cend, node - L, s0, s1);
+            const int cnt = (int)(s1 - s0);
             uint32_t t = mine ? (uint32_t)(kk[kKnn - 1] >> 32) : 0u;
             for (int off = 32; off > 0; off >>= 1) t = max(t, (uint32_t)__shfl_xor((int)t, off));
             bool keep = false;
@@ -2443,22 +2689,113 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
                 const float lb = D == 3 ? d2_of(m[0], m[1], m[2]) : d2_of(m[0], m[1]);
                 keep = __float_as_uint(lb) <= t;
             }
-            uint64_t cand = __ballot(keep);
-            while (cand) {
-                const int j = (int)__builtin_ctzll(cand);
-                cand &= cand - 1;
-                const float4 v = pts[s0 + j];
-                const float dx = v.x - px, dy = v.y - py, dz = v.z - pz;
-                const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
-                const uint64_t key = ((uint64_t)__float_as_uint(d2) << 32) | __float_as_uint(v.w);
-                if (mine && s0 + j != p && key < kk[kKnn - 1]) insert(key, (int32_t)(s0 + j));
+            pairs(s0, __ballot(keep), true);
+        };
+        // another leaf, for the lanes of needm: a leaf point is paired when it
+        // lies within the largest kKnn-th d2 of a taking-part group's lanes
+        // from that group's box
+        auto scan_other = [&](int node, bool need, uint64_t needm) {
+            int64_t s0, s1;
+            leaf_range(lt, lfirst, nl, cend, node - L, s0, s1);
+            const int cnt = (int)(s1 - s0);
+            uint32_t t = need ? (uint32_t)(kk[kKnn - 1] >> 32) : 0u;
+            t = max(t, (uint32_t)__shfl_xor((int)t, 1));
+            t = max(t, (uint32_t)__shfl_xor((int)t, 2));
+            t = max(t, (uint32_t)__shfl_xor((int)t, 4));
+            float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (lane < cnt) w = pts[s0 + lane];
+            bool keep = false;
+#pragma unroll 1
+            for (int g = 0; g < 8; ++g) {
+                if (((needm >> (8 * g)) & 0xffull) == 0ull) continue;
+                const uint32_t tg = (uint32_t)__builtin_amdgcn_readlane((int)t, 8 * g);
+                const float4 lo = gbox[2 * g], hi = gbox[2 * g + 1];
+                const float m0 = fmaxf(fmaxf(lo.x - w.x, w.x - hi.x), 0.f);
+                const float m1 = fmaxf(fmaxf(lo.y - w.y, w.y - hi.y), 0.f);
+                const float m2 = fmaxf(fmaxf(lo.z - w.z, w.z - hi.z), 0.f);
+                const float lb = D == 3 ? d2_of(m0, m1, m2) : d2_of(m0, m1);
+                keep = keep || __float_as_uint(lb) <= tg;
             }
+            pairs(s0, __ballot(keep && lane < cnt), false);
         };
         // the leaves holding the packet's points of this cloud
         const int first = (int)__builtin_ctzll(minem), last = 63 - (int)__builtin_clzll(minem);
         const int64_t pf = (int64_t)blockIdx.x * 64 + first, pl = (int64_t)blockIdx.x * 64 + last;
-        const int leaf_lo = (int)(L + (pf - cpos) / bucket), leaf_hi = (int)(L + (pl - cpos) / bucket);
-        for (int lf = leaf_lo; lf <= leaf_hi; ++lf) scan_leaf(lf);
+        const int leaf_lo = (int)(L + (lt.of[pf] - lfirst)), leaf_hi = (int)(L + (lt.of[pl] - lfirst));
+        for (int lf = leaf_lo; lf <= leaf_hi; ++lf) scan_own(lf);
+        // gather the other leaves within the largest kKnn-th d2 of the packet's box
+        uint32_t R = mine ? (uint32_t)(kk[kKnn - 1] >> 32) : 0u;
+        for (int off = 32; off > 0; off >>= 1) R = max(R, (uint32_t)__shfl_xor((int)R, off));
+        int head = 0, tail = 1, ngl = 0;
+        bool over = false;
+        if (lane == 0) work[0] = 1;
+        __builtin_amdgcn_wave_barrier();
+        while (head < tail) {
+            const int take = min(64, tail - head);
+            const int nd = lane < take ? work[head + lane] : 0;
+            head += take;
+            st_nodes += (uint32_t)take;
+            bool pass = false, leaf = false;
+            uint32_t bd = 0;
+            if (lane < take) {
+                const Node x = tree[nd];
+                float m[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) m[k] = fmaxf(fmaxf(x.lo[k] - qhi[k], qlo[k] - x.hi[k]), 0.f);
+                bd = __float_as_uint(D == 3 ? d2_of(m[0], m[1], m[2]) : d2_of(m[0], m[1]));
+                leaf = nd >= L;
+                pass = bd <= R && !(leaf && nd >= leaf_lo && nd <= leaf_hi);
+            }
+            const uint64_t mi = __ballot(pass && !leaf), ml = __ballot(pass && leaf);
+            const int ni = __popcll(mi), nl = __popcll(ml);
+            if (tail + 2 * ni > kGatherWork || ngl + nl > kGatherLeaves) {
+                over = true;
+                break;
+            }
+            const uint64_t lt = lanemask_lt();
+            if (pass && !leaf) {
+                const int r = tail + 2 * __popcll(mi & lt);
+                work[r] = 2 * nd;
+                work[r + 1] = 2 * nd + 1;
+            }
+            if (pass && leaf) gathered[ngl + __popcll(ml & lt)] = ((uint64_t)bd << 32) | (uint32_t)nd;
+            tail += 2 * ni;
+            ngl += nl;
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (!over) {
+            // nearest first, 64 leaves at a time
+            for (int c0 = 0; c0 < ngl; c0 += 64) {
+                const int cn = min(64, ngl - c0);
+                const uint64_t e = wave_sort_u64(lane < cn ? gathered[c0 + lane] : kEmptyKey);
+                for (int i = 0; i < cn; ++i) {
+                    const int node = __builtin_amdgcn_readlane((int)(uint32_t)e, i);
+                    const Node x = tree[node];
+                    ++st_nodes;
+                    const bool need = needed(x, __float_as_uint(x.pad[0]));
+                    const uint64_t needm = __ballot(need);
+                    if (needm != 0ull) scan_other(node, need, needm);
+                }
+            }
+            continue;
+        }
+        // the per-lane walk: from the root, nearest child first (from the
+        // packet's centroid), the own leaves skipped
+        float cx = mine ? px : 0.f, cy = mine ? py : 0.f, cz = mine ? pz : 0.f;
+        float na = mine ? 1.f : 0.f;
+        for (int off = 32; off > 0; off >>= 1) {
+            cx += __shfl_xor(cx, off);
+            cy += __shfl_xor(cy, off);
+            cz += __shfl_xor(cz, off);
+            na += __shfl_xor(na, off);
+        }
+        cx /= na; cy /= na; cz /= na;
+        auto cdist = [&](const Node& x) {
+            const float dxx = fmaxf(fmaxf(x.lo[0] - cx, cx - x.hi[0]), 0.f);
+            const float dyy = fmaxf(fmaxf(x.lo[1] - cy, cy - x.hi[1]), 0.f);
+            const float dzz = fmaxf(fmaxf(x.lo[2] - cz, cz - x.hi[2]), 0.f);
+            return dxx * dxx + dyy * dyy + dzz * dzz;
+        };
         int sp = 1;
         int ver = 0;
         if (lane == 0) {
@@ -2476,16 +2813,19 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
             } else {
                 const Node nd = tree[node];
                 need = needed(nd, __float_as_uint(nd.pad[0]));
+                ++st_nodes;
             }
-            if (__ballot(need) == 0ull) continue;
+            const uint64_t needm = __ballot(need);
+            if (needm == 0ull) continue;
             if (node >= L) {
                 if (node >= leaf_lo && node <= leaf_hi) continue;
-                scan_leaf(node);
+                scan_other(node, need, needm);
                 ++ver;
             } else {
                 const bool leafkids = 2 * (int64_t)node >= L;
                 const int fc = leafkids ? 2 * node : 4 * node;
                 const int nk = leafkids ? 2 : 4;
+                st_nodes += (uint32_t)nk;
                 uint64_t mk[4];
                 float dk[4];
                 int id[4];
@@ -2520,6 +2860,11 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
                 __builtin_amdgcn_wave_barrier();
             }
         }
+    }
+    if (collect && lane == 0) {
+        atomicAdd(&C->knn_nodes, (unsigned long long)st_nodes);
+        atomicAdd(&C->knn_leaves, (unsigned long long)st_leaves);
+        atomicAdd(&C->knn_pairs, 64ull * st_steps);
     }
     if (!active) return;
 #pragma unroll
@@ -2960,17 +3305,17 @@ void launch_dup_groups(const uint64_t* keys, const float4* pts, int64_t n, int32
     hipLaunchKernelGGL(k_dup_finish, dim3(1), dim3(1), 0, s, C);
 }
 
-void launch_build_tree(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, int64_t Lmax, int bucket,
+void launch_build_tree(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, int64_t Lmax, LeafTable lt,
                        Node* nodes, const int32_t* rep, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_leaves, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, pts, cl, ncl, nleaves, bucket, nodes,
+    hipLaunchKernelGGL(k_leaves, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, pts, cl, ncl, nleaves, lt, nodes,
                        rep);
     for (int64_t lo = Lmax / 2; lo >= 1; lo /= 2)
         hipLaunchKernelGGL(k_level, dim3(nblocks((int64_t)ncl * lo)), dim3(kBlock), 0, s, nodes, cl, ncl, lo);
 }
 
 void launch_cone_search(int dim, int cdim, const float4* pts, const Node* nodes, const Cloud* cl, int ncl,
-                        int bucket, int64_t n, const int32_t* qlist, int64_t nq, const int32_t* label,
+                        int bucket, LeafTable lt, int64_t n, const int32_t* qlist, int64_t nq, const int32_t* label,
                         bool check_labels, const int32_t* inv, const float* seed_r, float seed_scale,
                         uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
                         uint64_t* first_key, Counters* C, bool collect_stats, DupState dup,
@@ -2981,7 +3326,7 @@ void launch_cone_search(int dim, int cdim, const float4* pts, const Node* nodes,
     const int cs = collect_stats ? 1 : 0;
     auto kern = dim == 3 ? (cdim == 2 ? k_cone_search<3, 2> : k_cone_search<3, 3>) : k_cone_search<2, 2>;
     if (!split || qlist || split->after < 0 || split->after >= kMaxCappedPasses) {
-        hipLaunchKernelGGL(kern, dim3(g), dim3(64), 0, s, pts, nodes, cl, ncl, bucket, n, qlist, nq, label,
+        hipLaunchKernelGGL(kern, dim3(g), dim3(64), 0, s, pts, nodes, cl, ncl, bucket, lt, n, qlist, nq, label,
                            check_labels ? 1 : 0, inv, seed_r, seed_scale, cand_key, cand_pos, revive, first_corr,
                            first_key, C, cs, dup, 0, kMaxCappedPasses, (uint64_t*)nullptr,
                            (const uint32_t*)nullptr, (const uint64_t*)nullptr, (int64_t)0);
@@ -2989,25 +3334,38 @@ void launch_cone_search(int dim, int cdim, const float4* pts, const Node* nodes,
     }
     // passes 0..after over every query, then the open ones compacted into full waves
     const int64_t P = (int64_t)g;
-    hipLaunchKernelGGL(kern, dim3(g), dim3(64), 0, s, pts, nodes, cl, ncl, bucket, n, qlist, nq, label,
+    hipLaunchKernelGGL(kern, dim3(g), dim3(64), 0, s, pts, nodes, cl, ncl, bucket, lt, n, qlist, nq, label,
                        check_labels ? 1 : 0, inv, seed_r, seed_scale, cand_key, cand_pos, revive, first_corr,
                        first_key, C, cs, dup, 0, split->after, split->open_mask, (const uint32_t*)nullptr,
                        (const uint64_t*)nullptr, (int64_t)0);
     hipLaunchKernelGGL(k_open_count, dim3(nblocks(P + 1)), dim3(kBlock), 0, s, split->open_mask, P, split->cnt);
     exclusive_scan_u32(split->cnt, split->off, P + 1, split->tmp, s);
     hipLaunchKernelGGL(k_open_fill, dim3((unsigned)P), dim3(64), 0, s, split->open_mask, split->off, split->list);
-    hipLaunchKernelGGL(kern, dim3(g), dim3(64), 0, s, pts, nodes, cl, ncl, bucket, n,
+    hipLaunchKernelGGL(kern, dim3(g), dim3(64), 0, s, pts, nodes, cl, ncl, bucket, lt, n,
                        (const int32_t*)split->list, nq, label, check_labels ? 1 : 0, inv, seed_r, seed_scale,
                        cand_key, cand_pos, revive, first_corr, first_key, C, cs, dup, split->after + 1, kMaxCappedPasses,
                        (uint64_t*)nullptr, (const uint32_t*)(split->off + P), (const uint64_t*)split->open_mask,
                        (int64_t)(split->compact_frac * (double)nq));
 }
 
-void launch_node_min_ids(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, int bucket, Node* nodes,
+void launch_leaves(const uint64_t* keys, int64_t n, const Cloud* cl, int ncl, int bucket, uint32_t* mark,
+                   uint32_t* idx, uint32_t* tmp, int32_t* start, int32_t* of, int32_t* cloud_lf, hipStream_t s)
+{
+    if (n > 0) {
+        hipLaunchKernelGGL(k_leaf_marks, dim3(nblocks(n)), dim3(kBlock), 0, s, keys, n, cl, ncl, bucket, mark);
+        exclusive_scan_u32(mark, idx, n, tmp, s);
+        hipLaunchKernelGGL(k_leaf_fill, dim3(nblocks(n)), dim3(kBlock), 0, s, (const uint32_t*)mark,
+                           (const uint32_t*)idx, n, start, of);
+    }
+    hipLaunchKernelGGL(k_cloud_leaves, dim3(nblocks(ncl + 1)), dim3(kBlock), 0, s, cl, ncl, (const uint32_t*)mark,
+                       (const uint32_t*)idx, n, cloud_lf);
+}
+
+void launch_node_min_ids(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, LeafTable lt, Node* nodes,
                          unsigned long long* visit, hipStream_t s)
 {
     if (nleaves <= 0) return;
-    hipLaunchKernelGGL(k_nmin_tree, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, pts, cl, ncl, nleaves, bucket, nodes,
+    hipLaunchKernelGGL(k_nmin_tree, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, pts, cl, ncl, nleaves, lt, nodes,
                        visit);
 }
 
@@ -3019,7 +3377,7 @@ void launch_comp_seed_init(int64_t n, const float4* pts, int32_t* csize, uint32_
 
 void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsigned long long* visit,
                        const Cloud* cl, int ncl, int64_t nleaves,
-                       int bucket,
+                       int bucket, LeafTable lt,
                        const int32_t* label, int32_t* corr, uint64_t* corr_key, uint64_t* comp1, const int32_t* inv,
                        float seed_scale, const SplitScratch& sc, const CompSeed& cs, const NbrList& nb,
                        const int32_t* eu, const int32_t* eround, Counters* C, hipStream_t s, hipEvent_t* ev)
@@ -3029,7 +3387,7 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsig
     if (cs.features_on)
         hipLaunchKernelGGL(k_comp_merge, dim3(g), dim3(kBlock), 0, s, n, label, eu, eround, cs.csize, cs.cbox,
                            (const Counters*)C);
-    hipLaunchKernelGGL(k_nlab_tree, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, label, cl, ncl, nleaves, bucket,
+    hipLaunchKernelGGL(k_nlab_tree, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, label, cl, ncl, nleaves, lt,
                        nodes, visit, (const Counters*)C);
     const int64_t P = (n + 63) / 64;
     hipLaunchKernelGGL(k_comp_keep, dim3(g), dim3(kBlock), 0, s, n, pts, label, corr, corr_key, comp1, nb,
@@ -3041,7 +3399,7 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsig
     const float* seed_r = nullptr;
     if (cs.features_on) {
         hipLaunchKernelGGL(k_comp_features, dim3(g), dim3(kBlock), 0, s, cs.samples != nullptr ? 1 : 0,
-                           cs.model != nullptr ? 1 : 0, dim, pts, nodes, cl, ncl, bucket,
+                           cs.model != nullptr ? 1 : 0, dim, pts, nodes, cl, ncl, lt,
                            (const int32_t*)sc.list, nq, label, (const uint64_t*)corr_key, (const uint64_t*)comp1,
                            (const int32_t*)cs.csize, (const uint32_t*)cs.cbox, (const Counters*)C, cs.X, cs.LS,
                            cs.nq_keep);
@@ -3059,15 +3417,15 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsig
     // <2, 64> instance measured 7% slower on the 2D power-law cloud)
     // pass 0 over the searching points; their open packets' lanes into open_mask
     if (ev) (void)hipEventRecord(ev[0], s);
-    hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, (const Node*)nodes, cl, ncl, bucket,
+    hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, (const Node*)nodes, cl, ncl, bucket, lt,
                        (const int32_t*)sc.list, nq, label, inv, seed_scale, corr, corr_key, comp1, 0, 0,
-                       sc.open_mask, seed_r, C);
+                       sc.open_mask, seed_r, C, ev != nullptr ? 1 : 0);
     if (ev) (void)hipEventRecord(ev[1], s);
     open_compact(sc.open_mask, P, sc.off, sc.tmp, (const int32_t*)sc.list, sc.list2, s);
     if (ev) (void)hipEventRecord(ev[2], s);
-    hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, (const Node*)nodes, cl, ncl, bucket,
+    hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, (const Node*)nodes, cl, ncl, bucket, lt,
                        (const int32_t*)sc.list2, (const uint32_t*)(sc.off + P), label, inv, seed_scale, corr,
-                       corr_key, comp1, 1, kMaxCappedPasses, (uint64_t*)nullptr, seed_r, C);
+                       corr_key, comp1, 1, kMaxCappedPasses, (uint64_t*)nullptr, seed_r, C, ev != nullptr ? 1 : 0);
     if (ev) (void)hipEventRecord(ev[3], s);
     // comp1 now holds every component's edge: the sample log of this round
     if (cs.features_on && cs.samples != nullptr)
@@ -3078,16 +3436,18 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsig
 }
 
 void launch_knn_first(int dim, int64_t n, const float4* pts, const Node* nodes, const Cloud* cl, int ncl,
-                      int bucket, int32_t* corr, uint64_t* corr_key, const NbrList& nb, hipStream_t s)
+                      int bucket, LeafTable lt, int32_t* corr, uint64_t* corr_key, const NbrList& nb, Counters* C, bool collect,
+                      hipStream_t s)
 {
     if (n <= 0) return;
     const unsigned P = (unsigned)((n + 63) / 64);
+    const int cs = collect ? 1 : 0;
     if (dim == 3)
-        hipLaunchKernelGGL(k_knn_first<3>, dim3(P), dim3(64), 0, s, n, pts, nodes, cl, ncl, bucket, corr, corr_key,
-                           nb);
+        hipLaunchKernelGGL(k_knn_first<3>, dim3(P), dim3(64), 0, s, n, pts, nodes, cl, ncl, bucket, lt, corr, corr_key,
+                           nb, C, cs);
     else
-        hipLaunchKernelGGL(k_knn_first<2>, dim3(P), dim3(64), 0, s, n, pts, nodes, cl, ncl, bucket, corr, corr_key,
-                           nb);
+        hipLaunchKernelGGL(k_knn_first<2>, dim3(P), dim3(64), 0, s, n, pts, nodes, cl, ncl, bucket, lt, corr, corr_key,
+                           nb, C, cs);
 }
 
 void launch_dup_fill(int dim, int64_t n, const int32_t* label, const float4* pts, const int32_t* inv,

@@ -328,7 +328,7 @@ __global__ void k_adam(float* __restrict__ P, float* __restrict__ G, float* __re
 // positions (Morton order).
 __global__ __launch_bounds__(256) void k_radius_features(int dim, const float4* __restrict__ pts,
                                                         const Node* __restrict__ nodes, int64_t L,
-                                                        int bucket, int64_t n, float* __restrict__ X,
+                                                        LeafTable lt, int64_t n, float* __restrict__ X,
                                                         float* __restrict__ LS)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -336,15 +336,17 @@ __global__ __launch_bounds__(256) void k_radius_features(int dim, const float4* 
     const float4 me = pts[p];
     const float pc[3] = {me.x, me.y, me.z};
     float sp[4];
-    int64_t node = L + p / bucket;
+    const int64_t nl = lt.of[n - 1] + 1;        // one cloud: leaves 0 .. nl - 1
+    int64_t node = L + lt.of[p];
     Node leaf{};
     for (int k = 0; k < 4; ++k) {
         const Node nd = nodes[node];
         if (k == 0) leaf = nd;
-        // points under this node: its leaf range clipped to n
+        // points under this node: its leaves' range (slots past the last leaf are empty)
         int64_t first = node, last = node;
         while (first < L) { first = 2 * first; last = 2 * last + 1; }
-        const int64_t a = (first - L) * bucket, b = min(n, (last - L + 1) * bucket);
+        const int64_t a = first - L < nl ? lt.start[first - L] : n;
+        const int64_t b = last - L < nl ? lt.start[last - L + 1] : n;
         const float cnt = (float)max<int64_t>(b - a, 1);
         float vol = 1.f, ext = 0.f;
         for (int d = 0; d < dim; ++d) {
@@ -431,12 +433,12 @@ __global__ __launch_bounds__(256) void k_radius_seed(int64_t n, const float* __r
     R[p] = expf(LS[p] + fminf(Y[p], 20.f));
 }
 
-void launch_radius_features(int dim, const float4* pts, const Node* nodes, int64_t L, int bucket, int64_t n,
+void launch_radius_features(int dim, const float4* pts, const Node* nodes, int64_t L, LeafTable lt, int64_t n,
                             float* X, float* LS, hipStream_t s)
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_radius_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dim, pts, nodes, L,
-                       bucket, n, X, LS);
+                       lt, n, X, LS);
 }
 
 void launch_radius_targets(int K, int64_t n, const uint64_t* cand_key, const float* LS, float* T, hipStream_t s)

@@ -234,7 +234,9 @@ int emst_get_cell_grid(emst_ctx* ctx, int32_t level_radius, int32_t* ext_dual,
  * learned seed's features and inference when set, and the candidates ranked
  * for the rounds), [1] round-1 cone search, [2] Boruvka rounds, [3] total,
  * [4] the per-round component search's search dispatches (k_comp_search)
- * summed over the rounds, in traced runs (options.trace; 0 otherwise).
+ * summed over the rounds, [5] the component search's neighbour lists at
+ * build (k_knn_first); [4] and [5] in traced runs (options.trace; 0
+ * otherwise).
  * n_times: capacity in, count out. */
 int emst_get_times(emst_ctx* ctx, float* times, int32_t* n_times);
 
@@ -249,7 +251,12 @@ int emst_get_times(emst_ctx* ctx, float* times, int32_t* n_times);
  * emst_options.search_split; 0 when the search ran as one launch),
  * [9] queries over all the per-round component search's dispatches of the
  * run so far (round_search 1; a query resumed after its first pass counts
- * twice).
+ * twice), [10] hierarchy nodes those dispatches tested (one per node a
+ * 64-query packet examined), [11] leaf cells they entered, [12] point pairs
+ * they evaluated (leaf points x queries that needed the leaf), [13]-[15] the
+ * same three for the neighbour lists of the component search's build
+ * (pairs there: leaf points examined x 64 lanes); [10]-[15] are counted in
+ * traced runs (options.trace) only, 0 otherwise.
  * No counterpart in the reference (its per-phase timers are in
  * SolutionEMSTOperators.cu:166-373).  n_stats: capacity in, count out. */
 int emst_get_search_stats(emst_ctx* ctx, int64_t* stats, int32_t* n_stats);
