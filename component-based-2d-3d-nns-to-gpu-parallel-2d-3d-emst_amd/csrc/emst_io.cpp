@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <new>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -57,11 +58,23 @@ bool write_file(const std::string& path, const std::string& body)
     return std::fclose(f) == 0 && ok;
 }
 
+// Bytes of the open file (its size bounds the counts its header may claim:
+// every coordinate takes at least two characters).
+long long file_bytes(std::ifstream& fi)
+{
+    const std::streampos here = fi.tellg();
+    fi.seekg(0, std::ios::end);
+    const long long n = (long long)fi.tellg();
+    fi.seekg(here);
+    return n < 0 ? 0 : n;
+}
+
 // readUniform (InputRW.h:516-623): first token N, then N points
 int read_uniform(std::ifstream& fi, int dim, std::vector<float>& out)
 {
+    const long long bytes = file_bytes(fi);
     long long size = 0;
-    if (!(fi >> size) || size < 0) return EMST_E_IO;
+    if (!(fi >> size) || size < 0 || size > bytes / (2 * dim)) return EMST_E_IO;
     out.resize((size_t)size * dim);
     for (long long i = 0; i < size * dim; ++i)
         if (!(fi >> out[(size_t)i])) return EMST_E_IO;
@@ -72,6 +85,7 @@ int read_uniform(std::ifstream& fi, int dim, std::vector<float>& out)
 // header lines, then "index x y" per city
 int read_tsp(std::ifstream& fi, std::vector<float>& xy)
 {
+    const long long bytes = file_bytes(fi);
     std::string line;
     if (!std::getline(fi, line) || line.find("NAME") == std::string::npos) return EMST_E_IO;
     for (int l = 1; l <= 3; ++l)
@@ -81,6 +95,7 @@ int read_tsp(std::ifstream& fi, std::vector<float>& xy)
     char w1[256] = {0}, w2[256] = {0};
     long long size = 0;
     if (std::sscanf(line.c_str() + d, "%255s%255s%lld", w1, w2, &size) != 3 || size < 0) return EMST_E_IO;
+    size = std::min(size, bytes / 6);       // an "index x y" row takes at least 6 characters
     for (int l = 1; l <= 2; ++l)
         if (!std::getline(fi, line)) return EMST_E_IO;
     // a file holding fewer cities than DIMENSION ends at its "EOF" line: the
@@ -104,10 +119,13 @@ int read_tsp(std::ifstream& fi, std::vector<float>& xy)
 // is parsed as written here.)
 int read_points_list(std::ifstream& fi, int dim, std::vector<float>& out)
 {
+    const long long bytes = file_bytes(fi);
     std::string w, eq;
     long long width = 0, height = 0;
     if (!(fi >> w >> eq >> width) || !(fi >> w >> eq >> height) || width < 0 || height < 0) return EMST_E_IO;
+    if (width > bytes || height > bytes) return EMST_E_IO;
     const long long size = width * height;
+    if (size > bytes / (2 * dim)) return EMST_E_IO;
     out.resize((size_t)size * dim);
     for (long long i = 0; i < size; ++i)
         for (int k = 0; k < dim; ++k)
@@ -119,7 +137,22 @@ int read_points_list(std::ifstream& fi, int dim, std::vector<float>& out)
 
 extern "C" {
 
+static int read_points_impl(const char* path, int32_t dim, float* xyz, int64_t* n);
+
+// The readers and writers never let an exception leave the C ABI: a failed
+// allocation is EMST_E_OOM, anything else a file error.
 int emst_read_points(const char* path, int32_t dim, float* xyz, int64_t* n)
+{
+    try {
+        return read_points_impl(path, dim, xyz, n);
+    } catch (const std::bad_alloc&) {
+        return EMST_E_OOM;
+    } catch (...) {
+        return EMST_E_IO;
+    }
+}
+
+static int read_points_impl(const char* path, int32_t dim, float* xyz, int64_t* n)
 {
     if (!path || !n || (dim != 2 && dim != 3)) return EMST_E_ARG;
     const std::string file(path);
@@ -175,7 +208,20 @@ int emst_read_points(const char* path, int32_t dim, float* xyz, int64_t* n)
     return EMST_OK;
 }
 
+static int write_solution_impl(emst_ctx* ctx, const char* path);
+
 int emst_write_solution(emst_ctx* ctx, const char* path)
+{
+    try {
+        return write_solution_impl(ctx, path);
+    } catch (const std::bad_alloc&) {
+        return EMST_E_OOM;
+    } catch (...) {
+        return EMST_E_IO;
+    }
+}
+
+static int write_solution_impl(emst_ctx* ctx, const char* path)
 {
     if (!ctx || !path) return EMST_E_ARG;
     const std::string file(path);

@@ -77,3 +77,21 @@ def test_cli_reports_unsupported_modes(pkg, tmp_path):
     r = subprocess.run([pkg.CLI_PATH, golden_path("1000randomCity.2d"), str(tmp_path / "o"),
                         str(tmp_path / "o.stats"), str(cfg)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and "UNSUPPORTED FUNCTIONMODE=2 !!" in r.stdout
+
+
+@pytest.mark.parametrize("name,text", [
+    ("huge.2d", "99999999999999\n1 2\n3 4\n"),
+    ("neg.3d", "-5\n1 2 3\n"),
+    ("short.2d", "3\n1 2\n3 4\n"),
+    ("big.points", "Width = 4000000000 Height = 4000000000\n1 2 3\n"),
+    ("over.points", "Width = 3037000500 Height = 3037000500\n1 2 3\n"),
+])
+def test_malformed_headers_are_file_errors(pkg, tmp_path, name, text):
+    """A header claiming more points than the file can hold (or a negative or
+    overflowing count) is a file error, not an allocation or a crash."""
+    path = tmp_path / name
+    path.write_text(text)
+    dim = 2 if name.endswith(".2d") else 3
+    with pytest.raises(pkg.EmstError) as e:
+        pkg.read_points(str(path), dim)
+    assert e.value.code == 8
