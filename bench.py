@@ -23,22 +23,116 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 
-def profiled_traffic(args, kernel="k_cone_search"):
-    """HBM bytes per launch of `kernel` (for k_comp_search: all its dispatches
-    of one EMST) from the committed rocprofv3 summary of this workload
-    (FETCH_SIZE + WRITE_SIZE, each scaled by a plain copy), or None when no
-    summary covers it."""
-    tags = {(1, 10_000_000, 3): "gauss64_1e7_3d", (0, 1_000_000, 3): "uniform_1e6_3d"}
-    tag = tags.get((args.kind, args.n, args.dim))
-    if tag is None or args.cone_dim not in (0, args.dim):
-        return None
+def profiled(kernel, tag, field):
+    """A figure of the committed rocprofv3 summary of `kernel` on the workload
+    `tag` (profiles/<kernel>_<tag>.md, a line "<field>: value"), or None."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{tag}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return int(float(line.split(":")[1]))
+        if line.startswith(field + ":"):
+            return float(line.split(":")[1])
     return None
+
+
+WORKLOAD_TAGS = {(1, 10_000_000, 3): "gauss64_1e7_3d", (2, 10_000_000, 2): "powerlaw16_1e7_2d",
+                 (0, 1_000_000, 3): "uniform_1e6_3d"}
+NAMES = {0: "uniform", 1: "gauss64+10%bg", 2: "powerlaw16"}
+DESC = {"cone_plain": "cone search, plain seed", "cone_learned": "cone search, learned seed",
+        "comp_plain": "per-round component search, plain seed",
+        "comp_learned": "per-round component search, learned seed"}
+
+
+def load_model(pkg_dir, tag, dim):
+    """(model, training metadata) of models/radius_<tag><dim>d.npy, or (None, None)."""
+    import numpy as np
+    path = os.path.join(pkg_dir, "models", f"radius_{tag}{dim}d.npy")
+    if not os.path.exists(path):
+        return None, None
+    meta = os.path.join(pkg_dir, "models", f"train_radius{'_comp' if tag else ''}.json")
+    return np.load(path), (json.load(open(meta)) if os.path.exists(meta) else None)
+
+
+def measure_cloud(pkg, ge, kind, n, dim, seed, steps, warmup, bucket=0, cone_dim=0, plain_only=False,
+                  families=("cone", "comp")):
+    """Every form on the same resident points: W untimed and K timed steps
+    each (one step = set_points from device memory + run + weight), all
+    forms asserted to build the same tree.  Returns the forms and the models."""
+    import numpy as np
+    import torch
+
+    pts = pkg.synth(kind, n, dim, seed)
+    dev = torch.from_numpy(pts).to("cuda:0")
+    torch.cuda.synchronize()
+    cdim = cone_dim or dim
+    models = {}
+    for fam, tag in (("cone", ""), ("comp", "comp_")):
+        m, meta = (None, None) if plain_only or cdim != dim else load_model(ge.PKG_DIR, tag, dim)
+        models[fam] = (m, meta)
+
+    def timed(model, round_search):
+        ctx = pkg.Calculateur(dim=dim, bucket_size=bucket, cone_dim=cone_dim, round_search=round_search)
+        if model is not None:
+            ctx.set_radius_model(model)
+        search_ms, total_ms = [], []
+
+        def step():
+            ctx.initialize_device(dev.data_ptr(), n, dim)
+            ctx.run()
+            return ctx.evaluate()
+
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            w, m = step()
+            t = ctx.times()
+            search_ms.append(t[1])
+            total_ms.append(t[3])
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        u, v, _ = ctx.getLinks()
+        samples = ctx.radius_samples() if (model is not None and not round_search) else None
+        ctx.close()
+        assert m == n - 1, f"tree has {m} edges"
+        med = lambda xs: sorted(xs)[len(xs) // 2]
+        return {"dt": dt, "w": w, "m": m, "search_ms": med(search_ms), "total_ms": med(total_ms),
+                "edges_digest": hash((u.tobytes(), v.tobytes())), "samples": samples,
+                "mpts": n * steps / dt / 1e6}
+
+    forms = {}
+    if "cone" in families:
+        forms["cone_plain"] = timed(None, 0)
+        if models["cone"][0] is not None:
+            forms["cone_learned"] = timed(models["cone"][0], 0)
+    if "comp" in families and cdim == dim:
+        forms["comp_plain"] = timed(None, 1)
+        if models["comp"][0] is not None:
+            forms["comp_learned"] = timed(models["comp"][0], 1)
+    ref = next(iter(forms.values()))
+    for name, f in forms.items():
+        assert f["edges_digest"] == ref["edges_digest"] and f["w"] == ref["w"], f"{name} built a different tree"
+    return {"forms": forms, "models": models, "dev": dev, "pts": pts, "cdim": cdim}
+
+
+def traced_work(pkg, dev, n, dim, model, round_search, bucket=0):
+    """One traced run: per-kernel device times (HIP events on the context's
+    stream) and the work counters."""
+    with pkg.Calculateur(dim=dim, bucket_size=bucket, round_search=round_search, trace=True) as c:
+        if model is not None:
+            c.set_radius_model(model)
+        c.initialize_device(dev.data_ptr(), n, dim)
+        c.run()
+        return c.times(), pkg.search_stats(c)
+
+
+def learned_report(forms, fam):
+    plain, learned = forms.get(f"{fam}_plain"), forms.get(f"{fam}_learned")
+    if plain is None or learned is None:
+        return None
+    return {"plain_mpts": round(plain["mpts"], 3), "learned_mpts": round(learned["mpts"], 3),
+            "learned_vs_plain": round(plain["dt"] / learned["dt"], 3)}
 
 
 def main():
@@ -53,12 +147,16 @@ def main():
     ap.add_argument("--bucket", type=int, default=0)
     ap.add_argument("--cone-dim", type=int, default=0,
                     help="2 with --dim 3: the 2.5D mode (wedges over the xy plane, 3D distances)")
-    ap.add_argument("--plain", action="store_true", help="plain seed only (skip the learned form)")
+    ap.add_argument("--plain", action="store_true", help="plain seed only (skip the learned forms)")
+    ap.add_argument("--forms", default="cone,comp", help="search families timed: cone, comp or both")
     ap.add_argument("--round-search", type=int, default=-1, choices=(-1, 0, 1),
                     help="headline form: the fastest of the cone search's forms (0), of the per-round "
                          "component search's (1), or of all (-1)")
     ap.add_argument("--cpu-sample", type=int, default=200_000,
                     help="points of the same distribution timed on the CPU oracle (0 = skip)")
+    ap.add_argument("--extra", type=int, default=1,
+                    help="1: also time the 10^7-point 2D power-law cloud (seed 4) and the 4096-cloud batch "
+                         "(seed 5) into config")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (scales rocprof byte counters)")
     ap.add_argument("--batch", type=int, default=0, metavar="CLOUDS",
@@ -84,169 +182,115 @@ def main():
               "(the EMST engine runs one context per GPU)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-    import torch
+    # torch (device buffers for the resident inputs) is loaded before the
+    # library: both link the HIP runtime under one soname, and the copy the
+    # process loads first then serves both
+    import torch  # noqa: F401
+
     import __graft_entry__ as ge
 
     pkg = ge.load_package()
     pkg.lib()
     if args.batch:
-        run_batch(args, pkg)
+        print(json.dumps(run_batch(args, pkg, args.batch, args.batch_seed, args.steps, args.warmup)))
         return
     if args.tsplib:
         run_tsplib(args, pkg)
         return
-    names = {0: "uniform", 1: "gauss64+10%bg", 2: "powerlaw16"}
-    pts = pkg.synth(args.kind, args.n, args.dim, args.seed)
-    dev = torch.from_numpy(pts).to("cuda:0")
-    torch.cuda.synchronize()
 
-    cdim = args.cone_dim or args.dim
+    fams = tuple(f for f in args.forms.split(",") if f)
+    res = measure_cloud(pkg, ge, args.kind, args.n, args.dim, args.seed, args.steps, args.warmup,
+                        bucket=args.bucket, cone_dim=args.cone_dim, plain_only=args.plain, families=fams)
+    forms, models, cdim = res["forms"], res["models"], res["cdim"]
     K = 2 * cdim * (1 << (cdim - 1))
-
-    def load_model(tag):
-        path = os.path.join(ge.PKG_DIR, "models", f"radius_{tag}{args.dim}d.npy")
-        if args.plain or cdim != args.dim or not os.path.exists(path):
-            return None, None
-        meta = os.path.join(ge.PKG_DIR, "models", f"train_radius{'_comp' if tag else ''}.json")
-        return np.load(path), (json.load(open(meta)).get("clouds") if os.path.exists(meta) else None)
-
-    cone_model, model_clouds = load_model("")
-    comp_model, _ = load_model("comp_")
-
-    def timed(form_model, round_search):
-        ctx = pkg.Calculateur(dim=args.dim, bucket_size=args.bucket, cone_dim=args.cone_dim,
-                              round_search=round_search)
-        if form_model is not None:
-            ctx.set_radius_model(form_model)
-        search_ms, total_ms = [], []
-
-        def step():
-            ctx.initialize_device(dev.data_ptr(), args.n, args.dim)
-            ctx.run()
-            return ctx.evaluate()
-
-        for _ in range(args.warmup):
-            w, m = step()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            w, m = step()
-            t = ctx.times()
-            search_ms.append(t[1])
-            total_ms.append(t[3])
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        u, v, _ = ctx.getLinks()
-        st = pkg.search_stats(ctx)
-        samples = ctx.radius_samples() if (form_model is not None and not round_search) else None
-        ctx.close()
-        assert m == args.n - 1, f"tree has {m} edges"
-        med = lambda xs: sorted(xs)[len(xs) // 2]
-        return {"dt": dt, "w": w, "m": m, "search_ms": med(search_ms), "total_ms": med(total_ms),
-                "edges_digest": hash((u.tobytes(), v.tobytes())), "stats": st, "samples": samples,
-                "mpts": args.n * args.steps / dt / 1e6}
-
-    # every form on the same points; each must build the same tree
-    forms = {"cone_plain": timed(None, 0)}
-    if cone_model is not None:
-        forms["cone_learned"] = timed(cone_model, 0)
-    if cdim == args.dim:
-        forms["comp_plain"] = timed(None, 1)
-        if comp_model is not None:
-            forms["comp_learned"] = timed(comp_model, 1)
-    ref = forms["cone_plain"]
-    for name, f in forms.items():
-        assert f["edges_digest"] == ref["edges_digest"] and f["w"] == ref["w"], f"{name} built a different tree"
     # the headline form: the fastest measured (--round-search: the fastest of
     # that search's forms)
     if args.round_search < 0:
         head_name = max(forms, key=lambda k: forms[k]["mpts"])
     else:
         fam = "comp" if args.round_search == 1 else "cone"
-        head_name = max((k for k in forms if k.startswith(fam)), key=lambda k: forms[k]["mpts"],
-                        default="cone_plain")
+        head_name = max((k for k in forms if k.startswith(fam)), key=lambda k: forms[k]["mpts"])
     fam = "comp" if head_name.startswith("comp") else "cone"
-    rs = 1 if fam == "comp" else 0
     head = forms[head_name]
-    plain, learned = forms["cone_plain"], forms.get("cone_learned")
-    seed_report = None
-    if learned is not None:
-        # hit rate: points whose seed already covers every non-empty cone's
-        # nearest point (pass 0 closes them all); savings: traversal work
-        X, T = learned["samples"]
-        has = T > -19.0
-        y = pkg.mlp_eval(cone_model, X) + cone_model[-1]
-        ps, ls = plain["stats"], learned["stats"]
-        seed_report = {
-            "hit_rate_learned": round(float(np.mean((y >= T)[has])), 4),
-            "hit_rate_plain": round(float(np.mean((np.log(3.0) >= T)[has])), 4),
-            "nodes_tested_per_packet": [round(ps["nodes"] / ps["packets"], 1), round(ls["nodes"] / ls["packets"], 1)],
-            "pairs_per_point": [round(ps["pairs"] / args.n, 1), round(ls["pairs"] / args.n, 1)],
-            "node_test_savings": round(1 - ls["nodes"] / ps["nodes"], 4),
-            "pair_savings": round(1 - ls["pairs"] / ps["pairs"], 4),
-        }
-    ms_per_step = head["dt"] * 1e3 / args.steps
-    value = head["mpts"]
-    w, m = head["w"], head["m"]
+    tag = WORKLOAD_TAGS.get((args.kind, args.n, args.dim)) if args.cone_dim in (0, args.dim) else None
 
     if fam == "cone":
-        # dominant kernel: the round-1 cone search; algorithmic bytes per
-        # point = 8*D (own coordinates) + 4 + 8*K (one (id, d2) per cone out);
-        # time from HIP events on the context stream
-        alg_bytes = args.n * (8 * args.dim + 4 + 8 * K)
-        k_ms = head["search_ms"]
-        kernel = "k_cone_search"
+        # the round-1 cone search; algorithmic bytes per point = 8*D (own
+        # coordinates) + 4 + 8*K (one (id, d2) per cone out); time from HIP
+        # events on the context stream
+        kern = {"k_cone_search": {"alg_bytes": args.n * (8 * args.dim + 4 + 8 * K), "ms": head["search_ms"]}}
+        work = None
     else:
-        # dominant kernel: the component search's dispatches of one EMST (two
-        # per round); algorithmic bytes per query and dispatch = 44: its point
-        # (16), label (4), previous key (8) and list entry (4) in, its result
-        # (corr 4, key 8) out.  Time from HIP events around each dispatch, in
-        # one traced (host-issued) run on the context stream
-        with pkg.Calculateur(dim=args.dim, bucket_size=args.bucket, round_search=1, trace=True) as c:
-            if head_name == "comp_learned":
-                c.set_radius_model(comp_model)
-            c.initialize_device(dev.data_ptr(), args.n, args.dim)
-            c.run()
-            k_ms = c.times()[4]
-            queries = pkg.search_stats(c)["comp_queries"]
-        alg_bytes = 44 * queries
-        kernel = "k_comp_search"
-    achieved = alg_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+        # the NNS kernels of the component search, from one traced run (HIP
+        # events around each dispatch on the context stream):
+        #   k_knn_first, once per EMST: 101 algorithmic bytes per point (its
+        #   point 16 in; 6 list entries of 12 bytes, target 12 and cursor 1 out);
+        #   k_comp_search, two dispatches per round: 44 bytes per query and
+        #   dispatch (point 16, label 4, previous key 8, list entry 4 in;
+        #   result 12 out)
+        t, st = traced_work(pkg, res["dev"], args.n, args.dim,
+                            models["comp"][0] if head_name == "comp_learned" else None, 1, args.bucket)
+        kern = {"k_knn_first": {"alg_bytes": 101 * args.n, "ms": t[5]},
+                "k_comp_search": {"alg_bytes": 44 * st["comp_queries"], "ms": t[4]}}
+        work = {k: st[k] for k in ("comp_queries", "comp_nodes", "comp_leaves", "comp_pairs", "knn_nodes",
+                                   "knn_leaves", "knn_pairs")}
+    for name, kd in kern.items():
+        kd["achieved_gbs"] = round(kd["alg_bytes"] / (kd["ms"] * 1e-3) / 1e9, 1) if kd["ms"] > 0 else 0.0
+        kd["ms"] = round(kd["ms"], 3)
+        tr = [profiled(name, tag, f) for f in ("traffic_bytes_per_launch", "valu_busy")] if tag else [None, None]
+        kd["traffic"] = int(tr[0]) if tr[0] is not None else None
+        kd["valu_busy"] = tr[1]
+    tot_bytes = sum(kd["alg_bytes"] for kd in kern.values())
+    tot_ms = sum(kd["ms"] for kd in kern.values())
+    achieved = tot_bytes / (tot_ms * 1e-3) / 1e9 if tot_ms > 0 else 0.0
+    traffics = [kd["traffic"] for kd in kern.values()]
+    busy = [kd["valu_busy"] for kd in kern.values() if kd["valu_busy"] is not None]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-                "frac": round(achieved / 8000.0, 4), "traffic": profiled_traffic(args, kernel),
-                "kernel": kernel, "kernel_ms": round(k_ms, 3)}
+                "frac": round(achieved / 8000.0, 4),
+                "traffic": sum(traffics) if all(x is not None for x in traffics) else None,
+                "kernel": "+".join(kern), "kernel_ms": round(tot_ms, 3), "kernels": kern,
+                # the HBM roofline is the one the survey prices these kernels
+                # on; the counters put their limit elsewhere: instruction issue
+                "limiter": "VALU issue" if busy else None,
+                "valu_busy": round(sum(busy) / len(busy), 3) if busy else None,
+                "work": work}
+
+    extra = {}
+    if args.extra and args.n == 10_000_000 and args.kind == 1 and args.dim == 3 and not args.cone_dim:
+        # the 10^7-point 2D power-law cloud, every form
+        r4 = measure_cloud(pkg, ge, 2, 10_000_000, 2, 4, args.steps, args.warmup, plain_only=args.plain)
+        f4 = r4["forms"]
+        h4 = max(f4, key=lambda k: f4[k]["mpts"])
+        extra["powerlaw_2d_1e7"] = {
+            "workload": "EMST 10000000 pts 2D powerlaw16 seed 4", "headline_form": DESC[h4],
+            "mpts": round(f4[h4]["mpts"], 3), "ms_per_step": round(f4[h4]["dt"] * 1e3 / args.steps, 3),
+            "mpts_by_form": {k: round(v["mpts"], 3) for k, v in f4.items()},
+            "learned_vs_plain": {fam_: learned_report(f4, fam_) for fam_ in ("comp", "cone")},
+            "same_tree_all_forms": True}
+        del r4
+        # the 4096-cloud batch
+        extra["batch_4096"] = run_batch(args, pkg, 4096, 5, args.steps, args.warmup)
 
     cpu = cpu_baseline(args, pkg, cdim) if args.cpu_sample > 0 else None
-
-    desc = {"cone_plain": "cone search, plain seed", "cone_learned": "cone search, learned seed",
-            "comp_plain": "per-round component search, plain seed",
-            "comp_learned": "per-round component search, learned seed"}
-    mp = lambda k: round(forms[k]["mpts"], 3) if k in forms else None
+    cone_meta = models["cone"][1] or {}
+    comp_meta = models["comp"][1] or {}
     out = {
         "metric": "EMST Mpoints/s at 10^7 points (fastest exact form; learned-radius vs plain in config)",
-        "value": round(value, 3), "unit": "Mpoints/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "value": round(head["mpts"], 3), "unit": "Mpoints/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(head["dt"] * 1e3 / args.steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (project generator, seeded)",
         "config": {"workload": f"EMST {args.n} pts {args.dim}D{' (2.5D: xy wedges)' if cdim < args.dim else ''} "
-                               f"{names.get(args.kind)} seed {args.seed}",
-                   "n": args.n, "dim": args.dim, "form": desc[head_name],
-                   "edges": int(m), "weight": w,
+                               f"{NAMES.get(args.kind)} seed {args.seed}",
+                   "n": args.n, "dim": args.dim, "form": DESC[head_name],
+                   "edges": int(head["m"]), "weight": head["w"],
                    "gpu_total_ms_median": round(head["total_ms"], 3),
-                   "mpts_by_form": {k: mp(k) for k in desc},
+                   "mpts_by_form": {k: (round(forms[k]["mpts"], 3) if k in forms else None) for k in DESC},
                    "same_tree_all_forms": True,
-                   # learned vs plain: the cone search's seed model (the
-                   # component search ships none: its model measured slower)
-                   "plain_mpts": mp("cone_plain"),
-                   "plain_search_ms": round(plain["search_ms"], 3),
-                   "learned_mpts": mp("cone_learned"),
-                   "learned_search_ms": round(learned["search_ms"], 3) if learned else None,
-                   "learned_vs_plain": round(plain["dt"] / learned["dt"], 3) if learned else None,
-                   "radius_model_training_clouds": model_clouds,
-                   "learned_seed": seed_report,
-                   "cone_dim": cdim, "cycles_cut": int(forms["cone_plain"]["stats"]["cycles_cut"]),
-                   "round_search": rs},
+                   "learned_vs_plain": {fam_: learned_report(forms, fam_) for fam_ in ("comp", "cone")},
+                   "radius_model_training": {"cone": cone_meta.get("clouds"), "comp": comp_meta.get("clouds")},
+                   "cone_dim": cdim, "round_search": 1 if fam == "comp" else 0,
+                   **extra},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -366,19 +410,28 @@ def run_tsplib(args, pkg):
     print(json.dumps(out))
 
 
-def run_batch(args, pkg):
+def run_batch(args, pkg, B, seed, steps, warmup):
     """Batched EMST: the 2D clouds and the 3D clouds each run as one batch
     context (emst_set_clouds), the two contexts concurrently from two host
-    threads on their own streams.  value = all points / wall time of a step;
-    per-cloud latency = build + cone search + the rounds up to the cloud's
-    completion round, from the contexts' HIP events in a traced step."""
+    threads on their own streams, in two forms:
+      autotuned   leaf size, seed and search form per dimension from
+                  emst_autotune, the build issued kernel by kernel and the
+                  rounds replayed 4 at a time from a captured graph;
+      one_launch  the per-round component search with options.one_launch
+                  (and the autotuned leaf size and seed where the autotuner
+                  chose that search): each context's whole run is one graph
+                  launch.
+    Throughput = all points / wall time of a step; value = the faster form's.
+    Per-cloud latency = time from the run's start to the end of the cloud's
+    completion round: for one_launch from the events inside the launch
+    (emst_get_round_ends), for autotuned from a traced step.  Returns the
+    result as a dict (the JSON line of --batch)."""
     import threading
 
     import numpy as np
     import torch
 
-    B = args.batch
-    sizes, dims, kinds, make = pkg.batch_workload(B, args.batch_seed)
+    sizes, dims, kinds, make = pkg.batch_workload(B, seed)
     groups = []
     for d in (2, 3):
         idx = np.nonzero(dims == d)[0]
@@ -387,11 +440,6 @@ def run_batch(args, pkg):
         groups.append({"dim": d, "idx": idx, "sizes": sizes[idx], "dev": packed})
     torch.cuda.synchronize()
     total = int(sizes.sum())
-
-    tuned = [{} for _ in groups]
-
-    def make(trace):
-        return [pkg.Calculateur(dim=g["dim"], **dict(t, trace=trace)) for g, t in zip(groups, tuned)]
 
     def step(ctxs):
         errs = []
@@ -411,32 +459,43 @@ def run_batch(args, pkg):
         if errs:
             raise errs[0]
 
-    ctxs = make(False)
+    def timed(opts):
+        ctxs = [pkg.Calculateur(dim=g["dim"], **o) for g, o in zip(groups, opts)]
+        for _ in range(warmup):
+            step(ctxs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(ctxs)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        edges = 0
+        lat = np.zeros(B)
+        for c, g in zip(ctxs, groups):
+            w, e, last = c.cloud_results()
+            assert np.array_equal(e, np.maximum(g["sizes"] - 1, 0)), "a cloud did not finish its tree"
+            edges += int(e.sum())
+            ends = c.round_ends()
+            if len(ends):
+                t = c.times()
+                lat[g["idx"]] = np.where(last > 0, ends[np.maximum(last, 1) - 1], t[0])
+            c.close()
+        return dt, edges, lat
+
+    # autotuned (untimed tuning)
+    tuned = [{} for _ in groups]
     if not args.no_autotune:
-        # untimed: leaf size and seed scale per dimension from the autotuner
-        for i, (c, g) in enumerate(zip(ctxs, groups)):
-            c.initialize_clouds_device(g["dev"].data_ptr(), g["sizes"])
-            c.autotune()
-            tuned[i] = {k: v for k, v in c.options().items()
-                        if k in ("bucket_size", "seed_scale", "search_split", "round_search")}
-    for _ in range(args.warmup):
-        step(ctxs)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(ctxs)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    edges = 0
-    for c, g in zip(ctxs, groups):
-        w, e, r = c.cloud_results()
-        assert np.array_equal(e, np.maximum(g["sizes"] - 1, 0)), "a cloud did not finish its tree"
-        edges += int(e.sum())
-        c.close()
-    # latency from a traced step
-    tctx = make(True)
+        for i, g in enumerate(groups):
+            with pkg.Calculateur(dim=g["dim"]) as c:
+                c.initialize_clouds_device(g["dev"].data_ptr(), g["sizes"])
+                c.autotune()
+                tuned[i] = {k: v for k, v in c.options().items()
+                            if k in ("bucket_size", "seed_scale", "search_split", "round_search")}
+    dt_a, edges, _ = timed(tuned)
+    # its latency from a traced step
+    tctx = [pkg.Calculateur(dim=g["dim"], **dict(t, trace=True)) for g, t in zip(groups, tuned)]
     step(tctx)
-    lat = np.zeros(B)
+    lat_a = np.zeros(B)
     for c, g in zip(tctx, groups):
         t = c.times()
         rows = c.trace()
@@ -444,22 +503,34 @@ def run_batch(args, pkg):
                      r["time_flatten"] for r in rows]
         cum = np.concatenate([[0.0], np.cumsum(per_round)])
         _, _, last = c.cloud_results()
-        lat[g["idx"]] = t[0] + t[1] + cum[np.minimum(last, len(per_round))]
+        lat_a[g["idx"]] = t[0] + t[1] + cum[np.minimum(last, len(per_round))]
         c.close()
-    out = {
-        "metric": f"batched EMST Mpoints/s ({B} clouds)", "value": round(total * args.steps / dt / 1e6, 3),
-        "unit": "Mpoints/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(dt * 1e3 / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+    # one launch per context
+    dt_o, edges_o, lat_o = timed([dict(t, round_search=1, one_launch=1) if t.get("round_search", 1) == 1
+                                  else {"round_search": 1, "one_launch": 1} for t in tuned])
+    assert edges_o == edges
+    forms = {"autotuned": (dt_a, lat_a), "one_launch": (dt_o, lat_o)}
+    best = min(forms, key=lambda k: forms[k][0])
+    dt, lat = forms[best]
+    rep = lambda d, l: {"mpts": round(total * steps / d / 1e6, 3), "ms_per_step": round(d * 1e3 / steps, 3),
+                        "latency_ms_p50": round(float(np.percentile(l, 50)), 3),
+                        "latency_ms_p99": round(float(np.percentile(l, 99)), 3),
+                        "latency_ms_max": round(float(l.max()), 3)}
+    return {
+        "metric": f"batched EMST Mpoints/s ({B} clouds)", "value": round(total * steps / dt / 1e6, 3),
+        "unit": "Mpoints/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": round(dt * 1e3 / steps, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (project generator, seeded)",
-        "config": {"workload": f"{B} clouds, sizes log-uniform 1e2-1e5, half 2D half 3D, uniform or 8 Gaussians",
+        "config": {"workload": f"{B} clouds, sizes log-uniform 1e2-1e5, half 2D half 3D, uniform or 8 Gaussians, "
+                               f"seed {seed}",
                    "clouds": B, "points": total, "edges": edges,
                    "clouds_2d": int((dims == 2).sum()), "points_2d": int(sizes[dims == 2].sum()),
-                   "autotuned": {f"{g['dim']}d": t for g, t in zip(groups, tuned)},
+                   "form": best, "autotuned": {f"{g['dim']}d": t for g, t in zip(groups, tuned)},
+                   "by_form": {k: rep(*v) for k, v in forms.items()},
                    "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
                    "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
                    "latency_ms_max": round(float(lat.max()), 3)},
     }
-    print(json.dumps(out))
 
 
 if __name__ == "__main__":

@@ -28,7 +28,7 @@ EXPORTS = [
     "emst_set_radius_model", "emst_get_radius_samples", "emst_mlp_train", "emst_mlp_eval", "emst_mlp_grad",
     "emst_set_clouds", "emst_get_cloud_results", "emst_autotune", "emst_get_shape", "emst_get_options",
     "emst_read_points",
-    "emst_write_solution", "emst_weight_noalloc",
+    "emst_write_solution", "emst_weight_noalloc", "emst_get_round_ends",
 ]
 CLI_PATH = os.path.join(_HERE, "emst_cli")
 
@@ -47,7 +47,7 @@ class Options(ctypes.Structure):
                 ("trace", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("radius_mode", ctypes.c_int32), ("seed_scale", ctypes.c_float),
                 ("cone_dim", ctypes.c_int32), ("search_split", ctypes.c_int32), ("round_search", ctypes.c_int32),
-                ("log_samples", ctypes.c_int32)]
+                ("log_samples", ctypes.c_int32), ("one_launch", ctypes.c_int32)]
 
 
 class RoundTrace(ctypes.Structure):
@@ -105,6 +105,7 @@ def lib():
     L.emst_read_points.argtypes = [ctypes.c_char_p, i32, P, P]
     L.emst_write_solution.argtypes = [P, ctypes.c_char_p]
     L.emst_weight_noalloc.argtypes = [P, i32, P, P, i64, i32, P, P]
+    L.emst_get_round_ends.argtypes = [P, P, P]
     L.emst_version.restype = ctypes.c_char_p
     _lib = L
     return L
@@ -161,7 +162,7 @@ class Calculateur:
     """One EMST context on one device (CalculateurEMST / SolutionEMST)."""
 
     def __init__(self, dim=3, level_radius=1, max_rounds=100, bucket_size=0, trace=False, device=0,
-                 seed_scale=0.0, cone_dim=0, search_split=0, round_search=0, log_samples=0):
+                 seed_scale=0.0, cone_dim=0, search_split=0, round_search=0, log_samples=0, one_launch=0):
         """cone_dim=2 with dim=3: the 2.5D mode (SolutionEMST<3,2>, 3D points,
         wedges over the xy plane)."""
         L = lib()
@@ -174,6 +175,7 @@ class Calculateur:
         o.search_split = search_split
         o.round_search = round_search
         o.log_samples = log_samples
+        o.one_launch = one_launch
         self._L = L
         self._c = L.emst_create(ctypes.byref(o))
         if not self._c:
@@ -207,11 +209,13 @@ class Calculateur:
         n, dim = pts.shape
         self._chk(self._L.emst_set_points(self._c, _p(pts), n, dim, 0), "set_points")
         self.n, self.dim = n, dim
+        self.offsets = np.array([0, n], np.int64)
 
     def initialize_device(self, dev_ptr, n, dim):
         """Same, for points already resident on the device (AoS float32)."""
         self._chk(self._L.emst_set_points(self._c, ctypes.c_void_p(dev_ptr), n, dim, 1), "set_points")
         self.n, self.dim = n, dim
+        self.offsets = np.array([0, n], np.int64)
 
     def initialize_clouds(self, clouds):
         """A batch of independent clouds of this context's dimension (list of
@@ -255,7 +259,8 @@ class Calculateur:
         self._chk(self._L.emst_get_options(self._c, ctypes.byref(o)), "options")
         return {"level_radius": o.level_radius, "max_rounds": o.max_rounds, "bucket_size": o.bucket_size,
                 "trace": bool(o.trace), "device": o.device, "seed_scale": o.seed_scale, "cone_dim": o.cone_dim,
-                "search_split": o.search_split, "round_search": o.round_search, "log_samples": o.log_samples}
+                "search_split": o.search_split, "round_search": o.round_search, "log_samples": o.log_samples,
+                "one_launch": o.one_launch}
 
     def cloud_results(self):
         """Per cloud: (FP64 weight, edge count, completion round)."""
@@ -349,6 +354,15 @@ class Calculateur:
         T = np.empty(m.value, np.float32)
         self._chk(self._L.emst_get_radius_samples(self._c, _p(X), _p(T), ctypes.byref(m)), "samples")
         return X, T
+
+    def round_ends(self):
+        """One-launch runs: ms from the run's start to the end of each round."""
+        k = ctypes.c_int32(0)
+        self._chk(self._L.emst_get_round_ends(self._c, None, ctypes.byref(k)), "round_ends")
+        out = np.empty(k.value, np.float32)
+        if k.value:
+            self._chk(self._L.emst_get_round_ends(self._c, _p(out), ctypes.byref(k)), "round_ends")
+        return out
 
     def times(self):
         t = (ctypes.c_float * 6)()

@@ -146,6 +146,14 @@ struct emst_ctx {
     bool groups_settled = false;      // no group spans two components any more
     hipGraphExec_t graph = nullptr;   // captured rounds (run_graph)
     std::vector<uintptr_t> graph_key;
+    hipGraphExec_t graph1 = nullptr;  // a whole captured run (one_launch)
+    std::vector<uintptr_t> graph1_key;
+    // one_launch: device clock at the run's start [0], at the end of each
+    // round [1..kOneLaunchRounds], after the build [kOneLaunchRounds + 1] and
+    // at the end [kOneLaunchRounds + 2]
+    DevBuf<unsigned long long> stamps;
+    int clock_khz = 0;                 // hipDeviceAttributeWallClockRate
+    std::vector<float> round_end_ms;   // the last one-launch run: start to the end of each round
     int64_t dup_points = 0, dup_groups = 0;
     Params* P = nullptr;
     Counters* C = nullptr;
@@ -348,7 +356,7 @@ int make_leaves(emst_ctx* c)
         return rc;
     // the sort's second key and value buffers are free here
     launch_leaves(c->keys0.p, n, c->dcl.p, c->ncl, c->bucket, c->vals1.p, reinterpret_cast<uint32_t*>(c->keys1.p),
-                  c->scan_tmp.p, c->lstart.p, c->leaf_of.p, c->cloud_lf.p, s);
+                  c->scan_tmp.p, c->lstart.p, c->leaf_of.p, c->cloud_lf.p, c->P, c->C, false, s);
     LAUNCHCHK(c);
     std::vector<int32_t> lf(c->ncl + 1);
     HIPCHK(c, hipMemcpyAsync(lf.data(), c->cloud_lf.p, sizeof(int32_t) * (c->ncl + 1), hipMemcpyDeviceToHost, s));
@@ -545,14 +553,16 @@ int build(emst_ctx* c)
         // searches (launch_comp_round)
         HIPCHK(c, hipMemsetAsync(c->visit.p, 0, sizeof(unsigned long long) * 2 * c->nleaves, s));
         launch_node_min_ids(c->pts.p, c->dcl.p, c->ncl, c->nleaves, leaf_table(c), c->nodes.p, c->visit.p, s);
-        // round 1's targets and the neighbour lists; the learned component
-        // seed keeps round 1's searches (its sample log starts there)
-        c->knn_built = !comp_features_on(c);
+        // round 1's targets and the neighbour lists; a run that logs the
+        // learned seed's training pairs searches in round 1 as well (its log
+        // starts there); the seed itself applies from round 3 on, past the
+        // rounds the lists answer
+        c->knn_built = c->opt.log_samples != 1;
         const bool tr = c->opt.trace != 0;
         if (tr) HIPCHK(c, hipEventRecord(c->ev[12], s));
         if (c->knn_built)
-            launch_knn_first(c->dim, n, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, leaf_table(c), c->corr.p,
-                             c->corr_key.p, nbr_list(c), c->C, tr, s);
+            launch_knn_first(c->dim, n, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, leaf_table(c), c->inv.p,
+                             c->corr.p, c->corr_key.p, nbr_list(c), c->C, tr, s);
         if (tr) HIPCHK(c, hipEventRecord(c->ev[13], s));
         int rc1 = ensure_comp_seed(c);
         if (rc1) return rc1;
@@ -675,10 +685,16 @@ int weight_of(emst_ctx* c, int64_t m, double* w)
     return EMST_OK;
 }
 
+// rounds captured in a one-launch run (options.one_launch); a run that needs
+// more finishes with the replayed round graph.  Rounds past the end cost
+// their launches (about 0.15 ms each at 10^7 points), and 10^7-point clouds
+// finish in 10-12 rounds.
+constexpr int kOneLaunchRounds = 16;
+
 // The kernels of one Boruvka round (SolutionEMST::activate).  Round control
 // lives on the device (k_round_begin / k_round_end), so a round issued after
 // the run has finished does nothing.  trace: events after each phase.
-void issue_round(emst_ctx* c, bool tr)
+void issue_round(emst_ctx* c, bool tr, unsigned long long* stamps = nullptr)
 {
     hipStream_t s = c->stream;
     launch_round_begin(c->C, s);
@@ -710,7 +726,7 @@ void issue_round(emst_ctx* c, bool tr)
                            c->ev_.p, c->ed2.p, c->eround.p, cy, s);
         if (tr) (void)hipEventRecord(c->ev[4 + ph], s);
     }
-    launch_round_end(c->C, c->grouped, s);
+    launch_round_end(c->C, c->grouped, s, stamps, kOneLaunchRounds);
 }
 
 // A flatten that stopped at its step cap paused the rounds (done == 2):
@@ -741,12 +757,11 @@ void absorb_counters(emst_ctx* c, const Counters& h)
     c->rounds = (int)h.round;
     c->stats[7] = (int64_t)h.cycles_cut;
     c->stats[9] = (int64_t)h.comp_queries;
-    c->stats[10] = (int64_t)h.comp_nodes;
-    c->stats[11] = (int64_t)h.comp_leaves;
-    c->stats[12] = (int64_t)h.comp_pairs;
-    c->stats[13] = (int64_t)h.knn_nodes;
-    c->stats[14] = (int64_t)h.knn_leaves;
-    c->stats[15] = (int64_t)h.knn_pairs;
+    for (int k = 0; k < 6; ++k) {
+        unsigned long long t = 0;
+        for (int j = 0; j < kWorkSlots; ++j) t += h.work[k][j];
+        c->stats[10 + k] = (int64_t)t;
+    }
     if (h.done == 1u) c->finished = true;
 }
 
@@ -831,6 +846,7 @@ std::vector<uintptr_t> graph_key(emst_ctx* c)
             (uintptr_t)c->knn_key.p, (uintptr_t)c->knn_cur.p, (uintptr_t)c->knn_built, (uintptr_t)c->open_mask.p, (uintptr_t)c->open_cnt.p,
             (uintptr_t)c->open_tmp.p, (uintptr_t)c->list.p, (uintptr_t)c->nodes.p, (uintptr_t)c->dcl.p,
             (uintptr_t)c->nleaves, (uintptr_t)c->lstart.p, (uintptr_t)c->leaf_of.p, (uintptr_t)c->bucket,
+            (uintptr_t)__builtin_bit_cast(uint32_t, c->seed_scale),
             (uintptr_t)y.j[0], (uintptr_t)y.kh[0], (uintptr_t)y.kl[0], (uintptr_t)y.onc, (uintptr_t)y.cut,
             (uintptr_t)c->grouped, (uintptr_t)d.npoints,
             (uintptr_t)d.ngroups, (uintptr_t)c->C, (uintptr_t)c->pts.p, (uintptr_t)c->cand_key.p,
@@ -874,6 +890,181 @@ int run_graph(emst_ctx* c)
     return EMST_OK;
 }
 
+// ---- one launch: the whole run as one captured graph (options.one_launch)
+
+bool one_launch_on(const emst_ctx* c)
+{
+    return c->opt.one_launch == 1 && c->opt.round_search == 1 && !c->opt.trace && !comp_features_on(c) &&
+           c->n > 0;
+}
+
+// The build of the component search with no host decision in it: each
+// cloud's slot capacity is fixed from its size, a power of two of at least
+// one slot per bucket/4 points (maximal prefix cells of at most 64 points
+// hold 25-50 points on average on uniform, clustered and power-law clouds of
+// 10^2 to 3.10^5 points; a cloud with more leaves than that is flagged on the
+// device and the run redone eagerly).  The device then lays the hierarchies
+// out inside that capacity exactly as the eager build does (k_cloud_leaves),
+// and the launches over leaf slots cover the capacity, their threads past a
+// cloud's slots idle.  The checks the eager build makes on the host (NaN or
+// Inf input) leave the rounds idle through C->invalid.
+int one_launch_layout(emst_ctx* c)
+{
+    int64_t lbase = 0, Lmax = 1;
+    const int64_t per_slot = std::max(1, c->bucket / 4);
+    for (Cloud& cl : c->hcl) {
+        const int64_t want = std::max<int64_t>(1, (cl.n + per_slot - 1) / per_slot);
+        int64_t L = 1;
+        while (L < want) L <<= 1;
+        cl.L = L;
+        cl.lbase = lbase;
+        cl.lfirst = 0;
+        cl.nl = 0;
+        lbase += L;
+        Lmax = std::max(Lmax, L);
+    }
+    c->nleaves = lbase;
+    c->Lmax = Lmax;
+    c->L = c->hcl[0].L;
+    int rc;
+    if ((rc = ensure(c, c->nodes, 2 * (size_t)lbase)) || (rc = ensure(c, c->visit, 2 * (size_t)lbase)) ||
+        (rc = ensure(c, c->lstart, (size_t)c->n + 1)) || (rc = ensure(c, c->leaf_of, (size_t)c->n)) ||
+        (rc = ensure(c, c->cloud_lf, (size_t)c->ncl + 1)) || (rc = ensure(c, c->stamps, kOneLaunchRounds + 3)))
+        return rc;
+    if (c->clock_khz <= 0) {
+        HIPCHK(c, hipDeviceGetAttribute(&c->clock_khz, hipDeviceAttributeWallClockRate, c->opt.device));
+        if (c->clock_khz <= 0) return fail(c, EMST_E_HIP, "no device clock rate");
+    }
+    HIPCHK(c, hipMemcpyAsync(c->dcl.p, c->hcl.data(), sizeof(Cloud) * c->ncl, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return EMST_OK;
+}
+
+void issue_one_launch_build(emst_ctx* c)
+{
+    const int64_t n = c->n;
+    hipStream_t s = c->stream;
+    launch_bbox(c->xyz, c->dcl.p, c->ncl, n, c->dim, c->P, s);
+    launch_params(c->P, c->dcl.p, c->ncl, c->dim, c->mbits, s);
+    launch_rescale_morton(c->xyz, c->dcl.p, c->ncl, n, c->dim, c->mbits, c->P, c->resc.p, c->keys0.p, s);
+    int key_bits = c->mbits * c->dim;
+    if (c->ncl > 1) {
+        int cb = 0;
+        while ((1ll << cb) < c->ncl) ++cb;
+        key_bits += cb;
+    }
+    radix_sort(c->keys0.p, c->vals0.p, c->keys1.p, c->vals1.p, n, key_bits, true, c->hist.p, c->scan_tmp.p, s);
+    launch_gather_sorted(c->resc.p, c->vals0.p, n, c->dim, c->pts.p, c->inv.p, s);
+    launch_leaves(c->keys0.p, n, c->dcl.p, c->ncl, c->bucket, c->vals1.p, reinterpret_cast<uint32_t*>(c->keys1.p),
+                  c->scan_tmp.p, c->lstart.p, c->leaf_of.p, c->cloud_lf.p, c->P, c->C, true, s);
+    // dup_runs .. cycles_cut start at zero (a kernel: no memset nodes in graphs)
+    launch_zero_u32(&c->C->dup_runs, 8, s);
+    launch_build_tree(c->pts.p, c->dcl.p, c->ncl, c->nleaves, c->Lmax, leaf_table(c), c->nodes.p, nullptr, s);
+    launch_round_init(n, c->label.p, c->label2.p, c->fixed.p, c->eu.p, c->corr.p, c->corr_key.p, c->comp1.p,
+                      c->comp2.p, c->C, (uint32_t)c->target_edges, (uint32_t)c->opt.max_rounds, s);
+    launch_zero_u32(reinterpret_cast<uint32_t*>(c->visit.p), 4 * c->nleaves, s);
+    launch_node_min_ids(c->pts.p, c->dcl.p, c->ncl, c->nleaves, leaf_table(c), c->nodes.p, c->visit.p, s);
+    launch_knn_first(c->dim, n, c->pts.p, c->nodes.p, c->dcl.p, c->ncl, c->bucket, leaf_table(c), c->inv.p,
+                     c->corr.p, c->corr_key.p, NbrList{c->knn_pos.p, c->knn_key.p, c->knn_cur.p}, c->C, false, s);
+}
+
+int run_rounds(emst_ctx* c);
+
+int run_one_launch(emst_ctx* c)
+{
+    hipStream_t s = c->stream;
+    int rc = one_launch_layout(c);
+    if (rc) return rc;
+    c->grouped = false;
+    c->groups_settled = false;
+    c->no_grouping = false;
+    c->knn_built = true;
+    c->dup_points = c->dup_groups = 0;
+    std::vector<uintptr_t> key = graph_key(c);
+    key.push_back((uintptr_t)c->xyz);
+    key.push_back((uintptr_t)c->ncl);
+    key.push_back((uintptr_t)c->resc.p);
+    key.push_back((uintptr_t)c->keys0.p);
+    key.push_back((uintptr_t)c->target_edges);
+    key.push_back((uintptr_t)c->opt.max_rounds);
+    key.push_back((uintptr_t)c->mbits);
+    if (!c->graph1 || key != c->graph1_key) {
+        if (c->graph1) {
+            (void)hipGraphExecDestroy(c->graph1);
+            c->graph1 = nullptr;
+        }
+        unsigned long long* st = c->stamps.p;
+        HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        launch_stamp(st, s);
+        issue_one_launch_build(c);
+        launch_stamp(st + kOneLaunchRounds + 1, s);
+        for (int r = 0; r < kOneLaunchRounds; ++r) issue_round(c, false, st);
+        launch_stamp(st + kOneLaunchRounds + 2, s);
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamEndCapture(s, &g));
+        hipError_t e = hipGraphInstantiate(&c->graph1, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            c->graph1 = nullptr;
+            return fail(c, EMST_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        c->graph1_key = key;
+    }
+    HIPCHK(c, hipEventRecord(c->ev[0], s));
+    HIPCHK(c, hipGraphLaunch(c->graph1, s));
+    HIPCHK(c, hipEventRecord(c->ev[3], s));
+    Counters h{};
+    if ((rc = read_counters(c, &h))) return rc;
+    if (h.invalid) {
+        std::vector<Params> hp(c->ncl);
+        HIPCHK(c, hipMemcpyAsync(hp.data(), c->P, sizeof(Params) * c->ncl, hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        for (int k = 0; k < c->ncl; ++k)
+            if (hp[k].nonfinite) {
+                c->built = false;
+                return fail(c, EMST_E_NONFINITE,
+                            c->ncl == 1 ? std::string("input holds a NaN or Inf coordinate")
+                                        : "cloud " + std::to_string(k) + " holds a NaN or Inf coordinate");
+            }
+        // more leaves than slots: the eager build lays the hierarchy out exactly
+        c->built = false;
+        if ((rc = build(c))) return rc;
+        return run_rounds(c);
+    }
+    c->zero_or_empty = 0;
+    c->comp_search_ms = 0.f;
+    c->knn_ms = 0.f;
+    for (auto& x : c->stats) x = 0;
+    c->resumed_h = 0;
+    c->built = true;
+    c->rounds = 0;
+    c->edges = 0;
+    c->finished = c->target_edges == 0;
+    c->trace.clear();
+    if (h.done == 2u && (rc = resume_flatten(c, &h))) return rc;
+    absorb_counters(c, h);
+    unsigned long long st[kOneLaunchRounds + 3];
+    HIPCHK(c, hipMemcpyAsync(st, c->stamps.p, sizeof st, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    const double per_ms = (double)c->clock_khz;
+    const float whole = elapsed(c->ev[0], c->ev[3]);
+    c->times[0] = (float)((st[kOneLaunchRounds + 1] - st[0]) / per_ms);
+    c->times[1] = 0.f;
+    c->times[2] = std::max(0.f, whole - c->times[0]);
+    c->round_end_ms.clear();
+    for (int r = 1; r <= std::min(c->rounds, kOneLaunchRounds); ++r)
+        c->round_end_ms.push_back((float)((st[r] - st[0]) / per_ms));
+    if (!c->finished) {
+        HIPCHK(c, hipEventRecord(c->ev[2], s));
+        if ((rc = run_graph(c))) return rc;
+        HIPCHK(c, hipEventRecord(c->ev[3], s));
+        HIPCHK(c, hipEventSynchronize(c->ev[3]));
+        c->times[2] += elapsed(c->ev[2], c->ev[3]);
+    }
+    c->times[3] = c->times[0] + c->times[1] + c->times[2];
+    return EMST_OK;
+}
+
 int need_points(emst_ctx* c)
 {
     if (!c) return EMST_E_ARG;
@@ -893,6 +1084,7 @@ int ensure_built(emst_ctx* c)
 int run_rounds(emst_ctx* c)
 {
     int rc;
+    c->round_end_ms.clear();
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
     if (!c->opt.trace && (c->grouped || c->zero_or_empty == 0)) {
         rc = run_graph(c);
@@ -995,6 +1187,7 @@ int emst_destroy(emst_ctx* c)
     hipSetDevice(c->opt.device);
     if (c->stream) hipStreamSynchronize(c->stream);
     if (c->graph) hipGraphExecDestroy(c->graph);
+    if (c->graph1) hipGraphExecDestroy(c->graph1);
     if (c->C) hipFree(c->C);
     for (auto& e : c->ev) if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -1176,6 +1369,7 @@ int emst_run(emst_ctx* c)
     int rc = need_points(c);
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->opt.device));
+    if (!c->built && one_launch_on(c)) return run_one_launch(c);
     rc = ensure_built(c);
     if (rc) return rc;
     return run_rounds(c);
@@ -1427,6 +1621,20 @@ int emst_get_times(emst_ctx* c, float* times, int32_t* n_times)
     const int32_t k = std::min<int32_t>(*n_times, 6);
     for (int i = 0; i < k; ++i) times[i] = i < 4 ? c->times[i] : (i == 4 ? c->comp_search_ms : c->knn_ms);
     *n_times = k;
+    return EMST_OK;
+}
+
+int emst_get_round_ends(emst_ctx* c, float* ms, int32_t* n)
+{
+    if (!c || !n) return EMST_E_ARG;
+    const int32_t have = (int32_t)c->round_end_ms.size();
+    if (!ms) {
+        *n = have;
+        return EMST_OK;
+    }
+    const int32_t k = std::min(*n, have);
+    for (int32_t i = 0; i < k; ++i) ms[i] = c->round_end_ms[i];
+    *n = k;
     return EMST_OK;
 }
 

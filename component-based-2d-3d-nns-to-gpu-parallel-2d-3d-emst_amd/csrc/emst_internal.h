@@ -21,6 +21,9 @@ struct Params {
     uint32_t pad[3];
 };
 
+constexpr int kWorkSlots = 64;
+enum { kWorkCompNodes, kWorkCompLeaves, kWorkCompPairs, kWorkKnnNodes, kWorkKnnLeaves, kWorkKnnPairs };
+
 // Round counters, one device struct per context.
 struct Counters {
     uint32_t edges;         // total forest edges
@@ -51,11 +54,16 @@ struct Counters {
     uint32_t nroots;        // roots listed by k_cycle_init this round (2.5D cycle rule)
     uint32_t nsamples;      // seed samples logged by the component search over the run
     unsigned long long comp_queries;    // queries over the component search's dispatches, over the run
-    // work of the component search's dispatches over the run: node tests
-    // (one per node a packet examines), leaf visits, point pairs evaluated
-    unsigned long long comp_nodes, comp_leaves, comp_pairs;
-    // the same for the neighbour lists at build (k_knn_first)
-    unsigned long long knn_nodes, knn_leaves, knn_pairs;
+    // work of the component search's dispatches over the run (node tests,
+    // one per node a packet examines; leaf visits; point pairs evaluated),
+    // then the same for the neighbour lists at build (k_knn_first); each
+    // spread over kWorkSlots words (a packet adds to slot blockIdx % kWorkSlots,
+    // so that the adds do not queue on one address), summed by the host
+    unsigned long long work[6][kWorkSlots];
+    // a build issued without host checks (one launch) found a NaN or Inf
+    // coordinate or a cloud with more leaves than its slots: the rounds do
+    // nothing and the host redoes or refuses the run
+    uint32_t invalid;
 };
 
 // Coincident-point groups.  Points with identical rescaled coordinates form a
@@ -174,8 +182,15 @@ void launch_build_tree(const float4* pts, const Cloud* cl, int ncl, int64_t nlea
 // scratch, tmp scan_tmp_words(n); start (n + 1 words at most) and of (n) are
 // written, and cloud_lf[c] = the first leaf of cloud c, cloud_lf[ncl] = the
 // number of leaves.
-void launch_leaves(const uint64_t* keys, int64_t n, const Cloud* cl, int ncl, int bucket, uint32_t* mark,
-                   uint32_t* idx, uint32_t* tmp, int32_t* start, int32_t* of, int32_t* cloud_lf, hipStream_t s);
+// layout: each cloud's slot count L is fixed beforehand (a bound); the
+// leaves found are written into the cloud table on the device, and
+// C->invalid set when a cloud has more leaves than slots or a NaN or Inf
+// coordinate (P).  Without layout the host lays the hierarchy out from
+// cloud_lf.
+void launch_leaves(const uint64_t* keys, int64_t n, Cloud* cl, int ncl, int bucket, uint32_t* mark,
+                   uint32_t* idx, uint32_t* tmp, int32_t* start, int32_t* of, int32_t* cloud_lf, const Params* P,
+                   Counters* C, bool layout, hipStream_t s);
+void launch_zero_u32(uint32_t* p, int64_t n, hipStream_t s);
 // Cone search over queries [0,n) (qlist == nullptr) or qlist[0..nq).
 // check_labels: skip zero-distance points of the query's own component.
 // collect_stats: add the traversal counters to C.
@@ -246,8 +261,8 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsig
 // launch_node_min_ids).  Writes corr / corr_key (every component is one
 // point, so the nearest neighbour is the round's exact target) and nb.
 void launch_knn_first(int dim, int64_t n, const float4* pts, const Node* nodes, const Cloud* cl, int ncl,
-                      int bucket, LeafTable lt, int32_t* corr, uint64_t* corr_key, const NbrList& nb, Counters* C, bool collect,
-                      hipStream_t s);
+                      int bucket, LeafTable lt, const int32_t* inv, int32_t* corr, uint64_t* corr_key,
+                      const NbrList& nb, Counters* C, bool collect, hipStream_t s);
 // (ev: 4 events recorded around the two k_comp_search dispatches, or nullptr)
 // Sizes and boxes of the single-point components (the start of a run).
 void launch_comp_seed_init(int64_t n, const float4* pts, int32_t* csize, uint32_t* cbox, hipStream_t s);
@@ -287,7 +302,10 @@ void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const ui
 // Round control: begin (termination test, round += 1) and end (no progress
 // -> finished; flatten unfinished -> paused; groups settled).
 void launch_round_begin(Counters* C, hipStream_t s);
-void launch_round_end(Counters* C, bool grouped, hipStream_t s);
+void launch_round_end(Counters* C, bool grouped, hipStream_t s, unsigned long long* stamps = nullptr,
+                      uint32_t cap = 0);
+// the device's constant-rate clock (hipDeviceAttributeWallClockRate) into *at
+void launch_stamp(unsigned long long* at, hipStream_t s);
 // Queue the points holding a zero-distance or no candidate.
 void launch_flag_research(int K, int64_t n, const uint64_t* cand_key, int32_t* list,
                           Counters* C, hipStream_t s);

@@ -337,12 +337,15 @@ __global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* __restrict__ out,
 }
 
 // ------------------------------------------------------------ radix sort
-// 8-bit digits, tiles of 4096 keys per 256-thread block.  count: per-tile
+// 8-bit digits, tiles of 2048 keys per 256-thread block.  count: per-tile
 // digit histogram (LDS atomics); scan over (digit, tile); scatter: each wave
-// owns 1024 consecutive keys, ranks equal digits with 8 ballots, so the sort
-// is stable.
-constexpr int kRsTile = 4096;
+// owns 512 consecutive keys of the tile and ranks equal digits with 8
+// ballots, so the sort is stable; the tile is first sorted by digit in LDS
+// and then written out in that order, so that the stores of consecutive
+// threads fall on consecutive addresses of each digit's run.
+constexpr int kRsTile = 2048;
 constexpr int kRsWaveKeys = kRsTile / 4;
+constexpr int kRsChunks = kRsWaveKeys / 64;
 
 __global__ __launch_bounds__(kBlock) void k_rs_count(const uint64_t* __restrict__ keys, int64_t n,
                                                     int shift, uint32_t* __restrict__ hist,
@@ -367,49 +370,83 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint64_t* __restric
                                                       int shift, const uint32_t* __restrict__ hist,
                                                       int64_t nblk, int identity)
 {
-    __shared__ uint32_t cnt[4][256];
-    __shared__ uint32_t base[4][256];
+    __shared__ uint32_t cnt[4][256];        // per wave and digit; then the wave's next tile slot
+    __shared__ uint32_t lstart[256];        // tile slot of each digit's first key
+    __shared__ uint32_t gstart[256];        // output position of the tile's first key of each digit
+    __shared__ uint32_t wsum[4];
+    __shared__ uint64_t skey[kRsTile];
+    __shared__ uint32_t sval[kRsTile];
     const int w = threadIdx.x / 64;
     const int lane = __lane_id();
     for (int k = threadIdx.x; k < 4 * 256; k += kBlock) (&cnt[0][0])[k] = 0;
     __syncthreads();
-    const int64_t wbase = (int64_t)blockIdx.x * kRsTile + (int64_t)w * kRsWaveKeys;
-    for (int c = 0; c < kRsWaveKeys / 64; ++c) {
+    const int64_t tbase = (int64_t)blockIdx.x * kRsTile;
+    const int64_t wbase = tbase + (int64_t)w * kRsWaveKeys;
+    uint64_t key[kRsChunks];
+    uint32_t val[kRsChunks];
+#pragma unroll
+    for (int c = 0; c < kRsChunks; ++c) {
         const int64_t i = wbase + c * 64 + lane;
-        if (i < n) atomicAdd(&cnt[w][(kin[i] >> shift) & 255u], 1u);
+        const bool valid = i < n;
+        key[c] = valid ? kin[i] : 0ull;
+        val[c] = identity ? (uint32_t)i : (valid ? vin[i] : 0u);
+        if (valid) atomicAdd(&cnt[w][(key[c] >> shift) & 255u], 1u);
     }
     __syncthreads();
     {
+        // exclusive scan of the tile's digit totals: within each wave, then
+        // across the 4 waves' sums
         const int d = threadIdx.x;
-        uint32_t run = hist[(int64_t)d * nblk + blockIdx.x];
-        for (int ww = 0; ww < 4; ++ww) {
-            base[ww][d] = run;
-            run += cnt[ww][d];
+        const uint32_t c0 = cnt[0][d], c1 = cnt[1][d], c2 = cnt[2][d], c3 = cnt[3][d];
+        const uint32_t tot = c0 + c1 + c2 + c3;
+        uint32_t inc = tot;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t o = (uint32_t)__shfl_up((int)inc, off);
+            if (lane >= off) inc += o;
         }
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        uint32_t before = 0;
+        for (int ww = 0; ww < w; ++ww) before += wsum[ww];
+        const uint32_t ls = before + inc - tot;
+        lstart[d] = ls;
+        gstart[d] = hist[(int64_t)d * nblk + blockIdx.x];
+        cnt[0][d] = ls;
+        cnt[1][d] = ls + c0;
+        cnt[2][d] = ls + c0 + c1;
+        cnt[3][d] = ls + c0 + c1 + c2;
     }
     __syncthreads();
     const uint64_t lt = lanemask_lt();
-    for (int c = 0; c < kRsWaveKeys / 64; ++c) {
+#pragma unroll
+    for (int c = 0; c < kRsChunks; ++c) {
         const int64_t i = wbase + c * 64 + lane;
         const bool valid = i < n;
-        const uint64_t key = valid ? kin[i] : 0ull;
-        const uint32_t d = (uint32_t)(key >> shift) & 255u;
+        const uint32_t d = (uint32_t)(key[c] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
         for (int b = 0; b < 8; ++b) {
             const bool bit = (d >> b) & 1u;
             const uint64_t bb = __ballot(valid && bit);
             peers &= bit ? bb : ~bb;
         }
-        uint32_t pos = 0;
         if (valid) {
-            const uint32_t rank = (uint32_t)__popcll(peers & lt);
-            pos = base[w][d] + rank;
-            kout[pos] = key;
-            vout[pos] = identity ? (uint32_t)i : vin[i];
+            const uint32_t slot = cnt[w][d] + (uint32_t)__popcll(peers & lt);
+            skey[slot] = key[c];
+            sval[slot] = val[c];
         }
         __builtin_amdgcn_wave_barrier();
-        if (valid && (peers & lt) == 0ull) base[w][d] += (uint32_t)__popcll(peers);
+        if (valid && (peers & lt) == 0ull) cnt[w][d] += (uint32_t)__popcll(peers);
         __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    const int tn = (int)min<int64_t>(kRsTile, n - tbase);
+    for (int s = threadIdx.x; s < tn; s += kBlock) {
+        const uint64_t kk = skey[s];
+        const uint32_t d = (uint32_t)(kk >> shift) & 255u;
+        const uint32_t pos = gstart[d] + (uint32_t)s - lstart[d];
+        kout[pos] = kk;
+        vout[pos] = sval[s];
     }
 }
 
@@ -526,14 +563,62 @@ __global__ __launch_bounds__(kBlock) void k_leaf_fill(const uint32_t* __restrict
     if (i == n - 1) start[g + 1] = (int32_t)n;
 }
 
-// first leaf of every cloud (every non-empty cloud starts a leaf), and the count
-__global__ void k_cloud_leaves(const Cloud* __restrict__ cl, int ncl, const uint32_t* __restrict__ mark,
-                               const uint32_t* __restrict__ idx, int64_t n, int32_t* __restrict__ out)
+// First leaf of every cloud (every non-empty cloud starts a leaf), and the
+// count.  layout: also lay the clouds' hierarchies out in the cloud table as
+// the host does after reading the counts (L = the power of two at or above
+// the cloud's leaf count, lbase = the sum of the earlier clouds' L), inside
+// slot counts the host fixed beforehand (the table's L on entry);
+// C->invalid flags a cloud with more leaves than those slots, or with a NaN
+// or Inf coordinate.  One block.
+__global__ __launch_bounds__(1024) void k_cloud_leaves(Cloud* __restrict__ cl, int ncl,
+                                                      const uint32_t* __restrict__ mark,
+                                                      const uint32_t* __restrict__ idx, int64_t n,
+                                                      int32_t* __restrict__ out, const Params* __restrict__ P,
+                                                      Counters* C, int layout)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c > ncl) return;
+    __shared__ int64_t part[1024];
+    if (threadIdx.x == 0) C->invalid = 0u;
     const int32_t total = n > 0 ? (int32_t)(idx[n - 1] + mark[n - 1]) : 0;
-    out[c] = (c < ncl && cl[c].pos < n) ? (int32_t)idx[cl[c].pos] : total;
+    for (int c = threadIdx.x; c <= ncl; c += blockDim.x)
+        out[c] = (c < ncl && cl[c].pos < n) ? (int32_t)idx[cl[c].pos] : total;
+    __syncthreads();
+    if (!layout) return;
+    // each thread a run of consecutive clouds
+    const int per = (ncl + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int c0 = min(ncl, (int)threadIdx.x * per), c1 = min(ncl, c0 + per);
+    int64_t sum = 0;
+    for (int c = c0; c < c1; ++c) {
+        const int64_t nl = (int64_t)out[c + 1] - out[c];
+        if (nl > cl[c].L || P[c].nonfinite) atomicOr(&C->invalid, 1u);
+        int64_t L = 1;
+        while (L < nl) L <<= 1;
+        cl[c].lfirst = out[c];
+        cl[c].nl = nl;
+        cl[c].L = L;
+        sum += L;
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t run = 0;
+        for (int t = 0; t < (int)blockDim.x; ++t) {
+            const int64_t v = part[t];
+            part[t] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    int64_t run = part[threadIdx.x];
+    for (int c = c0; c < c1; ++c) {
+        cl[c].lbase = run;
+        run += cl[c].L;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero_u32(uint32_t* __restrict__ p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 // rep[] = -1 everywhere; the first position of every run of >= 2 equal keys is queued
@@ -679,6 +764,7 @@ __global__ __launch_bounds__(kBlock) void k_leaves(const float4* __restrict__ pt
     if (g >= nleaves) return;
     const int c = find_cloud_leaf(cl, ncl, g);
     const int64_t j = g - cl[c].lbase;          // the cloud's leaf slot j
+    if (j >= cl[c].L) return;               // past the cloud's slots (one-launch capacity)
     const bool grouped = rep != nullptr;
     Node nd;
     for (int k = 0; k < 3; ++k) { nd.lo[k] = INFINITY; nd.hi[k] = -INFINITY; }
@@ -877,12 +963,14 @@ constexpr int kMaxCappedPasses = 6;
 // resumes its own open lanes (resume_mask).  Resuming after pass k rebuilds
 // the state exactly: R_k by the same float products, and a cone is open after
 // pass k iff its best bound exceeds R_k^2.
-// amdgpu_waves_per_eu(5): the search waits on dependent box and point loads,
-// so occupancy pays; with 8 cones the kernel fits 5 waves per SIMD in 96
-// VGPRs without spills (+8% on the 2D power-law cloud, +9% in the 2.5D
-// mode).  With 24 cones the cone bests' 12 KB of LDS hold it at 3 anyway.
+// Occupancy: the search waits on dependent box and point loads, so it pays;
+// with 8 cones the kernel fits 5 waves per SIMD in 96 VGPRs without spills
+// (+8% on the 2D power-law cloud, +9% in the 2.5D mode).  With 24 cones the
+// cone bests take 12 KB of LDS per 64-query block, which allows 12 blocks per
+// CU, 3 waves per SIMD, whatever the registers; the attribute asks for that
+// (160 VGPRs, no VGPR spills).
 template <int D, int DC>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_cone_search(const float4* __restrict__ pts,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DC == 3 ? 3 : 5))) void k_cone_search(const float4* __restrict__ pts,
                                                    const Node* __restrict__ nodes,
                                                    const Cloud* __restrict__ clouds, int ncl,
                                                    int bucket, LeafTable lt, int64_t n,
@@ -1424,12 +1512,11 @@ __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __res
     if (i == 0) {
         C->edges = 0; C->research = 0; C->zero_or_empty = 0; C->unflat = 0;
         C->packets = C->nodes = C->leaves = C->leaf_points = C->pairs = 0;
-        C->round = 0; C->done = 0; C->edges_prev = 0; C->dup_settled = 0; C->nsamples = 0;
+        C->round = 0; C->done = C->invalid ? 1u : 0u; C->edges_prev = 0; C->dup_settled = 0; C->nsamples = 0;
         C->comp_queries = 0;
-        C->comp_nodes = C->comp_leaves = C->comp_pairs = 0;
-        C->knn_nodes = C->knn_leaves = C->knn_pairs = 0;
         C->target = target; C->max_rounds = max_rounds;
     }
+    if (i < 6 * kWorkSlots) C->work[i / kWorkSlots][i % kWorkSlots] = 0;
     if (i >= n) return;
     label[i] = (int32_t)i;          // K_EMST_initDisjointSet (EMSTOperators.h:330-343)
     label2[i] = (int32_t)i;
@@ -1460,16 +1547,24 @@ __global__ void k_round_begin(Counters* C)
 // A round that added no edge leaves a forest with no outgoing candidate: done.
 // A flatten that hit its step cap pauses the rounds until the host has
 // finished it.  Coincident groups stop being refreshed once none spans two
-// components (components only merge).
-__global__ void k_round_end(Counters* C, int grouped)
+// components (components only merge).  stamps (one-launch runs): the
+// device's constant-rate clock at the end of each round that ran, at
+// stamps[round].
+__global__ void k_round_end(Counters* C, int grouped, unsigned long long* stamps, uint32_t cap)
 {
     if (C->done) return;
+    if (stamps != nullptr && C->round <= cap) stamps[C->round] = wall_clock64();
     if (C->unflat) {
         C->done = 2;
         return;
     }
     if (grouped && !C->dup_settled && C->dup_split == 0u) C->dup_settled = 1;
     if (C->edges == C->edges_prev) C->done = 1;
+}
+
+__global__ void k_stamp(unsigned long long* at)
+{
+    *at = wall_clock64();
 }
 
 // Points that must search again next round: a zero-distance cone or no cone
@@ -1839,6 +1934,7 @@ __global__ __launch_bounds__(kBlock) void k_nlab_tree(const int32_t* __restrict_
     if (g >= nleaves) return;
     const int c = find_cloud_leaf(cl, ncl, g);
     const int64_t j = g - cl[c].lbase;
+    if (j >= cl[c].L) return;               // past the cloud's slots (one-launch capacity)
     int64_t s0, s1;
     leaf_range(lt, cl[c].lfirst, cl[c].nl, cl[c].pos + cl[c].n, j, s0, s1);
     int32_t lab = kNlabEmpty;
@@ -1861,6 +1957,7 @@ __global__ __launch_bounds__(kBlock) void k_nmin_tree(const float4* __restrict__
     if (g >= nleaves) return;
     const int c = find_cloud_leaf(cl, ncl, g);
     const int64_t j = g - cl[c].lbase;
+    if (j >= cl[c].L) return;               // past the cloud's slots (one-launch capacity)
     int64_t s0, s1;
     leaf_range(lt, cl[c].lfirst, cl[c].nl, cl[c].pos + cl[c].n, j, s0, s1);
     uint32_t m = 0xffffffffu;
@@ -2142,6 +2239,20 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m)
     return ((uint64_t)hi << 32) | lo;
 }
 
+// A gathering starts from every node of depth min(6, leaf depth), 64 at most,
+// one per lane: the levels above would each take a dependent step for a few
+// nodes.  Returns the count written to work[].
+__device__ __forceinline__ int gather_start(int64_t L, int32_t* work)
+{
+    int d0 = 0;
+    while (d0 < 6 && ((int64_t)2 << d0) <= L) ++d0;
+    const int cnt = 1 << d0;
+    const int lane = __lane_id();
+    if (lane < cnt) work[lane] = cnt + lane;
+    __builtin_amdgcn_wave_barrier();
+    return cnt;
+}
+
 // ascending bitonic sort of one 64-bit key per lane across the wave
 __device__ __forceinline__ uint64_t wave_sort_u64(uint64_t e)
 {
@@ -2168,7 +2279,7 @@ __device__ __forceinline__ uint64_t wave_sort_u64(uint64_t e)
 // beyond it cannot hold the component's edge, and it stops with that d2 as
 // its lower bound (corr = -1).  Exact results enter comp1 as they close.
 template <int D, int BK>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_comp_search(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_comp_search(
     const float4* __restrict__ pts, const Node* __restrict__ nodes,
     const Cloud* __restrict__ clouds, int ncl, int bucket_rt, LeafTable lt, const int32_t* __restrict__ qlist,
     const uint32_t* __restrict__ nq_dev, const int32_t* __restrict__ label, const int32_t* __restrict__ inv,
@@ -2382,9 +2493,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                 const int fo = (int)__builtin_ctzll(__ballot(open));
                 const int32_t lab0 = __builtin_amdgcn_readfirstlane(__shfl(mylab, fo));
                 const int32_t skip_lab = __ballot(open && mylab != lab0) == 0ull ? lab0 : (int32_t)0x80000000;
-                int head = 0, tail = 1, ngl = 0;
+                int head = 0, tail = gather_start(L, work), ngl = 0;
                 bool over = false;
-                if (lane == 0) work[0] = 1;
                 __builtin_amdgcn_wave_barrier();
                 while (head < tail) {
                     const int take = min(64, tail - head);
@@ -2474,48 +2584,29 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                         visit_leaf(node, need, needm);
                         ++ver;
                     } else {
-                        const bool leafkids = 2 * (int64_t)node >= L;
-                        const int first = leafkids ? 2 * node : 4 * node;
-                        const int nk = leafkids ? 2 : 4;
-                        st_nodes += (uint32_t)nk;
-                        Node bx[4];
-                        int32_t bl[4];
-                        uint32_t bm[4];
-    #pragma unroll
-                        for (int i = 0; i < 4; ++i)
-                            if (i < nk) {
-                                bx[i] = tree[first + i];
-                                bl[i] = __float_as_int(bx[i].pad[1]);
-                                bm[i] = __float_as_uint(bx[i].pad[0]);
-                            }
-                        uint64_t mk[4];
-                        float dk[4];
-                        int id[4];
-    #pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            mk[i] = i < nk ? __ballot(needed(bx[i], bl[i], bm[i], lim, bk)) : 0ull;
-                            dk[i] = mk[i] ? cdist(bx[i]) : -1.f;
-                            id[i] = first + i;
-                        }
+                        // (rare: an unbounded pass, or a gathering that
+                        // outgrew its lists) both children, the nearer last
+                        // on the stack
+                        const int first = 2 * node;
+                        st_nodes += 2u;
+                        const Node b0 = tree[first], b1 = tree[first + 1];
+                        const uint64_t m0 =
+                            __ballot(needed(b0, __float_as_int(b0.pad[1]), __float_as_uint(b0.pad[0]), lim, bk));
+                        const uint64_t m1 =
+                            __ballot(needed(b1, __float_as_int(b1.pad[1]), __float_as_uint(b1.pad[0]), lim, bk));
+                        const float d0 = m0 ? cdist(b0) : -1.f, d1 = m1 ? cdist(b1) : -1.f;
                         // pushed farthest first, so the nearest pops first; at
-                        // equal distance the lower node (smaller ids in a run of
-                        // coincident points) pops first
-                        auto ce = [&](int a, int b) {
-                            if (dk[a] < dk[b] || (dk[a] == dk[b] && id[a] < id[b])) {
-                                const float td = dk[a]; dk[a] = dk[b]; dk[b] = td;
-                                const uint64_t tm = mk[a]; mk[a] = mk[b]; mk[b] = tm;
-                                const int ti = id[a]; id[a] = id[b]; id[b] = ti;
-                            }
-                        };
-                        ce(0, 1); ce(2, 3); ce(0, 2); ce(1, 3); ce(1, 2);
+                        // equal distance the lower node (smaller ids in a run
+                        // of coincident points) pops first
+                        const bool swap = d1 > d0;
+                        const int ia = swap ? first + 1 : first, ib = swap ? first : first + 1;
+                        const uint64_t ma = swap ? m1 : m0, mb = swap ? m0 : m1;
                         if (lane == 0) {
                             int t = sp;
-    #pragma unroll
-                            for (int i = 0; i < 4; ++i)
-                                if (mk[i]) { stk_id[t] = id[i]; stk_ver[t] = ver; stk_mask[t] = mk[i]; ++t; }
+                            if (ma) { stk_id[t] = ia; stk_ver[t] = ver; stk_mask[t] = ma; ++t; }
+                            if (mb) { stk_id[t] = ib; stk_ver[t] = ver; stk_mask[t] = mb; ++t; }
                         }
-    #pragma unroll
-                        for (int i = 0; i < 4; ++i) sp += mk[i] != 0ull ? 1 : 0;
+                        sp += (ma != 0ull ? 1 : 0) + (mb != 0ull ? 1 : 0);
                         __builtin_amdgcn_wave_barrier();
                     }
                 }
@@ -2550,9 +2641,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         if (lane == 0) open_out[blockIdx.x] = m;
     }
     if (collect && lane == 0) {
-        atomicAdd(&C->comp_nodes, (unsigned long long)st_nodes);
-        atomicAdd(&C->comp_leaves, (unsigned long long)st_leaves);
-        atomicAdd(&C->comp_pairs, (unsigned long long)st_pairs);
+        const int slot = blockIdx.x % kWorkSlots;
+        atomicAdd(&C->work[kWorkCompNodes][slot], (unsigned long long)st_nodes);
+        atomicAdd(&C->work[kWorkCompLeaves][slot], (unsigned long long)st_leaves);
+        atomicAdd(&C->work[kWorkCompPairs][slot], (unsigned long long)st_pairs);
     }
     if (active && !left_open) {
         const uint64_t k = best[lane];
@@ -2580,12 +2672,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
 // gathering outgrows the LDS lists walks the hierarchy per lane instead,
 // nearest child first.  In the pair loop each lane is one query and the
 // leaf's points are read in turn at one address for the whole wave.
+//
+// The list holds keys only, biased by 2^52: (d2 bits << 32 | id) + 2^52 with
+// d2 bits <= 0x7f800000 is a positive normal double whose order is the key's,
+// so an insertion is a min/max network of kKnn FP64 min/max pairs without
+// branches or selects (the empty key is the largest finite double).  The
+// positions come from inv at the end.
+constexpr uint64_t kKnnBias = 1ull << 52;
+constexpr uint64_t kKnnEmpty = 0x7fefffffffffffffull;
+constexpr uint32_t kKnnBiasHi = (uint32_t)(kKnnBias >> 32);
+
+// returns min(a, x) and leaves max(a, x) in x, for doubles that are never NaN:
+// v_min_f64 / v_max_f64 without the canonicalising v_max_f64 the compiler
+// puts before each input of fmin / fmax
+__device__ __forceinline__ double minmax_f64(double a, double& x)
+{
+    double lo, hi;
+    asm("v_min_f64 %0, %2, %3\n\tv_max_f64 %1, %2, %3" : "=&v"(lo), "=&v"(hi) : "v"(a), "v"(x));
+    x = hi;
+    return lo;
+}
 template <int D>
 __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __restrict__ pts,
                                                  const Node* __restrict__ nodes, const Cloud* __restrict__ clouds,
-                                                 int ncl, int bucket, LeafTable lt, int32_t* __restrict__ corr,
-                                                 uint64_t* __restrict__ corr_key, NbrList nb, Counters* C,
-                                                 int collect)
+                                                 int ncl, int bucket, LeafTable lt, const int32_t* __restrict__ inv,
+                                                 int32_t* __restrict__ corr, uint64_t* __restrict__ corr_key,
+                                                 NbrList nb, Counters* C, int collect)
 {
     __shared__ int32_t stk_id[64];
     __shared__ int32_t stk_ver[64];
@@ -2593,33 +2705,26 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
     __shared__ int32_t work[kGatherWork];
     __shared__ unsigned long long gathered[kGatherLeaves];
     __shared__ float4 gbox[16];             // per 8-lane group: lo (xyz), hi (xyz)
+    if (C->done) return;                    // a build whose input was refused on the device
     const int lane = __lane_id();
     const int64_t p = (int64_t)blockIdx.x * 64 + lane;
     const bool active = p < n;
     const float4 me = pts[active ? p : 0];
     const float px = me.x, py = me.y, pz = me.z;
     const int my_cl = active ? find_cloud(clouds, ncl, p) : -1;
-    uint64_t kk[kKnn];
-    int32_t kp[kKnn];
+    double kd[kKnn];                // biased keys as doubles, ascending
 #pragma unroll
-    for (int i = 0; i < kKnn; ++i) {
-        kk[i] = kEmptyKey;
-        kp[i] = -1;
-    }
-    // sorted insertion (keys are distinct: the id is in the low word)
-    auto insert = [&](uint64_t key, int32_t pos) {
+    for (int i = 0; i < kKnn; ++i) kd[i] = __longlong_as_double((long long)kKnnEmpty);
+    auto last = [&]() { return (uint64_t)__double_as_longlong(kd[kKnn - 1]); };
+    // sorted insertion of a biased key (keys are distinct: the id is in the
+    // low word)
+    auto insert = [&](uint64_t key) {
+        double x = __longlong_as_double((long long)key);
 #pragma unroll
-        for (int i = kKnn - 1; i > 0; --i) {
-            const bool sh = key < kk[i - 1];
-            const bool here = key < kk[i];
-            kp[i] = sh ? kp[i - 1] : (here ? pos : kp[i]);
-            kk[i] = sh ? kk[i - 1] : (here ? key : kk[i]);
-        }
-        if (key < kk[0]) {
-            kk[0] = key;
-            kp[0] = pos;
-        }
+        for (int i = 0; i < kKnn; ++i) kd[i] = minmax_f64(kd[i], x);
     };
+    // the kKnn-th d2 bits (an empty list: above every finite d2)
+    auto bound = [&]() { return (uint32_t)(last() >> 32) - kKnnBiasHi; };
     uint32_t st_nodes = 0, st_leaves = 0, st_steps = 0;     // work counters of this packet
     uint64_t pending = __ballot(active);
     while (pending) {
@@ -2652,7 +2757,7 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
 #pragma unroll
             for (int k = 0; k < 3; ++k) m[k] = fmaxf(fmaxf(nd.lo[k] - q[k], q[k] - nd.hi[k]), 0.f);
             const float lb = D == 3 ? d2_of(m[0], m[1], m[2]) : d2_of(m[0], m[1]);
-            return (((uint64_t)__float_as_uint(lb) << 32) | nm) < kk[kKnn - 1];
+            return (((uint64_t)(__float_as_uint(lb) + kKnnBiasHi) << 32) | nm) < last();
         };
         // one leaf point per candidate step, at one address for the wave
         auto pairs = [&](int64_t s0, uint64_t cand, bool own) {
@@ -2665,10 +2770,8 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
                 const float dx = v.x - px, dy = v.y - py, dz = v.z - pz;
                 const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
                 const uint32_t db = __float_as_uint(d2);
-                if (mine && db <= (uint32_t)(kk[kKnn - 1] >> 32) && (!own || s0 + j != p)) {
-                    const uint64_t key = ((uint64_t)db << 32) | __float_as_uint(v.w);
-                    if (key < kk[kKnn - 1]) insert(key, (int32_t)(s0 + j));
-                }
+                const uint64_t key = ((uint64_t)(db + kKnnBiasHi) << 32) | __float_as_uint(v.w);
+                if (mine && key < last() && (!own || s0 + j != p)) insert(key);
             }
         };
         // the packet's own leaves: a leaf point farther from the packet's box
@@ -2677,7 +2780,7 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
             int64_t s0, s1;
             leaf_range(lt, lfirst, nl, cend, node - L, s0, s1);
             const int cnt = (int)(s1 - s0);
-            uint32_t t = mine ? (uint32_t)(kk[kKnn - 1] >> 32) : 0u;
+            uint32_t t = mine ? bound() : 0u;
             for (int off = 32; off > 0; off >>= 1) t = max(t, (uint32_t)__shfl_xor((int)t, off));
             bool keep = false;
             if (lane < cnt) {
@@ -2698,7 +2801,7 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
             int64_t s0, s1;
             leaf_range(lt, lfirst, nl, cend, node - L, s0, s1);
             const int cnt = (int)(s1 - s0);
-            uint32_t t = need ? (uint32_t)(kk[kKnn - 1] >> 32) : 0u;
+            uint32_t t = need ? bound() : 0u;
             t = max(t, (uint32_t)__shfl_xor((int)t, 1));
             t = max(t, (uint32_t)__shfl_xor((int)t, 2));
             t = max(t, (uint32_t)__shfl_xor((int)t, 4));
@@ -2724,11 +2827,10 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
         const int leaf_lo = (int)(L + (lt.of[pf] - lfirst)), leaf_hi = (int)(L + (lt.of[pl] - lfirst));
         for (int lf = leaf_lo; lf <= leaf_hi; ++lf) scan_own(lf);
         // gather the other leaves within the largest kKnn-th d2 of the packet's box
-        uint32_t R = mine ? (uint32_t)(kk[kKnn - 1] >> 32) : 0u;
+        uint32_t R = mine ? bound() : 0u;
         for (int off = 32; off > 0; off >>= 1) R = max(R, (uint32_t)__shfl_xor((int)R, off));
-        int head = 0, tail = 1, ngl = 0;
+        int head = 0, tail = gather_start(L, work), ngl = 0;
         bool over = false;
-        if (lane == 0) work[0] = 1;
         __builtin_amdgcn_wave_barrier();
         while (head < tail) {
             const int take = min(64, tail - head);
@@ -2862,18 +2964,29 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
         }
     }
     if (collect && lane == 0) {
-        atomicAdd(&C->knn_nodes, (unsigned long long)st_nodes);
-        atomicAdd(&C->knn_leaves, (unsigned long long)st_leaves);
-        atomicAdd(&C->knn_pairs, 64ull * st_steps);
+        const int slot = blockIdx.x % kWorkSlots;
+        atomicAdd(&C->work[kWorkKnnNodes][slot], (unsigned long long)st_nodes);
+        atomicAdd(&C->work[kWorkKnnLeaves][slot], (unsigned long long)st_leaves);
+        atomicAdd(&C->work[kWorkKnnPairs][slot], 64ull * st_steps);
     }
     if (!active) return;
+    int32_t pos0 = -2;
+    uint64_t key0 = kEmptyKey;
 #pragma unroll
     for (int i = 0; i < kKnn; ++i) {
-        nb.pos[(int64_t)i * n + p] = kp[i];
-        nb.key[(int64_t)i * n + p] = kk[i];
+        const uint64_t b = (uint64_t)__double_as_longlong(kd[i]);
+        const bool has = b != kKnnEmpty;
+        const uint64_t key = has ? b - kKnnBias : kEmptyKey;
+        const int32_t pos = has ? inv[(uint32_t)key] : -1;
+        if (i == 0 && has) {
+            pos0 = pos;
+            key0 = key;
+        }
+        nb.pos[(int64_t)i * n + p] = pos;
+        nb.key[(int64_t)i * n + p] = key;
     }
-    corr[p] = kp[0] >= 0 ? kp[0] : -2;
-    corr_key[p] = kk[0];
+    corr[p] = pos0;
+    corr_key[p] = key0;
     nb.cur[p] = 0;
 }
 
@@ -3348,17 +3461,24 @@ void launch_cone_search(int dim, int cdim, const float4* pts, const Node* nodes,
                        (int64_t)(split->compact_frac * (double)nq));
 }
 
-void launch_leaves(const uint64_t* keys, int64_t n, const Cloud* cl, int ncl, int bucket, uint32_t* mark,
-                   uint32_t* idx, uint32_t* tmp, int32_t* start, int32_t* of, int32_t* cloud_lf, hipStream_t s)
+void launch_leaves(const uint64_t* keys, int64_t n, Cloud* cl, int ncl, int bucket, uint32_t* mark,
+                   uint32_t* idx, uint32_t* tmp, int32_t* start, int32_t* of, int32_t* cloud_lf, const Params* P,
+                   Counters* C, bool layout, hipStream_t s)
 {
     if (n > 0) {
-        hipLaunchKernelGGL(k_leaf_marks, dim3(nblocks(n)), dim3(kBlock), 0, s, keys, n, cl, ncl, bucket, mark);
+        hipLaunchKernelGGL(k_leaf_marks, dim3(nblocks(n)), dim3(kBlock), 0, s, keys, n, (const Cloud*)cl, ncl,
+                           bucket, mark);
         exclusive_scan_u32(mark, idx, n, tmp, s);
         hipLaunchKernelGGL(k_leaf_fill, dim3(nblocks(n)), dim3(kBlock), 0, s, (const uint32_t*)mark,
                            (const uint32_t*)idx, n, start, of);
     }
-    hipLaunchKernelGGL(k_cloud_leaves, dim3(nblocks(ncl + 1)), dim3(kBlock), 0, s, cl, ncl, (const uint32_t*)mark,
-                       (const uint32_t*)idx, n, cloud_lf);
+    hipLaunchKernelGGL(k_cloud_leaves, dim3(1), dim3(1024), 0, s, cl, ncl, (const uint32_t*)mark,
+                       (const uint32_t*)idx, n, cloud_lf, P, C, layout ? 1 : 0);
+}
+
+void launch_zero_u32(uint32_t* p, int64_t n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_zero_u32, dim3(nblocks(n)), dim3(kBlock), 0, s, p, n);
 }
 
 void launch_node_min_ids(const float4* pts, const Cloud* cl, int ncl, int64_t nleaves, LeafTable lt, Node* nodes,
@@ -3436,18 +3556,18 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsig
 }
 
 void launch_knn_first(int dim, int64_t n, const float4* pts, const Node* nodes, const Cloud* cl, int ncl,
-                      int bucket, LeafTable lt, int32_t* corr, uint64_t* corr_key, const NbrList& nb, Counters* C, bool collect,
-                      hipStream_t s)
+                      int bucket, LeafTable lt, const int32_t* inv, int32_t* corr, uint64_t* corr_key,
+                      const NbrList& nb, Counters* C, bool collect, hipStream_t s)
 {
     if (n <= 0) return;
     const unsigned P = (unsigned)((n + 63) / 64);
     const int cs = collect ? 1 : 0;
     if (dim == 3)
-        hipLaunchKernelGGL(k_knn_first<3>, dim3(P), dim3(64), 0, s, n, pts, nodes, cl, ncl, bucket, lt, corr, corr_key,
-                           nb, C, cs);
+        hipLaunchKernelGGL(k_knn_first<3>, dim3(P), dim3(64), 0, s, n, pts, nodes, cl, ncl, bucket, lt, inv, corr,
+                           corr_key, nb, C, cs);
     else
-        hipLaunchKernelGGL(k_knn_first<2>, dim3(P), dim3(64), 0, s, n, pts, nodes, cl, ncl, bucket, lt, corr, corr_key,
-                           nb, C, cs);
+        hipLaunchKernelGGL(k_knn_first<2>, dim3(P), dim3(64), 0, s, n, pts, nodes, cl, ncl, bucket, lt, inv, corr,
+                           corr_key, nb, C, cs);
 }
 
 void launch_dup_fill(int dim, int64_t n, const int32_t* label, const float4* pts, const int32_t* inv,
@@ -3471,7 +3591,7 @@ void launch_round_init(int64_t n, int32_t* label, int32_t* label2, int32_t* fixe
                        uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, Counters* C, uint32_t target,
                        uint32_t max_rounds, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_round_init, dim3(nblocks(std::max<int64_t>(n, 1))), dim3(kBlock), 0, s, n, label,
+    hipLaunchKernelGGL(k_round_init, dim3(nblocks(std::max<int64_t>(n, 6 * kWorkSlots))), dim3(kBlock), 0, s, n, label,
                        label2, fixed, eu, corr, corr_key, comp1, comp2, C, target, max_rounds);
 }
 
@@ -3480,9 +3600,14 @@ void launch_round_begin(Counters* C, hipStream_t s)
     hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(1), 0, s, C);
 }
 
-void launch_round_end(Counters* C, bool grouped, hipStream_t s)
+void launch_round_end(Counters* C, bool grouped, hipStream_t s, unsigned long long* stamps, uint32_t cap)
 {
-    hipLaunchKernelGGL(k_round_end, dim3(1), dim3(1), 0, s, C, grouped ? 1 : 0);
+    hipLaunchKernelGGL(k_round_end, dim3(1), dim3(1), 0, s, C, grouped ? 1 : 0, stamps, cap);
+}
+
+void launch_stamp(unsigned long long* at, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, s, at);
 }
 
 void launch_flag_research(int K, int64_t n, const uint64_t* cand_key, int32_t* list, Counters* C,

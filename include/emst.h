@@ -83,6 +83,14 @@ typedef struct {
                                learned seed predicts each component's edge length per round */
     int32_t log_samples;    /* 1 (round_search 1): log the learned seed's training pairs during
                                runs (emst_get_radius_samples) */
+    int32_t one_launch;     /* 1 (round_search 1, untraced, no seed model or sample log): emst_run
+                               issues the whole run -- bounding boxes, rescale, sort, hierarchy,
+                               neighbour lists and the first 16 Boruvka rounds -- as one HIP graph
+                               launch with one host synchronisation at its end (captured on first
+                               use, replayed while the point layout is the same); rounds past those
+                               continue as usual.  The results are those of the default run.  For
+                               batches of many small clouds and for callers that replay the same
+                               shapes; not for emst_step. */
 } emst_options;
 
 /* One row of the per-round trace: the columns of the reference's stats file
@@ -239,6 +247,14 @@ int emst_get_cell_grid(emst_ctx* ctx, int32_t level_radius, int32_t* ext_dual,
  * otherwise).
  * n_times: capacity in, count out. */
 int emst_get_times(emst_ctx* ctx, float* times, int32_t* n_times);
+
+/* Device time from the start of the last one_launch run (options.one_launch)
+ * to the end of each of its captured rounds, in milliseconds: round r's entry
+ * is ms[r - 1], so a cloud that completed in round r (emst_get_cloud_results)
+ * had its forest that long after the run began.  Other runs leave no entries.
+ * ms NULL: *n = the count; otherwise *n is the capacity in, count out.  No
+ * counterpart in the reference. */
+int emst_get_round_ends(emst_ctx* ctx, float* ms, int32_t* n);
 
 /* Counters of the last full cone search (the work behind its bytes):
  * [0] packets (64-point waves), [1] hierarchy nodes tested, [2] leaf cells

@@ -12,10 +12,16 @@ C4  10^7 2D power-law points, seed 4: every form (cone search plain and with
     every point's nearest neighbour (scipy k-d tree, independent of both),
     and Kruskal over the tree plus the 3 nearest neighbours of every point
     picks exactly the tree (no neighbour edge can replace a tree edge).
+    C4 also equals the oracle's restatement of the reference ring search
+    (mode 1) through committed digests of its edges and d2 bits
+    (tests/golden/oracle_digests.json, made by make_oracle_digests.py: the
+    oracle takes an hour on it).
 C5  the 4096-cloud batch (seed 5): every cloud gets n-1 edges, every 64th
     cloud equals its own single-cloud run, every cloud of at most 5000
     points equals the oracle's complete-graph MST.
 """
+import hashlib
+import json
 import os
 from concurrent.futures import ThreadPoolExecutor
 
@@ -25,6 +31,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 WTOL = 1e-9
+DIGESTS = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_digests.json")))
 
 
 def gpu_tree(pkg, pts, model=None, **kw):
@@ -161,6 +168,15 @@ def test_c3_c4_full_size(pkg, oracle_mod, name, kind, dim, seed):
         assert t["w"] == ref["w"], k
     rescaled = check_tree(oracle_mod, pts, ref)
     check_neighbours(oracle_mod, rescaled, ref)
+    dg = [v for v in DIGESTS.values() if (v["kind"], v["n"], v["dim"], v["seed"]) == (kind, n, dim, seed)]
+    if name == "C4":
+        assert len(dg) == 1
+    for want in dg:
+        e = edges(ref).astype("<i4")
+        assert len(e) == want["edges"]
+        assert hashlib.sha256(e.tobytes()).hexdigest() == want["edges_sha256"]
+        assert hashlib.sha256(ref["d"].astype("<f4").tobytes()).hexdigest() == want["d2_sha256"]
+        assert abs(ref["w"] - want["weight"]) <= WTOL * want["weight"]
 
 
 def test_c5_batch_4096(pkg, oracle_mod):
