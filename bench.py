@@ -233,7 +233,7 @@ def main():
         kern = {"k_knn_first": {"alg_bytes": 101 * args.n, "ms": t[5]},
                 "k_comp_search": {"alg_bytes": 44 * st["comp_queries"], "ms": t[4]}}
         work = {k: st[k] for k in ("comp_queries", "comp_nodes", "comp_leaves", "comp_pairs", "knn_nodes",
-                                   "knn_leaves", "knn_pairs")}
+                                   "knn_leaves", "knn_pairs", "comp_walks", "knn_walks")}
     for name, kd in kern.items():
         kd["achieved_gbs"] = round(kd["alg_bytes"] / (kd["ms"] * 1e-3) / 1e9, 1) if kd["ms"] > 0 else 0.0
         kd["ms"] = round(kd["ms"], 3)

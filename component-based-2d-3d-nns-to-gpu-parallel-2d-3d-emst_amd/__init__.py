@@ -395,12 +395,13 @@ def mlp_eval(model, X, device=0):
 
 def _stats_names():
     return ["packets", "nodes", "leaves", "leaf_points", "pairs", "group_points", "grouped", "cycles_cut", "resumed",
-            "comp_queries", "comp_nodes", "comp_leaves", "comp_pairs", "knn_nodes", "knn_leaves", "knn_pairs"]
+            "comp_queries", "comp_nodes", "comp_leaves", "comp_pairs", "knn_nodes", "knn_leaves", "knn_pairs",
+            "comp_walks", "knn_walks"]
 
 
 def search_stats(calc):
-    t = (ctypes.c_int64 * 16)()
-    k = ctypes.c_int32(16)
+    t = (ctypes.c_int64 * 18)()
+    k = ctypes.c_int32(18)
     calc._chk(calc._L.emst_get_search_stats(calc._c, t, ctypes.byref(k)), "stats")
     return dict(zip(_stats_names(), list(t)[: k.value]))
 

@@ -95,7 +95,7 @@ struct emst_ctx {
     uint32_t zero_or_empty = 0;
     std::vector<emst_round_trace> trace;
     float times[4] = {0, 0, 0, 0};
-    int64_t stats[16] = {};
+    int64_t stats[18] = {};
     uint32_t resumed_h = 0;           // queries resumed after the first search pass
     float seed_scale = 3.0f;          // plain seed: multiple of the leaf cell's mean spacing
     hipEvent_t ev[14] = {};             // 8..11: around the component search's dispatches, 12..13:
@@ -757,7 +757,7 @@ void absorb_counters(emst_ctx* c, const Counters& h)
     c->rounds = (int)h.round;
     c->stats[7] = (int64_t)h.cycles_cut;
     c->stats[9] = (int64_t)h.comp_queries;
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < kWorkKinds; ++k) {
         unsigned long long t = 0;
         for (int j = 0; j < kWorkSlots; ++j) t += h.work[k][j];
         c->stats[10 + k] = (int64_t)t;
@@ -799,6 +799,7 @@ int round_once(emst_ctx* c, int32_t* finished)
         LAUNCHCHK(c);
         searched = h.zero_or_empty;
     }
+    const int64_t queries_before = c->stats[9];
     issue_round(c, tr);
     LAUNCHCHK(c);
     Counters h{};
@@ -808,6 +809,8 @@ int round_once(emst_ctx* c, int32_t* finished)
         c->comp_search_ms += elapsed(c->ev[8], c->ev[9]) + elapsed(c->ev[10], c->ev[11]);
     if (h.done == 2u && (rc = resume_flatten(c, &h))) return rc;
     absorb_counters(c, h);
+    // the component search: queries of this round's two dispatches
+    if (c->opt.round_search == 1) searched = c->stats[9] - queries_before;
     c->groups_settled = h.dup_settled != 0u;
     if (tr) {
         emst_round_trace row{};
@@ -1706,7 +1709,7 @@ int emst_get_radius_samples(emst_ctx* c, float* feat, float* target, int64_t* m)
 int emst_get_search_stats(emst_ctx* c, int64_t* stats, int32_t* n_stats)
 {
     if (!c || !stats || !n_stats) return EMST_E_ARG;
-    const int32_t k = std::min<int32_t>(*n_stats, 16);
+    const int32_t k = std::min<int32_t>(*n_stats, 18);
     for (int i = 0; i < k; ++i) stats[i] = c->stats[i];
     *n_stats = k;
     return EMST_OK;

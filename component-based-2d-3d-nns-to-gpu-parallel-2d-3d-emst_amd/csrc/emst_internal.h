@@ -22,7 +22,10 @@ struct Params {
 };
 
 constexpr int kWorkSlots = 64;
-enum { kWorkCompNodes, kWorkCompLeaves, kWorkCompPairs, kWorkKnnNodes, kWorkKnnLeaves, kWorkKnnPairs };
+enum {
+    kWorkCompNodes, kWorkCompLeaves, kWorkCompPairs, kWorkKnnNodes, kWorkKnnLeaves, kWorkKnnPairs,
+    kWorkCompWalks, kWorkKnnWalks, kWorkKinds
+};
 
 // Round counters, one device struct per context.
 struct Counters {
@@ -56,10 +59,12 @@ struct Counters {
     unsigned long long comp_queries;    // queries over the component search's dispatches, over the run
     // work of the component search's dispatches over the run (node tests,
     // one per node a packet examines; leaf visits; point pairs evaluated),
-    // then the same for the neighbour lists at build (k_knn_first); each
+    // then the same for the neighbour lists at build (k_knn_first), then the
+    // passes of each that walked the tree per query because their gathering
+    // outgrew its lists or had no finite bound; each
     // spread over kWorkSlots words (a packet adds to slot blockIdx % kWorkSlots,
     // so that the adds do not queue on one address), summed by the host
-    unsigned long long work[6][kWorkSlots];
+    unsigned long long work[kWorkKinds][kWorkSlots];
     // a build issued without host checks (one launch) found a NaN or Inf
     // coordinate or a cloud with more leaves than its slots: the rounds do
     // nothing and the host redoes or refuses the run

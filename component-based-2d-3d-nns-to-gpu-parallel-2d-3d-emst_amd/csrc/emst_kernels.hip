@@ -63,6 +63,28 @@ __device__ __forceinline__ int find_cloud(const Cloud* cl, int ncl, int64_t p)
 }
 
 // largest c with cl[c].lbase <= g (the cloud holding global leaf g)
+// This file is built with -fno-honor-nans (the kernels then skip the
+// canonicalising moves the compiler puts before fminf / fmaxf inputs): every
+// coordinate is checked finite before any search runs, and the tests for NaN
+// and infinity look at the bits, passed through an empty asm statement so
+// that the compiler cannot turn them back into floating-point class tests
+// (which it would then fold under the no-NaN assumption).
+__device__ __forceinline__ uint32_t opaque_bits(float x)
+{
+    uint32_t b = __float_as_uint(x);
+    asm("" : "+v"(b));
+    return b;
+}
+__device__ __forceinline__ bool finite_bits(float x)
+{
+    return (opaque_bits(x) & 0x7f800000u) != 0x7f800000u;
+}
+// x > 0 and finite
+__device__ __forceinline__ bool positive_finite(float x)
+{
+    return opaque_bits(x) - 1u < 0x7f7fffffu;
+}
+
 __device__ __forceinline__ int find_cloud_leaf(const Cloud* cl, int ncl, int64_t g)
 {
     int lo = 0, hi = ncl - 1;
@@ -151,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_bbox(const float* __restrict__ xyz, 
                 const float* q = xyz + i * D;
                 for (int k = 0; k < D; ++k) {
                     const float v = q[k];
-                    bad |= !__builtin_isfinite(v);
+                    bad |= !finite_bits(v);
                     const uint32_t o = f2ord(v);
                     lo[k] = min(lo[k], o);
                     hi[k] = max(hi[k], o);
@@ -1056,6 +1078,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DC == 3 ? 3 
             // not this traversal's query
         } else if (seed_r) {
             r = seed_r[p];
+            if (!positive_finite(r)) r = 1e-6f;
         } else {
             const int64_t lf = lt.of[p];
             const Node own = tree[L + (lf - lfirst)];
@@ -1067,7 +1090,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DC == 3 ? 3 
                 ext = fmaxf(ext, e);
             }
             float sp = vol > 0.f ? powf(vol / cnt, 1.f / D) : ext / cnt;
-            if (!(sp > 0.f)) sp = 1e-6f;
+            if (!positive_finite(sp)) sp = 1e-6f;
             r = seed_scale * sp;
         }
         float r2 = r * r;
@@ -1516,7 +1539,7 @@ __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __res
         C->comp_queries = 0;
         C->target = target; C->max_rounds = max_rounds;
     }
-    if (i < 6 * kWorkSlots) C->work[i / kWorkSlots][i % kWorkSlots] = 0;
+    if (i < kWorkKinds * kWorkSlots) C->work[i / kWorkSlots][i % kWorkSlots] = 0;
     if (i >= n) return;
     label[i] = (int32_t)i;          // K_EMST_initDisjointSet (EMSTOperators.h:330-343)
     label2[i] = (int32_t)i;
@@ -1937,8 +1960,24 @@ __global__ __launch_bounds__(kBlock) void k_nlab_tree(const int32_t* __restrict_
     if (j >= cl[c].L) return;               // past the cloud's slots (one-launch capacity)
     int64_t s0, s1;
     leaf_range(lt, cl[c].lfirst, cl[c].nl, cl[c].pos + cl[c].n, j, s0, s1);
+    // the leaf's common label (kNlabMixed when two differ), its loads
+    // independent of each other (a serial combine waited on each load: the
+    // kernel took 0.12 ms per round at 10^7 points)
     int32_t lab = kNlabEmpty;
-    for (int64_t s = s0; s < s1; ++s) lab = LabelOp::combine(lab, label[s]);
+    if (s1 > s0) {
+        lab = label[s0];
+        bool diff = false;
+        int64_t s = s0 + 1;
+        for (; s + 8 <= s1; s += 8) {
+            int32_t v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = label[s + k];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) diff |= v[k] != lab;
+        }
+        for (; s < s1; ++s) diff |= label[s] != lab;
+        if (diff) lab = kNlabMixed;
+    }
     Node* base = nodes + 2 * cl[c].lbase;
     auto store = [&](int64_t i, int32_t v) { base[i].pad[1] = __int_as_float(v); };
     store(cl[c].L + j, lab);
@@ -1961,7 +2000,14 @@ __global__ __launch_bounds__(kBlock) void k_nmin_tree(const float4* __restrict__
     int64_t s0, s1;
     leaf_range(lt, cl[c].lfirst, cl[c].nl, cl[c].pos + cl[c].n, j, s0, s1);
     uint32_t m = 0xffffffffu;
-    for (int64_t s = s0; s < s1; ++s) m = min(m, __float_as_uint(pts[s].w));
+    int64_t s = s0;
+    for (; s + 4 <= s1; s += 4) {
+        uint32_t v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = __float_as_uint(pts[s + j].w);
+        m = min(m, min(min(v[0], v[1]), min(v[2], v[3])));
+    }
+    for (; s < s1; ++s) m = min(m, __float_as_uint(pts[s].w));
     Node* base = nodes + 2 * cl[c].lbase;
     auto store = [&](int64_t i, uint32_t v) { base[i].pad[0] = __uint_as_float(v); };
     store(cl[c].L + j, m);
@@ -2231,6 +2277,10 @@ __global__ __launch_bounds__(kBlock) void k_comp_samples(const int32_t* __restri
 // the work list, and gathered leaves as (box d2 bits << 32) | leaf
 constexpr int kGatherWork = 512;
 constexpr int kGatherLeaves = 128;
+// the neighbour lists' gatherings (k_knn_first): their radius is the packet's
+// largest kKnn-th distance after its own leaves, so they reach further
+constexpr int kKnnGatherWork = 1024;
+constexpr int kKnnGatherLeaves = 256;
 
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m)
 {
@@ -2315,7 +2365,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
     const uint32_t pid = __float_as_uint(me.w);
     bool bounded = false, left_open = false;
     int32_t lbnd = 0;
-    uint32_t st_nodes = 0, st_leaves = 0;       // work counters of this packet
+    uint32_t st_nodes = 0, st_leaves = 0, st_walks = 0;   // work counters of this packet
     uint64_t st_pairs = 0;
     uint64_t pending = __ballot(active);
     while (pending) {
@@ -2340,7 +2390,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
                 ext = fmaxf(ext, e);
             }
             float sp = vol > 0.f ? powf(vol / cnt, 1.f / D) : ext / cnt;
-            if (!(sp > 0.f)) sp = 1e-6f;
+            if (!positive_finite(sp)) sp = 1e-6f;
             const bool has_lb = (uint32_t)(prev >> 32) != 0xffffffffu;
             const uint32_t dbits = (uint32_t)(prev >> 32);
             if (has_lb) inner = (int32_t)dbits - 1;
@@ -2352,7 +2402,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
                 r = seed_r[p];
                 if (has_lb) r = fmaxf(r, sqrtf(__uint_as_float(dbits)));
             }
-            if (!(r > 0.f)) r = 1e-6f;
+            if (!positive_finite(r)) r = 1e-6f;
         }
         float r2 = r * r;
         // a launch that resumes at pass_lo: the state the earlier passes left
@@ -2553,6 +2603,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
                 }
             }
             if (!done_by_gather) {
+                ++st_walks;
                 if (lane == 0) {
                     stk_id[0] = 1;
                     stk_ver[0] = -1;
@@ -2645,6 +2696,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
         atomicAdd(&C->work[kWorkCompNodes][slot], (unsigned long long)st_nodes);
         atomicAdd(&C->work[kWorkCompLeaves][slot], (unsigned long long)st_leaves);
         atomicAdd(&C->work[kWorkCompPairs][slot], (unsigned long long)st_pairs);
+        atomicAdd(&C->work[kWorkCompWalks][slot], (unsigned long long)st_walks);
     }
     if (active && !left_open) {
         const uint64_t k = best[lane];
@@ -2702,8 +2754,8 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
     __shared__ int32_t stk_id[64];
     __shared__ int32_t stk_ver[64];
     __shared__ unsigned long long stk_mask[64];
-    __shared__ int32_t work[kGatherWork];
-    __shared__ unsigned long long gathered[kGatherLeaves];
+    __shared__ int32_t work[kKnnGatherWork];
+    __shared__ unsigned long long gathered[kKnnGatherLeaves];
     __shared__ float4 gbox[16];             // per 8-lane group: lo (xyz), hi (xyz)
     if (C->done) return;                    // a build whose input was refused on the device
     const int lane = __lane_id();
@@ -2725,7 +2777,7 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
     };
     // the kKnn-th d2 bits (an empty list: above every finite d2)
     auto bound = [&]() { return (uint32_t)(last() >> 32) - kKnnBiasHi; };
-    uint32_t st_nodes = 0, st_leaves = 0, st_steps = 0;     // work counters of this packet
+    uint32_t st_nodes = 0, st_leaves = 0, st_steps = 0, st_walks = 0;   // work counters of this packet
     uint64_t pending = __ballot(active);
     while (pending) {
         const int cl = __builtin_amdgcn_readfirstlane(__shfl(my_cl, (int)__builtin_ctzll(pending)));
@@ -2850,7 +2902,7 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
             }
             const uint64_t mi = __ballot(pass && !leaf), ml = __ballot(pass && leaf);
             const int ni = __popcll(mi), nl = __popcll(ml);
-            if (tail + 2 * ni > kGatherWork || ngl + nl > kGatherLeaves) {
+            if (tail + 2 * ni > kKnnGatherWork || ngl + nl > kKnnGatherLeaves) {
                 over = true;
                 break;
             }
@@ -2883,6 +2935,7 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
         }
         // the per-lane walk: from the root, nearest child first (from the
         // packet's centroid), the own leaves skipped
+        ++st_walks;
         float cx = mine ? px : 0.f, cy = mine ? py : 0.f, cz = mine ? pz : 0.f;
         float na = mine ? 1.f : 0.f;
         for (int off = 32; off > 0; off >>= 1) {
@@ -2968,6 +3021,7 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
         atomicAdd(&C->work[kWorkKnnNodes][slot], (unsigned long long)st_nodes);
         atomicAdd(&C->work[kWorkKnnLeaves][slot], (unsigned long long)st_leaves);
         atomicAdd(&C->work[kWorkKnnPairs][slot], 64ull * st_steps);
+        atomicAdd(&C->work[kWorkKnnWalks][slot], (unsigned long long)st_walks);
     }
     if (!active) return;
     int32_t pos0 = -2;
@@ -3591,7 +3645,7 @@ void launch_round_init(int64_t n, int32_t* label, int32_t* label2, int32_t* fixe
                        uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, Counters* C, uint32_t target,
                        uint32_t max_rounds, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_round_init, dim3(nblocks(std::max<int64_t>(n, 6 * kWorkSlots))), dim3(kBlock), 0, s, n, label,
+    hipLaunchKernelGGL(k_round_init, dim3(nblocks(std::max<int64_t>(n, kWorkKinds * kWorkSlots))), dim3(kBlock), 0, s, n, label,
                        label2, fixed, eu, corr, corr_key, comp1, comp2, C, target, max_rounds);
 }
 

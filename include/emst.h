@@ -105,7 +105,9 @@ typedef struct {
     float time_find_next_closest;   /* cone search + per-point candidate min */
     float time_find_min_pair;       /* per-component minimum */
     float time_connect_union;       /* hooking */
-    int64_t searched;               /* points whose cone search ran this round */
+    int64_t searched;               /* points whose cone search ran this round; with
+                                       round_search 1, the queries of the round's
+                                       component search (first and resumed dispatch) */
 } emst_round_trace;
 
 typedef struct emst_ctx emst_ctx;
@@ -271,8 +273,11 @@ int emst_get_round_ends(emst_ctx* ctx, float* ms, int32_t* n);
  * 64-query packet examined), [11] leaf cells they entered, [12] point pairs
  * they evaluated (leaf points x queries that needed the leaf), [13]-[15] the
  * same three for the neighbour lists of the component search's build
- * (pairs there: leaf points examined x 64 lanes); [10]-[15] are counted in
- * traced runs (options.trace) only, 0 otherwise.
+ * (pairs there: leaf points examined x 64 lanes), [16] passes of the
+ * component search's packets that walked the hierarchy per query (an
+ * unbounded pass, or a gathering of candidate leaves that outgrew its lists),
+ * [17] packets of the neighbour lists' build that did; [10]-[17] are counted
+ * in traced runs (options.trace) only, 0 otherwise.
  * No counterpart in the reference (its per-phase timers are in
  * SolutionEMSTOperators.cu:166-373).  n_stats: capacity in, count out. */
 int emst_get_search_stats(emst_ctx* ctx, int64_t* stats, int32_t* n_stats);
