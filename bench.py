@@ -240,19 +240,24 @@ def main():
         tr = [profiled(name, tag, f) for f in ("traffic_bytes_per_launch", "valu_busy")] if tag else [None, None]
         kd["traffic"] = int(tr[0]) if tr[0] is not None else None
         kd["valu_busy"] = tr[1]
+        # from the committed SQ counters of the same launch: VALU busy most of
+        # the time means instruction issue; idle mostly, waits on dependent loads
+        kd["limiter"] = None if tr[1] is None else ("VALU issue" if tr[1] >= 0.5 else "load latency")
     tot_bytes = sum(kd["alg_bytes"] for kd in kern.values())
     tot_ms = sum(kd["ms"] for kd in kern.values())
     achieved = tot_bytes / (tot_ms * 1e-3) / 1e9 if tot_ms > 0 else 0.0
     traffics = [kd["traffic"] for kd in kern.values()]
-    busy = [kd["valu_busy"] for kd in kern.values() if kd["valu_busy"] is not None]
+    timed_busy = [(kd["valu_busy"], kd["ms"]) for kd in kern.values() if kd["valu_busy"] is not None]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                 "frac": round(achieved / 8000.0, 4),
                 "traffic": sum(traffics) if all(x is not None for x in traffics) else None,
                 "kernel": "+".join(kern), "kernel_ms": round(tot_ms, 3), "kernels": kern,
                 # the HBM roofline is the one the survey prices these kernels
-                # on; the counters put their limit elsewhere: instruction issue
-                "limiter": "VALU issue" if busy else None,
-                "valu_busy": round(sum(busy) / len(busy), 3) if busy else None,
+                # on; the counters put their limits elsewhere (per kernel)
+                "limiter": "; ".join(f"{k}: {kd['limiter']}" for k, kd in kern.items() if kd.get("limiter"))
+                           or None,
+                "valu_busy": round(sum(b * m for b, m in timed_busy) / sum(m for _, m in timed_busy), 3)
+                             if timed_busy and sum(m for _, m in timed_busy) > 0 else None,
                 "work": work}
 
     extra = {}
