@@ -134,6 +134,7 @@ struct emst_ctx {
     DevBuf<uint8_t> cy_onc, cy_cut;
     DevBuf<unsigned long long> visit;  // arrival state of the bottom-up node passes
     DevBuf<int32_t> list2;             // queries open after pass 0 (per-round component search)
+    DevBuf<uint64_t> defer;            // queries handed to the per-query pass (per-round component search)
     DevBuf<int32_t> knn_pos;           // neighbour lists of the component search (NbrList)
     DevBuf<uint64_t> knn_key;
     DevBuf<uint8_t> knn_cur;
@@ -270,6 +271,7 @@ int alloc_all(emst_ctx* c)
     ENS(dup_gslot, n);
     if (comp) {
         ENS(list2, n);
+        ENS(defer, n);
         ENS(knn_pos, (size_t)kKnn * n);
         ENS(knn_key, (size_t)kKnn * n);
         ENS(knn_cur, n);
@@ -462,6 +464,7 @@ SplitScratch split_scratch(emst_ctx* c)
     sc.tmp = c->open_tmp.p;
     sc.list = c->list.p;
     sc.list2 = c->list2.p;
+    sc.defer = c->defer.p;
     sc.after = 0;
     sc.compact_frac = 0.10;
     return sc;
@@ -845,7 +848,7 @@ std::vector<uintptr_t> graph_key(emst_ctx* c)
             (uintptr_t)c->opt.round_search, (uintptr_t)comp_features_on(c), (uintptr_t)c->learned,
             (uintptr_t)c->model.p, (uintptr_t)c->csize.p, (uintptr_t)c->cbox.p, (uintptr_t)c->feat.p, (uintptr_t)c->ls.p,
             (uintptr_t)c->pred.p, (uintptr_t)c->seed.p, (uintptr_t)c->samples.p, (uintptr_t)c->nq_keep.p,
-            (uintptr_t)c->opt.log_samples, (uintptr_t)c->visit.p, (uintptr_t)c->list2.p, (uintptr_t)c->knn_pos.p,
+            (uintptr_t)c->opt.log_samples, (uintptr_t)c->visit.p, (uintptr_t)c->list2.p, (uintptr_t)c->defer.p, (uintptr_t)c->knn_pos.p,
             (uintptr_t)c->knn_key.p, (uintptr_t)c->knn_cur.p, (uintptr_t)c->knn_built, (uintptr_t)c->open_mask.p, (uintptr_t)c->open_cnt.p,
             (uintptr_t)c->open_tmp.p, (uintptr_t)c->list.p, (uintptr_t)c->nodes.p, (uintptr_t)c->dcl.p,
             (uintptr_t)c->nleaves, (uintptr_t)c->lstart.p, (uintptr_t)c->leaf_of.p, (uintptr_t)c->bucket,

@@ -69,6 +69,9 @@ struct Counters {
     // coordinate or a cloud with more leaves than its slots: the rounds do
     // nothing and the host redoes or refuses the run
     uint32_t invalid;
+    // queries of this round's component search handed to the per-query pass
+    // (k_comp_search_deferred), entries of the defer list
+    uint32_t deferred;
 };
 
 // Coincident-point groups.  Points with identical rescaled coordinates form a
@@ -208,6 +211,7 @@ struct SplitScratch {
     uint32_t* tmp;          // scan_tmp_words(P + 1)
     int32_t* list;          // n
     int32_t* list2;         // n (component search: the queries left open after pass 0)
+    uint64_t* defer;        // n (component search: queries handed to the per-query pass)
     int after;              // the first launch runs passes 0..after
     double compact_frac;    // resume from a compacted list when at most this share of the queries is open
 };

@@ -504,6 +504,18 @@ __device__ __forceinline__ bool same_xyz(const float4& a, const float4& b)
 }
 
 // append the lanes holding `take` to list, one atomic per wave
+__device__ __forceinline__ void wave_append_u64(bool take, uint64_t v, uint64_t* list, uint32_t* counter)
+{
+    const uint64_t m = __ballot(take);
+    if (m == 0ull) return;
+    const int leader = __builtin_ctzll(m);
+    uint32_t base = 0;
+    if (__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, leader);
+    if (take) list[base + (uint32_t)__popcll(m & lanemask_lt())] = v;
+}
+
+// append the lanes holding `take` to list, one atomic per wave
 __device__ __forceinline__ void wave_append(bool take, int32_t v, int32_t* list, uint32_t* counter)
 {
     const uint64_t m = __ballot(take);
@@ -1537,6 +1549,7 @@ __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __res
         C->packets = C->nodes = C->leaves = C->leaf_points = C->pairs = 0;
         C->round = 0; C->done = C->invalid ? 1u : 0u; C->edges_prev = 0; C->dup_settled = 0; C->nsamples = 0;
         C->comp_queries = 0;
+        C->deferred = 0;
         C->target = target; C->max_rounds = max_rounds;
     }
     if (i < kWorkKinds * kWorkSlots) C->work[i / kWorkSlots][i % kWorkSlots] = 0;
@@ -1565,6 +1578,7 @@ __global__ void k_round_begin(Counters* C)
     C->round += 1;
     C->edges_prev = C->edges;
     C->nroots = 0;
+    C->deferred = 0;
 }
 
 // A round that added no edge leaves a forest with no outgoing candidate: done.
@@ -2277,6 +2291,10 @@ __global__ __launch_bounds__(kBlock) void k_comp_samples(const int32_t* __restri
 // the work list, and gathered leaves as (box d2 bits << 32) | leaf
 constexpr int kGatherWork = 512;
 constexpr int kGatherLeaves = 128;
+// dispatches of the component search with at most this many queries (1024
+// packets, one per SIMD of the chip) hand their overflowing packets' queries
+// to the per-query pass (k_comp_search_deferred)
+constexpr int64_t kDeferQueries = 65536;
 // the neighbour lists' gatherings (k_knn_first): their radius is the packet's
 // largest kKnn-th distance after its own leaves, so they reach further
 constexpr int kKnnGatherWork = 1024;
@@ -2335,7 +2353,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
     const uint32_t* __restrict__ nq_dev, const int32_t* __restrict__ label, const int32_t* __restrict__ inv,
     float seed_scale, int32_t* __restrict__ corr, uint64_t* __restrict__ corr_key,
     uint64_t* __restrict__ comp1, int pass_lo, int pass_hi,
-    uint64_t* __restrict__ open_out, const float* __restrict__ seed_r, Counters* C, int collect)
+    uint64_t* __restrict__ open_out, const float* __restrict__ seed_r, uint64_t* __restrict__ defer, Counters* C,
+    int collect)
 {
     // the leaf size: a compile-time constant for the default sizes
     const int bucket = BK ? BK : bucket_rt;
@@ -2363,7 +2382,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
     best[lane] = kEmptyKey;
     const int my_cl = active ? find_cloud(clouds, ncl, p) : -1;
     const uint32_t pid = __float_as_uint(me.w);
-    bool bounded = false, left_open = false;
+    bool bounded = false, left_open = false, deferred = false;
     int32_t lbnd = 0;
     uint32_t st_nodes = 0, st_leaves = 0, st_walks = 0;   // work counters of this packet
     uint64_t st_pairs = 0;
@@ -2602,7 +2621,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
                     }
                 }
             }
-            if (!done_by_gather) {
+            if (!done_by_gather && defer != nullptr && B < 0x7f800000u && nq <= kDeferQueries) {
+                // a gathering that outgrew its lists, in a dispatch of fewer
+                // packets than the chip has SIMDs: the dispatch takes as long
+                // as its slowest packet, and this packet's walk would visit
+                // the union of its queries' neighbourhoods in one wave; the
+                // open queries go to the per-query pass instead, one wave
+                // each, from this pass on (no leaf has been visited in it
+                // yet, so their bests are empty)
+                ++st_walks;
+                wave_append_u64(open, ((uint64_t)(uint32_t)pass << 32) | (uint32_t)p, defer, &C->deferred);
+                if (open) {
+                    open = false;
+                    deferred = true;
+                }
+            } else if (!done_by_gather) {
                 ++st_walks;
                 if (lane == 0) {
                     stk_id[0] = 1;
@@ -2698,7 +2731,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
         atomicAdd(&C->work[kWorkCompPairs][slot], (unsigned long long)st_pairs);
         atomicAdd(&C->work[kWorkCompWalks][slot], (unsigned long long)st_walks);
     }
-    if (active && !left_open) {
+    if (active && !left_open && !deferred) {
         const uint64_t k = best[lane];
         if (bounded) {
             corr[p] = -1;
@@ -2707,6 +2740,266 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
             corr[p] = k == kEmptyKey ? -2 : inv[(uint32_t)k];
             corr_key[p] = k;
         }
+    }
+}
+
+// The per-query pass of the component search: the queries a packet handed
+// over (defer: position | pass << 32, count in C->deferred), one wave per
+// query, from the pass it reached to the last.  The same search as
+// k_comp_search's, for one query: the wave gathers the candidate leaves
+// around the query point by box tests of 64 nodes a step, then visits them
+// nearest first with one leaf point per lane and a wave minimum of the keys;
+// a gathering that outgrows its lists walks the tree, nearest child first.
+constexpr int kDeferWaves = 4;          // waves per block
+constexpr int kDeferBlocks = 2048;      // a grid that fills the chip once; waves loop over the queries
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint64_t o = shfl_xor_u64(v, off);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+template <int D>
+__global__ __launch_bounds__(64 * kDeferWaves) void k_comp_search_deferred(
+    const float4* __restrict__ pts, const Node* __restrict__ nodes, const Cloud* __restrict__ clouds, int ncl,
+    LeafTable lt, const uint64_t* __restrict__ defer, const int32_t* __restrict__ label,
+    const int32_t* __restrict__ inv, float seed_scale, int32_t* __restrict__ corr,
+    uint64_t* __restrict__ corr_key, uint64_t* __restrict__ comp1, const float* __restrict__ seed_r, Counters* C,
+    int collect)
+{
+    if (C->done) return;
+    const uint32_t cnt = C->deferred;
+    __shared__ int32_t work_w[kDeferWaves][kGatherWork];
+    __shared__ unsigned long long gathered_w[kDeferWaves][kGatherLeaves];
+    __shared__ int32_t stk_w[kDeferWaves][64];
+    const int w = (int)threadIdx.x / 64;
+    const int lane = __lane_id();
+    int32_t* work = work_w[w];
+    unsigned long long* gathered = gathered_w[w];
+    int32_t* stk = stk_w[w];
+    uint32_t st_nodes = 0, st_leaves = 0;
+    uint64_t st_pairs = 0;
+    for (uint32_t qi = blockIdx.x * kDeferWaves + w; qi < cnt; qi += gridDim.x * kDeferWaves) {
+        const uint64_t e = defer[qi];
+        const int64_t p = (int64_t)(uint32_t)e;
+        const int pass0 = (int)(e >> 32);
+        const float4 me = pts[p];
+        const float px = me.x, py = me.y, pz = me.z;
+        const uint32_t pid = __float_as_uint(me.w);
+        const int32_t mylab = label[p];
+        const uint64_t prev = corr_key[p];
+        const int c = find_cloud(clouds, ncl, p);
+        const int64_t cend = clouds[c].pos + clouds[c].n;
+        const int64_t L = clouds[c].L, lfirst = clouds[c].lfirst, nl = clouds[c].nl;
+        const Node* __restrict__ tree = nodes + 2 * clouds[c].lbase;
+        // the seed and the state the earlier passes left, as in k_comp_search
+        int32_t inner = -1;
+        float r;
+        {
+            const int64_t lf = lt.of[p];
+            const Node own = tree[L + (lf - lfirst)];
+            const float cntl = (float)(lt.start[lf + 1] - lt.start[lf]);
+            float vol = 1.f, ext = 0.f;
+            for (int k = 0; k < D; ++k) {
+                const float ee = own.hi[k] - own.lo[k];
+                vol *= ee;
+                ext = fmaxf(ext, ee);
+            }
+            float sp = vol > 0.f ? powf(vol / cntl, 1.f / D) : ext / cntl;
+            if (!positive_finite(sp)) sp = 1e-6f;
+            const bool has_lb = (uint32_t)(prev >> 32) != 0xffffffffu;
+            const uint32_t dbits = (uint32_t)(prev >> 32);
+            if (has_lb) inner = (int32_t)dbits - 1;
+            r = seed_scale * sp;
+            if (has_lb) r = fmaxf(r, 2.f * sqrtf(__uint_as_float(dbits)));
+            if (seed_r != nullptr && C->round >= kLearnedFrom) {
+                r = seed_r[p];
+                if (has_lb) r = fmaxf(r, sqrtf(__uint_as_float(dbits)));
+            }
+            if (!positive_finite(r)) r = 1e-6f;
+        }
+        float r2 = r * r;
+        for (int k = 0; k < pass0; ++k) {
+            inner = max(inner, (int32_t)__float_as_uint(r2));
+            r2 *= 8.f;
+        }
+        bool open = true, bounded = false;
+        int32_t lbnd = 0;
+        uint64_t bk = kEmptyKey;
+        auto box_d2 = [&](const Node& x, float& far) {
+            float m[3], u[3];
+            const float lo[3] = {x.lo[0] - px, x.lo[1] - py, x.lo[2] - pz};
+            const float hi[3] = {x.hi[0] - px, x.hi[1] - py, x.hi[2] - pz};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                m[k] = fmaxf(fmaxf(lo[k], -hi[k]), 0.f);
+                u[k] = fmaxf(-lo[k], hi[k]);
+            }
+            far = D == 3 ? d2_of(u[0], u[1], u[2]) : d2_of(u[0], u[1]);
+            return D == 3 ? d2_of(m[0], m[1], m[2]) : d2_of(m[0], m[1]);
+        };
+        for (int pass = pass0; pass <= kMaxCappedPasses && open; ++pass) {
+            const bool last = pass == kMaxCappedPasses;
+            const int32_t cbb = key_bound(comp1[mylab]);
+            const int32_t cap = last || cbb != 0x7fffffff ? 0x7fffffff : (int32_t)__float_as_uint(r2);
+            if (cbb <= inner) {
+                open = false;
+                bounded = true;
+                lbnd = inner;
+                break;
+            }
+            const int32_t lim = min(cap, cbb);
+            int32_t bnd = min(key_bound(bk), lim);
+            // a node is needed when it holds another component, its nearest
+            // point is within the limit and may beat the best so far under
+            // (d2, id), and its farthest point lies beyond `inner`
+            auto needed = [&](const Node& x) {
+                const int32_t xl = __float_as_int(x.pad[1]);
+                if (xl == mylab || xl == kNlabEmpty) return false;
+                float far;
+                const uint32_t lbb = __float_as_uint(box_d2(x, far));
+                return (int32_t)lbb <= lim && (((uint64_t)lbb << 32) | __float_as_uint(x.pad[0])) < bk &&
+                       (int32_t)__float_as_uint(far) > inner;
+            };
+            // one leaf point per lane, the wave's minimum key
+            auto visit_leaf = [&](int node) {
+                int64_t s0, s1;
+                leaf_range(lt, lfirst, nl, cend, node - L, s0, s1);
+                const int cn = (int)(s1 - s0);
+                ++st_leaves;
+                st_pairs += (uint64_t)cn;
+                uint64_t key = kEmptyKey;
+                if (lane < cn) {
+                    const float4 v = pts[s0 + lane];
+                    const float dx = v.x - px, dy = v.y - py, dz = v.z - pz;
+                    const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
+                    const int32_t db = (int32_t)__float_as_uint(d2);
+                    if (label[s0 + lane] != mylab && db <= bnd && db > inner)
+                        key = ((uint64_t)__float_as_uint(d2) << 32) | __float_as_uint(v.w);
+                }
+                const uint64_t km = wave_min_u64(key);
+                if (km < bk) {
+                    bk = km;
+                    bnd = min(key_bound(bk), lim);
+                }
+            };
+            bool done_by_gather = false;
+            if ((uint32_t)bnd < 0x7f800000u) {
+                const uint32_t B = (uint32_t)bnd;
+                int head = 0, tail = gather_start(L, work), ngl = 0;
+                bool over = false;
+                while (head < tail) {
+                    const int take = min(64, tail - head);
+                    const int nd = lane < take ? work[head + lane] : 0;
+                    head += take;
+                    st_nodes += (uint32_t)take;
+                    bool pass_ = false, leaf = false;
+                    uint32_t bd = 0;
+                    if (lane < take) {
+                        const Node x = tree[nd];
+                        const int32_t xl = __float_as_int(x.pad[1]);
+                        float far;
+                        bd = __float_as_uint(box_d2(x, far));
+                        leaf = nd >= L;
+                        pass_ = xl != kNlabEmpty && xl != mylab && bd <= B && (int32_t)__float_as_uint(far) > inner;
+                    }
+                    const uint64_t mi = __ballot(pass_ && !leaf), ml = __ballot(pass_ && leaf);
+                    const int ni = __popcll(mi), nlf = __popcll(ml);
+                    if (tail + 2 * ni > kGatherWork || ngl + nlf > kGatherLeaves) {
+                        over = true;
+                        break;
+                    }
+                    const uint64_t lt_ = lanemask_lt();
+                    if (pass_ && !leaf) {
+                        const int rr = tail + 2 * __popcll(mi & lt_);
+                        work[rr] = 2 * nd;
+                        work[rr + 1] = 2 * nd + 1;
+                    }
+                    if (pass_ && leaf) gathered[ngl + __popcll(ml & lt_)] = ((uint64_t)bd << 32) | (uint32_t)nd;
+                    tail += 2 * ni;
+                    ngl += nlf;
+                    __builtin_amdgcn_wave_barrier();
+                }
+                if (!over) {
+                    done_by_gather = true;
+                    for (int c0 = 0; c0 < ngl; c0 += 64) {
+                        const int cn = min(64, ngl - c0);
+                        const uint64_t ge = wave_sort_u64(lane < cn ? gathered[c0 + lane] : kEmptyKey);
+                        for (int i = 0; i < cn; ++i) {
+                            const int node = __builtin_amdgcn_readlane((int)(uint32_t)ge, i);
+                            const Node x = tree[node];
+                            ++st_nodes;
+                            if (needed(x)) visit_leaf(node);
+                        }
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+            if (!done_by_gather) {
+                // the walk: a wave-uniform stack, nearest child popped first
+                int sp = 1;
+                if (lane == 0) stk[0] = 1;
+                __builtin_amdgcn_wave_barrier();
+                while (sp > 0) {
+                    --sp;
+                    const int node = __builtin_amdgcn_readfirstlane(stk[sp]);
+                    __builtin_amdgcn_wave_barrier();
+                    const Node x = tree[node];
+                    ++st_nodes;
+                    if (!needed(x)) continue;
+                    if (node >= L) {
+                        visit_leaf(node);
+                        continue;
+                    }
+                    const int first = 2 * node;
+                    const Node b0 = tree[first], b1 = tree[first + 1];
+                    float f0, f1;
+                    const float d0 = box_d2(b0, f0), d1 = box_d2(b1, f1);
+                    // pushed farther first; at equal distance the lower node pops first
+                    const bool swap = d1 >= d0;
+                    if (lane == 0) {
+                        stk[sp] = swap ? first + 1 : first;
+                        stk[sp + 1] = swap ? first : first + 1;
+                    }
+                    sp += 2;
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            // as in k_comp_search: exact when the best is within lim, else
+            // nothing lies within lim, and with lim = cbb the point stops on
+            // its lower bound
+            if (key_bound(bk) <= lim) {
+                open = false;
+                if (bk != kEmptyKey && lane == 0)
+                    atomic_min_if_lower(&comp1[mylab],
+                                        (bk & 0xffffffff00000000ull) | (uint64_t)min(pid, (uint32_t)bk));
+            } else if (cbb <= cap) {
+                open = false;
+                bounded = true;
+                lbnd = cbb;
+            }
+            inner = max(inner, cap);
+            r2 *= 8.f;
+        }
+        if (lane == 0) {
+            if (bounded) {
+                corr[p] = -1;
+                corr_key[p] = ((uint64_t)(uint32_t)lbnd << 32) | 0xffffffffull;
+            } else {
+                corr[p] = bk == kEmptyKey ? -2 : inv[(uint32_t)bk];
+                corr_key[p] = bk;
+            }
+        }
+    }
+    if (collect && lane == 0) {
+        const int slot = (blockIdx.x * kDeferWaves + w) % kWorkSlots;
+        atomicAdd(&C->work[kWorkCompNodes][slot], (unsigned long long)st_nodes);
+        atomicAdd(&C->work[kWorkCompLeaves][slot], (unsigned long long)st_leaves);
+        atomicAdd(&C->work[kWorkCompPairs][slot], st_pairs);
     }
 }
 
@@ -3593,13 +3886,21 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsig
     if (ev) (void)hipEventRecord(ev[0], s);
     hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, (const Node*)nodes, cl, ncl, bucket, lt,
                        (const int32_t*)sc.list, nq, label, inv, seed_scale, corr, corr_key, comp1, 0, 0,
-                       sc.open_mask, seed_r, C, ev != nullptr ? 1 : 0);
+                       sc.open_mask, seed_r, sc.defer, C, ev != nullptr ? 1 : 0);
     if (ev) (void)hipEventRecord(ev[1], s);
     open_compact(sc.open_mask, P, sc.off, sc.tmp, (const int32_t*)sc.list, sc.list2, s);
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, (const Node*)nodes, cl, ncl, bucket, lt,
                        (const int32_t*)sc.list2, (const uint32_t*)(sc.off + P), label, inv, seed_scale, corr,
-                       corr_key, comp1, 1, kMaxCappedPasses, (uint64_t*)nullptr, seed_r, C, ev != nullptr ? 1 : 0);
+                       corr_key, comp1, 1, kMaxCappedPasses, (uint64_t*)nullptr, seed_r, sc.defer, C,
+                       ev != nullptr ? 1 : 0);
+    // the queries the packets handed over, one wave each
+    if (sc.defer != nullptr) {
+        auto dk = dim == 3 ? k_comp_search_deferred<3> : k_comp_search_deferred<2>;
+        hipLaunchKernelGGL(dk, dim3(kDeferBlocks), dim3(64 * kDeferWaves), 0, s, pts, (const Node*)nodes, cl, ncl, lt,
+                           (const uint64_t*)sc.defer, label, inv, seed_scale, corr, corr_key, comp1, seed_r, C,
+                           ev != nullptr ? 1 : 0);
+    }
     if (ev) (void)hipEventRecord(ev[3], s);
     // comp1 now holds every component's edge: the sample log of this round
     if (cs.features_on && cs.samples != nullptr)

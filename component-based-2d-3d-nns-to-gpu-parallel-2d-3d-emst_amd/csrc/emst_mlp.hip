@@ -7,7 +7,8 @@
 // deepening of k_cone_search, so the search result never depends on it.
 //
 // Arithmetic: BF16 weights and activations, FP32 accumulation on the matrix
-// cores (v_mfma_f32_16x16x16bf16_1k), FP32 master weights and Adam.
+// cores (v_mfma_f32_16x16x32_bf16 for the 64-deep products, 16x16x16 for the
+// 16 padded features), FP32 master weights and Adam.
 //
 // Model layout (EMST_MLP_PARAMS floats): W1[8][64] | b1[64] | W2[64][64] |
 // b2[64] | w3[64] | b3 | feature mean[8] | feature 1/std[8] | seed offset.
@@ -35,6 +36,7 @@ constexpr float kXClamp = 4.f;   // normalised features are clamped to [-4, 4]
 static_assert(kParams == EMST_MLP_PARAMS, "model layout");
 
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float f)
@@ -45,30 +47,54 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-// One wave: C[MT*16][NT*16] += A[M][K] * B[K][N] from LDS bf16 with strides.
-// Lane l holds A(m = 16mt + l%16, k = 16kt + 4(l/16) + j), B(k, n = 16nt + l%16)
-// and C(m = 16mt + 4(l/16) + j, n = 16nt + l%16), j = 0..3.
+// One wave: C[MT*16][NT*16] += A[M][K] * B[K][N] from LDS bf16 with strides,
+// K = 16 KT.  An even KT runs gfx950's 16x16x32 BF16 product (half the
+// matrix instructions): lane l holds A(m = 16mt + l%16, k = 32kt + 8(l/16) + j)
+// and B(k, n = 16nt + l%16), j = 0..7; KT = 1 (the 16 padded features) runs
+// the 16x16x16 product, lane l holding k = 16kt + 4(l/16) + j, j = 0..3.  Both
+// leave C(m = 16mt + 4(l/16) + j, n = 16nt + l%16), j = 0..3, in lane l.
 template <int MT, int NT, int KT>
 __device__ __forceinline__ void wave_gemm(const uint16_t* A, int sam, int sak, const uint16_t* B, int sbk,
                                           int sbn, f4 (&acc)[MT][NT])
 {
     const int l = __lane_id();
     const int r = l & 15, q = l >> 4;
+    if constexpr (KT % 2 == 0) {
 #pragma unroll
-    for (int kt = 0; kt < KT; ++kt) {
+        for (int kt = 0; kt < KT / 2; ++kt) {
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-            s4 a;
+            for (int mt = 0; mt < MT; ++mt) {
+                bf8 a;
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                a[j] = (short)A[(mt * 16 + r) * sam + (kt * 16 + 4 * q + j) * sak];
+                for (int j = 0; j < 8; ++j)
+                    a[j] = __builtin_bit_cast(__bf16, A[(mt * 16 + r) * sam + (kt * 32 + 8 * q + j) * sak]);
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                s4 b;
+                for (int nt = 0; nt < NT; ++nt) {
+                    bf8 b;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        b[j] = __builtin_bit_cast(__bf16, B[(kt * 32 + 8 * q + j) * sbk + (nt * 16 + r) * sbn]);
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[mt][nt], 0, 0, 0);
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int kt = 0; kt < KT; ++kt) {
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                s4 a;
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    b[j] = (short)B[(kt * 16 + 4 * q + j) * sbk + (nt * 16 + r) * sbn];
-                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[mt][nt], 0, 0, 0);
+                    a[j] = (short)A[(mt * 16 + r) * sam + (kt * 16 + 4 * q + j) * sak];
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    s4 b;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        b[j] = (short)B[(kt * 16 + 4 * q + j) * sbk + (nt * 16 + r) * sbn];
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[mt][nt], 0, 0, 0);
+                }
             }
         }
     }
