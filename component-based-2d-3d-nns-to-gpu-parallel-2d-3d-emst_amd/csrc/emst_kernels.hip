@@ -1385,7 +1385,7 @@ __global__ __launch_bounds__(kBlock) void k_open_scan_local(const uint64_t* __re
 
 // (one wave per packet, grid-stride: the lists are built every round, and a
 // launch of one block per packet costs more than its work when few are open)
-__global__ __launch_bounds__(kBlock) void k_open_fill_scan(const uint64_t* __restrict__ open_mask, int64_t P,
+__global__ __launch_bounds__(kBlock) void k_open_fill_scan(uint64_t* __restrict__ open_mask, int64_t P,
                                                           uint32_t* __restrict__ off,
                                                           const uint32_t* __restrict__ sums,
                                                           const int32_t* __restrict__ src, int32_t* __restrict__ dst)
@@ -1396,13 +1396,16 @@ __global__ __launch_bounds__(kBlock) void k_open_fill_scan(const uint64_t* __res
     if (w0 == 0 && lane == 0) off[P] += sums[P / kScanTile];
     for (int64_t b = w0; b < P; b += waves) {
         const uint64_t m = open_mask[b];
+        // read once: cleared, so that the search that writes the next mask
+        // need only write the words of its packets
+        if (lane == 0 && m != 0ull) open_mask[b] = 0ull;
         if (!((m >> lane) & 1ull)) continue;
         const uint32_t o = off[b] + sums[b / kScanTile] + (uint32_t)__popcll(m & lanemask_lt());
         dst[o] = src != nullptr ? src[b * 64 + lane] : (int32_t)(b * 64 + lane);
     }
 }
 
-void open_compact(const uint64_t* open_mask, int64_t P, uint32_t* off, uint32_t* sums, const int32_t* src,
+void open_compact(uint64_t* open_mask, int64_t P, uint32_t* off, uint32_t* sums, const int32_t* src,
                   int32_t* dst, hipStream_t s)
 {
     const int64_t nb = (P + 1 + kScanTile - 1) / kScanTile;
@@ -2295,6 +2298,15 @@ constexpr int kGatherLeaves = 128;
 // packets, one per SIMD of the chip) hand their overflowing packets' queries
 // to the per-query pass (k_comp_search_deferred)
 constexpr int64_t kDeferQueries = 65536;
+// the resumed dispatch of the component search runs on at most this many
+// one-packet blocks, looping over its packets: its queries are few (most
+// close in the first pass), and a grid of one block per packet of the whole
+// query list cost 34 us per round at 10^7 points while idle
+constexpr int64_t kResumeBlocks = 4096;
+// the first dispatch: at most this many blocks (far more than the chip holds,
+// so the hardware still balances the packets); the compaction clears the
+// mask, so a block need only write the words of the packets it searched
+constexpr int64_t kSearchBlocks = 65536;
 // the neighbour lists' gatherings (k_knn_first): their radius is the packet's
 // largest kKnn-th distance after its own leaves, so they reach further
 constexpr int kKnnGatherWork = 1024;
@@ -2361,384 +2373,388 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
     if (C->done) return;
     const int64_t nq = *nq_dev;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&C->comp_queries, (unsigned long long)nq);
-    if ((int64_t)blockIdx.x * 64 >= nq) {
-        if (open_out != nullptr && threadIdx.x == 0) open_out[blockIdx.x] = 0ull;
-        return;
-    }
     __shared__ unsigned long long best[64];
     __shared__ int32_t stk_id[64];
     __shared__ int32_t stk_ver[64];
     __shared__ unsigned long long stk_mask[64];
     __shared__ int32_t work[kGatherWork];
     __shared__ unsigned long long gathered[kGatherLeaves];
-    const int lane = __lane_id();
-    const int64_t qi = (int64_t)blockIdx.x * 64 + lane;
-    const bool active = qi < nq;
-    const int64_t p = active ? (int64_t)qlist[qi] : 0;
-    const float4 me = pts[p];
-    const float px = me.x, py = me.y, pz = me.z;
-    const int32_t mylab = active ? label[p] : -1;
-    const uint64_t prev = active ? corr_key[p] : kEmptyKey;
-    best[lane] = kEmptyKey;
-    const int my_cl = active ? find_cloud(clouds, ncl, p) : -1;
-    const uint32_t pid = __float_as_uint(me.w);
-    bool bounded = false, left_open = false, deferred = false;
-    int32_t lbnd = 0;
-    uint32_t st_nodes = 0, st_leaves = 0, st_walks = 0;   // work counters of this packet
-    uint64_t st_pairs = 0;
-    uint64_t pending = __ballot(active);
-    while (pending) {
-        const int cl = __builtin_amdgcn_readfirstlane(__shfl(my_cl, (int)__builtin_ctzll(pending)));
-        const bool mine = my_cl == cl;
-        pending &= ~__ballot(mine);
-        const int64_t cend = clouds[cl].pos + clouds[cl].n;
-        const int64_t L = clouds[cl].L, lfirst = clouds[cl].lfirst, nl = clouds[cl].nl;
-        const Node* __restrict__ tree = nodes + 2 * clouds[cl].lbase;
-        // seed: twice the previous target's distance, at least seed_scale
-        // times the mean spacing of the point's leaf cell
-        float r = 1.f;
-        int32_t inner = -1;
-        if (mine) {
-            const int64_t lf = lt.of[p];
-            const Node own = tree[L + (lf - lfirst)];
-            const float cnt = (float)(lt.start[lf + 1] - lt.start[lf]);
-            float vol = 1.f, ext = 0.f;
-            for (int k = 0; k < D; ++k) {
-                const float e = own.hi[k] - own.lo[k];
-                vol *= e;
-                ext = fmaxf(ext, e);
-            }
-            float sp = vol > 0.f ? powf(vol / cnt, 1.f / D) : ext / cnt;
-            if (!positive_finite(sp)) sp = 1e-6f;
-            const bool has_lb = (uint32_t)(prev >> 32) != 0xffffffffu;
-            const uint32_t dbits = (uint32_t)(prev >> 32);
-            if (has_lb) inner = (int32_t)dbits - 1;
-            r = seed_scale * sp;
-            if (has_lb) r = fmaxf(r, 2.f * sqrtf(__uint_as_float(dbits)));
-            // learned: the predicted length of the component's edge, so
-            // that the first pass can already find it
-            if (seed_r != nullptr && C->round >= kLearnedFrom) {
-                r = seed_r[p];
-                if (has_lb) r = fmaxf(r, sqrtf(__uint_as_float(dbits)));
-            }
-            if (!positive_finite(r)) r = 1e-6f;
+    // one packet per block; a grid smaller than the packets (the resumed
+    // dispatch, whose queries are few but unknown to the host) loops
+    for (int64_t pk = blockIdx.x;; pk += gridDim.x) {
+        if (pk * 64 >= nq) {
+            if (open_out != nullptr && threadIdx.x == 0 && pk == (int64_t)blockIdx.x) open_out[pk] = 0ull;
+            break;
         }
-        float r2 = r * r;
-        // a launch that resumes at pass_lo: the state the earlier passes left
-        for (int k = 0; k < pass_lo; ++k) {
-            inner = max(inner, (int32_t)__float_as_uint(r2));
-            r2 *= 8.f;
-        }
-        bool open = mine;
-        float cx = mine ? px : 0.f, cy = mine ? py : 0.f, cz = mine ? pz : 0.f;
-        float na = mine ? 1.f : 0.f;
-        for (int off = 32; off > 0; off >>= 1) {
-            cx += __shfl_xor(cx, off);
-            cy += __shfl_xor(cy, off);
-            cz += __shfl_xor(cz, off);
-            na += __shfl_xor(na, off);
-        }
-        cx /= na; cy /= na; cz /= na;
-        auto cdist = [&](const Node& x) {
-            const float dxx = fmaxf(fmaxf(x.lo[0] - cx, cx - x.hi[0]), 0.f);
-            const float dyy = fmaxf(fmaxf(x.lo[1] - cy, cy - x.hi[1]), 0.f);
-            const float dzz = fmaxf(fmaxf(x.lo[2] - cz, cz - x.hi[2]), 0.f);
-            return dxx * dxx + dyy * dyy + dzz * dzz;
-        };
-        // a box is needed when it holds another component, its nearest point
-        // is within the limit and may beat the best so far under (d2, id)
-        // (at equal d2 only with a smaller id), and its farthest point lies
-        // beyond `inner`
-        auto needed = [&](const Node& nd, int32_t nl, uint32_t nm, int32_t lim, uint64_t bk) {
-            if (!open || nl == mylab || nl == kNlabEmpty) return false;
-            float m[3], u[3];
-            const float lo[3] = {nd.lo[0] - px, nd.lo[1] - py, nd.lo[2] - pz};
-            const float hi[3] = {nd.hi[0] - px, nd.hi[1] - py, nd.hi[2] - pz};
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                m[k] = fmaxf(fmaxf(lo[k], -hi[k]), 0.f);
-                u[k] = fmaxf(-lo[k], hi[k]);
-            }
-            const float lb = D == 3 ? d2_of(m[0], m[1], m[2]) : d2_of(m[0], m[1]);
-            const float far = D == 3 ? d2_of(u[0], u[1], u[2]) : d2_of(u[0], u[1]);
-            const uint32_t lbb = __float_as_uint(lb);
-            return (int32_t)lbb <= lim && (((uint64_t)lbb << 32) | nm) < bk &&
-                   (int32_t)__float_as_uint(far) > inner;
-        };
-        for (int pass = pass_lo; pass <= pass_hi; ++pass) {
-            const bool last = pass == kMaxCappedPasses;
-            const int32_t cbb = open ? key_bound(comp1[mylab]) : 0x7fffffff;
-            // a point whose component already has an edge (a kept target, or
-            // a member's result) searches straight out to it in one pass
-            // instead of deepening by 4x per pass
-            const int32_t cap = last || cbb != 0x7fffffff ? 0x7fffffff : (int32_t)__float_as_uint(r2);
-            // nothing outside the component lies within `inner`: with the
-            // component's edge at most that long, the point cannot hold it
-            if (open && cbb <= inner) {
-                open = false;
-                bounded = true;
-                lbnd = inner;
-            }
-            if (__ballot(open) == 0ull) break;
-            const int32_t lim = min(cap, cbb);
-            uint64_t bk = best[lane];
-            int32_t bnd = min(key_bound(bk), lim);
-            int sp = 0;
-            int ver = 0;
-            // first pass (3D): the leaf of the first open lane goes first, so
-            // that the packet's lanes enter the tree with bounds from their
-            // own neighbourhood; the traversal skips it when it comes again
-            // (clustered 3D -3.4%; in 2D +1%, so not there)
-            int pre = -1;
-            bool pre_done = false;
-            if (D == 3 && pass == 0) {
-                const int fl = (int)__builtin_ctzll(__ballot(open));
-                const int myleaf = (int)(L + (mine ? lt.of[p] - lfirst : 0));
-                pre = __builtin_amdgcn_readfirstlane(__shfl(myleaf, fl));
-            }
-            // a leaf: one (query, leaf point) pair per lane; the query's state by
-            // LDS permute (a v_readlane broadcast when 64 / bucket = 1, as in
-            // the cone search, measured 7% slower here, and four packets per
-            // block 6% slower)
-            auto visit_leaf = [&](int node, bool need, uint64_t needm) {
-                int64_t s0, s1;
-                leaf_range(lt, lfirst, nl, cend, node - L, s0, s1);
-                const int cnt = (int)(s1 - s0);
-                ++st_leaves;
-                st_pairs += (uint64_t)cnt * (uint64_t)__popcll(needm);
-                const int j = lane & (bucket - 1);
-                const int qslot = lane / bucket;
-                const int qpc = 64 / bucket;
-                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-                int32_t vlab = -1;
-                if (j < cnt) {
-                    v = pts[s0 + j];
-                    vlab = label[s0 + j];
+        const int lane = __lane_id();
+        const int64_t qi = pk * 64 + lane;
+        const bool active = qi < nq;
+        const int64_t p = active ? (int64_t)qlist[qi] : 0;
+        const float4 me = pts[p];
+        const float px = me.x, py = me.y, pz = me.z;
+        const int32_t mylab = active ? label[p] : -1;
+        const uint64_t prev = active ? corr_key[p] : kEmptyKey;
+        best[lane] = kEmptyKey;
+        const int my_cl = active ? find_cloud(clouds, ncl, p) : -1;
+        const uint32_t pid = __float_as_uint(me.w);
+        bool bounded = false, left_open = false, deferred = false;
+        int32_t lbnd = 0;
+        uint32_t st_nodes = 0, st_leaves = 0, st_walks = 0;   // work counters of this packet
+        uint64_t st_pairs = 0;
+        uint64_t pending = __ballot(active);
+        while (pending) {
+            const int cl = __builtin_amdgcn_readfirstlane(__shfl(my_cl, (int)__builtin_ctzll(pending)));
+            const bool mine = my_cl == cl;
+            pending &= ~__ballot(mine);
+            const int64_t cend = clouds[cl].pos + clouds[cl].n;
+            const int64_t L = clouds[cl].L, lfirst = clouds[cl].lfirst, nl = clouds[cl].nl;
+            const Node* __restrict__ tree = nodes + 2 * clouds[cl].lbase;
+            // seed: twice the previous target's distance, at least seed_scale
+            // times the mean spacing of the point's leaf cell
+            float r = 1.f;
+            int32_t inner = -1;
+            if (mine) {
+                const int64_t lf = lt.of[p];
+                const Node own = tree[L + (lf - lfirst)];
+                const float cnt = (float)(lt.start[lf + 1] - lt.start[lf]);
+                float vol = 1.f, ext = 0.f;
+                for (int k = 0; k < D; ++k) {
+                    const float e = own.hi[k] - own.lo[k];
+                    vol *= e;
+                    ext = fmaxf(ext, e);
                 }
-                uint64_t rem = needm;
-                while (rem) {
-                    int myq = -1;
-                    for (int k = 0; k < qpc; ++k) {
-                        const int l = rem ? (int)__builtin_ctzll(rem) : -1;
-                        if (rem) rem &= rem - 1;
-                        if (k == qslot) myq = l;
-                    }
-                    const int src = myq >= 0 ? myq : 0;
-                    const float qx = __shfl(px, src), qy = __shfl(py, src), qz = __shfl(pz, src);
-                    const int32_t qb = __shfl(bnd, src), qin = __shfl(inner, src), ql = __shfl(mylab, src);
-                    const float dx = v.x - qx, dy = v.y - qy, dz = v.z - qz;
-                    const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
-                    const int32_t db = (int32_t)__float_as_uint(d2);
-                    if (myq >= 0 && j < cnt && vlab != ql && db <= qb && db > qin)
-                        atomicMin(&best[myq],
-                                  ((unsigned long long)__float_as_uint(d2) << 32) | __float_as_uint(v.w));
+                float sp = vol > 0.f ? powf(vol / cnt, 1.f / D) : ext / cnt;
+                if (!positive_finite(sp)) sp = 1e-6f;
+                const bool has_lb = (uint32_t)(prev >> 32) != 0xffffffffu;
+                const uint32_t dbits = (uint32_t)(prev >> 32);
+                if (has_lb) inner = (int32_t)dbits - 1;
+                r = seed_scale * sp;
+                if (has_lb) r = fmaxf(r, 2.f * sqrtf(__uint_as_float(dbits)));
+                // learned: the predicted length of the component's edge, so
+                // that the first pass can already find it
+                if (seed_r != nullptr && C->round >= kLearnedFrom) {
+                    r = seed_r[p];
+                    if (has_lb) r = fmaxf(r, sqrtf(__uint_as_float(dbits)));
                 }
-                if (need) {
-                    bk = best[lane];
-                    bnd = min(key_bound(bk), lim);
-                }
+                if (!positive_finite(r)) r = 1e-6f;
+            }
+            float r2 = r * r;
+            // a launch that resumes at pass_lo: the state the earlier passes left
+            for (int k = 0; k < pass_lo; ++k) {
+                inner = max(inner, (int32_t)__float_as_uint(r2));
+                r2 *= 8.f;
+            }
+            bool open = mine;
+            float cx = mine ? px : 0.f, cy = mine ? py : 0.f, cz = mine ? pz : 0.f;
+            float na = mine ? 1.f : 0.f;
+            for (int off = 32; off > 0; off >>= 1) {
+                cx += __shfl_xor(cx, off);
+                cy += __shfl_xor(cy, off);
+                cz += __shfl_xor(cz, off);
+                na += __shfl_xor(na, off);
+            }
+            cx /= na; cy /= na; cz /= na;
+            auto cdist = [&](const Node& x) {
+                const float dxx = fmaxf(fmaxf(x.lo[0] - cx, cx - x.hi[0]), 0.f);
+                const float dyy = fmaxf(fmaxf(x.lo[1] - cy, cy - x.hi[1]), 0.f);
+                const float dzz = fmaxf(fmaxf(x.lo[2] - cz, cz - x.hi[2]), 0.f);
+                return dxx * dxx + dyy * dyy + dzz * dzz;
             };
-            // The gathering (a finite bound): every leaf whose box lies within
-            // the largest bound of the open queries from their box, of another
-            // component than all of them, and not inside all of their lower
-            // bounds, by box-to-box tests of 64 nodes a step; then those
-            // leaves nearest first, each tested per query.  A gathering that
-            // outgrows the LDS lists leaves the pass to the per-query walk.
-            bool done_by_gather = false;
-            uint32_t B = open ? (uint32_t)bnd : 0u;
-            for (int off = 32; off > 0; off >>= 1) B = max(B, (uint32_t)__shfl_xor((int)B, off));
-            if (B < 0x7f800000u) {
-                float qlo[3] = {open ? px : INFINITY, open ? py : INFINITY, open ? pz : INFINITY};
-                float qhi[3] = {open ? px : -INFINITY, open ? py : -INFINITY, open ? pz : -INFINITY};
-                int32_t imin = open ? inner : 0x7fffffff;
-                for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        qlo[k] = fminf(qlo[k], __shfl_xor(qlo[k], off));
-                        qhi[k] = fmaxf(qhi[k], __shfl_xor(qhi[k], off));
-                    }
-                    imin = min(imin, __shfl_xor(imin, off));
+            // a box is needed when it holds another component, its nearest point
+            // is within the limit and may beat the best so far under (d2, id)
+            // (at equal d2 only with a smaller id), and its farthest point lies
+            // beyond `inner`
+            auto needed = [&](const Node& nd, int32_t nl, uint32_t nm, int32_t lim, uint64_t bk) {
+                if (!open || nl == mylab || nl == kNlabEmpty) return false;
+                float m[3], u[3];
+                const float lo[3] = {nd.lo[0] - px, nd.lo[1] - py, nd.lo[2] - pz};
+                const float hi[3] = {nd.hi[0] - px, nd.hi[1] - py, nd.hi[2] - pz};
+    #pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    m[k] = fmaxf(fmaxf(lo[k], -hi[k]), 0.f);
+                    u[k] = fmaxf(-lo[k], hi[k]);
                 }
-                const int fo = (int)__builtin_ctzll(__ballot(open));
-                const int32_t lab0 = __builtin_amdgcn_readfirstlane(__shfl(mylab, fo));
-                const int32_t skip_lab = __ballot(open && mylab != lab0) == 0ull ? lab0 : (int32_t)0x80000000;
-                int head = 0, tail = gather_start(L, work), ngl = 0;
-                bool over = false;
-                __builtin_amdgcn_wave_barrier();
-                while (head < tail) {
-                    const int take = min(64, tail - head);
-                    const int nd = lane < take ? work[head + lane] : 0;
-                    head += take;
-                    st_nodes += (uint32_t)take;
-                    bool pass_ = false, leaf = false;
-                    uint32_t bd = 0;
-                    if (lane < take) {
-                        const Node x = tree[nd];
-                        const int32_t xl = __float_as_int(x.pad[1]);
-                        float m[3], u[3];
-#pragma unroll
-                        for (int k = 0; k < 3; ++k) {
-                            m[k] = fmaxf(fmaxf(x.lo[k] - qhi[k], qlo[k] - x.hi[k]), 0.f);
-                            u[k] = fmaxf(x.hi[k] - qlo[k], qhi[k] - x.lo[k]);
-                        }
-                        bd = __float_as_uint(D == 3 ? d2_of(m[0], m[1], m[2]) : d2_of(m[0], m[1]));
-                        const float far = D == 3 ? d2_of(u[0], u[1], u[2]) : d2_of(u[0], u[1]);
-                        leaf = nd >= L;
-                        pass_ = xl != kNlabEmpty && xl != skip_lab && bd <= B &&
-                                (int32_t)__float_as_uint(far) > imin && !(leaf && nd == pre);
-                    }
-                    const uint64_t mi = __ballot(pass_ && !leaf), ml = __ballot(pass_ && leaf);
-                    const int ni = __popcll(mi), nlf = __popcll(ml);
-                    if (tail + 2 * ni > kGatherWork || ngl + nlf > kGatherLeaves) {
-                        over = true;
-                        break;
-                    }
-                    const uint64_t lt_ = lanemask_lt();
-                    if (pass_ && !leaf) {
-                        const int r = tail + 2 * __popcll(mi & lt_);
-                        work[r] = 2 * nd;
-                        work[r + 1] = 2 * nd + 1;
-                    }
-                    if (pass_ && leaf) gathered[ngl + __popcll(ml & lt_)] = ((uint64_t)bd << 32) | (uint32_t)nd;
-                    tail += 2 * ni;
-                    ngl += nlf;
-                    __builtin_amdgcn_wave_barrier();
-                }
-                if (!over) {
-                    done_by_gather = true;
-                    auto try_leaf = [&](int node) {
-                        const Node x = tree[node];
-                        ++st_nodes;
-                        const bool need = needed(x, __float_as_int(x.pad[1]), __float_as_uint(x.pad[0]), lim, bk);
-                        const uint64_t needm = __ballot(need);
-                        if (needm != 0ull) visit_leaf(node, need, needm);
-                    };
-                    if (pre >= 0) try_leaf(pre);
-                    for (int c0 = 0; c0 < ngl; c0 += 64) {
-                        const int cn = min(64, ngl - c0);
-                        const uint64_t e = wave_sort_u64(lane < cn ? gathered[c0 + lane] : kEmptyKey);
-                        for (int i = 0; i < cn; ++i) try_leaf(__builtin_amdgcn_readlane((int)(uint32_t)e, i));
-                    }
-                }
-            }
-            if (!done_by_gather && defer != nullptr && B < 0x7f800000u && nq <= kDeferQueries) {
-                // a gathering that outgrew its lists, in a dispatch of fewer
-                // packets than the chip has SIMDs: the dispatch takes as long
-                // as its slowest packet, and this packet's walk would visit
-                // the union of its queries' neighbourhoods in one wave; the
-                // open queries go to the per-query pass instead, one wave
-                // each, from this pass on (no leaf has been visited in it
-                // yet, so their bests are empty)
-                ++st_walks;
-                wave_append_u64(open, ((uint64_t)(uint32_t)pass << 32) | (uint32_t)p, defer, &C->deferred);
-                if (open) {
-                    open = false;
-                    deferred = true;
-                }
-            } else if (!done_by_gather) {
-                ++st_walks;
-                if (lane == 0) {
-                    stk_id[0] = 1;
-                    stk_ver[0] = -1;
-                    if (pre >= 0) { stk_id[1] = pre; stk_ver[1] = -1; }
-                }
-                sp = pre >= 0 ? 2 : 1;
-                __builtin_amdgcn_wave_barrier();
-                while (sp > 0) {
-                    --sp;
-                    const int node = __builtin_amdgcn_readfirstlane(stk_id[sp]);
-                    const int pver = __builtin_amdgcn_readfirstlane(stk_ver[sp]);
-                    if (node == pre) {
-                        if (pre_done) continue;
-                        pre_done = true;
-                    }
-                    bool need;
-                    if (pver == ver) {
-                        need = (stk_mask[sp] >> lane) & 1ull;
-                    } else {
-                        {
-                            const Node nd = tree[node];
-                            need = needed(nd, __float_as_int(nd.pad[1]), __float_as_uint(nd.pad[0]), lim, bk);
-                        }
-                        ++st_nodes;
-                    }
-                    const uint64_t needm = __ballot(need);
-                    if (needm == 0ull) continue;
-                    if (node >= L) {
-                        visit_leaf(node, need, needm);
-                        ++ver;
-                    } else {
-                        // (rare: an unbounded pass, or a gathering that
-                        // outgrew its lists) both children, the nearer last
-                        // on the stack
-                        const int first = 2 * node;
-                        st_nodes += 2u;
-                        const Node b0 = tree[first], b1 = tree[first + 1];
-                        const uint64_t m0 =
-                            __ballot(needed(b0, __float_as_int(b0.pad[1]), __float_as_uint(b0.pad[0]), lim, bk));
-                        const uint64_t m1 =
-                            __ballot(needed(b1, __float_as_int(b1.pad[1]), __float_as_uint(b1.pad[0]), lim, bk));
-                        const float d0 = m0 ? cdist(b0) : -1.f, d1 = m1 ? cdist(b1) : -1.f;
-                        // pushed farthest first, so the nearest pops first; at
-                        // equal distance the lower node (smaller ids in a run
-                        // of coincident points) pops first
-                        const bool swap = d1 > d0;
-                        const int ia = swap ? first + 1 : first, ib = swap ? first : first + 1;
-                        const uint64_t ma = swap ? m1 : m0, mb = swap ? m0 : m1;
-                        if (lane == 0) {
-                            int t = sp;
-                            if (ma) { stk_id[t] = ia; stk_ver[t] = ver; stk_mask[t] = ma; ++t; }
-                            if (mb) { stk_id[t] = ib; stk_ver[t] = ver; stk_mask[t] = mb; ++t; }
-                        }
-                        sp += (ma != 0ull ? 1 : 0) + (mb != 0ull ? 1 : 0);
-                        __builtin_amdgcn_wave_barrier();
-                    }
-                }
-            }
-            // everything outside the component up to min(best, lim) has been
-            // seen: best is exact when within lim; else nothing lies within
-            // lim, and with lim = cbb the point stops on its lower bound
-            bool fresh = false;
-            uint64_t key1 = kEmptyKey;
-            if (open) {
-                const uint64_t kb = best[lane];
-                if (key_bound(kb) <= lim) {
-                    open = false;
-                    if (kb != kEmptyKey) {
-                        fresh = true;
-                        key1 = (kb & 0xffffffff00000000ull) | (uint64_t)min(pid, (uint32_t)kb);
-                    }
-                } else if (cbb <= cap) {
+                const float lb = D == 3 ? d2_of(m[0], m[1], m[2]) : d2_of(m[0], m[1]);
+                const float far = D == 3 ? d2_of(u[0], u[1], u[2]) : d2_of(u[0], u[1]);
+                const uint32_t lbb = __float_as_uint(lb);
+                return (int32_t)lbb <= lim && (((uint64_t)lbb << 32) | nm) < bk &&
+                       (int32_t)__float_as_uint(far) > inner;
+            };
+            for (int pass = pass_lo; pass <= pass_hi; ++pass) {
+                const bool last = pass == kMaxCappedPasses;
+                const int32_t cbb = open ? key_bound(comp1[mylab]) : 0x7fffffff;
+                // a point whose component already has an edge (a kept target, or
+                // a member's result) searches straight out to it in one pass
+                // instead of deepening by 4x per pass
+                const int32_t cap = last || cbb != 0x7fffffff ? 0x7fffffff : (int32_t)__float_as_uint(r2);
+                // nothing outside the component lies within `inner`: with the
+                // component's edge at most that long, the point cannot hold it
+                if (open && cbb <= inner) {
                     open = false;
                     bounded = true;
-                    lbnd = cbb;
+                    lbnd = inner;
                 }
+                if (__ballot(open) == 0ull) break;
+                const int32_t lim = min(cap, cbb);
+                uint64_t bk = best[lane];
+                int32_t bnd = min(key_bound(bk), lim);
+                int sp = 0;
+                int ver = 0;
+                // first pass (3D): the leaf of the first open lane goes first, so
+                // that the packet's lanes enter the tree with bounds from their
+                // own neighbourhood; the traversal skips it when it comes again
+                // (clustered 3D -3.4%; in 2D +1%, so not there)
+                int pre = -1;
+                bool pre_done = false;
+                if (D == 3 && pass == 0) {
+                    const int fl = (int)__builtin_ctzll(__ballot(open));
+                    const int myleaf = (int)(L + (mine ? lt.of[p] - lfirst : 0));
+                    pre = __builtin_amdgcn_readfirstlane(__shfl(myleaf, fl));
+                }
+                // a leaf: one (query, leaf point) pair per lane; the query's state by
+                // LDS permute (a v_readlane broadcast when 64 / bucket = 1, as in
+                // the cone search, measured 7% slower here, and four packets per
+                // block 6% slower)
+                auto visit_leaf = [&](int node, bool need, uint64_t needm) {
+                    int64_t s0, s1;
+                    leaf_range(lt, lfirst, nl, cend, node - L, s0, s1);
+                    const int cnt = (int)(s1 - s0);
+                    ++st_leaves;
+                    st_pairs += (uint64_t)cnt * (uint64_t)__popcll(needm);
+                    const int j = lane & (bucket - 1);
+                    const int qslot = lane / bucket;
+                    const int qpc = 64 / bucket;
+                    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                    int32_t vlab = -1;
+                    if (j < cnt) {
+                        v = pts[s0 + j];
+                        vlab = label[s0 + j];
+                    }
+                    uint64_t rem = needm;
+                    while (rem) {
+                        int myq = -1;
+                        for (int k = 0; k < qpc; ++k) {
+                            const int l = rem ? (int)__builtin_ctzll(rem) : -1;
+                            if (rem) rem &= rem - 1;
+                            if (k == qslot) myq = l;
+                        }
+                        const int src = myq >= 0 ? myq : 0;
+                        const float qx = __shfl(px, src), qy = __shfl(py, src), qz = __shfl(pz, src);
+                        const int32_t qb = __shfl(bnd, src), qin = __shfl(inner, src), ql = __shfl(mylab, src);
+                        const float dx = v.x - qx, dy = v.y - qy, dz = v.z - qz;
+                        const float d2 = D == 3 ? d2_of(dx, dy, dz) : d2_of(dx, dy);
+                        const int32_t db = (int32_t)__float_as_uint(d2);
+                        if (myq >= 0 && j < cnt && vlab != ql && db <= qb && db > qin)
+                            atomicMin(&best[myq],
+                                      ((unsigned long long)__float_as_uint(d2) << 32) | __float_as_uint(v.w));
+                    }
+                    if (need) {
+                        bk = best[lane];
+                        bnd = min(key_bound(bk), lim);
+                    }
+                };
+                // The gathering (a finite bound): every leaf whose box lies within
+                // the largest bound of the open queries from their box, of another
+                // component than all of them, and not inside all of their lower
+                // bounds, by box-to-box tests of 64 nodes a step; then those
+                // leaves nearest first, each tested per query.  A gathering that
+                // outgrows the LDS lists leaves the pass to the per-query walk.
+                bool done_by_gather = false;
+                uint32_t B = open ? (uint32_t)bnd : 0u;
+                for (int off = 32; off > 0; off >>= 1) B = max(B, (uint32_t)__shfl_xor((int)B, off));
+                if (B < 0x7f800000u) {
+                    float qlo[3] = {open ? px : INFINITY, open ? py : INFINITY, open ? pz : INFINITY};
+                    float qhi[3] = {open ? px : -INFINITY, open ? py : -INFINITY, open ? pz : -INFINITY};
+                    int32_t imin = open ? inner : 0x7fffffff;
+                    for (int off = 32; off > 0; off >>= 1) {
+    #pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            qlo[k] = fminf(qlo[k], __shfl_xor(qlo[k], off));
+                            qhi[k] = fmaxf(qhi[k], __shfl_xor(qhi[k], off));
+                        }
+                        imin = min(imin, __shfl_xor(imin, off));
+                    }
+                    const int fo = (int)__builtin_ctzll(__ballot(open));
+                    const int32_t lab0 = __builtin_amdgcn_readfirstlane(__shfl(mylab, fo));
+                    const int32_t skip_lab = __ballot(open && mylab != lab0) == 0ull ? lab0 : (int32_t)0x80000000;
+                    int head = 0, tail = gather_start(L, work), ngl = 0;
+                    bool over = false;
+                    __builtin_amdgcn_wave_barrier();
+                    while (head < tail) {
+                        const int take = min(64, tail - head);
+                        const int nd = lane < take ? work[head + lane] : 0;
+                        head += take;
+                        st_nodes += (uint32_t)take;
+                        bool pass_ = false, leaf = false;
+                        uint32_t bd = 0;
+                        if (lane < take) {
+                            const Node x = tree[nd];
+                            const int32_t xl = __float_as_int(x.pad[1]);
+                            float m[3], u[3];
+    #pragma unroll
+                            for (int k = 0; k < 3; ++k) {
+                                m[k] = fmaxf(fmaxf(x.lo[k] - qhi[k], qlo[k] - x.hi[k]), 0.f);
+                                u[k] = fmaxf(x.hi[k] - qlo[k], qhi[k] - x.lo[k]);
+                            }
+                            bd = __float_as_uint(D == 3 ? d2_of(m[0], m[1], m[2]) : d2_of(m[0], m[1]));
+                            const float far = D == 3 ? d2_of(u[0], u[1], u[2]) : d2_of(u[0], u[1]);
+                            leaf = nd >= L;
+                            pass_ = xl != kNlabEmpty && xl != skip_lab && bd <= B &&
+                                    (int32_t)__float_as_uint(far) > imin && !(leaf && nd == pre);
+                        }
+                        const uint64_t mi = __ballot(pass_ && !leaf), ml = __ballot(pass_ && leaf);
+                        const int ni = __popcll(mi), nlf = __popcll(ml);
+                        if (tail + 2 * ni > kGatherWork || ngl + nlf > kGatherLeaves) {
+                            over = true;
+                            break;
+                        }
+                        const uint64_t lt_ = lanemask_lt();
+                        if (pass_ && !leaf) {
+                            const int r = tail + 2 * __popcll(mi & lt_);
+                            work[r] = 2 * nd;
+                            work[r + 1] = 2 * nd + 1;
+                        }
+                        if (pass_ && leaf) gathered[ngl + __popcll(ml & lt_)] = ((uint64_t)bd << 32) | (uint32_t)nd;
+                        tail += 2 * ni;
+                        ngl += nlf;
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                    if (!over) {
+                        done_by_gather = true;
+                        auto try_leaf = [&](int node) {
+                            const Node x = tree[node];
+                            ++st_nodes;
+                            const bool need = needed(x, __float_as_int(x.pad[1]), __float_as_uint(x.pad[0]), lim, bk);
+                            const uint64_t needm = __ballot(need);
+                            if (needm != 0ull) visit_leaf(node, need, needm);
+                        };
+                        if (pre >= 0) try_leaf(pre);
+                        for (int c0 = 0; c0 < ngl; c0 += 64) {
+                            const int cn = min(64, ngl - c0);
+                            const uint64_t e = wave_sort_u64(lane < cn ? gathered[c0 + lane] : kEmptyKey);
+                            for (int i = 0; i < cn; ++i) try_leaf(__builtin_amdgcn_readlane((int)(uint32_t)e, i));
+                        }
+                    }
+                }
+                if (!done_by_gather && defer != nullptr && B < 0x7f800000u && nq <= kDeferQueries) {
+                    // a gathering that outgrew its lists, in a dispatch of fewer
+                    // packets than the chip has SIMDs: the dispatch takes as long
+                    // as its slowest packet, and this packet's walk would visit
+                    // the union of its queries' neighbourhoods in one wave; the
+                    // open queries go to the per-query pass instead, one wave
+                    // each, from this pass on (no leaf has been visited in it
+                    // yet, so their bests are empty)
+                    ++st_walks;
+                    wave_append_u64(open, ((uint64_t)(uint32_t)pass << 32) | (uint32_t)p, defer, &C->deferred);
+                    if (open) {
+                        open = false;
+                        deferred = true;
+                    }
+                } else if (!done_by_gather) {
+                    ++st_walks;
+                    if (lane == 0) {
+                        stk_id[0] = 1;
+                        stk_ver[0] = -1;
+                        if (pre >= 0) { stk_id[1] = pre; stk_ver[1] = -1; }
+                    }
+                    sp = pre >= 0 ? 2 : 1;
+                    __builtin_amdgcn_wave_barrier();
+                    while (sp > 0) {
+                        --sp;
+                        const int node = __builtin_amdgcn_readfirstlane(stk_id[sp]);
+                        const int pver = __builtin_amdgcn_readfirstlane(stk_ver[sp]);
+                        if (node == pre) {
+                            if (pre_done) continue;
+                            pre_done = true;
+                        }
+                        bool need;
+                        if (pver == ver) {
+                            need = (stk_mask[sp] >> lane) & 1ull;
+                        } else {
+                            {
+                                const Node nd = tree[node];
+                                need = needed(nd, __float_as_int(nd.pad[1]), __float_as_uint(nd.pad[0]), lim, bk);
+                            }
+                            ++st_nodes;
+                        }
+                        const uint64_t needm = __ballot(need);
+                        if (needm == 0ull) continue;
+                        if (node >= L) {
+                            visit_leaf(node, need, needm);
+                            ++ver;
+                        } else {
+                            // (rare: an unbounded pass, or a gathering that
+                            // outgrew its lists) both children, the nearer last
+                            // on the stack
+                            const int first = 2 * node;
+                            st_nodes += 2u;
+                            const Node b0 = tree[first], b1 = tree[first + 1];
+                            const uint64_t m0 =
+                                __ballot(needed(b0, __float_as_int(b0.pad[1]), __float_as_uint(b0.pad[0]), lim, bk));
+                            const uint64_t m1 =
+                                __ballot(needed(b1, __float_as_int(b1.pad[1]), __float_as_uint(b1.pad[0]), lim, bk));
+                            const float d0 = m0 ? cdist(b0) : -1.f, d1 = m1 ? cdist(b1) : -1.f;
+                            // pushed farthest first, so the nearest pops first; at
+                            // equal distance the lower node (smaller ids in a run
+                            // of coincident points) pops first
+                            const bool swap = d1 > d0;
+                            const int ia = swap ? first + 1 : first, ib = swap ? first : first + 1;
+                            const uint64_t ma = swap ? m1 : m0, mb = swap ? m0 : m1;
+                            if (lane == 0) {
+                                int t = sp;
+                                if (ma) { stk_id[t] = ia; stk_ver[t] = ver; stk_mask[t] = ma; ++t; }
+                                if (mb) { stk_id[t] = ib; stk_ver[t] = ver; stk_mask[t] = mb; ++t; }
+                            }
+                            sp += (ma != 0ull ? 1 : 0) + (mb != 0ull ? 1 : 0);
+                            __builtin_amdgcn_wave_barrier();
+                        }
+                    }
+                }
+                // everything outside the component up to min(best, lim) has been
+                // seen: best is exact when within lim; else nothing lies within
+                // lim, and with lim = cbb the point stops on its lower bound
+                bool fresh = false;
+                uint64_t key1 = kEmptyKey;
+                if (open) {
+                    const uint64_t kb = best[lane];
+                    if (key_bound(kb) <= lim) {
+                        open = false;
+                        if (kb != kEmptyKey) {
+                            fresh = true;
+                            key1 = (kb & 0xffffffff00000000ull) | (uint64_t)min(pid, (uint32_t)kb);
+                        }
+                    } else if (cbb <= cap) {
+                        open = false;
+                        bounded = true;
+                        lbnd = cbb;
+                    }
+                }
+                atomic_min_grouped(comp1, mylab, key1, fresh);
+                inner = max(inner, cap);
+                r2 *= 8.f;
             }
-            atomic_min_grouped(comp1, mylab, key1, fresh);
-            inner = max(inner, cap);
-            r2 *= 8.f;
+            if (mine) left_open = open;
         }
-        if (mine) left_open = open;
-    }
-    if (open_out != nullptr) {
-        const uint64_t m = __ballot(left_open);
-        if (lane == 0) open_out[blockIdx.x] = m;
-    }
-    if (collect && lane == 0) {
-        const int slot = blockIdx.x % kWorkSlots;
-        atomicAdd(&C->work[kWorkCompNodes][slot], (unsigned long long)st_nodes);
-        atomicAdd(&C->work[kWorkCompLeaves][slot], (unsigned long long)st_leaves);
-        atomicAdd(&C->work[kWorkCompPairs][slot], (unsigned long long)st_pairs);
-        atomicAdd(&C->work[kWorkCompWalks][slot], (unsigned long long)st_walks);
-    }
-    if (active && !left_open && !deferred) {
-        const uint64_t k = best[lane];
-        if (bounded) {
-            corr[p] = -1;
-            corr_key[p] = ((uint64_t)(uint32_t)lbnd << 32) | 0xffffffffull;
-        } else {
-            corr[p] = k == kEmptyKey ? -2 : inv[(uint32_t)k];
-            corr_key[p] = k;
+        if (open_out != nullptr) {
+            const uint64_t m = __ballot(left_open);
+            if (lane == 0) open_out[pk] = m;
+        }
+        if (collect && lane == 0) {
+            const int slot = (int)(pk % kWorkSlots);
+            atomicAdd(&C->work[kWorkCompNodes][slot], (unsigned long long)st_nodes);
+            atomicAdd(&C->work[kWorkCompLeaves][slot], (unsigned long long)st_leaves);
+            atomicAdd(&C->work[kWorkCompPairs][slot], (unsigned long long)st_pairs);
+            atomicAdd(&C->work[kWorkCompWalks][slot], (unsigned long long)st_walks);
+        }
+        if (active && !left_open && !deferred) {
+            const uint64_t k = best[lane];
+            if (bounded) {
+                corr[p] = -1;
+                corr_key[p] = ((uint64_t)(uint32_t)lbnd << 32) | 0xffffffffull;
+            } else {
+                corr[p] = k == kEmptyKey ? -2 : inv[(uint32_t)k];
+                corr_key[p] = k;
+            }
         }
     }
 }
@@ -3884,13 +3900,15 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsig
     // <2, 64> instance measured 7% slower on the 2D power-law cloud)
     // pass 0 over the searching points; their open packets' lanes into open_mask
     if (ev) (void)hipEventRecord(ev[0], s);
-    hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, (const Node*)nodes, cl, ncl, bucket, lt,
+    hipLaunchKernelGGL(kern, dim3((unsigned)std::min<int64_t>(P, kSearchBlocks)), dim3(64), 0, s, pts,
+                       (const Node*)nodes, cl, ncl, bucket, lt,
                        (const int32_t*)sc.list, nq, label, inv, seed_scale, corr, corr_key, comp1, 0, 0,
                        sc.open_mask, seed_r, sc.defer, C, ev != nullptr ? 1 : 0);
     if (ev) (void)hipEventRecord(ev[1], s);
     open_compact(sc.open_mask, P, sc.off, sc.tmp, (const int32_t*)sc.list, sc.list2, s);
     if (ev) (void)hipEventRecord(ev[2], s);
-    hipLaunchKernelGGL(kern, dim3((unsigned)P), dim3(64), 0, s, pts, (const Node*)nodes, cl, ncl, bucket, lt,
+    hipLaunchKernelGGL(kern, dim3((unsigned)std::min<int64_t>(P, kResumeBlocks)), dim3(64), 0, s, pts,
+                       (const Node*)nodes, cl, ncl, bucket, lt,
                        (const int32_t*)sc.list2, (const uint32_t*)(sc.off + P), label, inv, seed_scale, corr,
                        corr_key, comp1, 1, kMaxCappedPasses, (uint64_t*)nullptr, seed_r, sc.defer, C,
                        ev != nullptr ? 1 : 0);
