@@ -108,6 +108,7 @@ struct emst_ctx {
     DevBuf<uint32_t> vals0, vals1, hist, scan_tmp, comp2;
     DevBuf<float4> pts;
     DevBuf<int32_t> inv, cand_pos, label, label2, corr, fixed, list, eu, ev_, eround, geo;
+    DevBuf<uint32_t> ids;              // input id of each sorted position (the w of pts, dense)
     DevBuf<Node> nodes;
     DevBuf<int32_t> lstart, leaf_of, cloud_lf;     // LeafTable and each cloud's first leaf
     DevBuf<float> ed2;
@@ -240,6 +241,7 @@ int alloc_all(emst_ctx* c)
     ENS(scan_tmp, scan_tmp_words((int64_t)radix_hist_words(n)) + scan_tmp_words(n) + 16);
     ENS(pts, n);
     ENS(inv, n);
+    ENS(ids, n);
     ENS(cand_key, n * K);
     ENS(cand_pos, n * K);
     ENS(label, n);
@@ -518,7 +520,7 @@ int build(emst_ctx* c)
         key_bits += cb;
     }
     radix_sort(c->keys0.p, c->vals0.p, c->keys1.p, c->vals1.p, n, key_bits, true, c->hist.p, c->scan_tmp.p, s);
-    launch_gather_sorted(c->resc.p, c->vals0.p, n, c->dim, c->pts.p, c->inv.p, s);
+    launch_gather_sorted(c->resc.p, c->vals0.p, n, c->dim, c->pts.p, c->inv.p, c->ids.p, s);
     int rcl = make_leaves(c);
     if (rcl) return rcl;
     // coincident-point groups: dup_runs .. cycles_cut start at zero
@@ -716,14 +718,14 @@ void issue_round(emst_ctx* c, bool tr, unsigned long long* stamps = nullptr)
     const CycleState cy = cycle_state(c);
     for (int ph = 0; ph < 4; ++ph) {
         if (ph == 0 && c->opt.round_search == 1) {
-            launch_comp_round(c->dim, c->n, c->pts.p, c->nodes.p, c->visit.p, c->dcl.p, c->ncl,
+            launch_comp_round(c->dim, c->n, c->pts.p, c->ids.p, c->nodes.p, c->visit.p, c->dcl.p, c->ncl,
                               c->nleaves, c->bucket, leaf_table(c), c->label.p, c->corr.p, c->corr_key.p, c->comp1.p,
                               c->inv.p, c->seed_scale, split_scratch(c), comp_seed(c), nbr_list(c), c->eu.p,
                               c->eround.p, c->C, s, tr ? c->ev + 8 : nullptr);
             if (tr) (void)hipEventRecord(c->ev[4], s);
             continue;
         }
-        launch_round_phase(ph, c->K, c->n, c->pts.p, c->rkey.p, c->rpos.p, c->cursor.p,
+        launch_round_phase(ph, c->K, c->n, c->ids.p, c->rkey.p, c->rpos.p, c->cursor.p,
                            c->label.p, c->label2.p,
                            c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->fixed.p, c->C, c->eu.p,
                            c->ev_.p, c->ed2.p, c->eround.p, cy, s);
@@ -741,7 +743,7 @@ int resume_flatten(emst_ctx* c, Counters* h)
     for (int again = 0; h->unflat; ++again) {
         if (again == 64) return fail(c, EMST_E_HIP, "flatten did not converge (cycle in the hook forest)");
         HIPCHK(c, hipMemsetAsync(&c->C->unflat, 0, sizeof(uint32_t), s));
-        launch_round_phase(3, c->K, c->n, c->pts.p, c->rkey.p, c->rpos.p, c->cursor.p,
+        launch_round_phase(3, c->K, c->n, c->ids.p, c->rkey.p, c->rpos.p, c->cursor.p,
                            c->label.p, c->label2.p,
                            c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->fixed.p, c->C, c->eu.p,
                            c->ev_.p, c->ed2.p, c->eround.p, CycleState{}, s);
@@ -960,7 +962,7 @@ void issue_one_launch_build(emst_ctx* c)
         key_bits += cb;
     }
     radix_sort(c->keys0.p, c->vals0.p, c->keys1.p, c->vals1.p, n, key_bits, true, c->hist.p, c->scan_tmp.p, s);
-    launch_gather_sorted(c->resc.p, c->vals0.p, n, c->dim, c->pts.p, c->inv.p, s);
+    launch_gather_sorted(c->resc.p, c->vals0.p, n, c->dim, c->pts.p, c->inv.p, c->ids.p, s);
     launch_leaves(c->keys0.p, n, c->dcl.p, c->ncl, c->bucket, c->vals1.p, reinterpret_cast<uint32_t*>(c->keys1.p),
                   c->scan_tmp.p, c->lstart.p, c->leaf_of.p, c->cloud_lf.p, c->P, c->C, true, s);
     // dup_runs .. cycles_cut start at zero (a kernel: no memset nodes in graphs)

@@ -176,7 +176,7 @@ void radix_sort(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, int64_t 
 size_t radix_hist_words(int64_t n);
 size_t scan_tmp_words(int64_t n);
 void launch_gather_sorted(const float* resc, const uint32_t* perm, int64_t n, int dim, float4* pts,
-                          int32_t* inv, hipStream_t s);
+                          int32_t* inv, uint32_t* ids, hipStream_t s);
 // Coincident-point groups from the sorted keys and points: fills rep[] (all
 // n entries), list[], gslot[] (representatives) and C->dup_* / C->grouped.
 // runs: n words of scratch.  The dup counters must be zero on entry.
@@ -260,7 +260,8 @@ struct NbrList {
     uint64_t* key;          // kKnn * n
     uint8_t* cur;           // n
 };
-void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsigned long long* visit,
+void launch_comp_round(int dim, int64_t n, const float4* pts, const uint32_t* ids, Node* nodes,
+                       unsigned long long* visit,
                        const Cloud* cl, int ncl, int64_t nleaves, int bucket, LeafTable lt, const int32_t* label, int32_t* corr, uint64_t* corr_key, uint64_t* comp1, const int32_t* inv,
                        float seed_scale, const SplitScratch& sc, const CompSeed& cs, const NbrList& nb,
                        const int32_t* eu, const int32_t* eround, Counters* C, hipStream_t s,
@@ -302,7 +303,7 @@ void launch_rank_cones(int dim, int64_t n, const uint64_t* cand_key, const int32
                        const int32_t* qlist, int64_t nq, uint64_t* rkey, int32_t* rpos, uint8_t* cursor,
                        const Counters* C, hipStream_t s);
 // rkey / rpos: the ranked candidates of k_rank_cones (point-major, K per point)
-void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const uint64_t* rkey,
+void launch_round_phase(int phase, int K, int64_t n, const uint32_t* ids, const uint64_t* rkey,
                         const int32_t* rpos, uint8_t* cursor, int32_t* label,
                         int32_t* label2, int32_t* corr,
                         uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, int32_t* fixed,

@@ -477,11 +477,12 @@ template <int D>
 __global__ __launch_bounds__(kBlock) void k_gather_sorted(const float* __restrict__ resc,
                                                          const uint32_t* __restrict__ perm,
                                                          int64_t n, float4* __restrict__ pts,
-                                                         int32_t* __restrict__ inv)
+                                                         int32_t* __restrict__ inv, uint32_t* __restrict__ ids)
 {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const uint32_t o = perm[s];
+    ids[s] = o;
     float4 p;
     p.x = resc[(int64_t)o * D + 0];
     p.y = resc[(int64_t)o * D + 1];
@@ -1728,7 +1729,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_cones(int64_t n, const uint64_t
 // differs: the cones are ranked once (k_rank_cones) and cursor[p] skips the
 // ones already found inside the component, which never leave it (components
 // only merge), so each candidate is looked at about once over all rounds.
-__global__ __launch_bounds__(kBlock) void k_round_min(int K, int64_t n, const float4* __restrict__ pts,
+__global__ __launch_bounds__(kBlock) void k_round_min(int K, int64_t n, const uint32_t* __restrict__ ids,
                                                      const uint64_t* __restrict__ rkey,
                                                      const int32_t* __restrict__ rpos,
                                                      uint8_t* __restrict__ cursor,
@@ -1798,7 +1799,7 @@ __global__ __launch_bounds__(kBlock) void k_round_min(int K, int64_t n, const fl
                 corr_key[p] = ck;
             }
             if (c >= 0) {
-                const uint32_t pid = __float_as_uint(pts[p].w);
+                const uint32_t pid = ids[p];
                 const uint32_t qid = (uint32_t)ck;
                 key1 = (ck & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
                 valid = true;
@@ -1809,7 +1810,7 @@ __global__ __launch_bounds__(kBlock) void k_round_min(int K, int64_t n, const fl
 }
 
 // second phase: among members matching (d2, min id), the smallest max id.
-__global__ __launch_bounds__(kBlock) void k_round_comp2(int64_t n, const float4* __restrict__ pts,
+__global__ __launch_bounds__(kBlock) void k_round_comp2(int64_t n, const uint32_t* __restrict__ ids,
                                                        const int32_t* __restrict__ label,
                                                        const int32_t* __restrict__ corr,
                                                        const uint64_t* __restrict__ corr_key,
@@ -1821,14 +1822,14 @@ __global__ __launch_bounds__(kBlock) void k_round_comp2(int64_t n, const float4*
     if (p >= n || corr[p] < 0) return;
     const int32_t lp = label[p];
     const uint64_t ck = corr_key[p];
-    const uint32_t pid = __float_as_uint(pts[p].w), qid = (uint32_t)ck;
+    const uint32_t pid = ids[p], qid = (uint32_t)ck;
     const uint64_t key1 = (ck & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
     if (key1 == comp1[lp]) atomicMin(&comp2[lp], max(pid, qid));
 }
 
 // winner flag (fixedMap, NodeEMST.h:947-949), stamped with the round number:
 // fixed[p] == round marks this round's winners, and older stamps need no reset
-__global__ __launch_bounds__(kBlock) void k_round_winner(int64_t n, const float4* __restrict__ pts,
+__global__ __launch_bounds__(kBlock) void k_round_winner(int64_t n, const uint32_t* __restrict__ ids,
                                                         const int32_t* __restrict__ label,
                                                         const int32_t* __restrict__ corr,
                                                         const uint64_t* __restrict__ corr_key,
@@ -1841,7 +1842,7 @@ __global__ __launch_bounds__(kBlock) void k_round_winner(int64_t n, const float4
     if (p >= n || corr[p] < 0) return;
     const int32_t lp = label[p];
     const uint64_t ck = corr_key[p];
-    const uint32_t pid = __float_as_uint(pts[p].w), qid = (uint32_t)ck;
+    const uint32_t pid = ids[p], qid = (uint32_t)ck;
     const uint64_t key1 = (ck & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
     if (key1 == comp1[lp] && max(pid, qid) == comp2[lp]) fixed[p] = (int32_t)C->round;
 }
@@ -1857,7 +1858,7 @@ __global__ __launch_bounds__(kBlock) void k_round_winner(int64_t n, const float4
 // count is reduced per block.
 constexpr int kHookBlock = 1024;
 
-__global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const float4* __restrict__ pts,
+__global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const uint32_t* __restrict__ ids,
                                                           const int32_t* __restrict__ label,
                                                           int32_t* __restrict__ label2,
                                                           const int32_t* __restrict__ corr,
@@ -1879,7 +1880,7 @@ __global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const floa
     if (p < n && fixed[p] == round && !(cut && round > 1 && cut[label[p]])) {
         const int32_t q = corr[p];
         const uint64_t ck = corr_key[p];
-        const uint32_t pid = __float_as_uint(pts[p].w), qid = (uint32_t)ck;
+        const uint32_t pid = ids[p], qid = (uint32_t)ck;
         const int32_t rp = label[p], rq = label[q];
         int32_t absorbed = -1;
         if (fixed[q] != round || corr[q] != (int32_t)p) {
@@ -2039,7 +2040,7 @@ __global__ __launch_bounds__(kBlock) void k_nmin_tree(const float4* __restrict__
 //               on the d2 of the nearest outside point (all ones: none);
 //   corr == -2  nothing outside the component in its cloud (final).
 // Keep-previous: the component minimum of the kept targets.
-__global__ __launch_bounds__(kBlock) void k_comp_keep(int64_t n, const float4* __restrict__ pts,
+__global__ __launch_bounds__(kBlock) void k_comp_keep(int64_t n, const uint32_t* __restrict__ ids,
                                                      const int32_t* __restrict__ label,
                                                      int32_t* __restrict__ corr,
                                                      uint64_t* __restrict__ corr_key,
@@ -2084,7 +2085,7 @@ __global__ __launch_bounds__(kBlock) void k_comp_keep(int64_t n, const float4* _
         }
         if (c >= 0 && label[c] != lp) {
             if (!have_ck) ck = corr_key[p];
-            const uint32_t pid = __float_as_uint(pts[p].w), qid = (uint32_t)ck;
+            const uint32_t pid = ids[p], qid = (uint32_t)ck;
             key1 = (ck & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
             valid = true;
         }
@@ -2095,8 +2096,14 @@ __global__ __launch_bounds__(kBlock) void k_comp_keep(int64_t n, const float4* _
 // The points that must search, flagged per 64-position packet: those without
 // a kept target whose lower bound does not already exceed the component's
 // kept minimum (such a point cannot hold the component's shortest edge this
-// round; it keeps its bound for the next).
-__global__ __launch_bounds__(kBlock) void k_comp_prep(int64_t n, const int32_t* __restrict__ label,
+// round; it keeps its bound for the next), and that do not sit at the same
+// location as the position before them in the same component.  Such a point
+// is dominated: equal keys sort stably, so the point before it has a smaller
+// id, the same nearest outside point at the same d2, and so a key (d2, min
+// id, max id) at least as small; the first of the run searches for both (the
+// coincident corners of a clamped cloud are runs of 10^4-10^5 points).
+__global__ __launch_bounds__(kBlock) void k_comp_prep(int64_t n, const float4* __restrict__ pts,
+                                                     const int32_t* __restrict__ label,
                                                      int32_t* __restrict__ corr,
                                                      const uint64_t* __restrict__ corr_key,
                                                      const uint64_t* __restrict__ comp1,
@@ -2112,6 +2119,7 @@ __global__ __launch_bounds__(kBlock) void k_comp_prep(int64_t n, const int32_t* 
             const uint32_t lb = (uint32_t)(corr_key[p] >> 32);
             const int32_t lbb = lb == 0xffffffffu ? -1 : (int32_t)lb;
             need = lbb <= key_bound(comp1[lp]);
+            if (need && p > 0 && label[p - 1] == lp && same_xyz(pts[p - 1], pts[p])) need = false;
             if (!need) corr[p] = -1;    // the absorbed target's d2 stays as the bound
         }
     }
@@ -3762,11 +3770,13 @@ void radix_sort(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, int64_t 
 }
 
 void launch_gather_sorted(const float* resc, const uint32_t* perm, int64_t n, int dim, float4* pts,
-                          int32_t* inv, hipStream_t s)
+                          int32_t* inv, uint32_t* ids, hipStream_t s)
 {
     if (n <= 0) return;
-    if (dim == 3) hipLaunchKernelGGL(k_gather_sorted<3>, dim3(nblocks(n)), dim3(kBlock), 0, s, resc, perm, n, pts, inv);
-    else hipLaunchKernelGGL(k_gather_sorted<2>, dim3(nblocks(n)), dim3(kBlock), 0, s, resc, perm, n, pts, inv);
+    if (dim == 3)
+        hipLaunchKernelGGL(k_gather_sorted<3>, dim3(nblocks(n)), dim3(kBlock), 0, s, resc, perm, n, pts, inv, ids);
+    else
+        hipLaunchKernelGGL(k_gather_sorted<2>, dim3(nblocks(n)), dim3(kBlock), 0, s, resc, perm, n, pts, inv, ids);
 }
 
 void launch_dup_groups(const uint64_t* keys, const float4* pts, int64_t n, int32_t* rep,
@@ -3858,7 +3868,8 @@ void launch_comp_seed_init(int64_t n, const float4* pts, int32_t* csize, uint32_
     hipLaunchKernelGGL(k_comp_seed_init, dim3(nblocks(n)), dim3(kBlock), 0, s, n, pts, csize, cbox);
 }
 
-void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsigned long long* visit,
+void launch_comp_round(int dim, int64_t n, const float4* pts, const uint32_t* ids, Node* nodes,
+                       unsigned long long* visit,
                        const Cloud* cl, int ncl, int64_t nleaves,
                        int bucket, LeafTable lt,
                        const int32_t* label, int32_t* corr, uint64_t* corr_key, uint64_t* comp1, const int32_t* inv,
@@ -3873,9 +3884,9 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, Node* nodes, unsig
     hipLaunchKernelGGL(k_nlab_tree, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, label, cl, ncl, nleaves, lt,
                        nodes, visit, (const Counters*)C);
     const int64_t P = (n + 63) / 64;
-    hipLaunchKernelGGL(k_comp_keep, dim3(g), dim3(kBlock), 0, s, n, pts, label, corr, corr_key, comp1, nb,
+    hipLaunchKernelGGL(k_comp_keep, dim3(g), dim3(kBlock), 0, s, n, ids, label, corr, corr_key, comp1, nb,
                        (const Counters*)C);
-    hipLaunchKernelGGL(k_comp_prep, dim3(g), dim3(kBlock), 0, s, n, label, corr, (const uint64_t*)corr_key,
+    hipLaunchKernelGGL(k_comp_prep, dim3(g), dim3(kBlock), 0, s, n, pts, label, corr, (const uint64_t*)corr_key,
                        (const uint64_t*)comp1, sc.open_mask, (const Counters*)C);
     open_compact(sc.open_mask, P, sc.off, sc.tmp, nullptr, sc.list, s);
     const uint32_t* nq = sc.off + P;
@@ -3999,7 +4010,7 @@ void launch_rank_cones(int dim, int64_t n, const uint64_t* cand_key, const int32
                        cand_key, cand_pos, qlist, nq, rkey, rpos, cursor, C);
 }
 
-void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const uint64_t* rkey,
+void launch_round_phase(int phase, int K, int64_t n, const uint32_t* ids, const uint64_t* rkey,
                         const int32_t* rpos, uint8_t* cursor, int32_t* label,
                         int32_t* label2, int32_t* corr,
                         uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, int32_t* fixed,
@@ -4010,13 +4021,13 @@ void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const ui
     const unsigned g = nblocks(n);
     switch (phase) {
     case 0:
-        hipLaunchKernelGGL(k_round_min, dim3(g), dim3(kBlock), 0, s, K, n, pts, rkey, rpos, cursor,
+        hipLaunchKernelGGL(k_round_min, dim3(g), dim3(kBlock), 0, s, K, n, ids, rkey, rpos, cursor,
                            label, corr, corr_key, comp1, C);
         break;
     case 1:
-        hipLaunchKernelGGL(k_round_comp2, dim3(g), dim3(kBlock), 0, s, n, pts, label, corr, corr_key, comp1,
+        hipLaunchKernelGGL(k_round_comp2, dim3(g), dim3(kBlock), 0, s, n, ids, label, corr, corr_key, comp1,
                            comp2, C);
-        hipLaunchKernelGGL(k_round_winner, dim3(g), dim3(kBlock), 0, s, n, pts, label, corr, corr_key,
+        hipLaunchKernelGGL(k_round_winner, dim3(g), dim3(kBlock), 0, s, n, ids, label, corr, corr_key,
                            comp1, comp2, fixed, C);
         break;
     case 2:
@@ -4029,7 +4040,7 @@ void launch_round_phase(int phase, int K, int64_t n, const float4* pts, const ui
             hipLaunchKernelGGL(k_cycle_mark, dim3(gc), dim3(kBlock), 0, s, cy, C);
             hipLaunchKernelGGL(k_cycle_cut, dim3(gc), dim3(kBlock), 0, s, comp1, comp2, cy, C);
         }
-        hipLaunchKernelGGL(k_round_hook, dim3(nblocks(n, kHookBlock)), dim3(kHookBlock), 0, s, n, pts, label,
+        hipLaunchKernelGGL(k_round_hook, dim3(nblocks(n, kHookBlock)), dim3(kHookBlock), 0, s, n, ids, label,
                            label2, corr, corr_key, fixed, C, eu, ev, ed2, eround, cy.j0 ? cy.cut : nullptr);
         break;
     case 3:
