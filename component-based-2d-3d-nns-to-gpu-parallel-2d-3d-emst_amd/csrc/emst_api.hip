@@ -107,7 +107,7 @@ struct emst_ctx {
     DevBuf<uint64_t> keys0, keys1, cand_key, corr_key, comp1;
     DevBuf<uint32_t> vals0, vals1, hist, scan_tmp, comp2;
     DevBuf<float4> pts;
-    DevBuf<int32_t> inv, cand_pos, label, label2, corr, fixed, list, eu, ev_, eround, geo;
+    DevBuf<int32_t> inv, cand_pos, label, label2, corr, list, eu, ev_, eround, geo;
     DevBuf<uint32_t> ids;              // input id of each sorted position (the w of pts, dense)
     DevBuf<Node> nodes;
     DevBuf<int32_t> lstart, leaf_of, cloud_lf;     // LeafTable and each cloud's first leaf
@@ -247,7 +247,6 @@ int alloc_all(emst_ctx* c)
     ENS(label, n);
     ENS(label2, n);
     ENS(corr, n);
-    ENS(fixed, n);
     ENS(list, n);
     ENS(corr_key, n);
     ENS(comp1, n);
@@ -550,7 +549,7 @@ int build(emst_ctx* c)
     }
     const DupState dup = dup_state(c);
     launch_build_tree(c->pts.p, c->dcl.p, c->ncl, c->nleaves, c->Lmax, leaf_table(c), c->nodes.p, dup.rep, s);
-    launch_round_init(n, c->label.p, c->label2.p, c->fixed.p, c->eu.p, c->corr.p, c->corr_key.p, c->comp1.p,
+    launch_round_init(n, c->label.p, c->label2.p, c->eu.p, c->corr.p, c->corr_key.p, c->comp1.p,
                       c->comp2.p, c->C, (uint32_t)c->target_edges, (uint32_t)c->opt.max_rounds, s);
     LAUNCHCHK(c);
     if (comp) {
@@ -727,7 +726,7 @@ void issue_round(emst_ctx* c, bool tr, unsigned long long* stamps = nullptr)
         }
         launch_round_phase(ph, c->K, c->n, c->ids.p, c->rkey.p, c->rpos.p, c->cursor.p,
                            c->label.p, c->label2.p,
-                           c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->fixed.p, c->C, c->eu.p,
+                           c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->C, c->eu.p,
                            c->ev_.p, c->ed2.p, c->eround.p, cy, s);
         if (tr) (void)hipEventRecord(c->ev[4 + ph], s);
     }
@@ -745,7 +744,7 @@ int resume_flatten(emst_ctx* c, Counters* h)
         HIPCHK(c, hipMemsetAsync(&c->C->unflat, 0, sizeof(uint32_t), s));
         launch_round_phase(3, c->K, c->n, c->ids.p, c->rkey.p, c->rpos.p, c->cursor.p,
                            c->label.p, c->label2.p,
-                           c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->fixed.p, c->C, c->eu.p,
+                           c->corr.p, c->corr_key.p, c->comp1.p, c->comp2.p, c->C, c->eu.p,
                            c->ev_.p, c->ed2.p, c->eround.p, CycleState{}, s);
         LAUNCHCHK(c);
         if ((rc = read_counters(c, h))) return rc;
@@ -859,7 +858,7 @@ std::vector<uintptr_t> graph_key(emst_ctx* c)
             (uintptr_t)c->grouped, (uintptr_t)d.npoints,
             (uintptr_t)d.ngroups, (uintptr_t)c->C, (uintptr_t)c->pts.p, (uintptr_t)c->cand_key.p,
             (uintptr_t)c->cand_pos.p, (uintptr_t)c->label.p, (uintptr_t)c->label2.p, (uintptr_t)c->corr.p,
-            (uintptr_t)c->corr_key.p, (uintptr_t)c->comp1.p, (uintptr_t)c->comp2.p, (uintptr_t)c->fixed.p,
+            (uintptr_t)c->corr_key.p, (uintptr_t)c->comp1.p, (uintptr_t)c->comp2.p,
             (uintptr_t)c->eu.p, (uintptr_t)c->ev_.p, (uintptr_t)c->ed2.p, (uintptr_t)c->eround.p,
             (uintptr_t)c->inv.p, (uintptr_t)c->rkey.p, (uintptr_t)c->rpos.p, (uintptr_t)c->cursor.p, (uintptr_t)d.rep, (uintptr_t)d.list, (uintptr_t)d.gslot, (uintptr_t)d.m2,
             (uintptr_t)d.bkey, (uintptr_t)d.bpos};
@@ -968,7 +967,7 @@ void issue_one_launch_build(emst_ctx* c)
     // dup_runs .. cycles_cut start at zero (a kernel: no memset nodes in graphs)
     launch_zero_u32(&c->C->dup_runs, 8, s);
     launch_build_tree(c->pts.p, c->dcl.p, c->ncl, c->nleaves, c->Lmax, leaf_table(c), c->nodes.p, nullptr, s);
-    launch_round_init(n, c->label.p, c->label2.p, c->fixed.p, c->eu.p, c->corr.p, c->corr_key.p, c->comp1.p,
+    launch_round_init(n, c->label.p, c->label2.p, c->eu.p, c->corr.p, c->corr_key.p, c->comp1.p,
                       c->comp2.p, c->C, (uint32_t)c->target_edges, (uint32_t)c->opt.max_rounds, s);
     launch_zero_u32(reinterpret_cast<uint32_t*>(c->visit.p), 4 * c->nleaves, s);
     launch_node_min_ids(c->pts.p, c->dcl.p, c->ncl, c->nleaves, leaf_table(c), c->nodes.p, c->visit.p, s);

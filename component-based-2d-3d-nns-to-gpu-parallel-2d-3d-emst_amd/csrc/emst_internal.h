@@ -291,11 +291,11 @@ void launch_node_min_ids(const float4* pts, const Cloud* cl, int ncl, int64_t nl
 void launch_dup_fill(int dim, int64_t n, const int32_t* label, const float4* pts, const int32_t* inv,
                      uint64_t* cand_key, int32_t* cand_pos, int32_t* revive, int32_t* first_corr,
                      uint64_t* first_key, Counters* C, DupState dup, hipStream_t s);
-void launch_round_init(int64_t n, int32_t* label, int32_t* label2, int32_t* fixed, int32_t* eu, int32_t* corr,
+void launch_round_init(int64_t n, int32_t* label, int32_t* label2, int32_t* eu, int32_t* corr,
                        uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, Counters* C, uint32_t target,
                        uint32_t max_rounds, hipStream_t s);
 // Phase of a round: 0 per-point target + component min (first key), 1 second
-// key + winners, 2 hook, 3 flatten.
+// key, 2 winners and hook, 3 flatten.
 // Ranks each point's cones by key for k_round_min (all points, or the qlist
 // points); C != nullptr: skipped once the rounds are done or groups settled.
 // dim: the cone dimension.
@@ -306,7 +306,7 @@ void launch_rank_cones(int dim, int64_t n, const uint64_t* cand_key, const int32
 void launch_round_phase(int phase, int K, int64_t n, const uint32_t* ids, const uint64_t* rkey,
                         const int32_t* rpos, uint8_t* cursor, int32_t* label,
                         int32_t* label2, int32_t* corr,
-                        uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, int32_t* fixed,
+                        uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2,
                         Counters* C, int32_t* eu, int32_t* ev, float* ed2, int32_t* eround,
                         const CycleState& cy, hipStream_t s);
 // Round control: begin (termination test, round += 1) and end (no progress

@@ -1539,7 +1539,6 @@ __global__ __launch_bounds__(kBlock) void k_dup_fill(int64_t n, const int32_t* _
 // ------------------------------------------------------- Boruvka rounds
 __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __restrict__ label,
                                                       int32_t* __restrict__ label2,
-                                                      int32_t* __restrict__ fixed,
                                                       int32_t* __restrict__ eu,
                                                       int32_t* __restrict__ corr,
                                                       uint64_t* __restrict__ corr_key,
@@ -1560,7 +1559,6 @@ __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __res
     if (i >= n) return;
     label[i] = (int32_t)i;          // K_EMST_initDisjointSet (EMSTOperators.h:330-343)
     label2[i] = (int32_t)i;
-    fixed[i] = 0;                   // round stamps start at 1
     eu[i] = -1;                     // no edge absorbed this root yet
     corr[i] = -1;                   // correspondenceMap reset (SolutionEMST.cu:464-466)
     corr_key[i] = kEmptyKey;
@@ -1809,6 +1807,12 @@ __global__ __launch_bounds__(kBlock) void k_round_min(int K, int64_t n, const ui
     }
 }
 
+// a member's edge key, high part: (d2, min id); the max id completes it
+__device__ __forceinline__ uint64_t edge_key1(uint64_t ck, uint32_t pid)
+{
+    return (ck & 0xffffffff00000000ull) | (uint64_t)min(pid, (uint32_t)ck);
+}
+
 // second phase: among members matching (d2, min id), the smallest max id.
 __global__ __launch_bounds__(kBlock) void k_round_comp2(int64_t n, const uint32_t* __restrict__ ids,
                                                        const int32_t* __restrict__ label,
@@ -1822,40 +1826,24 @@ __global__ __launch_bounds__(kBlock) void k_round_comp2(int64_t n, const uint32_
     if (p >= n || corr[p] < 0) return;
     const int32_t lp = label[p];
     const uint64_t ck = corr_key[p];
-    const uint32_t pid = ids[p], qid = (uint32_t)ck;
-    const uint64_t key1 = (ck & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
-    if (key1 == comp1[lp]) atomicMin(&comp2[lp], max(pid, qid));
-}
-
-// winner flag (fixedMap, NodeEMST.h:947-949), stamped with the round number:
-// fixed[p] == round marks this round's winners, and older stamps need no reset
-__global__ __launch_bounds__(kBlock) void k_round_winner(int64_t n, const uint32_t* __restrict__ ids,
-                                                        const int32_t* __restrict__ label,
-                                                        const int32_t* __restrict__ corr,
-                                                        const uint64_t* __restrict__ corr_key,
-                                                        const uint64_t* __restrict__ comp1,
-                                                        const uint32_t* __restrict__ comp2,
-                                                        int32_t* __restrict__ fixed, const Counters* C)
-{
-    if (C->done) return;
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n || corr[p] < 0) return;
-    const int32_t lp = label[p];
-    const uint64_t ck = corr_key[p];
-    const uint32_t pid = ids[p], qid = (uint32_t)ck;
-    const uint64_t key1 = (ck & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
-    if (key1 == comp1[lp] && max(pid, qid) == comp2[lp]) fixed[p] = (int32_t)C->round;
+    const uint32_t pid = ids[p];
+    if (edge_key1(ck, pid) == comp1[lp]) atomicMin(&comp2[lp], max(pid, (uint32_t)ck));
 }
 
 // hook/union (EMSTOperators.h:1267-1303): roots read from the flattened
 // labels, written into label2.  label2 is a parent forest whose chase from
 // any point ends at its component root (k_round_init starts it as the
 // identity; only roots are written here and the flatten only halves paths),
-// so it needs no copy of the labels between rounds.  Non-mutual: root(p) -> root(q).  Mutual pair:
-// the endpoint with the smaller id keeps its root.  Each hook absorbs exactly
-// one root and a root is absorbed at most once over the run, so the edge is
-// stored at the absorbed root's index (eu = -1 marks an empty slot); the edge
-// count is reduced per block.
+// so it needs no copy of the labels between rounds.
+// The winner of a component (fixedMap, NodeEMST.h:947-949) is the member
+// whose edge key (d2, min id, max id) equals the component's comp1 / comp2;
+// the pair of ids makes it unique, so the test is made here, for p and for
+// its target q, with no per-point flag between passes.
+// Non-mutual: root(p) -> root(q).  Mutual pair (q won its component with an
+// edge back to p): the endpoint with the smaller id keeps its root.  Each
+// hook absorbs exactly one root and a root is absorbed at most once over the
+// run, so the edge is stored at the absorbed root's index (eu = -1 marks an
+// empty slot); the edge count is reduced per block.
 constexpr int kHookBlock = 1024;
 
 __global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const uint32_t* __restrict__ ids,
@@ -1863,7 +1851,8 @@ __global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const uint
                                                           int32_t* __restrict__ label2,
                                                           const int32_t* __restrict__ corr,
                                                           const uint64_t* __restrict__ corr_key,
-                                                          const int32_t* __restrict__ fixed,
+                                                          const uint64_t* __restrict__ comp1,
+                                                          const uint32_t* __restrict__ comp2,
                                                           Counters* C, int32_t* __restrict__ eu,
                                                           int32_t* __restrict__ ev,
                                                           float* __restrict__ ed2,
@@ -1877,25 +1866,35 @@ __global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const uint
     __syncthreads();
     uint32_t added = 0;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n && fixed[p] == round && !(cut && round > 1 && cut[label[p]])) {
-        const int32_t q = corr[p];
+    const int32_t q = p < n ? corr[p] : -1;
+    if (q >= 0) {
+        const int32_t rp = label[p];
         const uint64_t ck = corr_key[p];
         const uint32_t pid = ids[p], qid = (uint32_t)ck;
-        const int32_t rp = label[p], rq = label[q];
-        int32_t absorbed = -1;
-        if (fixed[q] != round || corr[q] != (int32_t)p) {
-            label2[rp] = rq;
-            absorbed = rp;
-        } else if (pid < qid) {
-            label2[rq] = rp;
-            absorbed = rq;
-        }
-        if (absorbed >= 0) {
-            ++added;
-            eu[absorbed] = (int32_t)min(pid, qid);
-            ev[absorbed] = (int32_t)max(pid, qid);
-            ed2[absorbed] = __uint_as_float((uint32_t)(ck >> 32));
-            eround[absorbed] = round;
+        const uint64_t key1 = edge_key1(ck, pid);
+        if (key1 == comp1[rp] && max(pid, qid) == comp2[rp] && !(cut && round > 1 && cut[rp])) {
+            const int32_t rq = label[q];
+            bool mutual = false;            // q won its component with the edge back to p
+            if (corr[q] == (int32_t)p) {
+                const uint64_t cq = corr_key[q];
+                const uint32_t qq = ids[q];
+                mutual = edge_key1(cq, qq) == comp1[rq] && max(qq, (uint32_t)cq) == comp2[rq];
+            }
+            int32_t absorbed = -1;
+            if (!mutual) {
+                label2[rp] = rq;
+                absorbed = rp;
+            } else if (pid < qid) {
+                label2[rq] = rp;
+                absorbed = rq;
+            }
+            if (absorbed >= 0) {
+                ++added;
+                eu[absorbed] = (int32_t)min(pid, qid);
+                ev[absorbed] = (int32_t)max(pid, qid);
+                ed2[absorbed] = __uint_as_float((uint32_t)(key1 >> 32));
+                eround[absorbed] = round;
+            }
         }
     }
     for (int off = 32; off > 0; off >>= 1) added += (uint32_t)__shfl_xor((int)added, off);
@@ -2054,36 +2053,49 @@ __global__ __launch_bounds__(kBlock) void k_comp_keep(int64_t n, const uint32_t*
     if (p < n) {
         lp = label[p];
         int32_t c = corr[p];
+        const int j0 = nb.pos != nullptr ? (int)nb.cur[p] : kKnn;
+        bool outside = c >= 0 && label[c] != lp;
         uint64_t ck = 0;
         bool have_ck = false;
         // a target that joined the component: the next listed point outside
         // it (components only grow, so the listed points before it are inside
-        // for good); the list used up, its last key bounds the search
-        if (c >= 0 && label[c] == lp && nb.pos != nullptr) {
-            int j = nb.cur[p];
-            if (j < kKnn) {
-                int32_t e = -1;
-                for (++j; j < kKnn; ++j) {
-                    e = nb.pos[(int64_t)j * n + p];
-                    if (e < 0 || label[e] != lp) break;
+        // for good); the list used up, its last key bounds the search.  The
+        // remaining entries, then their labels, load at once: a scan entry by
+        // entry waited on two dependent loads per entry (the kernel took up to
+        // 0.36 ms per round at 10^7 points, when every target has just joined).
+        if (c >= 0 && !outside && j0 < kKnn) {
+            int32_t e[kKnn], le[kKnn];
+#pragma unroll
+            for (int k = 1; k < kKnn; ++k) e[k] = k > j0 ? nb.pos[(int64_t)k * n + p] : -1;
+            // (unconditional loads, an absent entry reading p's own label, so
+            // that they issue together rather than one branch and wait each)
+#pragma unroll
+            for (int k = 1; k < kKnn; ++k) le[k] = label[e[k] >= 0 ? e[k] : (int32_t)p];
+            int j = kKnn;
+            int32_t f = -1;
+#pragma unroll
+            for (int k = kKnn - 1; k >= 1; --k)
+                if (k > j0 && (e[k] < 0 || le[k] != lp)) {
+                    j = k;
+                    f = e[k];
                 }
-                if (j == kKnn) {
-                    c = -1;
-                    ck = (nb.key[(int64_t)(kKnn - 1) * n + p] & 0xffffffff00000000ull) | 0xffffffffull;
-                } else if (e < 0) {
-                    c = -2;             // the cloud's other points are all listed, all inside
-                    ck = kEmptyKey;
-                } else {
-                    c = e;
-                    ck = nb.key[(int64_t)j * n + p];
-                    have_ck = true;
-                }
-                nb.cur[p] = (uint8_t)j;
-                corr[p] = c;
-                corr_key[p] = ck;
+            if (j == kKnn) {
+                c = -1;
+                ck = (nb.key[(int64_t)(kKnn - 1) * n + p] & 0xffffffff00000000ull) | 0xffffffffull;
+            } else if (f < 0) {
+                c = -2;             // the cloud's other points are all listed, all inside
+                ck = kEmptyKey;
+            } else {
+                c = f;
+                ck = nb.key[(int64_t)j * n + p];
+                have_ck = true;
+                outside = true;
             }
+            nb.cur[p] = (uint8_t)j;
+            corr[p] = c;
+            corr_key[p] = ck;
         }
-        if (c >= 0 && label[c] != lp) {
+        if (outside) {
             if (!have_ck) ck = corr_key[p];
             const uint32_t pid = ids[p], qid = (uint32_t)ck;
             key1 = (ck & 0xffffffff00000000ull) | (uint64_t)min(pid, qid);
@@ -3500,7 +3512,10 @@ __global__ __launch_bounds__(kBlock) void k_cycle_cut(const uint64_t* __restrict
 // flatten (EMSTOperators.h:412-433): chase label2 to the root, with path
 // halving on label2 (every write stores an ancestor, so concurrent chases stay
 // valid and label2 stays a parent forest); reset the component minima for the
-// next round.
+// next round.  Only this round's roots (label[p] == p) can hold minima (every
+// update goes to a point's root), and a label that stays is not rewritten:
+// later rounds absorb few roots, and the stores of unchanged values took most
+// of the pass.
 // Each launch takes at most kFlattenSteps steps per point; a point still off
 // its root raises C->unflat and the host launches the kernel again, so every
 // wave ends even if the hook forest were malformed.
@@ -3514,6 +3529,7 @@ __global__ __launch_bounds__(kBlock) void k_round_flatten(int64_t n, int32_t* __
     if (C->done == 1u) return;      // a paused round (done == 2) still flattens
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    const int32_t was = label[p];
     int32_t r = label2[p];
     int32_t nx = label2[r];
     int steps = 0;
@@ -3528,9 +3544,11 @@ __global__ __launch_bounds__(kBlock) void k_round_flatten(int64_t n, int32_t* __
     // p itself now points at the root (or the farthest ancestor reached), so
     // the next round's chase from p starts there
     if (steps > 0) label2[p] = r;
-    label[p] = r;
-    comp1[p] = kEmptyKey;
-    comp2[p] = 0xffffffffu;
+    if (r != was) label[p] = r;
+    if (was == (int32_t)p) {
+        comp1[p] = kEmptyKey;
+        comp2[p] = 0xffffffffu;
+    }
 }
 
 // ------------------------------------------------------------- weight
@@ -3971,12 +3989,12 @@ void launch_dup_fill(int dim, int64_t n, const int32_t* label, const float4* pts
                            revive, first_corr, first_key, C, dup);
 }
 
-void launch_round_init(int64_t n, int32_t* label, int32_t* label2, int32_t* fixed, int32_t* eu, int32_t* corr,
+void launch_round_init(int64_t n, int32_t* label, int32_t* label2, int32_t* eu, int32_t* corr,
                        uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, Counters* C, uint32_t target,
                        uint32_t max_rounds, hipStream_t s)
 {
     hipLaunchKernelGGL(k_round_init, dim3(nblocks(std::max<int64_t>(n, kWorkKinds * kWorkSlots))), dim3(kBlock), 0, s, n, label,
-                       label2, fixed, eu, corr, corr_key, comp1, comp2, C, target, max_rounds);
+                       label2, eu, corr, corr_key, comp1, comp2, C, target, max_rounds);
 }
 
 void launch_round_begin(Counters* C, hipStream_t s)
@@ -4013,7 +4031,7 @@ void launch_rank_cones(int dim, int64_t n, const uint64_t* cand_key, const int32
 void launch_round_phase(int phase, int K, int64_t n, const uint32_t* ids, const uint64_t* rkey,
                         const int32_t* rpos, uint8_t* cursor, int32_t* label,
                         int32_t* label2, int32_t* corr,
-                        uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2, int32_t* fixed,
+                        uint64_t* corr_key, uint64_t* comp1, uint32_t* comp2,
                         Counters* C, int32_t* eu, int32_t* ev, float* ed2, int32_t* eround,
                         const CycleState& cy, hipStream_t s)
 {
@@ -4027,8 +4045,6 @@ void launch_round_phase(int phase, int K, int64_t n, const uint32_t* ids, const 
     case 1:
         hipLaunchKernelGGL(k_round_comp2, dim3(g), dim3(kBlock), 0, s, n, ids, label, corr, corr_key, comp1,
                            comp2, C);
-        hipLaunchKernelGGL(k_round_winner, dim3(g), dim3(kBlock), 0, s, n, ids, label, corr, corr_key,
-                           comp1, comp2, fixed, C);
         break;
     case 2:
         if (cy.j0) {
@@ -4041,7 +4057,7 @@ void launch_round_phase(int phase, int K, int64_t n, const uint32_t* ids, const 
             hipLaunchKernelGGL(k_cycle_cut, dim3(gc), dim3(kBlock), 0, s, comp1, comp2, cy, C);
         }
         hipLaunchKernelGGL(k_round_hook, dim3(nblocks(n, kHookBlock)), dim3(kHookBlock), 0, s, n, ids, label,
-                           label2, corr, corr_key, fixed, C, eu, ev, ed2, eround, cy.j0 ? cy.cut : nullptr);
+                           label2, corr, corr_key, comp1, comp2, C, eu, ev, ed2, eround, cy.j0 ? cy.cut : nullptr);
         break;
     case 3:
         hipLaunchKernelGGL(k_round_flatten, dim3(g), dim3(kBlock), 0, s, n, label, label2, comp1, comp2, C);
