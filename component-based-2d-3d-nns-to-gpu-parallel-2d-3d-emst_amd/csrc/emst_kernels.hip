@@ -49,6 +49,17 @@ __device__ __forceinline__ uint64_t lanemask_lt()
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// maximum over each aligned group of 8 lanes, by DPP moves within rows (no
+// LDS round trip): swap neighbours, swap pairs, then mirror the half row
+// (lane i of the 8 reads lane 7 - i)
+__device__ __forceinline__ uint32_t group8_max_u32(uint32_t v)
+{
+    v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false));     // quad_perm [1,0,3,2]
+    v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false));     // quad_perm [2,3,0,1]
+    v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false));    // row_half_mirror
+    return v;
+}
+
 // ------------------------------------------------------------ clouds
 // largest c with cl[c].pos <= p (the cloud holding global point p)
 __device__ __forceinline__ int find_cloud(const Cloud* cl, int ncl, int64_t p)
@@ -1963,18 +1974,29 @@ struct MinOp {
     __device__ static uint32_t combine(uint32_t a, uint32_t b) { return min(a, b); }
 };
 
-// component label of every node: the common label of its points
+// component label of every node: the common label of its points.  A block
+// whose kBlock leaf slots are one aligned subtree of a cloud (every block of a
+// cloud with at least kBlock slots) combines the subtree's 8 lower levels in
+// LDS, and only the subtree's root climbs with atomics: the climb of every
+// leaf through those levels was a chain of dependent atomics per level.  The
+// arrival counts of the nodes inside such a subtree are then left alone,
+// which keeps their parity for the other climbs (k_nmin_tree).
 __global__ __launch_bounds__(kBlock) void k_nlab_tree(const int32_t* __restrict__ label,
                                                      const Cloud* __restrict__ cl, int ncl, int64_t nleaves,
                                                      LeafTable lt, Node* __restrict__ nodes,
                                                      unsigned long long* __restrict__ visit, const Counters* C)
 {
     if (C->done) return;
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nleaves) return;
-    const int c = find_cloud_leaf(cl, ncl, g);
+    __shared__ int32_t sub_lab[kBlock];
+    const int64_t g0 = (int64_t)blockIdx.x * kBlock;
+    const int cb = find_cloud_leaf(cl, ncl, g0);
+    const int64_t jb = g0 - cl[cb].lbase;
+    const bool subtree = (jb & (kBlock - 1)) == 0 && jb + kBlock <= cl[cb].L;
+    const int64_t g = g0 + threadIdx.x;
+    if (!subtree && g >= nleaves) return;
+    const int c = subtree ? cb : find_cloud_leaf(cl, ncl, g);
     const int64_t j = g - cl[c].lbase;
-    if (j >= cl[c].L) return;               // past the cloud's slots (one-launch capacity)
+    if (!subtree && j >= cl[c].L) return;   // past the cloud's slots (one-launch capacity)
     int64_t s0, s1;
     leaf_range(lt, cl[c].lfirst, cl[c].nl, cl[c].pos + cl[c].n, j, s0, s1);
     // the leaf's common label (kNlabMixed when two differ), its loads
@@ -1997,8 +2019,27 @@ __global__ __launch_bounds__(kBlock) void k_nlab_tree(const int32_t* __restrict_
     }
     Node* base = nodes + 2 * cl[c].lbase;
     auto store = [&](int64_t i, int32_t v) { base[i].pad[1] = __int_as_float(v); };
-    store(cl[c].L + j, lab);
-    climb<LabelOp>(lab, store, visit + 2 * cl[c].lbase, cl[c].L + j);
+    const int64_t L = cl[c].L;
+    store(L + j, lab);
+    if (!subtree) {
+        climb<LabelOp>(lab, store, visit + 2 * cl[c].lbase, L + j);
+        return;
+    }
+    // the subtree's levels: at level k (leaves at 0), thread t < kBlock >> k
+    // holds node ((L + jb) >> k) + t
+    const int t = (int)threadIdx.x;
+    sub_lab[t] = lab;
+    int32_t v = lab;
+    for (int k = 1, w = kBlock / 2; w >= 1; ++k, w >>= 1) {
+        __syncthreads();
+        if (t < w) v = LabelOp::combine(sub_lab[2 * t], sub_lab[2 * t + 1]);
+        __syncthreads();
+        if (t < w) {
+            sub_lab[t] = v;
+            store(((L + jb) >> k) + t, v);
+        }
+    }
+    if (t == 0) climb<LabelOp>(v, store, visit + 2 * cl[c].lbase, (L + jb) / kBlock);
 }
 
 // smallest point id under every node (static per build): with ties on d2
@@ -3065,13 +3106,14 @@ constexpr uint32_t kKnnBiasHi = (uint32_t)(kKnnBias >> 32);
 
 // returns min(a, x) and leaves max(a, x) in x, for doubles that are never NaN:
 // v_min_f64 / v_max_f64 without the canonicalising v_max_f64 the compiler
-// puts before each input of fmin / fmax
+// puts before each input of fmin / fmax.  The minimum is written over a's
+// register, so a network kd[i] = minmax_f64(kd[i], x) needs no copies.
 __device__ __forceinline__ double minmax_f64(double a, double& x)
 {
-    double lo, hi;
-    asm("v_min_f64 %0, %2, %3\n\tv_max_f64 %1, %2, %3" : "=&v"(lo), "=&v"(hi) : "v"(a), "v"(x));
+    double hi;
+    asm("v_max_f64 %1, %0, %2\n\tv_min_f64 %0, %0, %2" : "+v"(a), "=&v"(hi) : "v"(x));
     x = hi;
-    return lo;
+    return a;
 }
 template <int D>
 __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __restrict__ pts,
@@ -3182,10 +3224,7 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
             int64_t s0, s1;
             leaf_range(lt, lfirst, nl, cend, node - L, s0, s1);
             const int cnt = (int)(s1 - s0);
-            uint32_t t = need ? bound() : 0u;
-            t = max(t, (uint32_t)__shfl_xor((int)t, 1));
-            t = max(t, (uint32_t)__shfl_xor((int)t, 2));
-            t = max(t, (uint32_t)__shfl_xor((int)t, 4));
+            const uint32_t t = group8_max_u32(need ? bound() : 0u);
             float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
             if (lane < cnt) w = pts[s0 + lane];
             bool keep = false;
