@@ -72,6 +72,10 @@ struct Counters {
     // queries of this round's component search handed to the per-query pass
     // (k_comp_search_deferred), entries of the defer list
     uint32_t deferred;
+    // round 1's targets were written before the rounds (k_knn_first): round 1
+    // then needs no keep, prep, node labels or second key, and its hook takes
+    // every point with a target as its singleton component's winner
+    uint32_t knn_round1;
 };
 
 // Coincident-point groups.  Points with identical rescaled coordinates form a

@@ -1564,6 +1564,7 @@ __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __res
         C->round = 0; C->done = C->invalid ? 1u : 0u; C->edges_prev = 0; C->dup_settled = 0; C->nsamples = 0;
         C->comp_queries = 0;
         C->deferred = 0;
+        C->knn_round1 = 0;
         C->target = target; C->max_rounds = max_rounds;
     }
     if (i < kWorkKinds * kWorkSlots) C->work[i / kWorkSlots][i % kWorkSlots] = 0;
@@ -1576,6 +1577,9 @@ __global__ __launch_bounds__(kBlock) void k_round_init(int64_t n, int32_t* __res
     comp1[i] = kEmptyKey;
     comp2[i] = 0xffffffffu;
 }
+
+// round 1 with its targets from k_knn_first (Counters::knn_round1)
+__device__ __forceinline__ bool knn_round1(const Counters* C) { return C->round == 1u && C->knn_round1 != 0u; }
 
 // Termination is tested before a round (SolutionEMSTOperators.cu:194-227 and
 // the 100-round cap): the tree is complete or the cap is reached.  While
@@ -1832,7 +1836,7 @@ __global__ __launch_bounds__(kBlock) void k_round_comp2(int64_t n, const uint32_
                                                        const uint64_t* __restrict__ comp1,
                                                        uint32_t* __restrict__ comp2, const Counters* C)
 {
-    if (C->done) return;
+    if (C->done || knn_round1(C)) return;       // one member per component (see k_round_hook)
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n || corr[p] < 0) return;
     const int32_t lp = label[p];
@@ -1850,6 +1854,10 @@ __global__ __launch_bounds__(kBlock) void k_round_comp2(int64_t n, const uint32_
 // whose edge key (d2, min id, max id) equals the component's comp1 / comp2;
 // the pair of ids makes it unique, so the test is made here, for p and for
 // its target q, with no per-point flag between passes.
+// In round 1 every component is one point, whose edge is its own: with the
+// targets from k_knn_first, every point with a target wins, and a pair is
+// mutual when the target's target is the point; the component minima are
+// then not needed (k_comp_keep and k_round_comp2 skip that round).
 // Non-mutual: root(p) -> root(q).  Mutual pair (q won its component with an
 // edge back to p): the endpoint with the smaller id keeps its root.  Each
 // hook absorbs exactly one root and a root is absorbed at most once over the
@@ -1872,6 +1880,7 @@ __global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const uint
 {
     if (C->done) return;
     const int round = (int)C->round;
+    const bool first = knn_round1(C);
     __shared__ uint32_t cnt;
     if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
@@ -1879,17 +1888,20 @@ __global__ __launch_bounds__(kHookBlock) void k_round_hook(int64_t n, const uint
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t q = p < n ? corr[p] : -1;
     if (q >= 0) {
-        const int32_t rp = label[p];
+        const int32_t rp = first ? (int32_t)p : label[p];
         const uint64_t ck = corr_key[p];
         const uint32_t pid = ids[p], qid = (uint32_t)ck;
         const uint64_t key1 = edge_key1(ck, pid);
-        if (key1 == comp1[rp] && max(pid, qid) == comp2[rp] && !(cut && round > 1 && cut[rp])) {
-            const int32_t rq = label[q];
+        if (first || (key1 == comp1[rp] && max(pid, qid) == comp2[rp] && !(cut && round > 1 && cut[rp]))) {
+            const int32_t rq = first ? q : label[q];
             bool mutual = false;            // q won its component with the edge back to p
             if (corr[q] == (int32_t)p) {
-                const uint64_t cq = corr_key[q];
-                const uint32_t qq = ids[q];
-                mutual = edge_key1(cq, qq) == comp1[rq] && max(qq, (uint32_t)cq) == comp2[rq];
+                mutual = true;
+                if (!first) {
+                    const uint64_t cq = corr_key[q];
+                    const uint32_t qq = ids[q];
+                    mutual = edge_key1(cq, qq) == comp1[rq] && max(qq, (uint32_t)cq) == comp2[rq];
+                }
             }
             int32_t absorbed = -1;
             if (!mutual) {
@@ -1986,7 +1998,7 @@ __global__ __launch_bounds__(kBlock) void k_nlab_tree(const int32_t* __restrict_
                                                      LeafTable lt, Node* __restrict__ nodes,
                                                      unsigned long long* __restrict__ visit, const Counters* C)
 {
-    if (C->done) return;
+    if (C->done || knn_round1(C)) return;       // round 1 searches nothing
     __shared__ int32_t sub_lab[kBlock];
     const int64_t g0 = (int64_t)blockIdx.x * kBlock;
     const int cb = find_cloud_leaf(cl, ncl, g0);
@@ -2086,7 +2098,7 @@ __global__ __launch_bounds__(kBlock) void k_comp_keep(int64_t n, const uint32_t*
                                                      uint64_t* __restrict__ corr_key,
                                                      uint64_t* __restrict__ comp1, NbrList nb, const Counters* C)
 {
-    if (C->done) return;
+    if (C->done || knn_round1(C)) return;       // every target is outside and kept
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = false;
     int32_t lp = 0;
@@ -2165,7 +2177,7 @@ __global__ __launch_bounds__(kBlock) void k_comp_prep(int64_t n, const float4* _
     if (C->done) return;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool need = false;
-    if (p < n) {
+    if (p < n && !knn_round1(C)) {          // else every point holds its nearest neighbour
         const int32_t lp = label[p];
         const int32_t c = corr[p];
         if (c == -1 || (c >= 0 && label[c] == lp)) {
@@ -3130,6 +3142,7 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
     __shared__ float4 gbox[16];             // per 8-lane group: lo (xyz), hi (xyz)
     if (C->done) return;                    // a build whose input was refused on the device
     const int lane = __lane_id();
+    if (blockIdx.x == 0 && lane == 0) C->knn_round1 = 1u;
     const int64_t p = (int64_t)blockIdx.x * 64 + lane;
     const bool active = p < n;
     const float4 me = pts[active ? p : 0];
