@@ -2406,6 +2406,32 @@ __device__ __forceinline__ int gather_start(int64_t L, int32_t* work)
     return cnt;
 }
 
+// The gathering's next nodes from the inner nodes that passed: each hands on
+// its 4 grandchildren (one 128-byte line of the heap), or its 2 children when
+// those are leaves.  Every gathering test (box distance, far side, component
+// label, empty) is monotone under box inclusion, so a grandchild whose parent
+// would have failed fails too: the same nodes pass as level by level, in half
+// the dependent steps.  Writes the entries at work[tail ..] and their count to
+// add; false (nothing written) when they would pass cap.
+__device__ __forceinline__ bool push_inner(bool pass, int nd, int64_t L, int32_t* work, int tail, int cap,
+                                           int& add)
+{
+    const bool deep = pass && 2 * (int64_t)nd < L;     // its children are inner nodes
+    const uint64_t m4 = __ballot(deep), m2 = __ballot(pass && !deep);
+    add = 4 * __popcll(m4) + 2 * __popcll(m2);
+    if (tail + add > cap) return false;
+    const uint64_t lt = lanemask_lt();
+    const int r = tail + 4 * __popcll(m4 & lt) + 2 * __popcll(m2 & lt);
+    if (deep) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) work[r + k] = 4 * nd + k;
+    } else if (pass) {
+        work[r] = 2 * nd;
+        work[r + 1] = 2 * nd + 1;
+    }
+    return true;
+}
+
 // ascending bitonic sort of one 64-bit key per lane across the wave
 __device__ __forceinline__ uint64_t wave_sort_u64(uint64_t e)
 {
@@ -2675,20 +2701,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
                             pass_ = xl != kNlabEmpty && xl != skip_lab && bd <= B &&
                                     (int32_t)__float_as_uint(far) > imin && !(leaf && nd == pre);
                         }
-                        const uint64_t mi = __ballot(pass_ && !leaf), ml = __ballot(pass_ && leaf);
-                        const int ni = __popcll(mi), nlf = __popcll(ml);
-                        if (tail + 2 * ni > kGatherWork || ngl + nlf > kGatherLeaves) {
+                        const uint64_t ml = __ballot(pass_ && leaf);
+                        const int nlf = __popcll(ml);
+                        int add;
+                        if (!push_inner(pass_ && !leaf, nd, L, work, tail, kGatherWork, add) ||
+                            ngl + nlf > kGatherLeaves) {
                             over = true;
                             break;
                         }
-                        const uint64_t lt_ = lanemask_lt();
-                        if (pass_ && !leaf) {
-                            const int r = tail + 2 * __popcll(mi & lt_);
-                            work[r] = 2 * nd;
-                            work[r + 1] = 2 * nd + 1;
-                        }
-                        if (pass_ && leaf) gathered[ngl + __popcll(ml & lt_)] = ((uint64_t)bd << 32) | (uint32_t)nd;
-                        tail += 2 * ni;
+                        if (pass_ && leaf)
+                            gathered[ngl + __popcll(ml & lanemask_lt())] = ((uint64_t)bd << 32) | (uint32_t)nd;
+                        tail += add;
                         ngl += nlf;
                         __builtin_amdgcn_wave_barrier();
                     }
@@ -2996,20 +3019,17 @@ __global__ __launch_bounds__(64 * kDeferWaves) void k_comp_search_deferred(
                         leaf = nd >= L;
                         pass_ = xl != kNlabEmpty && xl != mylab && bd <= B && (int32_t)__float_as_uint(far) > inner;
                     }
-                    const uint64_t mi = __ballot(pass_ && !leaf), ml = __ballot(pass_ && leaf);
-                    const int ni = __popcll(mi), nlf = __popcll(ml);
-                    if (tail + 2 * ni > kGatherWork || ngl + nlf > kGatherLeaves) {
+                    const uint64_t ml = __ballot(pass_ && leaf);
+                    const int nlf = __popcll(ml);
+                    int add;
+                    if (!push_inner(pass_ && !leaf, nd, L, work, tail, kGatherWork, add) ||
+                        ngl + nlf > kGatherLeaves) {
                         over = true;
                         break;
                     }
-                    const uint64_t lt_ = lanemask_lt();
-                    if (pass_ && !leaf) {
-                        const int rr = tail + 2 * __popcll(mi & lt_);
-                        work[rr] = 2 * nd;
-                        work[rr + 1] = 2 * nd + 1;
-                    }
-                    if (pass_ && leaf) gathered[ngl + __popcll(ml & lt_)] = ((uint64_t)bd << 32) | (uint32_t)nd;
-                    tail += 2 * ni;
+                    if (pass_ && leaf)
+                        gathered[ngl + __popcll(ml & lanemask_lt())] = ((uint64_t)bd << 32) | (uint32_t)nd;
+                    tail += add;
                     ngl += nlf;
                     __builtin_amdgcn_wave_barrier();
                 }
@@ -3281,20 +3301,15 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
                 leaf = nd >= L;
                 pass = bd <= R && !(leaf && nd >= leaf_lo && nd <= leaf_hi);
             }
-            const uint64_t mi = __ballot(pass && !leaf), ml = __ballot(pass && leaf);
-            const int ni = __popcll(mi), nl = __popcll(ml);
-            if (tail + 2 * ni > kKnnGatherWork || ngl + nl > kKnnGatherLeaves) {
+            const uint64_t ml = __ballot(pass && leaf);
+            const int nl = __popcll(ml);
+            int add;
+            if (!push_inner(pass && !leaf, nd, L, work, tail, kKnnGatherWork, add) || ngl + nl > kKnnGatherLeaves) {
                 over = true;
                 break;
             }
-            const uint64_t lt = lanemask_lt();
-            if (pass && !leaf) {
-                const int r = tail + 2 * __popcll(mi & lt);
-                work[r] = 2 * nd;
-                work[r + 1] = 2 * nd + 1;
-            }
-            if (pass && leaf) gathered[ngl + __popcll(ml & lt)] = ((uint64_t)bd << 32) | (uint32_t)nd;
-            tail += 2 * ni;
+            if (pass && leaf) gathered[ngl + __popcll(ml & lanemask_lt())] = ((uint64_t)bd << 32) | (uint32_t)nd;
+            tail += add;
             ngl += nl;
             __builtin_amdgcn_wave_barrier();
         }
