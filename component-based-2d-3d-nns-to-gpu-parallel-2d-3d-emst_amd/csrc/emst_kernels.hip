@@ -2380,6 +2380,13 @@ constexpr int64_t kResumeBlocks = 4096;
 // so the hardware still balances the packets); the compaction clears the
 // mask, so a block need only write the words of the packets it searched
 constexpr int64_t kSearchBlocks = 65536;
+// Queries per packet.  A dispatch of few queries (rounds where the searching
+// points are scattered) has fewer 64-query packets than the chip holds waves,
+// and each packet walks the union of its queries' neighbourhoods: it takes
+// kSmallPacket queries per packet instead, so that more, narrower packets run
+// at once.
+constexpr int64_t kSmallDispatch = 786432;
+constexpr int kSmallPacket = 8;
 // the neighbour lists' gatherings (k_knn_first): their radius is the packet's
 // largest kKnn-th distance after its own leaves, so they reach further
 constexpr int kKnnGatherWork = 1024;
@@ -2480,14 +2487,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
     __shared__ unsigned long long gathered[kGatherLeaves];
     // one packet per block; a grid smaller than the packets (the resumed
     // dispatch, whose queries are few but unknown to the host) loops
+    const int qpp = nq <= kSmallDispatch ? kSmallPacket : 64;
     for (int64_t pk = blockIdx.x;; pk += gridDim.x) {
-        if (pk * 64 >= nq) {
+        if (pk * qpp >= nq) {
             if (open_out != nullptr && threadIdx.x == 0 && pk == (int64_t)blockIdx.x) open_out[pk] = 0ull;
             break;
         }
         const int lane = __lane_id();
-        const int64_t qi = pk * 64 + lane;
-        const bool active = qi < nq;
+        const int64_t qi = pk * qpp + lane;
+        const bool active = lane < qpp && qi < nq;
         const int64_t p = active ? (int64_t)qlist[qi] : 0;
         const float4 me = pts[p];
         const float px = me.x, py = me.y, pz = me.z;
@@ -2833,7 +2841,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
         }
         if (open_out != nullptr) {
             const uint64_t m = __ballot(left_open);
-            if (lane == 0) open_out[pk] = m;
+            if (qpp == 64) {
+                if (lane == 0) open_out[pk] = m;
+            } else if (lane == 0 && m != 0ull) {
+                // several packets share a mask word (the compaction cleared it)
+                atomicOr((unsigned long long*)&open_out[(pk * qpp) >> 6], (unsigned long long)(m << ((pk * qpp) & 63)));
+            }
         }
         if (collect && lane == 0) {
             const int slot = (int)(pk % kWorkSlots);
