@@ -527,6 +527,27 @@ __device__ __forceinline__ void wave_append_u64(bool take, uint64_t v, uint64_t*
     if (take) list[base + (uint32_t)__popcll(m & lanemask_lt())] = v;
 }
 
+// wave_append_u64 into a list of cap entries: one atomic per wave reserves
+// the slots, and the lanes whose slot lies past the end are left out (the
+// count may pass cap; its readers take min(count, cap), and every slot below
+// cap is written by the lane that reserved it).  Returns this lane's taking
+// part.  (A compare-and-swap loop that reserved only when all fitted queued
+// the thousands of waves of one round on the count: 48 ms a round.)
+__device__ __forceinline__ bool wave_append_capped_u64(bool take, uint64_t v, uint64_t* list, uint32_t* counter,
+                                                       uint32_t cap)
+{
+    const uint64_t m = __ballot(take);
+    if (m == 0ull) return false;
+    const int leader = __builtin_ctzll(m);
+    uint32_t base = 0;
+    if (__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, leader);
+    const uint32_t slot = base + (uint32_t)__popcll(m & lanemask_lt());
+    const bool in = take && slot < cap;
+    if (in) list[slot] = v;
+    return in;
+}
+
 // append the lanes holding `take` to list, one atomic per wave
 __device__ __forceinline__ void wave_append(bool take, int32_t v, int32_t* list, uint32_t* counter)
 {
@@ -2367,10 +2388,15 @@ __global__ __launch_bounds__(kBlock) void k_comp_samples(const int32_t* __restri
 // the work list, and gathered leaves as (box d2 bits << 32) | leaf
 constexpr int kGatherWork = 512;
 constexpr int kGatherLeaves = 128;
-// dispatches of the component search with at most this many queries (1024
-// packets, one per SIMD of the chip) hand their overflowing packets' queries
-// to the per-query pass (k_comp_search_deferred)
+// Packets whose gathering outgrows the LDS lists hand their open queries to
+// the per-query pass (k_comp_search_deferred) rather than walk the tree with
+// all of them: in 3D up to kDeferCap queries a round (a walk over the union
+// of 64 neighbourhoods is the tail of the large dispatches, and the per-query
+// pass takes about as long for 64k queries as one such walk), in 2D only in
+// dispatches of at most kDeferQueries queries (1024 packets, one per SIMD of
+// the chip; in the large 2D dispatches the walks cost less than the pass)
 constexpr int64_t kDeferQueries = 65536;
+constexpr uint32_t kDeferCap = 65536;
 // the resumed dispatch of the component search runs on at most this many
 // one-packet blocks, looping over its packets: its queries are few (most
 // close in the first pass), and a grid of one block per packet of the whole
@@ -2740,21 +2766,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
                         }
                     }
                 }
-                if (!done_by_gather && defer != nullptr && B < 0x7f800000u && nq <= kDeferQueries) {
-                    // a gathering that outgrew its lists, in a dispatch of fewer
-                    // packets than the chip has SIMDs: the dispatch takes as long
-                    // as its slowest packet, and this packet's walk would visit
-                    // the union of its queries' neighbourhoods in one wave; the
-                    // open queries go to the per-query pass instead, one wave
-                    // each, from this pass on (no leaf has been visited in it
-                    // yet, so their bests are empty)
-                    ++st_walks;
-                    wave_append_u64(open, ((uint64_t)(uint32_t)pass << 32) | (uint32_t)p, defer, &C->deferred);
-                    if (open) {
+                // a gathering that outgrew its lists: the dispatch takes as long
+                // as its slowest packet, and this packet's walk would visit the
+                // union of its queries' neighbourhoods in one wave; the open
+                // queries go to the per-query pass instead, one wave each, from
+                // this pass on (no leaf has been visited in it yet, so their
+                // bests are empty), while the round's list has room
+                bool handed = false;        // this lane's query went to the per-query pass
+                if (!done_by_gather && defer != nullptr && B < 0x7f800000u) {
+                    const uint64_t entry = ((uint64_t)(uint32_t)pass << 32) | (uint32_t)p;
+                    if (D == 3) {
+                        handed = wave_append_capped_u64(open, entry, defer, &C->deferred, kDeferCap);
+                    } else if (nq <= kDeferQueries) {
+                        wave_append_u64(open, entry, defer, &C->deferred);
+                        handed = open;
+                    }
+                    if (handed) {
                         open = false;
                         deferred = true;
                     }
-                } else if (!done_by_gather) {
+                }
+                const bool walk = !done_by_gather && __ballot(open) != 0ull;
+                if (!done_by_gather && !walk) {
+                    ++st_walks;
+                } else if (walk) {
                     ++st_walks;
                     if (lane == 0) {
                         stk_id[0] = 1;
@@ -2897,7 +2932,8 @@ __global__ __launch_bounds__(64 * kDeferWaves) void k_comp_search_deferred(
     int collect)
 {
     if (C->done) return;
-    const uint32_t cnt = C->deferred;
+    // (in 3D the count may pass the list's cap: the lanes past it walked)
+    const uint32_t cnt = D == 3 ? min(C->deferred, kDeferCap) : C->deferred;
     __shared__ int32_t work_w[kDeferWaves][kGatherWork];
     __shared__ unsigned long long gathered_w[kDeferWaves][kGatherLeaves];
     __shared__ int32_t stk_w[kDeferWaves][64];
