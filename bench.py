@@ -223,14 +223,17 @@ def main():
     else:
         # the NNS kernels of the component search, from one traced run (HIP
         # events around each dispatch on the context stream):
-        #   k_knn_first, once per EMST: 101 algorithmic bytes per point (its
-        #   point 16 in; 6 list entries of 12 bytes, target 12 and cursor 1 out);
+        #   k_knn_first, once per EMST: 16 + 12 * KL + 13 algorithmic bytes
+        #   per point (its point 16 in; KL list entries of 12 bytes, target 12
+        #   and cursor 1 out), KL = 10 in 3D and 6 in 2D (the library's list
+        #   length): 149 and 101;
         #   k_comp_search, two dispatches per round: 44 bytes per query and
         #   dispatch (point 16, label 4, previous key 8, list entry 4 in;
         #   result 12 out)
         t, st = traced_work(pkg, res["dev"], args.n, args.dim,
                             models["comp"][0] if head_name == "comp_learned" else None, 1, args.bucket)
-        kern = {"k_knn_first": {"alg_bytes": 101 * args.n, "ms": t[5]},
+        KL = 10 if args.dim == 3 else 6
+        kern = {"k_knn_first": {"alg_bytes": (16 + 12 * KL + 13) * args.n, "ms": t[5]},
                 "k_comp_search": {"alg_bytes": 44 * st["comp_queries"], "ms": t[4]}}
         work = {k: st[k] for k in ("comp_queries", "comp_nodes", "comp_leaves", "comp_pairs", "knn_nodes",
                                    "knn_leaves", "knn_pairs", "comp_walks", "knn_walks")}

@@ -273,8 +273,8 @@ int alloc_all(emst_ctx* c)
     if (comp) {
         ENS(list2, n);
         ENS(defer, n);
-        ENS(knn_pos, (size_t)kKnn * n);
-        ENS(knn_key, (size_t)kKnn * n);
+        ENS(knn_pos, (size_t)knn_len(c->dim) * n);
+        ENS(knn_key, (size_t)knn_len(c->dim) * n);
         ENS(knn_cur, n);
     }
     if (c->cdim < c->dim) {

@@ -253,15 +253,20 @@ struct CompSeed {
     int64_t per_round;      // pairs logged per round, at most (a stride over the searching points)
     uint32_t* nq_keep;      // 2 words: the counts the sample log and the model run on
 };
-// Neighbour lists of the component search: every point's kKnn nearest other
+// Neighbour lists of the component search: every point's K nearest other
 // points of its cloud under (d2, id), as positions and keys, list entry k of
 // point p at k * n + p (position -1, key all ones: the cloud has no more
-// points), and per point the entry its current target came from (kKnn: the
-// list is used up).
-constexpr int kKnn = 6;
+// points), and per point the entry its current target came from (K: the list
+// is used up).  K = knn_len(dim): longer lists in 3D, where they answer the
+// searches of the early rounds (clustered 3D 10^7: 6 entries 14.6 ms per
+// EMST, 8 13.7, 10 13.5, 12 14.9); 2D is fastest at 6 (12.3 ms; 8: 12.4,
+// 10: 12.6).
+constexpr int kKnn2 = 6;
+constexpr int kKnn3 = 10;
+__host__ __device__ constexpr int knn_len(int dim) { return dim == 3 ? kKnn3 : kKnn2; }
 struct NbrList {
-    int32_t* pos;           // kKnn * n
-    uint64_t* key;          // kKnn * n
+    int32_t* pos;           // K * n
+    uint64_t* key;          // K * n
     uint8_t* cur;           // n
 };
 void launch_comp_round(int dim, int64_t n, const float4* pts, const uint32_t* ids, Node* nodes,

@@ -2113,6 +2113,7 @@ __global__ __launch_bounds__(kBlock) void k_nmin_tree(const float4* __restrict__
 //               on the d2 of the nearest outside point (all ones: none);
 //   corr == -2  nothing outside the component in its cloud (final).
 // Keep-previous: the component minimum of the kept targets.
+template <int K>
 __global__ __launch_bounds__(kBlock) void k_comp_keep(int64_t n, const uint32_t* __restrict__ ids,
                                                      const int32_t* __restrict__ label,
                                                      int32_t* __restrict__ corr,
@@ -2127,7 +2128,7 @@ __global__ __launch_bounds__(kBlock) void k_comp_keep(int64_t n, const uint32_t*
     if (p < n) {
         lp = label[p];
         int32_t c = corr[p];
-        const int j0 = nb.pos != nullptr ? (int)nb.cur[p] : kKnn;
+        const int j0 = nb.pos != nullptr ? (int)nb.cur[p] : K;
         bool outside = c >= 0 && label[c] != lp;
         uint64_t ck = 0;
         bool have_ck = false;
@@ -2137,25 +2138,25 @@ __global__ __launch_bounds__(kBlock) void k_comp_keep(int64_t n, const uint32_t*
         // remaining entries, then their labels, load at once: a scan entry by
         // entry waited on two dependent loads per entry (the kernel took up to
         // 0.36 ms per round at 10^7 points, when every target has just joined).
-        if (c >= 0 && !outside && j0 < kKnn) {
-            int32_t e[kKnn], le[kKnn];
+        if (c >= 0 && !outside && j0 < K) {
+            int32_t e[K], le[K];
 #pragma unroll
-            for (int k = 1; k < kKnn; ++k) e[k] = k > j0 ? nb.pos[(int64_t)k * n + p] : -1;
+            for (int k = 1; k < K; ++k) e[k] = k > j0 ? nb.pos[(int64_t)k * n + p] : -1;
             // (unconditional loads, an absent entry reading p's own label, so
             // that they issue together rather than one branch and wait each)
 #pragma unroll
-            for (int k = 1; k < kKnn; ++k) le[k] = label[e[k] >= 0 ? e[k] : (int32_t)p];
-            int j = kKnn;
+            for (int k = 1; k < K; ++k) le[k] = label[e[k] >= 0 ? e[k] : (int32_t)p];
+            int j = K;
             int32_t f = -1;
 #pragma unroll
-            for (int k = kKnn - 1; k >= 1; --k)
+            for (int k = K - 1; k >= 1; --k)
                 if (k > j0 && (e[k] < 0 || le[k] != lp)) {
                     j = k;
                     f = e[k];
                 }
-            if (j == kKnn) {
+            if (j == K) {
                 c = -1;
-                ck = (nb.key[(int64_t)(kKnn - 1) * n + p] & 0xffffffff00000000ull) | 0xffffffffull;
+                ck = (nb.key[(int64_t)(K - 1) * n + p] & 0xffffffff00000000ull) | 0xffffffffull;
             } else if (f < 0) {
                 c = -2;             // the cloud's other points are all listed, all inside
                 ck = kEmptyKey;
@@ -2414,7 +2415,7 @@ constexpr int64_t kSearchBlocks = 65536;
 constexpr int64_t kSmallDispatch = 786432;
 constexpr int kSmallPacket = 8;
 // the neighbour lists' gatherings (k_knn_first): their radius is the packet's
-// largest kKnn-th distance after its own leaves, so they reach further
+// largest K-th distance after its own leaves, so they reach further
 constexpr int kKnnGatherWork = 1024;
 constexpr int kKnnGatherLeaves = 256;
 
@@ -3163,13 +3164,13 @@ __global__ __launch_bounds__(64 * kDeferWaves) void k_comp_search_deferred(
 
 // Round 1 of the component search and its neighbour lists.  Every component
 // is one point, so a point's nearest outside point is its nearest neighbour.
-// One wave per 64 consecutive positions keeps each lane's kKnn smallest keys
+// One wave per 64 consecutive positions keeps each lane's K smallest keys
 // (d2, id) in registers.  The leaves holding the packet's own points come
-// first; then every lane's kKnn-th key bounds its search.  The packet
+// first; then every lane's K-th key bounds its search.  The packet
 // gathers, 64 nodes per step (one box-to-box test per lane), every other leaf
 // whose box lies within the largest of those bounds of the packet's box, and
 // visits them nearest first.  At a leaf, a lane takes part only while the
-// leaf box may still beat its kKnn-th key, and a leaf point enters the pair
+// leaf box may still beat its K-th key, and a leaf point enters the pair
 // loop only when it lies within the bound of an 8-lane group of the packet
 // that takes part (the group's box and its largest bound).  A packet whose
 // gathering outgrows the LDS lists walks the hierarchy per lane instead,
@@ -3178,7 +3179,7 @@ __global__ __launch_bounds__(64 * kDeferWaves) void k_comp_search_deferred(
 //
 // The list holds keys only, biased by 2^52: (d2 bits << 32 | id) + 2^52 with
 // d2 bits <= 0x7f800000 is a positive normal double whose order is the key's,
-// so an insertion is a min/max network of kKnn FP64 min/max pairs without
+// so an insertion is a min/max network of K FP64 min/max pairs without
 // branches or selects (the empty key is the largest finite double).  The
 // positions come from inv at the end.
 constexpr uint64_t kKnnBias = 1ull << 52;
@@ -3217,18 +3218,19 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
     const float4 me = pts[active ? p : 0];
     const float px = me.x, py = me.y, pz = me.z;
     const int my_cl = active ? find_cloud(clouds, ncl, p) : -1;
-    double kd[kKnn];                // biased keys as doubles, ascending
+    constexpr int K = knn_len(D);   // list length
+    double kd[K];                   // biased keys as doubles, ascending
 #pragma unroll
-    for (int i = 0; i < kKnn; ++i) kd[i] = __longlong_as_double((long long)kKnnEmpty);
-    auto last = [&]() { return (uint64_t)__double_as_longlong(kd[kKnn - 1]); };
+    for (int i = 0; i < K; ++i) kd[i] = __longlong_as_double((long long)kKnnEmpty);
+    auto last = [&]() { return (uint64_t)__double_as_longlong(kd[K - 1]); };
     // sorted insertion of a biased key (keys are distinct: the id is in the
     // low word)
     auto insert = [&](uint64_t key) {
         double x = __longlong_as_double((long long)key);
 #pragma unroll
-        for (int i = 0; i < kKnn; ++i) kd[i] = minmax_f64(kd[i], x);
+        for (int i = 0; i < K; ++i) kd[i] = minmax_f64(kd[i], x);
     };
-    // the kKnn-th d2 bits (an empty list: above every finite d2)
+    // the K-th d2 bits (an empty list: above every finite d2)
     auto bound = [&]() { return (uint32_t)(last() >> 32) - kKnnBiasHi; };
     uint32_t st_nodes = 0, st_leaves = 0, st_steps = 0, st_walks = 0;   // work counters of this packet
     uint64_t pending = __ballot(active);
@@ -3280,7 +3282,7 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
             }
         };
         // the packet's own leaves: a leaf point farther from the packet's box
-        // than every lane's kKnn-th d2 cannot enter a list
+        // than every lane's K-th d2 cannot enter a list
         auto scan_own = [&](int node) {
             int64_t s0, s1;
             leaf_range(lt, lfirst, nl, cend, node - L, s0, s1);
@@ -3300,7 +3302,7 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
             pairs(s0, __ballot(keep), true);
         };
         // another leaf, for the lanes of needm: a leaf point is paired when it
-        // lies within the largest kKnn-th d2 of a taking-part group's lanes
+        // lies within the largest K-th d2 of a taking-part group's lanes
         // from that group's box
         auto scan_other = [&](int node, bool need, uint64_t needm) {
             int64_t s0, s1;
@@ -3328,7 +3330,7 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
         const int64_t pf = (int64_t)blockIdx.x * 64 + first, pl = (int64_t)blockIdx.x * 64 + last;
         const int leaf_lo = (int)(L + (lt.of[pf] - lfirst)), leaf_hi = (int)(L + (lt.of[pl] - lfirst));
         for (int lf = leaf_lo; lf <= leaf_hi; ++lf) scan_own(lf);
-        // gather the other leaves within the largest kKnn-th d2 of the packet's box
+        // gather the other leaves within the largest K-th d2 of the packet's box
         uint32_t R = mine ? bound() : 0u;
         for (int off = 32; off > 0; off >>= 1) R = max(R, (uint32_t)__shfl_xor((int)R, off));
         int head = 0, tail = gather_start(L, work), ngl = 0;
@@ -3472,7 +3474,7 @@ __global__ __launch_bounds__(64) void k_knn_first(int64_t n, const float4* __res
     int32_t pos0 = -2;
     uint64_t key0 = kEmptyKey;
 #pragma unroll
-    for (int i = 0; i < kKnn; ++i) {
+    for (int i = 0; i < K; ++i) {
         const uint64_t b = (uint64_t)__double_as_longlong(kd[i]);
         const bool has = b != kKnnEmpty;
         const uint64_t key = has ? b - kKnnBias : kEmptyKey;
@@ -4018,7 +4020,8 @@ void launch_comp_round(int dim, int64_t n, const float4* pts, const uint32_t* id
     hipLaunchKernelGGL(k_nlab_tree, dim3(nblocks(nleaves)), dim3(kBlock), 0, s, label, cl, ncl, nleaves, lt,
                        nodes, visit, (const Counters*)C);
     const int64_t P = (n + 63) / 64;
-    hipLaunchKernelGGL(k_comp_keep, dim3(g), dim3(kBlock), 0, s, n, ids, label, corr, corr_key, comp1, nb,
+    hipLaunchKernelGGL(dim == 3 ? k_comp_keep<kKnn3> : k_comp_keep<kKnn2>, dim3(g), dim3(kBlock), 0, s, n, ids,
+                       label, corr, corr_key, comp1, nb,
                        (const Counters*)C);
     hipLaunchKernelGGL(k_comp_prep, dim3(g), dim3(kBlock), 0, s, n, pts, label, corr, (const uint64_t*)corr_key,
                        (const uint64_t*)comp1, sc.open_mask, (const Counters*)C);

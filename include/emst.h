@@ -78,8 +78,9 @@ typedef struct {
                                (NodeSpiralSearchMD, NodeEMST.h:67-509, selected in the reference by
                                EMST_SPIRAL_SEARCH_KD_SLAB_2 0, SolutionEMST.h:55-56).  Both build
                                the same forests.  1 has no cone candidates and no 2.5D mode; it
-                               keeps each point's 6 nearest neighbours from the build (73 bytes
-                               per point), which answer the early rounds without a search; its
+                               keeps each point's nearest neighbours from the build (10 in 3D, 121
+                               bytes per point; 6 in 2D, 73 bytes), which answer the early rounds
+                               without a search; its
                                learned seed predicts each component's edge length per round */
     int32_t log_samples;    /* 1 (round_search 1): log the learned seed's training pairs during
                                runs (emst_get_radius_samples) */

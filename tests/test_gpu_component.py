@@ -246,16 +246,17 @@ def test_many_random_clouds_against_oracle(pkg, oracle_mod):
 
 @pytest.mark.parametrize("dim", [2, 3])
 def test_neighbour_lists_short_and_used_up(pkg, oracle_mod, dim):
-    """The round-1 lists hold each point's 6 nearest other points.  Clouds of
-    2-12 points leave them short (nothing outside once the listed points are
-    all inside), exactly full, or used up; 500 tight clusters of 10 points use
-    every list up inside its own cluster by round 2, so the searches start
-    from the lists' lower bounds.  Edges, per-round sizes and labels equal
-    the oracle's."""
-    cases = [pkg.synth(int(k % 3), k, dim, 1200 + k) for k in range(2, 13)]
+    """The round-1 lists hold each point's K nearest other points (K = 6 in
+    2D, 10 in 3D).  Clouds of 2 to K + 6 points leave them short (nothing
+    outside once the listed points are all inside), exactly full, or used up;
+    500 tight clusters of K + 4 points use every list up inside its own
+    cluster by round 2, so the searches start from the lists' lower bounds.
+    Edges, per-round sizes and labels equal the oracle's."""
+    K = 10 if dim == 3 else 6
+    cases = [pkg.synth(int(k % 3), k, dim, 1200 + k) for k in range(2, K + 7)]
     rng = np.random.default_rng(59 + dim)
     centres = rng.random((500, dim)).astype(np.float32)
-    jitter = (rng.random((500, 10, dim)) * 1e-4).astype(np.float32)
+    jitter = (rng.random((500, K + 4, dim)) * 1e-4).astype(np.float32)
     cases.append((centres[:, None, :] + jitter).reshape(-1, dim))
     for pts in cases:
         g = run(pkg, pts, round_search=1)
