@@ -84,12 +84,13 @@ struct emst_ctx {
     int ncl = 0;
     std::vector<Cloud> hcl;
     int64_t nleaves = 0, Lmax = 0;
-    int mbits = 21;                   // Morton bits per axis
+    int mbits = 16;                   // Morton bits per axis
     int64_t target_edges = 0;         // sum over clouds of (points - 1)
     const float* xyz = nullptr;       // input points on the device (AoS)
     bool have_points = false;
     bool built = false;
     bool finished = false;
+    uint64_t alloc_gen = 0;           // counts the buffer (re)allocations: part of every graph's key
     int rounds = 0;
     uint32_t edges = 0;
     uint32_t zero_or_empty = 0;
@@ -198,6 +199,7 @@ int ensure(emst_ctx* c, DevBuf<T>& b, size_t count)
         return fail(c, EMST_E_OOM, "hipMalloc of " + std::to_string(sizeof(T) * count) + " bytes failed");
     }
     b.cap = count;
+    ++c->alloc_gen;
     return EMST_OK;
 }
 
@@ -330,11 +332,17 @@ int layout_clouds(emst_ctx* c, int bucket)
         target += std::max<int64_t>(cl.n - 1, 0);
     }
     c->target_edges = target;
-    // Morton bits per axis: the cloud index takes the top key bits of a batch
+    // Morton bits per axis: the cloud index takes the top key bits of a batch.
+    // Keys of 48 bits (16 per axis in 3D, 24 in 2D, fewer in a batch) sort in
+    // 6 radix passes instead of 8; the leaves stay the same cells wherever a
+    // cell holds at most a leaf's points, down to 2^-16 of the box (a longer
+    // run of equal keys is cut by count, as a run of coincident points is):
+    // 10^7 points 0.2 ms faster per EMST in 2D and 3D.  Batches of more than
+    // 2^16 clouds use 64 bits.
     int cb = 0;
     if (c->ncl > 1)
         while ((1ll << cb) < c->ncl) ++cb;
-    c->mbits = std::min(c->dim == 3 ? 21 : 32, (64 - cb) / c->dim);
+    c->mbits = std::min(c->dim == 3 ? 16 : 24, ((cb <= 16 ? 48 : 64) - cb) / c->dim);
     int rc;
     if ((rc = alloc_all(c)) || (rc = ensure(c, c->dcl, (size_t)c->ncl)) ||
         (rc = ensure(c, c->params, (size_t)c->ncl)))
@@ -861,7 +869,7 @@ std::vector<uintptr_t> graph_key(emst_ctx* c)
             (uintptr_t)c->corr_key.p, (uintptr_t)c->comp1.p, (uintptr_t)c->comp2.p,
             (uintptr_t)c->eu.p, (uintptr_t)c->ev_.p, (uintptr_t)c->ed2.p, (uintptr_t)c->eround.p,
             (uintptr_t)c->inv.p, (uintptr_t)c->rkey.p, (uintptr_t)c->rpos.p, (uintptr_t)c->cursor.p, (uintptr_t)d.rep, (uintptr_t)d.list, (uintptr_t)d.gslot, (uintptr_t)d.m2,
-            (uintptr_t)d.bkey, (uintptr_t)d.bpos};
+            (uintptr_t)d.bkey, (uintptr_t)d.bpos, (uintptr_t)c->ids.p, (uintptr_t)c->alloc_gen};
 }
 
 int run_graph(emst_ctx* c)

@@ -1663,8 +1663,10 @@ __global__ __launch_bounds__(kBlock) void k_flag_research(int K, int64_t n,
     }
 }
 
-// Wave pre-reduction: lanes sharing the wave leader's label merge into one
-// atomic; the rest issue their own.
+// Component minima: one atomic per run of equal labels in a wave
+// (atomic_min_grouped).  In round 2 of the clustered 10^7 cloud, with
+// components of about 3 points, per-lane atomics took 0.15 of the keep pass's
+// 0.35 ms.
 // The minima only decrease, so a value read (even a stale one) that is
 // already <= the key makes the atomic redundant: late rounds, whose few large
 // components would otherwise take one atomic per wave on the same address,
@@ -1676,23 +1678,27 @@ __device__ __forceinline__ void atomic_min_if_lower(uint64_t* a, uint64_t key)
 
 __device__ __forceinline__ void atomic_min_grouped(uint64_t* arr, int32_t lab, uint64_t key, bool valid)
 {
-    const uint64_t vm = __ballot(valid);
-    if (vm == 0ull) return;
-    const int leader = __builtin_ctzll(vm);
-    const int32_t llab = __shfl(lab, leader);
-    const bool grp = valid && lab == llab;
-    const uint64_t gm = __ballot(grp);
-    uint64_t v = grp ? key : kEmptyKey;
-    if (__popcll(gm) > 1) {
-        for (int off = 32; off > 0; off >>= 1) {
-            const uint64_t o = (uint64_t)__shfl_xor((long long)v, off);
-            v = o < v ? o : v;
-        }
-        if (__lane_id() == leader) atomic_min_if_lower(&arr[llab], v);
-        if (valid && !grp) atomic_min_if_lower(&arr[lab], key);
-    } else if (valid) {
-        atomic_min_if_lower(&arr[lab], key);
+    if (__ballot(valid) == 0ull) return;
+    // runs of equal labels across consecutive lanes (the lanes hold
+    // consecutive positions or queries, and a component's points lie close
+    // together in position order): a segmented minimum from the right leaves
+    // each run's minimum in its first lane, which alone updates the entry;
+    // a lane past the run reads its own label, so runs never merge
+    const int lane = __lane_id();
+    const int32_t prev = __shfl_up(lab, 1);
+    const uint64_t heads = __ballot(lane == 0 || prev != lab) | 1ull;
+    const int seg = 63 - __builtin_clzll(heads & (~0ull >> (63 - lane)));
+    uint64_t v = valid ? key : kEmptyKey;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int src = lane + d < 64 ? lane + d : lane;
+        const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src);
+        const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src);
+        const int oseg = __shfl(seg, src);
+        const uint64_t o = ((uint64_t)hi << 32) | lo;
+        if (oseg == seg && o < v) v = o;
     }
+    if (seg == lane && v != kEmptyKey) atomic_min_if_lower(&arr[lab], v);
 }
 
 // Each point's cone candidates in key order, for the rounds (bitonic network
@@ -2914,6 +2920,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
 constexpr int kDeferWaves = 4;          // waves per block
 constexpr int kDeferBlocks = 2048;      // a grid that fills the chip once; waves loop over the queries
 
+// the first active lane's v in every lane.  A load of one address by the
+// whole wave is served in parts, so while other waves change the word its
+// lanes can read different values; a value that steers the wave's control
+// flow must then be one lane's (a partly active wave would also read zeros
+// from the idle lanes in the shuffles of wave_min_u64).
+__device__ __forceinline__ uint64_t wave_first_u64(uint64_t v)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
 {
 #pragma unroll
@@ -3006,7 +3024,9 @@ __global__ __launch_bounds__(64 * kDeferWaves) void k_comp_search_deferred(
         };
         for (int pass = pass0; pass <= kMaxCappedPasses && open; ++pass) {
             const bool last = pass == kMaxCappedPasses;
-            const int32_t cbb = key_bound(comp1[mylab]);
+            // (other waves lower comp1[mylab] meanwhile, and the lanes of one
+            // load can see different values: one lane's steers the wave)
+            const int32_t cbb = key_bound(wave_first_u64(comp1[mylab]));
             const int32_t cap = last || cbb != 0x7fffffff ? 0x7fffffff : (int32_t)__float_as_uint(r2);
             if (cbb <= inner) {
                 open = false;

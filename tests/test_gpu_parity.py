@@ -198,7 +198,7 @@ def _morton_keys(resc):
     """Host restatement of the Morton quantisation (per-axis cell index)."""
     resc = np.asarray(resc, np.float32)
     dim = resc.shape[1]
-    maxq = (1 << (21 if dim == 3 else 32)) - 1
+    maxq = (1 << (16 if dim == 3 else 24)) - 1
     mx = np.float32(resc.max())
     qs = np.float32(maxq / np.float64(mx))
     q = np.minimum((resc * qs).astype(np.float64), maxq).astype(np.uint64)
