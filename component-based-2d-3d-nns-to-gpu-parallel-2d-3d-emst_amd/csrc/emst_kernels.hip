@@ -289,6 +289,18 @@ __global__ __launch_bounds__(kBlock) void k_rescale_morton(const float* __restri
     keys[i] = code;
 }
 
+// the first active lane's v in every lane.  A load of one address by the
+// whole wave is served in parts, so while other waves change the word its
+// lanes can read different values; a value that steers the wave's control
+// flow must then be one lane's (a partly active wave would also read zeros
+// from the idle lanes in the shuffles of wave_min_u64).
+__device__ __forceinline__ uint64_t wave_first_u64(uint64_t v)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // ------------------------------------------------------------------ scan
 // Exclusive scan of uint32 in three passes: per-block scan of 1024 elements,
 // scan of block totals (one block, chunked), add-back.
@@ -1416,8 +1428,21 @@ __global__ __launch_bounds__(kBlock) void k_open_scan_local(const uint64_t* __re
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// (one wave per packet, grid-stride: the lists are built every round, and a
-// launch of one block per packet costs more than its work when few are open)
+// the tile totals of tiles [0, t) summed by the whole wave (a few hundred
+// tiles at most for 10^8 points: cheaper than a launch that scans them)
+__device__ __forceinline__ uint32_t tiles_before(const uint32_t* __restrict__ sums, int64_t t)
+{
+    uint32_t a = 0;
+    for (int64_t j = __lane_id(); j < t; j += 64) a += sums[j];
+    for (int o = 32; o > 0; o >>= 1) a += (uint32_t)__shfl_xor((int)a, o);
+    return a;
+}
+
+// (each wave takes a run of consecutive packets, the grid at most 2048
+// blocks: the lists are built every round, and a launch of one block per
+// packet costs more than its work when few are open; sums holds the tile
+// totals of k_open_scan_local: each wave adds up the ones before its run's
+// first tile, then the tiles it crosses)
 __global__ __launch_bounds__(kBlock) void k_open_fill_scan(uint64_t* __restrict__ open_mask, int64_t P,
                                                           uint32_t* __restrict__ off,
                                                           const uint32_t* __restrict__ sums,
@@ -1426,14 +1451,27 @@ __global__ __launch_bounds__(kBlock) void k_open_fill_scan(uint64_t* __restrict_
     const int lane = __lane_id();
     const int64_t waves = (int64_t)gridDim.x * (kBlock / 64);
     const int64_t w0 = (int64_t)blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
-    if (w0 == 0 && lane == 0) off[P] += sums[P / kScanTile];
-    for (int64_t b = w0; b < P; b += waves) {
-        const uint64_t m = open_mask[b];
+    if (w0 == 0) {
+        const uint32_t pre = tiles_before(sums, P / kScanTile);
+        if (lane == 0) off[P] += pre;
+    }
+    const int64_t per = (P + waves - 1) / waves;
+    const int64_t b0 = w0 * per, b1 = min(P, b0 + per);
+    if (b0 >= b1) return;
+    int64_t t = b0 / kScanTile;
+    uint32_t pre = tiles_before(sums, t);
+    for (int64_t b = b0; b < b1; ++b) {
+        if (b / kScanTile != t) {
+            pre += sums[t];
+            ++t;
+        }
+        const uint64_t m = wave_first_u64(open_mask[b]);
+        if (m == 0ull) continue;
         // read once: cleared, so that the search that writes the next mask
         // need only write the words of its packets
-        if (lane == 0 && m != 0ull) open_mask[b] = 0ull;
+        if (lane == 0) open_mask[b] = 0ull;
         if (!((m >> lane) & 1ull)) continue;
-        const uint32_t o = off[b] + sums[b / kScanTile] + (uint32_t)__popcll(m & lanemask_lt());
+        const uint32_t o = off[b] + pre + (uint32_t)__popcll(m & lanemask_lt());
         dst[o] = src != nullptr ? src[b * 64 + lane] : (int32_t)(b * 64 + lane);
     }
 }
@@ -1443,7 +1481,6 @@ void open_compact(uint64_t* open_mask, int64_t P, uint32_t* off, uint32_t* sums,
 {
     const int64_t nb = (P + 1 + kScanTile - 1) / kScanTile;
     hipLaunchKernelGGL(k_open_scan_local, dim3((unsigned)nb), dim3(kBlock), 0, s, open_mask, P, off, sums);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, sums, nb);
     const unsigned gf = (unsigned)std::min<int64_t>((P + kBlock / 64 - 1) / (kBlock / 64), 2048);
     hipLaunchKernelGGL(k_open_fill_scan, dim3(gf), dim3(kBlock), 0, s, open_mask, P, off, (const uint32_t*)sums,
                        src, dst);
@@ -2946,17 +2983,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
 constexpr int kDeferWaves = 4;          // waves per block
 constexpr int kDeferBlocks = 2048;      // a grid that fills the chip once; waves loop over the queries
 
-// the first active lane's v in every lane.  A load of one address by the
-// whole wave is served in parts, so while other waves change the word its
-// lanes can read different values; a value that steers the wave's control
-// flow must then be one lane's (a partly active wave would also read zeros
-// from the idle lanes in the shuffles of wave_min_u64).
-__device__ __forceinline__ uint64_t wave_first_u64(uint64_t v)
-{
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
 
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
 {
